@@ -1,0 +1,49 @@
+"""CPU FP64 oracle for the per-Newton-iteration assembly path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (``xkep-cae_amd/``) imports,
+calls or links anything under ``oracle/``.  The only permitted users are
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py``, and there only as the checker (or the timed CPU baseline), never
+as the thing measured or shipped.
+
+The oracle is a NumPy FP64 restatement of the reference's algorithm
+(gyp0bt/xkep-cae, pure Python/NumPy).  Each function cites the reference
+file:line it follows.  The reference's per-pair Python loops are restated as
+vectorised NumPy over a structure-of-arrays pair state; every per-pair update in
+those loops is independent of the other pairs, so the vectorised form computes
+the same values (the only difference is floating-point summation order in the
+global scatter, which the reference itself leaves to ``np.add.at`` / scipy).
+
+Modules
+-------
+beam      CR Timoshenko element f_int / analytical tangent / lumped mass
+contact   broadphase, candidate filters, narrowphase, frame transport,
+          hysteresis, Huber normal force, Coulomb return mapping, friction
+          tangents (material + geometric)
+assembly  per-iteration composition R = f_int - (f_c + f_fric) - f_ext,
+          K_T = K_beam + K_c - K_fric, Generalized-alpha effective terms
+newton    restated dynamic Newton driver (host spsolve) for Newton-count checks
+
+Pinning status of each function is listed in ``PINNING`` below and checked by
+``tests/test_oracle_*.py``.
+"""
+
+PINNING = {
+    # function: how it is pinned (closed forms / brute force / golden vectors)
+    "beam.cr_beam_element_batch.f": "rigid-body invariance, zero at u=0, small-u limit K_lin*u, "
+    "axial PL/EA, golden vectors from the reference",
+    "beam.cr_beam_element_batch.K": "small-u limit T^T Ke T, symmetry, central FD of f (rtol 1e-4), "
+    "golden vectors from the reference (exact algebra)",
+    "beam.rotvec_to_rotmat/rotmat_to_rotvec": "exp(log(R)) = R, orthogonality, Rodrigues closed form",
+    "beam.lumped_mass_diag": "closed form m/2, rho*Ip*L/2, m L^2/78 per node",
+    "contact.broadphase_aabb": "brute-force O(n^2) inclusive AABB overlap; cell-size invariance",
+    "contact.midpoint_prescreen": "golden vectors from the reference (DetectCandidates)",
+    "contact.closest_point_segments": "dense (s,t) grid minimisation; parallel/perpendicular closed forms",
+    "contact.build_contact_frame": "orthonormality, transport of t1 under rotation, golden vectors",
+    "contact.update_geometry": "hysteresis truth table (g_on/g_off), golden vectors",
+    "contact.huber_*": "closed-form Huber values/derivative, FD of f_c at fixed frame, golden vectors",
+    "contact.friction_*": "return-map closed forms (stick/slip, |q|=mu p_n), D_t eigenvalues, "
+    "golden vectors (sign quirks of the material block)",
+    "assembly.*": "golden vectors of R and K_T from the reference on a 7-wire state",
+    "newton.*": "Newton-iteration counts and active-set trajectory from the reference",
+}

@@ -1,0 +1,220 @@
+"""Write the golden fixtures of the assembly path from the reference itself.
+
+Runs ONLY in the build container, where the reference checkout is mounted at
+/root/reference (read-only; bytecode writing is disabled).  It imports the
+reference's own classes, feeds them inputs made by this repository's strand
+generator, and stores inputs and outputs as compressed ``.npz`` data next to
+this script.  The fixtures pin what closed forms leave open at the 1e-10 bar:
+the exact algebra of the symmetrised CR analytical tangent, the friction
+tangents (sign convention of the material block, geometric block), the
+friction-loop state quirks (stale p_n, status rewrite, z_t overwrite) and the
+composed R / K_T of whole Newton attempts across two increments.
+
+Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import importlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+
+sys.dont_write_bytecode = True
+sys.path.insert(0, REPO)
+sys.path.insert(0, REF)
+
+mesh_mod = importlib.import_module("xkep-cae_amd.mesh")
+
+from xkep_cae.contact._contact_pair import _ContactConfigInput, _ContactManagerInput  # noqa: E402
+from xkep_cae.contact._manager_process import (  # noqa: E402
+    DetectCandidatesInput,
+    DetectCandidatesProcess,
+    UpdateGeometryInput,
+    UpdateGeometryProcess,
+)
+from xkep_cae.contact.contact_force.strategy import HuberContactForceProcess  # noqa: E402
+from xkep_cae.contact.friction.strategy import CoulombReturnMappingProcess  # noqa: E402
+from xkep_cae.contact.solver._newton_steps import (  # noqa: E402
+    ContactForceAssemblyInput,
+    ContactForceAssemblyProcess,
+    TangentAssemblyInput,
+    TangentAssemblyProcess,
+)
+from xkep_cae.elements._beam_assembler import ULCRBeamAssembler  # noqa: E402
+from xkep_cae.elements._beam_cr import (  # noqa: E402
+    timo_beam3d_cr_internal_force,
+    timo_beam3d_cr_tangent_analytical,
+)
+from xkep_cae.time_integration.strategy import GeneralizedAlphaProcess  # noqa: E402
+from xkep_cae.core.diagnostics import ProcessExecutionLog  # noqa: E402
+
+# The reference writes a usage report under its own docs/ at interpreter exit;
+# the checkout must stay untouched, so that report is switched off here.
+ProcessExecutionLog.instance()._report_on_exit = False
+
+E, NU, RHO = 200e3, 0.3, 7.85e-9
+
+
+def section(r):
+    d = 2 * r
+    A = np.pi * r * r
+    Iv = np.pi * d**4 / 64.0
+    J = np.pi * d**4 / 32.0
+    kap = 6.0 * (1.0 + NU) / (7.0 + 6.0 * NU)
+    return dict(E=E, G=E / (2 * (1 + NU)), A=A, Iy=Iv, Iz=Iv, J=J, kappa_y=kap, kappa_z=kap)
+
+
+def csr_arrays(prefix, K):
+    K = K.tocsr()
+    K.sort_indices()
+    return {f"{prefix}_data": K.data, f"{prefix}_indices": K.indices.astype(np.int32),
+            f"{prefix}_indptr": K.indptr.astype(np.int64), f"{prefix}_shape": np.array(K.shape)}
+
+
+def pair_arrays(prefix, pairs):
+    key = np.array([(min(p.elem_a, p.elem_b), max(p.elem_a, p.elem_b)) for p in pairs], np.int64)
+    order = np.lexsort((key[:, 1], key[:, 0]))
+    P = [pairs[i] for i in order]
+    st = [p.state for p in P]
+    return {
+        f"{prefix}_key": key[order],
+        f"{prefix}_s": np.array([s.s for s in st]), f"{prefix}_t": np.array([s.t for s in st]),
+        f"{prefix}_gap": np.array([s.gap for s in st]),
+        f"{prefix}_normal": np.array([s.normal for s in st]).reshape(-1, 3),
+        f"{prefix}_tangent1": np.array([s.tangent1 for s in st]).reshape(-1, 3),
+        f"{prefix}_tangent2": np.array([s.tangent2 for s in st]).reshape(-1, 3),
+        f"{prefix}_p_n": np.array([s.p_n for s in st]),
+        f"{prefix}_z_t": np.array([s.z_t for s in st]).reshape(-1, 2),
+        f"{prefix}_k_pen": np.array([s.k_pen for s in st]),
+        f"{prefix}_k_t": np.array([s.k_t for s in st]),
+        f"{prefix}_q_trial_norm": np.array([s.q_trial_norm for s in st]),
+        f"{prefix}_status": np.array([s.status.value for s in st], np.int8),
+        f"{prefix}_stick": np.array([bool(s.stick) for s in st]),
+        f"{prefix}_dissipation": np.array([s.dissipation for s in st]),
+    }
+
+
+def element_fixture():
+    """CR element f_int / K on a small 7-wire strand at three rotation scales, plus
+    the reference's scalar element routines on a few elements."""
+    m = mesh_mod.make_strand(7, 1.0, 12.0, 12, lay_angle_deg=10.0)
+    sec = section(1.0)
+    asm = ULCRBeamAssembler(m.node_coords, m.connectivity, sec["E"], sec["G"], sec["A"],
+                            sec["Iy"], sec["Iz"], sec["J"], sec["kappa_y"], sec["kappa_z"])
+    out = {"X": m.node_coords, "conn": m.connectivity.astype(np.int64), "radius": np.array(1.0)}
+    rng = np.random.default_rng(2024)
+    ndof = 6 * m.n_nodes
+    scales = [(1e-3, 1e-3), (5e-2, 0.3), (0.2, 1.5)]
+    for k, (st, sr) in enumerate(scales):
+        u = np.zeros(ndof)
+        u.reshape(-1, 6)[:, :3] = rng.normal(0, st, (m.n_nodes, 3))
+        u.reshape(-1, 6)[:, 3:] = rng.normal(0, sr, (m.n_nodes, 3))
+        out[f"u{k}"] = u
+        out[f"f{k}"] = asm.assemble_internal_force(u)
+        out.update(csr_arrays(f"K{k}", asm.assemble_tangent(u)))
+    # scalar routines (Shepperd trace-first branch order) on elements 0..5 at scale 2
+    u = out["u2"]
+    fs, Ks = [], []
+    for e in range(6):
+        n1, n2 = m.connectivity[e]
+        ed = np.r_[6 * n1 + np.arange(6), 6 * n2 + np.arange(6)]
+        args = (m.node_coords[[n1, n2]], u[ed], sec["E"], sec["G"], sec["A"], sec["Iy"], sec["Iz"],
+                sec["J"], sec["kappa_y"], sec["kappa_z"])
+        fs.append(timo_beam3d_cr_internal_force(*args))
+        Ks.append(timo_beam3d_cr_tangent_analytical(*args))
+    out["scalar_f"] = np.array(fs)
+    out["scalar_K"] = np.array(Ks)
+    out["M_lumped"] = asm.assemble_mass(RHO, lumped=True).diagonal()
+    np.savez_compressed(os.path.join(HERE, "elem_small.npz"), **out)
+
+
+def assembly_fixture():
+    """Two increments x two Newton attempts of a 7-wire strand with contact and
+    friction, through the reference's own step Processes."""
+    r, L, ne = 1.0, 20.0, 20
+    m = mesh_mod.make_strand(7, r, L, ne, lay_angle_deg=10.0)
+    sec = section(r)
+    ndof = 6 * m.n_nodes
+    k_pen = 1e4 * E * r
+    sdelta = 5000.0 / r
+    mu = 0.3
+    rho_inf = 0.8
+    dt = 1e-5
+    asm = ULCRBeamAssembler(m.node_coords, m.connectivity, sec["E"], sec["G"], sec["A"],
+                            sec["Iy"], sec["Iz"], sec["J"], sec["kappa_y"], sec["kappa_z"])
+    M = asm.assemble_mass(RHO, lumped=True)
+    ga = GeneralizedAlphaProcess(M, rho_inf=rho_inf)
+    huber = HuberContactForceProcess(ndof, smoothing_delta=sdelta)
+    fric = CoulombReturnMappingProcess(ndof, k_pen=k_pen, k_t_ratio=0.5)
+    manager = _ContactManagerInput(config=_ContactConfigInput(mu=mu))
+    radii = m.radii
+    out = {"X": m.node_coords, "conn": m.connectivity.astype(np.int64), "radius": np.array(r),
+           "params": np.array([k_pen, sdelta, mu, rho_inf, dt, RHO, 0.5])}
+    d0 = DetectCandidatesProcess().process(DetectCandidatesInput(
+        manager=manager, node_coords=m.node_coords, connectivity=m.connectivity, radii=radii))
+    manager = d0.manager
+    out["cand0"] = np.array(sorted(d0.candidates), np.int64).reshape(-1, 2)
+    rng = np.random.default_rng(77)
+    nn = ne + 1
+    fixed = ((np.arange(7) * nn)[:, None] * 6 + np.arange(6)[None]).ravel()
+    f_ext = np.zeros(ndof)
+    f_ext[(np.arange(7) * nn + nn - 1) * 6 + 4] = 5.0
+    out["fixed"] = fixed
+    out["f_ext"] = f_ext
+    u_ref = np.zeros(ndof)
+    u = np.zeros(ndof)
+    for inc in range(2):
+        u_pred = ga.predict(u, dt)
+        base = u_pred + rng.normal(0, 2e-3 * r, ndof)
+        base[fixed] = 0.0
+        coords = m.node_coords + base.reshape(-1, 6)[:, :3]
+        dc = DetectCandidatesProcess().process(DetectCandidatesInput(
+            manager=manager, node_coords=coords, connectivity=m.connectivity, radii=radii))
+        manager = dc.manager
+        out[f"inc{inc}_cand"] = np.array(sorted(dc.candidates), np.int64).reshape(-1, 2)
+        ug = UpdateGeometryProcess().process(UpdateGeometryInput(manager=manager, node_coords=coords))
+        manager = ug.manager
+        out[f"inc{inc}_u_pred"] = u_pred.copy()
+        out.update(pair_arrays(f"inc{inc}_start", manager.pairs))
+        for it in range(2):
+            u = base + (rng.normal(0, 5e-4 * r, ndof) if it else 0.0)
+            u[fixed] = 0.0
+            out[f"inc{inc}_it{it}_u"] = u.copy()
+            fo = ContactForceAssemblyProcess().process(ContactForceAssemblyInput(
+                u=u, f_ext=f_ext, fixed_dofs=fixed, manager=manager, node_coords_ref=m.node_coords,
+                contact_force_strategy=huber, friction_strategy=fric, coating_strategy=None,
+                k_pen=k_pen, mu=mu, u_ref=u_ref, load_frac=1.0, load_frac_prev=0.0,
+                increment_display=inc + 1, ndof_per_node=6, use_coating=False,
+                assemble_internal_force=asm.assemble_internal_force))
+            R = fo.R_u
+            ga.correct(u, np.zeros_like(u), dt)
+            R = ga.effective_residual(R, dt)
+            R[fixed] = 0.0
+            to = TangentAssemblyProcess().process(TangentAssemblyInput(
+                u=u, manager=manager, contact_force_strategy=huber, friction_strategy=fric,
+                coating_strategy=None, assemble_tangent=asm.assemble_tangent, k_pen=k_pen, mu=mu,
+                ndof=ndof, coords_def=fo.coords_def, load_frac=1.0, load_frac_prev=0.0,
+                use_coating=False))
+            K = ga.effective_stiffness(to.K_T, dt)
+            p = f"inc{inc}_it{it}"
+            out[f"{p}_R"] = R
+            out[f"{p}_fc"] = fo.f_c
+            out[f"{p}_n_active"] = np.array(sum(1 for q in manager.pairs if q.state.p_n > 0.0))
+            out.update(csr_arrays(f"{p}_K", K))
+            out.update(pair_arrays(f"{p}_pairs", manager.pairs))
+        u_ref = u.copy()
+    np.savez_compressed(os.path.join(HERE, "assembly_7wire.npz"), **out)
+
+
+if __name__ == "__main__":
+    element_fixture()
+    assembly_fixture()
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))

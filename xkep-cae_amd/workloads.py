@@ -1,0 +1,115 @@
+"""Synthetic strand workloads of the per-Newton-iteration assembly path.
+
+Units are mm-ton-MPa.  Material E = 200e3 MPa, nu = 0.3, rho = 7.85e-9.
+Contact: k_pen = 1e4 E r, mu = 0.3, Huber smoothing_delta = 5000 / r
+(the reference's rule, numerical_tests/three_point_bend_jig.py:1121),
+k_t_ratio = 0.5, g_on = 0, g_off = 1e-6.
+
+State: u ~ N(0, (1e-3 r)^2) i.i.d. on all six DOFs of every node, from
+``numpy.random.default_rng(seed)``; u_ref = 0; Generalized-alpha predicted
+from rest (u_pred = v_pred = acc_old = 0).  Boundary: all DOFs of the first
+node of every wire fixed; f_ext = end bending moment on the last node of every
+wire.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mesh import StrandMesh, make_strand
+
+E_STEEL = 200e3
+NU_STEEL = 0.3
+RHO_STEEL = 7.85e-9
+
+
+@dataclass(frozen=True)
+class StrandCase:
+    name: str
+    n_wires: int
+    wire_radius: float
+    length: float
+    n_elems_per_wire: int
+    lay_angle_deg: float
+    seed: int
+    mu: float = 0.3
+    rho_inf: float = 0.8
+    end_moment: float = 0.0
+    E: float = E_STEEL
+    nu: float = NU_STEEL
+    rho: float = RHO_STEEL
+
+    @property
+    def k_pen(self) -> float:
+        return 1e4 * self.E * self.wire_radius
+
+    @property
+    def smoothing_delta(self) -> float:
+        return 5000.0 / self.wire_radius
+
+    def mesh(self) -> StrandMesh:
+        return make_strand(self.n_wires, self.wire_radius, self.length, self.n_elems_per_wire,
+                           lay_angle_deg=self.lay_angle_deg)
+
+    def dt(self) -> float:
+        """T1/40 of one wire's first bending mode, f1 = pi/(2 L^2) sqrt(EI/(rho A))
+        (numerical_tests/three_point_bend_jig.py:841-858)."""
+        r = self.wire_radius
+        A = math.pi * r * r
+        Iv = math.pi * (2 * r) ** 4 / 64.0
+        f1 = (math.pi / (2.0 * self.length**2)) * math.sqrt(self.E * Iv / (self.rho * A))
+        return 1.0 / f1 / 40.0
+
+
+C1 = StrandCase("C1_7wire", 7, 1.0, 40.0, 40, 10.0, seed=7)
+C2 = StrandCase("C2_37wire", 37, 0.5, 60.0, 100, 12.0, seed=37)
+C3 = StrandCase("C3_91wire", 91, 0.4, 80.0, 150, 12.0, seed=91)
+C5 = StrandCase("C5_1000wire", 1000, 0.3, 100.0, 100, 12.0, seed=1000)
+CASES = {c.name: c for c in (C1, C2, C3, C5)}
+
+
+@dataclass
+class StrandState:
+    case: StrandCase
+    mesh: StrandMesh
+    u: np.ndarray
+    u_ref: np.ndarray
+    f_ext: np.ndarray
+    fixed_dofs: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+def make_state(case: StrandCase, *, seed: int | None = None) -> StrandState:
+    mesh = case.mesh()
+    rng = np.random.default_rng(case.seed if seed is None else seed)
+    ndof = 6 * mesh.n_nodes
+    u = rng.normal(0.0, 1e-3 * case.wire_radius, size=ndof)
+    nn = mesh.n_elems_per_wire + 1
+    first = np.arange(mesh.n_wires) * nn
+    last = first + nn - 1
+    fixed = (first[:, None] * 6 + np.arange(6)[None, :]).ravel()
+    u[fixed] = 0.0
+    f_ext = np.zeros(ndof)
+    f_ext[last * 6 + 4] = case.end_moment
+    return StrandState(case=case, mesh=mesh, u=u, u_ref=np.zeros(ndof), f_ext=f_ext,
+                       fixed_dofs=fixed)
+
+
+def batched_cases(n_instances: int = 64, seed: int = 64) -> list[StrandCase]:
+    """The uneven batched workload: wires in {7, 19, 37, 61}, 50-150 elements per
+    wire, lay angle U(8, 15) deg, mu U(0.1, 0.5), random end moment, r = 0.5,
+    L = 60."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_instances):
+        nw = int(rng.choice([7, 19, 37, 61]))
+        m = int(rng.integers(50, 151))
+        ang = float(rng.uniform(8.0, 15.0))
+        mu = float(rng.uniform(0.1, 0.5))
+        mom = float(rng.uniform(-1.0, 1.0)) * 10.0
+        out.append(StrandCase(f"B{k}_{nw}w{m}", nw, 0.5, 60.0, m, ang, seed=seed * 1000 + k,
+                              mu=mu, end_moment=mom))
+    return out
