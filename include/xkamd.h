@@ -1,0 +1,189 @@
+/*
+ * xkamd.h -- C ABI of the MI355X (gfx950) per-Newton-iteration assembly path
+ * of the xkep-cae strand-contact solver.
+ *
+ * Plain pointers and sizes only.  Pointers documented as "device" are HIP
+ * device pointers (any allocator; the Python layer passes torch tensor
+ * storage).  "host" pointers are ordinary host memory.  Every call that
+ * launches work takes a hipStream_t passed as void* (NULL = default stream)
+ * and is asynchronous unless documented otherwise.
+ *
+ * Return codes: 0 = ok, > 0 = ok with a status flag (see XK_FLAG_*),
+ * < 0 = error; xk_last_error() returns the message of the last error on the
+ * calling thread.
+ *
+ * DOF layout: 6 per node [ux, uy, uz, thx, thy, thz] (the reference's
+ * ndof_per_node = 6).  Element i joins conn[2i] -> conn[2i+1].
+ *
+ * Tangent storage: K values are written in the order of a scalar CSR matrix
+ * whose pattern is a 3x3-block CSR over DOF groups (group 2n = translations
+ * of node n, group 2n+1 = rotations).  xk_pattern_export() returns that
+ * scalar CSR (indptr/indices) so the values form a standard CSR matrix.
+ */
+#ifndef XKAMD_H
+#define XKAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xk_ctx xk_ctx;
+
+enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1 };
+
+/* status flags returned (bitwise OR) by assembly calls */
+enum {
+    XK_FLAG_NONFINITE = 1,     /* a non-finite residual entry was produced */
+    XK_FLAG_DEGENERATE = 2     /* an element of (near) zero length was met */
+};
+
+/* pair-kernel modes (bitwise OR) */
+enum {
+    XK_PAIR_GEOM = 1,          /* narrowphase + frame (UpdateGeometryProcess) */
+    XK_PAIR_HYSTERESIS = 2,    /* active-set update (unfrozen UpdateGeometry) */
+    XK_PAIR_HUBER = 4,         /* HuberContactForceProcess.evaluate */
+    XK_PAIR_FRICTION = 8,      /* CoulombReturnMappingProcess.evaluate */
+    XK_PAIR_TANGENT_C = 16,    /* stage Huber tangent blocks */
+    XK_PAIR_TANGENT_F = 32     /* stage friction tangent blocks (material + geometric) */
+};
+
+/* Mesh, material and section.  Replaces ULCRBeamAssembler.__init__
+ * (xkep_cae/elements/_beam_assembler.py:36-70): node_coords (n_nodes x 3,
+ * host, float64), connectivity (n_elems x 2, host, int32), section
+ * [E, G, A, Iy, Iz, J, kappa_y, kappa_z], scf <= 0 meaning None, v_ref NULL
+ * meaning the automatic reference axis, elem_radius (n_elems, host) the
+ * contact radii of DetectCandidatesInput.radii
+ * (xkep_cae/contact/_manager_process.py:135-145). */
+int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coords,
+                  const int32_t* connectivity, const double* section8, double scf,
+                  const double* v_ref3, const double* elem_radius, int precision);
+int xk_ctx_destroy(xk_ctx* ctx);
+const char* xk_last_error(void);
+int xk_ctx_precision(const xk_ctx* ctx);
+
+/* Element assembly.  Replaces ULCRBeamAssembler.assemble_internal_force /
+ * assemble_tangent (_beam_assembler.py:73-115) via assemble_cr_beam3d
+ * (_beam_assembly.py:26-152): u (device, 6 n_nodes float64); f_int (device,
+ * float64, nullable) receives the internal force; K_vals (device, nullable,
+ * float64 or float32 by precision) receives the beam tangent in the current
+ * pattern's CSR order (zeros on contact-only blocks); K_cap is its length in
+ * entries and the call fails if the pattern holds more. */
+int xk_beam_assemble(xk_ctx* ctx, const double* u, double* f_int, void* K_vals, int64_t K_cap,
+                     void* stream);
+
+/* Pair list (the manager.pairs of _ContactManagerInput,
+ * xkep_cae/contact/_contact_pair.py:39-81), structure of arrays. */
+int xk_pairs_count(const xk_ctx* ctx, int64_t* n_pairs);
+/* Replace the whole pair list from host arrays (n rows each): elem_a, elem_b
+ * (int32), and optionally state: s, t, gap (n), normal, tangent1 (3n),
+ * p_n, k_pen, k_t, q_trial_norm, dissipation (n), z_t (2n), status, stick
+ * (int8).  NULL state pointers mean the fresh _ContactStateOutput(). */
+int xk_pairs_set(xk_ctx* ctx, int64_t n, const int32_t* elem_a, const int32_t* elem_b,
+                 const double* s, const double* t, const double* gap, const double* normal,
+                 const double* tangent1, const double* p_n, const double* k_pen,
+                 const double* k_t, const double* q_trial_norm, const double* dissipation,
+                 const double* z_t, const int8_t* status, const int8_t* stick);
+/* Copy the pair list to host arrays (each may be NULL).  Synchronous. */
+int xk_pairs_get(xk_ctx* ctx, int32_t* elem_a, int32_t* elem_b, double* s, double* t,
+                 double* gap, double* normal, double* tangent1, double* p_n, double* k_pen,
+                 double* k_t, double* q_trial_norm, double* dissipation, double* z_t,
+                 int8_t* status, int8_t* stick);
+
+/* DetectCandidatesProcess.process (xkep_cae/contact/_manager_process.py:172-257):
+ * GPU AABB broadphase over deformed coordinates (device, n_nodes x 3,
+ * float64) with inclusive-overlap output identical to _broadphase_aabb
+ * (_broadphase.py:13-96), shared-node / same-layer / midpoint filters
+ * (elem_layer: device int32 per element or NULL), existing pairs outside the
+ * candidates set INACTIVE, new candidates appended in (i<j) order.
+ * cell_size <= 0 selects the tuned cell size.  prescreen_extra is the
+ * midpoint cutoff margin (the caller computes prescreening_margin or
+ * 0.5 mean(r_a + r_b)); pass a negative value to disable the prescreen.
+ * Synchronous (returns counts).  n_candidates receives the filtered count. */
+int xk_detect_candidates(xk_ctx* ctx, const double* coords, double margin, double cell_size,
+                         const int32_t* elem_layer, double prescreen_extra,
+                         int64_t* n_raw, int64_t* n_candidates, void* stream);
+/* The raw broadphase alone (_broadphase_aabb): pairs_out (device int32,
+ * 2*cap) gets sorted (i<j) pairs.  Synchronous. */
+int xk_broadphase(xk_ctx* ctx, const double* coords, double margin, double cell_size,
+                  int32_t* pairs_out, int64_t cap, int64_t* n_out, void* stream);
+/* Candidate list of the last xk_detect_candidates (device int32 2*n). */
+int xk_last_candidates(xk_ctx* ctx, int32_t* pairs_out_host, int64_t cap);
+/* Sum of (r_a + r_b) over the shared-node-filtered candidates of the last
+ * broadphase, in sorted (i<j) order, copied to host (n entries), for the
+ * caller's mean.  Synchronous. */
+int xk_prescreen_radii(xk_ctx* ctx, const double* coords, double margin, double cell_size,
+                       const int32_t* elem_layer, double* rsum_host, int64_t cap,
+                       int64_t* n_out, void* stream);
+
+/* Run the pair kernel over all pairs with the given XK_PAIR_* modes.
+ * coords_ref (device, n_nodes x 3) plus u (device, nullable) give the
+ * deformed coordinates; u_ref (device) is the friction reference.  Covers
+ * UpdateGeometryProcess (_manager_process.py:327-403, hysteresis :475-494),
+ * HuberContactForceProcess.evaluate/tangent (contact_force/strategy.py:126-254),
+ * CoulombReturnMappingProcess.evaluate/tangent (friction/strategy.py:111-164). */
+int xk_pairs_run(xk_ctx* ctx, int modes, const double* coords_ref, const double* u,
+                 const double* u_ref, double k_pen, double smoothing_delta, double k_t_ratio,
+                 double mu_eff, double g_on, double g_off, int allow_deactivation,
+                 void* stream);
+/* Gather the pair forces staged by the last xk_pairs_run into f (device,
+ * float64, 6 n_nodes, overwritten) and/or the staged pair tangent blocks into
+ * K_vals (device, pattern order, overwritten, K_cap entries). */
+int xk_pairs_gather(xk_ctx* ctx, double* f, void* K_vals, int64_t K_cap, void* stream);
+
+/* (Re)build the tangent pattern: beam blocks plus the blocks of every pair
+ * that can contribute within the current increment (status != INACTIVE or
+ * p_n > 0).  Called automatically when stale.  Synchronous. */
+int xk_pattern_build(xk_ctx* ctx);
+int xk_pattern_info(xk_ctx* ctx, int64_t* nnz, int64_t* n_blocks, int64_t* n_contrib_pairs);
+/* Scalar CSR of the current pattern (host arrays of ndof+1 and nnz). */
+int xk_pattern_export(xk_ctx* ctx, int64_t* indptr, int32_t* indices);
+
+/* Generalized-alpha state for the fused assembly (time_integration/strategy.py:96-232):
+ * lumped mass diagonal (device, ndof), u_pred, v_pred, acc_old (device, ndof)
+ * are read; acc and vel (device, ndof) are written by xk_assemble_iteration. */
+typedef struct xk_dyn {
+    const double* M_diag;
+    const double* u_pred;
+    const double* v_pred;
+    const double* acc_old;
+    double* acc;
+    double* vel;
+    double dt;
+    double alpha_m;
+    double beta;
+    double gamma;
+} xk_dyn;
+
+/* One fused Newton-attempt assembly: ContactForceAssemblyProcess
+ * (contact/solver/_newton_steps.py:79-135) + GeneralizedAlpha correct /
+ * effective_residual (_newton_dynamic.py:189-192) + TangentAssemblyProcess
+ * (_newton_steps.py:289-313) + effective_stiffness (_newton_dynamic.py:314-315):
+ *   narrowphase (frozen active set) -> Huber -> friction -> element f and K ->
+ *   R = f_int - (f_c + f_fric) - f_ext, R[fixed] = 0, R += M a_mid,
+ *   R[fixed] = 0 -> K = K_beam + K_c - K_fric + (1 - alpha_m) c0 M.
+ * fixed_mask (device uint8, ndof); dyn may be NULL (static).  R (device),
+ * K_vals (device, nullable, K_cap entries), res_norm2 (device double, ||R||^2, nullable),
+ * n_active (device int32, #(p_n > 0), nullable). */
+int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
+                          const double* f_ext, const uint8_t* fixed_mask, const xk_dyn* dyn,
+                          double k_pen, double smoothing_delta, double k_t_ratio, double mu_eff,
+                          double* R, void* K_vals, int64_t K_cap, double* res_norm2, int32_t* n_active,
+                          void* stream);
+
+/* Device status word of the last calls (XK_FLAG_* bits); synchronous. */
+int xk_status(xk_ctx* ctx, int* flags);
+
+/* Per-stage timing of xk_assemble_iteration with HIP events recorded on the
+ * call's stream: begin allocates events for up to max_calls calls; end
+ * synchronises and returns the summed milliseconds of the four stages
+ * [pairs, elements, residual gather + norm, tangent gather] and the number of
+ * calls recorded.  Not for use under graph capture. */
+int xk_profile_begin(xk_ctx* ctx, int max_calls);
+int xk_profile_end(xk_ctx* ctx, double* stage_ms4, int* n_calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XKAMD_H */

@@ -1,0 +1,260 @@
+"""GPU parity of the gfx950 assembly path against the reference's own outputs
+(golden fixtures) and the FP64 oracle, through the C-ABI library.
+
+Bars: FP64 mode -- R and K within 1e-10 relative (Frobenius), identical
+candidate and active sets, identical pair statuses; mixed mode -- residual
+within 1e-6 relative, identical active set."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import golden, pkg
+from oracle import assembly as oasm
+from oracle import beam as obeam
+from oracle import contact as ocon
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_R_TOL = 1e-6
+MIXED_K_TOL = 1e-5
+
+
+def csr(g, p):
+    return sp.csr_matrix((g[p + "_data"], g[p + "_indices"], g[p + "_indptr"]), shape=tuple(g[p + "_shape"]))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def srel(A, B):
+    return float(sp.linalg.norm(A - B) / sp.linalg.norm(B))
+
+
+def section(r):
+    C = pkg("context")
+    return C.circle_section(r, 200e3, 0.3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_beam_force_and_tangent_match_reference(gpu, precision):
+    g = golden("elem_small.npz")
+    E = pkg("elements")
+    s = section(float(g["radius"]))
+    asm = E.ULCRBeamAssembler(g["X"], g["conn"], s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z,
+                              precision=precision)
+    for k in range(3):
+        f = asm.assemble_internal_force(g[f"u{k}"])
+        K = asm.assemble_tangent(g[f"u{k}"])
+        assert rel(f, g[f"f{k}"]) < 1e-12, k
+        assert srel(K, csr(g, f"K{k}")) < (FP64_TOL if precision == "fp64" else MIXED_K_TOL), k
+
+
+def test_beam_matches_oracle_at_37_wires(gpu):
+    wl = pkg("workloads")
+    E = pkg("elements")
+    st = wl.make_state(wl.C2)
+    s = section(wl.C2.wire_radius)
+    asm = E.ULCRBeamAssembler(st.mesh.node_coords, st.mesh.connectivity, s.E, s.G, s.A, s.Iy, s.Iz, s.J,
+                              s.kappa_y, s.kappa_z)
+    osec = obeam.circle_section(wl.C2.wire_radius, 200e3, 0.3)
+    Kr, fr = obeam.assemble_cr_beam3d(st.mesh.node_coords, st.mesh.connectivity, st.u, osec)
+    assert rel(asm.assemble_internal_force(st.u), fr) < FP64_TOL
+    assert srel(asm.assemble_tangent(st.u), Kr) < FP64_TOL
+
+
+def test_broadphase_is_the_exact_overlap_set(gpu):
+    wl = pkg("workloads")
+    C = pkg("context")
+    st = wl.make_state(wl.C2)
+    m = st.mesh
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, section(m.wire_radius), radii=m.radii)
+    x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
+    ref = ocon.broadphase_aabb(x[m.connectivity[:, 0]], x[m.connectivity[:, 1]], m.radii)
+    for cell in (None, 0.2, 1.0, 3.0):
+        got = ctx.broadphase(ctx.to_dev(x), cell_size=cell)
+        assert np.array_equal(got, ref), cell
+
+
+def test_detect_candidates_matches_reference_sets(gpu):
+    a = golden("assembly_7wire.npz")
+    C = pkg("context")
+    K = pkg("contact")
+    r = float(a["radius"])
+    radii = np.full(len(a["conn"]), r)
+    ctx = C.DeviceContext(a["X"], a["conn"], section(r), radii=radii)
+    mgr = K.ContactManager(ctx, K.ContactConfig(mu=0.3))
+    out = K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, a["X"], a["conn"], radii))
+    assert np.array_equal(np.array(out.candidates).reshape(-1, 2), a["cand0"])
+    x = a["X"] + a["inc0_it0_u"].reshape(-1, 6)[:, :3]
+    out = K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, x, a["conn"], radii))
+    assert np.array_equal(np.array(out.candidates).reshape(-1, 2), a["inc0_cand"])
+
+
+def compare_pairs(d, a, pre, tol):
+    key = np.stack([np.minimum(d["elem_a"], d["elem_b"]), np.maximum(d["elem_a"], d["elem_b"])], 1)
+    o = np.lexsort((key[:, 1], key[:, 0]))
+    assert np.array_equal(key[o], a[pre + "_key"])
+    assert np.array_equal(d["status"][o], a[pre + "_status"]), pre
+    assert np.array_equal(d["stick"][o].astype(bool), a[pre + "_stick"]), pre
+    for f in ["s", "t", "gap", "normal", "tangent1", "tangent2", "p_n", "z_t", "k_t", "q_trial_norm"]:
+        r = a[pre + "_" + f]
+        err = np.abs(d[f][o] - r).max() / max(np.abs(r).max(), 1e-300)
+        assert err < tol, (pre, f, err)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision):
+    """Two increments x two Newton attempts of a 7-wire strand with Huber contact
+    and Coulomb friction, against the reference's ContactForceAssembly /
+    TangentAssembly / GeneralizedAlpha outputs."""
+    a = golden("assembly_7wire.npz")
+    C, K, T, A = pkg("context"), pkg("contact"), pkg("time_integration"), pkg("assembly")
+    k_pen, sd, mu, rho_inf, dt, rho, ktr = a["params"]
+    r = float(a["radius"])
+    X, conn = a["X"], a["conn"]
+    radii = np.full(len(conn), r)
+    ctx = C.DeviceContext(X, conn, section(r), radii=radii, precision=precision)
+    mgr = K.ContactManager(ctx, K.ContactConfig(mu=mu, k_t_ratio=ktr))
+    K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, X, conn, radii))
+    E = pkg("elements")
+    s = section(r)
+    Md = E.ULCRBeamAssembler(X, conn, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z).lumped_mass_diag(rho)
+    ga = T.GeneralizedAlphaProcess(Md, rho_inf=rho_inf)
+    asm = A.StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=sd, k_t_ratio=ktr, mu=mu, f_ext=a["f_ext"],
+                           fixed_dofs=a["fixed"], config=mgr.config, time=ga, dt=dt)
+    torch = ctx.torch
+    u = torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev)
+    state_tol = 1e-12 if precision == "fp64" else 1e-9
+    for inc in range(2):
+        u_pred = ga.predict_device(u, dt).cpu().numpy()
+        assert rel(u_pred, a[f"inc{inc}_u_pred"]) < 1e-14
+        asm.begin_increment(a[f"inc{inc}_it0_u"])
+        compare_pairs(ctx.get_pairs(), a, f"inc{inc}_start", state_tol)
+        for it in range(2):
+            R, Kv, res2, nact = asm.assemble(a[f"inc{inc}_it{it}_u"])
+            pre = f"inc{inc}_it{it}"
+            Rh = R.cpu().numpy()
+            assert rel(Rh, a[pre + "_R"]) < (FP64_TOL if precision == "fp64" else MIXED_R_TOL), pre
+            assert float(res2.item()) == pytest.approx(float(Rh @ Rh), rel=1e-12)
+            assert srel(asm.K_csr(), csr(a, pre + "_K")) < (FP64_TOL if precision == "fp64" else MIXED_K_TOL), pre
+            assert int(nact.item()) == int(a[pre + "_n_active"])
+            compare_pairs(ctx.get_pairs(), a, pre + "_pairs", state_tol)
+        u = asm.u.clone()
+        asm.set_u_ref(u)
+
+
+def test_strategy_api_matches_oracle_on_37_wires(gpu):
+    """Drop-in Processes / Strategies (DetectCandidates, UpdateGeometry, Huber,
+    Coulomb evaluate + tangent) against the oracle, one call each."""
+    wl = pkg("workloads")
+    C, K = pkg("context"), pkg("contact")
+    case = wl.C2
+    st = wl.make_state(case)
+    m = st.mesh
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, section(m.wire_radius), radii=m.radii)
+    cfg = K.ContactConfig(mu=case.mu)
+    mgr = K.ContactManager(ctx, cfg)
+    ocfg = ocon.ContactConfig(mu=case.mu)
+    op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, ocfg)
+    K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, m.node_coords, m.connectivity, m.radii))
+    x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
+    op, ocand = ocon.detect_candidates(op, x, m.connectivity, m.radii, ocfg)
+    out = K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, x, m.connectivity, m.radii))
+    assert np.array_equal(np.array(out.candidates).reshape(-1, 2), ocand)
+    op = ocon.update_geometry(op, x, ocfg)
+    K.UpdateGeometryProcess().process(K.UpdateGeometryInput(mgr, x))
+    d = ctx.get_pairs()
+    assert np.array_equal(d["elem_a"], op.elem_a) and np.array_equal(d["elem_b"], op.elem_b)
+    assert np.array_equal(d["status"], op.status)
+    ndof = ctx.ndof
+    hub = K.HuberContactForceProcess(ndof, smoothing_delta=case.smoothing_delta)
+    fc, res = hub.evaluate(st.u, mgr, case.k_pen)
+    ofc, ores, op = ocon.huber_evaluate(op, case.k_pen, case.smoothing_delta, ndof)
+    assert rel(fc, ofc) < FP64_TOL and rel(res, ores) < FP64_TOL
+    fr = K.CoulombReturnMappingProcess(ndof, k_pen=case.k_pen, k_t_ratio=0.5)
+    ff, fres = fr.evaluate(st.u, mgr.pairs, case.mu, u_ref=st.u_ref)
+    ofr = ocon.friction_evaluate(op, st.u, st.u_ref, case.k_pen, 0.5, case.mu, ndof)
+    assert rel(ff, ofr.f_fric) < FP64_TOL and rel(fres, ofr.residual) < FP64_TOL
+    assert len(fr.friction_tangents) == int(ofr.mask.sum())
+    Kc = hub.tangent(st.u, mgr, case.k_pen)
+    assert srel(Kc, ocon.huber_tangent(ofr.pairs, case.k_pen, case.smoothing_delta, ndof)) < FP64_TOL
+    Kf = fr.tangent(st.u, mgr.pairs, case.mu)
+    assert srel(Kf, ocon.friction_tangent(ofr.pairs, ofr, ndof)) < FP64_TOL
+    d = ctx.get_pairs()
+    assert np.array_equal(d["status"], ofr.pairs.status)
+
+
+def test_mixed_precision_residual_and_active_set_at_91_wires(gpu):
+    """Full-size 91-wire workload: FP64 kernels against the oracle (1e-10) and
+    the mixed form against FP64 (residual 1e-6, identical active set)."""
+    wl = pkg("workloads")
+    C, A, T, E = pkg("context"), pkg("assembly"), pkg("time_integration"), pkg("elements")
+    case = wl.C3
+    st = wl.make_state(case)
+    m = st.mesh
+    s = section(case.wire_radius)
+    Md = E.ULCRBeamAssembler(m.node_coords, m.connectivity, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y,
+                             s.kappa_z).lumped_mass_diag(case.rho)
+    dt = case.dt()
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        ctx = C.DeviceContext(m.node_coords, m.connectivity, s, radii=m.radii, precision=prec)
+        ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
+        ga.predict_device(ctx.torch.zeros(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev), dt)
+        asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                               f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, time=ga, dt=dt)
+        asm.begin_increment(np.zeros(ctx.ndof))  # candidates on the reference configuration
+        asm.begin_increment(st.u)
+        R, Kv, res2, nact = asm.assemble(st.u)
+        outs[prec] = (R.cpu().numpy(), asm.K_csr(), int(nact.item()), ctx.get_pairs()["status"].copy())
+    # oracle
+    osec = obeam.circle_section(case.wire_radius, 200e3, 0.3)
+    cfg = ocon.ContactConfig(mu=case.mu)
+    prob = oasm.Problem(m.node_coords, m.connectivity, osec, case.k_pen, case.smoothing_delta, 0.5, case.mu, cfg)
+    op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, cfg)
+    x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
+    op, _ = ocon.detect_candidates(op, x, m.connectivity, m.radii, cfg)
+    op = ocon.update_geometry(op, x, cfg)
+    ga = oasm.GeneralizedAlpha(obeam.lumped_mass_diag(m.node_coords, m.connectivity, case.rho, osec), case.rho_inf)
+    ga.predict(np.zeros_like(st.u), dt)
+    ref = oasm.assemble_iteration(prob, op, st.u, st.u_ref, st.f_ext, st.fixed_dofs, ga, dt)
+    R64, K64, n64, s64 = outs["fp64"]
+    assert rel(R64, ref.R) < FP64_TOL
+    assert srel(K64, ref.K) < FP64_TOL
+    assert n64 == ref.n_active
+    assert np.array_equal(s64, ref.pairs.status)
+    Rm, Km, nm, sm = outs["mixed"]
+    assert rel(Rm, R64) < MIXED_R_TOL
+    assert srel(Km, K64) < MIXED_K_TOL
+    assert nm == n64 and np.array_equal(sm, s64)
+
+
+def test_assembly_is_bitwise_deterministic_and_graph_replay_matches(gpu):
+    wl = pkg("workloads")
+    C, A = pkg("context"), pkg("assembly")
+    case = wl.C1
+    st = wl.make_state(case)
+    m = st.mesh
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, section(case.wire_radius), radii=m.radii)
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                           f_ext=st.f_ext, fixed_dofs=st.fixed_dofs)
+    asm.begin_increment(st.u)
+    snap = ctx.get_pairs()
+    R1, K1, _, n1 = (t.clone() for t in asm.assemble(st.u))
+    # restore the pair state the first call consumed, then repeat
+    ctx.set_pairs(snap["elem_a"], snap["elem_b"], snap)
+    asm.K = ctx.new_K()
+    R2, K2, _, n2 = (t.clone() for t in asm.assemble(st.u))
+    assert ctx.torch.equal(R1, R2) and ctx.torch.equal(K1, K2) and int(n1.item()) == int(n2.item())
+    # graph: replay several times with eager work between, each against an eager run
+    asm.capture()
+    for rep in range(3):
+        ctx.set_pairs(snap["elem_a"], snap["elem_b"], snap)
+        ctx.pattern()
+        Rg, Kg, _, ng = (t.clone() for t in asm.replay())
+        ctx.torch.cuda.synchronize()
+        assert ctx.torch.equal(Rg, R1) and ctx.torch.equal(Kg, K1), rep
+        _ = (ctx.X * 2.0).sum().item()

@@ -1,0 +1,121 @@
+"""Fused per-Newton-iteration assembly of one strand on the GPU.
+
+One call of ``StrandAssembly.assemble`` performs what the reference does in
+``ContactForceAssemblyProcess`` + Generalized-alpha correct/effective_residual
++ ``TangentAssemblyProcess`` + effective_stiffness
+(contact/solver/_newton_steps.py:79-135, :289-313;
+contact/solver/_newton_dynamic.py:164-192, :294-315): narrowphase with the
+active set frozen, Huber force, Coulomb friction, element force and tangent,
+R = f_int - (f_c + f_fric) - f_ext with fixed DOFs zeroed, inertia terms,
+||R||^2 and #(p_n > 0) -- five kernel launches on one stream, no host
+synchronisation, capturable in a HIP graph.
+
+``begin_increment`` runs the once-per-increment part (contact/solver/process.py:406-429):
+deformed coordinates, DetectCandidates on the GPU, UpdateGeometry with
+hysteresis, and the tangent pattern of the increment.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .contact import ContactConfig
+from .context import DeviceContext
+from .time_integration import GeneralizedAlphaProcess
+
+
+class StrandAssembly:
+    def __init__(self, ctx: DeviceContext, *, k_pen: float, smoothing_delta: float, k_t_ratio: float = 0.5,
+                 mu: float = 0.3, f_ext=None, fixed_dofs=None, config: ContactConfig | None = None,
+                 time: GeneralizedAlphaProcess | None = None, dt: float = 0.0):
+        torch = ctx.torch
+        self.ctx = ctx
+        self.torch = torch
+        self.k_pen = float(k_pen)
+        self.smoothing_delta = float(smoothing_delta)
+        self.k_t_ratio = float(k_t_ratio)
+        self.mu = float(mu)
+        self.cfg = config or ContactConfig(mu=mu, k_t_ratio=k_t_ratio)
+        n = ctx.ndof
+        dev = ctx.dev
+        self.f_ext = ctx.to_dev(np.zeros(n) if f_ext is None else f_ext)
+        mask = np.zeros(n, np.uint8)
+        if fixed_dofs is not None:
+            mask[np.asarray(fixed_dofs, np.int64)] = 1
+        self.fixed_mask = torch.as_tensor(mask, device=dev)
+        self.u = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.u_ref = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.R = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.res2 = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.n_active = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.K = None
+        self.time = time
+        self.dt = float(dt)
+        self._dyn = time.dyn_struct(self.dt) if time is not None else None
+        self.graph = None
+
+    def set_u(self, u):
+        self.u.copy_(self.ctx.to_dev(u))
+
+    def set_u_ref(self, u_ref):
+        self.u_ref.copy_(self.ctx.to_dev(u_ref))
+
+    def deformed_coords(self, u=None):
+        """x = X + u[0:3] per node (``_deformed_coords``, contact/solver/_utils.py:16-28)."""
+        u = self.u if u is None else self.ctx.to_dev(u)
+        return (self.ctx.X + u.view(-1, 6)[:, :3]).contiguous()
+
+    def begin_increment(self, u=None, *, detect: bool = True, margin: float = 0.0, cell_size=None):
+        """Once-per-increment: GPU DetectCandidates + UpdateGeometry (unfrozen) on
+        x = X + u, then the tangent pattern of the increment."""
+        if u is not None:
+            self.set_u(u)
+        x = self.deformed_coords()
+        cfg = self.cfg
+        if detect:
+            layer = None
+            if cfg.exclude_same_layer and cfg.elem_layer_map is not None:
+                layer = np.array([cfg.elem_layer_map.get(e, -1) for e in range(self.ctx.n_elems)], np.int32)
+            self.ctx.detect_candidates(x, margin=margin, cell_size=cell_size, elem_layer=layer,
+                                       midpoint_prescreening=cfg.midpoint_prescreening,
+                                       prescreening_margin=cfg.prescreening_margin)
+        self.ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, coords=x, g_on=cfg.g_on, g_off=cfg.g_off)
+        self.K = self.ctx.new_K()  # builds the pattern (synchronous)
+        self.graph = None
+        return self
+
+    def assemble(self, u=None, *, tangent: bool = True):
+        """Launch one fused assembly; returns device tensors (R, K, ||R||^2, n_active)."""
+        if u is not None:
+            self.set_u(u)
+        if self.K is None:
+            self.K = self.ctx.new_K()
+        self.ctx.assemble_iteration(self.u, self.u_ref, self.f_ext, self.fixed_mask, self.R,
+                                    self.K if tangent else None, self._dyn, k_pen=self.k_pen,
+                                    smoothing_delta=self.smoothing_delta, k_t_ratio=self.k_t_ratio,
+                                    mu_eff=self.mu, res_norm2=self.res2, n_active=self.n_active)
+        return self.R, (self.K if tangent else None), self.res2, self.n_active
+
+    # -------------------------------------------------------------- HIP graph
+    def capture(self, *, tangent: bool = True):
+        """Capture one assembly (reading ``self.u``) into a HIP graph."""
+        torch = self.torch
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.assemble(tangent=tangent)  # warm (pattern already built)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.assemble(tangent=tangent)
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        return self.R, self.K, self.res2, self.n_active
+
+    def K_csr(self):
+        return self.ctx.to_csr(self.K)

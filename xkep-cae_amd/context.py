@@ -1,0 +1,306 @@
+"""Device context: one mesh + section + pair list on one GPU.
+
+Thin owner of an ``xk_ctx`` (include/xkamd.h).  Device memory for inputs and
+outputs comes from torch tensors on ``cuda``; the library keeps its own
+persistent pair state, staging and pattern buffers.  There is no CPU path:
+constructing a context without a ROCm GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("xkep-cae_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+@dataclass
+class Section:
+    """[E, G, A, Iy, Iz, J, kappa_y, kappa_z] of ULCRBeamAssembler
+    (xkep_cae/elements/_beam_assembler.py:36-62)."""
+
+    E: float
+    G: float
+    A: float
+    Iy: float
+    Iz: float
+    J: float
+    kappa_y: float
+    kappa_z: float
+
+    def as_array(self) -> np.ndarray:
+        return np.array([self.E, self.G, self.A, self.Iy, self.Iz, self.J, self.kappa_y, self.kappa_z])
+
+
+def circle_section(radius: float, E: float, nu: float) -> Section:
+    """Solid circle with the Cowper shear factor 6(1+nu)/(7+6nu)
+    (xkep_cae/elements/_beam_section.py:59-66, :129-135)."""
+    d = 2.0 * radius
+    Iv = np.pi * d**4 / 64.0
+    kap = 6.0 * (1.0 + nu) / (7.0 + 6.0 * nu)
+    return Section(E, E / (2.0 * (1.0 + nu)), np.pi * radius**2, Iv, Iv, np.pi * d**4 / 32.0, kap, kap)
+
+
+PAIR_FIELDS_F64 = ("s", "t", "gap", "p_n", "k_pen", "k_t", "q_trial_norm", "dissipation")
+
+
+class DeviceContext:
+    """Mesh, section, contact radii and pair list resident on the GPU."""
+
+    def __init__(self, node_coords, connectivity, section: Section, *, radii=None, scf=None, v_ref=None,
+                 precision: str = "fp64"):
+        torch = require_gpu()
+        self.torch = torch
+        self.precision = precision
+        X = np.ascontiguousarray(node_coords, dtype=np.float64)
+        conn = np.ascontiguousarray(connectivity, dtype=np.int32)
+        self.n_nodes = X.shape[0]
+        self.n_elems = conn.shape[0]
+        self.ndof = 6 * self.n_nodes
+        self.X_np = X
+        self.conn_np = conn
+        r = np.zeros(self.n_elems) if radii is None else (
+            np.full(self.n_elems, float(radii)) if np.isscalar(radii) else np.asarray(radii, float))
+        self.radii_np = np.ascontiguousarray(r)
+        sec = np.ascontiguousarray(section.as_array())
+        vr = None if v_ref is None else np.ascontiguousarray(v_ref, dtype=np.float64)
+        h = C.c_void_p()
+        L.check(L.lib().xk_ctx_create(C.byref(h), self.n_nodes, self.n_elems, X.ctypes.data, conn.ctypes.data,
+                                      sec.ctypes.data, float(scf) if scf else 0.0,
+                                      vr.ctypes.data if vr is not None else None, self.radii_np.ctypes.data,
+                                      L.PREC[precision]), "xk_ctx_create")
+        self.h = h
+        self.dev = torch.device("cuda")
+        self.X = torch.from_numpy(X).to(self.dev)
+        self.kdtype = torch.float64 if precision == "fp64" else torch.float32
+        self._pattern_epoch = 0
+        self._pattern_cache = None
+
+    # ------------------------------------------------------------ plumbing
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().xk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self.torch.cuda.current_stream().cuda_stream
+
+    def to_dev(self, a, dtype=None):
+        torch = self.torch
+        dtype = dtype or torch.float64
+        if isinstance(a, torch.Tensor):
+            return a.to(device=self.dev, dtype=dtype).contiguous()
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=self.dev)
+
+    def invalidate_pattern(self):
+        self._pattern_epoch += 1
+
+    # ------------------------------------------------------------ pattern
+    def pattern(self):
+        """(indptr, indices, nnz) of the current tangent pattern (scalar CSR)."""
+        if self._pattern_cache is not None and self._pattern_cache[0] == self._pattern_epoch:
+            return self._pattern_cache[1]
+        nnz = C.c_int64()
+        nb = C.c_int64()
+        ns = C.c_int64()
+        L.check(L.lib().xk_pattern_info(self.h, C.byref(nnz), C.byref(nb), C.byref(ns)), "xk_pattern_info")
+        indptr = np.empty(self.ndof + 1, np.int64)
+        indices = np.empty(max(nnz.value, 1), np.int32)
+        L.check(L.lib().xk_pattern_export(self.h, indptr.ctypes.data, indices.ctypes.data), "xk_pattern_export")
+        out = (indptr, indices[: nnz.value], nnz.value)
+        self._pattern_cache = (self._pattern_epoch, out)
+        return out
+
+    def new_K(self):
+        return self.torch.empty(max(self.pattern()[2], 1), dtype=self.kdtype, device=self.dev)
+
+    def to_csr(self, K_vals) -> sp.csr_matrix:
+        indptr, indices, nnz = self.pattern()
+        v = K_vals[:nnz].detach().to("cpu").numpy().astype(np.float64)
+        return sp.csr_matrix((v, indices, indptr), shape=(self.ndof, self.ndof))
+
+    # ------------------------------------------------------------ pairs
+    @property
+    def n_pairs(self) -> int:
+        n = C.c_int64()
+        L.check(L.lib().xk_pairs_count(self.h, C.byref(n)))
+        return n.value
+
+    def set_pairs(self, elem_a, elem_b, state: dict | None = None):
+        ea = np.ascontiguousarray(elem_a, dtype=np.int32)
+        eb = np.ascontiguousarray(elem_b, dtype=np.int32)
+        st = state or {}
+        keep = []
+
+        def p(name, dtype, shape=None):
+            a = st.get(name)
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data
+
+        L.check(L.lib().xk_pairs_set(
+            self.h, len(ea), ea.ctypes.data, eb.ctypes.data, p("s", np.float64), p("t", np.float64),
+            p("gap", np.float64), p("normal", np.float64), p("tangent1", np.float64), p("p_n", np.float64),
+            p("k_pen", np.float64), p("k_t", np.float64), p("q_trial_norm", np.float64),
+            p("dissipation", np.float64), p("z_t", np.float64), p("status", np.int8), p("stick", np.int8)),
+            "xk_pairs_set")
+        self.invalidate_pattern()
+
+    def get_pairs(self) -> dict:
+        n = self.n_pairs
+        out = {
+            "elem_a": np.empty(n, np.int32), "elem_b": np.empty(n, np.int32),
+            **{k: np.empty(n) for k in PAIR_FIELDS_F64},
+            "normal": np.empty((n, 3)), "tangent1": np.empty((n, 3)), "z_t": np.empty((n, 2)),
+            "status": np.empty(n, np.int8), "stick": np.empty(n, np.int8),
+        }
+        if n:
+            d = lambda k: out[k].ctypes.data
+            L.check(L.lib().xk_pairs_get(
+                self.h, d("elem_a"), d("elem_b"), d("s"), d("t"), d("gap"), d("normal"), d("tangent1"),
+                d("p_n"), d("k_pen"), d("k_t"), d("q_trial_norm"), d("dissipation"), d("z_t"), d("status"),
+                d("stick")), "xk_pairs_get")
+        out["tangent2"] = np.cross(out["normal"], out["tangent1"])
+        return out
+
+    # ------------------------------------------------------------ kernels
+    def _check_vec(self, a, n, name):
+        if a is not None and a.numel() != n:
+            raise ValueError(f"{name} has {a.numel()} entries, expected {n}")
+
+    @staticmethod
+    def _cap(K):
+        return 0 if K is None else K.numel()
+
+    def beam(self, u, f_int=None, K_vals=None):
+        self._check_vec(u, self.ndof, "u")
+        self._check_vec(f_int, self.ndof, "f_int")
+        L.check(L.lib().xk_beam_assemble(self.h, L.ptr(u), L.ptr(f_int), L.ptr(K_vals), self._cap(K_vals),
+                                         self.stream), "xk_beam_assemble")
+
+    def pairs_run(self, modes, *, coords=None, u=None, u_ref=None, k_pen=0.0, smoothing_delta=0.0,
+                  k_t_ratio=0.5, mu_eff=0.0, g_on=0.0, g_off=1e-6, allow_deactivation=True):
+        coords = self.X if coords is None else coords
+        if coords.numel() != 3 * self.n_nodes:
+            raise ValueError("coords must hold n_nodes x 3 entries")
+        self._check_vec(u, self.ndof, "u")
+        self._check_vec(u_ref, self.ndof, "u_ref")
+        L.check(L.lib().xk_pairs_run(self.h, modes, L.ptr(coords), L.ptr(u), L.ptr(u_ref), float(k_pen),
+                                     float(smoothing_delta), float(k_t_ratio), float(mu_eff), float(g_on),
+                                     float(g_off), int(bool(allow_deactivation)), self.stream), "xk_pairs_run")
+        if modes & L.PAIR_HYST:
+            self.invalidate_pattern()
+
+    def pairs_gather(self, f=None, K_vals=None):
+        self._check_vec(f, self.ndof, "f")
+        L.check(L.lib().xk_pairs_gather(self.h, L.ptr(f), L.ptr(K_vals), self._cap(K_vals), self.stream),
+                "xk_pairs_gather")
+
+    def broadphase(self, coords, *, margin=0.0, cell_size=None):
+        """Raw inclusive-AABB-overlap pairs (i<j), sorted, as an (n,2) int64 array."""
+        n = C.c_int64()
+        L.check(L.lib().xk_broadphase(self.h, L.ptr(coords), float(margin), float(cell_size or 0.0), None, 0,
+                                      C.byref(n), self.stream), "xk_broadphase")
+        out = self.torch.empty(2 * max(n.value, 1), dtype=self.torch.int32, device=self.dev)
+        L.check(L.lib().xk_broadphase(self.h, L.ptr(coords), float(margin), float(cell_size or 0.0),
+                                      L.ptr(out), out.numel(), C.byref(n), self.stream), "xk_broadphase")
+        a = out[: 2 * n.value].cpu().numpy().astype(np.int64).reshape(2, -1).T
+        return np.ascontiguousarray(a)
+
+    def detect_candidates(self, coords, *, margin=0.0, cell_size=None, elem_layer=None,
+                          midpoint_prescreening=True, prescreening_margin=0.0):
+        """DetectCandidatesProcess on the device; returns the candidate count.
+
+        The midpoint cutoff margin is ``prescreening_margin`` when positive,
+        else 0.5 * mean(r_a + r_b) over the shared-node-filtered candidates
+        with NumPy's mean (as _manager_process.py:280-283)."""
+        layer = None if elem_layer is None else self.to_dev(elem_layer, self.torch.int32)
+        n1 = C.c_int64()
+        extra = -1.0
+        if midpoint_prescreening:
+            if prescreening_margin > 0.0:
+                L.check(L.lib().xk_prescreen_radii(self.h, L.ptr(coords), float(margin), float(cell_size or 0.0),
+                                                   L.ptr(layer), None, 0, C.byref(n1), self.stream))
+                extra = float(prescreening_margin)
+            else:
+                uniform = self.radii_np.size and np.all(self.radii_np == self.radii_np[0])
+                if uniform:
+                    L.check(L.lib().xk_prescreen_radii(self.h, L.ptr(coords), float(margin),
+                                                       float(cell_size or 0.0), L.ptr(layer), None, 0,
+                                                       C.byref(n1), self.stream))
+                    r0 = float(self.radii_np[0])
+                    extra = float(np.mean(np.full(n1.value, r0 + r0))) * 0.5 if n1.value else 0.0
+                else:
+                    L.check(L.lib().xk_prescreen_radii(self.h, L.ptr(coords), float(margin),
+                                                       float(cell_size or 0.0), L.ptr(layer), None, 0,
+                                                       C.byref(n1), self.stream))
+                    rs = np.empty(max(n1.value, 1))
+                    L.check(L.lib().xk_prescreen_radii(self.h, L.ptr(coords), float(margin),
+                                                       float(cell_size or 0.0), L.ptr(layer), rs.ctypes.data,
+                                                       rs.size, C.byref(n1), self.stream))
+                    extra = float(np.mean(rs[: n1.value])) * 0.5 if n1.value else 0.0
+        nr = C.c_int64()
+        nc = C.c_int64()
+        L.check(L.lib().xk_detect_candidates(self.h, L.ptr(coords), float(margin), float(cell_size or 0.0),
+                                             L.ptr(layer), extra, C.byref(nr), C.byref(nc), self.stream),
+                "xk_detect_candidates")
+        self.invalidate_pattern()
+        self._n_cand = nc.value
+        return nc.value
+
+    def last_candidates(self) -> np.ndarray:
+        """Filtered candidate pairs (i<j) of the last ``detect_candidates``, sorted."""
+        n = getattr(self, "_n_cand", 0)
+        buf = np.empty(2 * max(n, 1), np.int32)
+        L.check(L.lib().xk_last_candidates(self.h, buf.ctypes.data, buf.size), "xk_last_candidates")
+        return np.ascontiguousarray(buf[: 2 * n].reshape(2, -1).T.astype(np.int64))
+
+    def assemble_iteration(self, u, u_ref, f_ext, fixed_mask, R, K_vals=None, dyn: "L.XkDyn | None" = None, *,
+                           k_pen, smoothing_delta, k_t_ratio, mu_eff, res_norm2=None, n_active=None):
+        for a, nm in ((u, "u"), (u_ref, "u_ref"), (f_ext, "f_ext"), (fixed_mask, "fixed_mask"), (R, "R")):
+            self._check_vec(a, self.ndof, nm)
+        L.check(L.lib().xk_assemble_iteration(
+            self.h, L.ptr(u), L.ptr(u_ref), L.ptr(f_ext), L.ptr(fixed_mask),
+            C.byref(dyn) if dyn is not None else None, float(k_pen), float(smoothing_delta), float(k_t_ratio),
+            float(mu_eff), L.ptr(R), L.ptr(K_vals), self._cap(K_vals), L.ptr(res_norm2), L.ptr(n_active),
+            self.stream), "xk_assemble_iteration")
+
+    def profile_begin(self, max_calls: int):
+        L.check(L.lib().xk_profile_begin(self.h, int(max_calls)), "xk_profile_begin")
+
+    def profile_end(self):
+        """(stage_ms[4] summed over calls, n_calls): pairs, elements, residual, tangent."""
+        ms = np.zeros(4)
+        n = C.c_int()
+        L.check(L.lib().xk_profile_end(self.h, ms.ctypes.data, C.byref(n)), "xk_profile_end")
+        return ms, n.value
+
+    def status_flags(self) -> int:
+        f = C.c_int()
+        L.check(L.lib().xk_status(self.h, C.byref(f)))
+        return f.value

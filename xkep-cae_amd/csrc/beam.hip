@@ -1,0 +1,374 @@
+// Corotational Timoshenko beam element: internal force and symmetrised
+// analytical tangent, one element per lane.
+//
+// Follows the reference's batched element math
+// (xkep_cae/elements/_beam_assembly.py:211-325 with the helpers of
+// xkep_cae/elements/_beam_cr.py:672-998).  Kinematics (local axes, minimal
+// rotation, exponential/logarithmic rotation maps), the corotated
+// displacements and the element force are FP64 in every precision mode; the
+// tangent (B operator, Ke B, material and geometric parts, symmetrisation) is
+// evaluated in TT = double (plain) or float (mixed).
+//
+// The tangent is formed without the dense 12x12 products: the nonzero rows
+// of B are {3,4,5,6,9,10,11} and every row of Ke B and of K_geo has the
+// column-block form [x, y, -x, z] (x: node-1 translations, y: node-1
+// rotations, z: node-2 rotations), so each 3-row block b of the unsymmetrised
+// tangent is carried as three 3x3 matrices X_b, Y_b, Z_b.
+//
+// Outputs are staged structure-of-arrays so that every store instruction is
+// coalesced across the wavefront:
+//   fe_stage[i * n_elems + e], i = 0..11   (global-frame element force)
+//   Ke_stage[(lb * 9 + a * 3 + b) * n_elems + e]
+// with lb = g_row * 4 + g_col over the element's DOF groups
+// {node1 trans, node1 rot, node2 trans, node2 rot}.
+
+#include "xk_common.h"
+
+namespace xk {
+
+// Unit quaternion (w, v) of a rotation vector, Taylor below |th| < 1e-12
+// (_beam_cr.py:685-726), returned as the rotation matrix.
+__device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th) {
+    double ang = norm(th);
+    double half = ang / 2.0;
+    double sinc, ch;
+    if (ang < 1e-12) {
+        double h2 = half * half;
+        sinc = 0.5 - h2 / 48.0;
+        ch = 1.0 - h2 / 2.0;
+    } else {
+        sinc = sin(half) / ang;
+        ch = cos(half);
+    }
+    double w = ch, x = sinc * th.x, y = sinc * th.y, z = sinc * th.z;
+    double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+    double wx = w * x, wy = w * y, wz = w * z;
+    M3<double> R;
+    R.m[0] = 1.0 - 2.0 * (yy + zz); R.m[1] = 2.0 * (xy - wz);       R.m[2] = 2.0 * (xz + wy);
+    R.m[3] = 2.0 * (xy + wz);       R.m[4] = 1.0 - 2.0 * (xx + zz); R.m[5] = 2.0 * (yz - wx);
+    R.m[6] = 2.0 * (xz - wy);       R.m[7] = 2.0 * (yz + wx);       R.m[8] = 1.0 - 2.0 * (xx + yy);
+    return R;
+}
+
+// Logarithmic map by Shepperd's method: branch = argmax(tr, R00, R11, R22)
+// (first on ties), w >= 0, normalise (floor 1e-15), 2 atan2(|v|, w)/|v| v,
+// 2 v below |v| < 1e-12 (_beam_cr.py:729-787).
+__device__ __forceinline__ V3<double> rotmat_to_rotvec(const M3<double>& R) {
+    double tr = R.m[0] + R.m[4] + R.m[8];
+    int best = 0;
+    double bv = tr;
+    if (R.m[0] > bv) { best = 1; bv = R.m[0]; }
+    if (R.m[4] > bv) { best = 2; bv = R.m[4]; }
+    if (R.m[8] > bv) { best = 3; }
+    double q0, q1, q2, q3, s;
+    if (best == 0) {
+        s = 2.0 * sqrt(tr + 1.0);
+        q0 = 0.25 * s;
+        q1 = (R.m[7] - R.m[5]) / s;
+        q2 = (R.m[2] - R.m[6]) / s;
+        q3 = (R.m[3] - R.m[1]) / s;
+    } else if (best == 1) {
+        s = 2.0 * sqrt(((1.0 + R.m[0]) - R.m[4]) - R.m[8]);
+        q0 = (R.m[7] - R.m[5]) / s;
+        q1 = 0.25 * s;
+        q2 = (R.m[1] + R.m[3]) / s;
+        q3 = (R.m[2] + R.m[6]) / s;
+    } else if (best == 2) {
+        s = 2.0 * sqrt(((1.0 + R.m[4]) - R.m[0]) - R.m[8]);
+        q0 = (R.m[2] - R.m[6]) / s;
+        q1 = (R.m[1] + R.m[3]) / s;
+        q2 = 0.25 * s;
+        q3 = (R.m[5] + R.m[7]) / s;
+    } else {
+        s = 2.0 * sqrt(((1.0 + R.m[8]) - R.m[0]) - R.m[4]);
+        q0 = (R.m[3] - R.m[1]) / s;
+        q1 = (R.m[2] + R.m[6]) / s;
+        q2 = (R.m[5] + R.m[7]) / s;
+        q3 = 0.25 * s;
+    }
+    if (q0 < 0.0) { q0 = -q0; q1 = -q1; q2 = -q2; q3 = -q3; }
+    double qn = sqrt(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
+    qn = qn > 1e-15 ? qn : 1e-15;
+    q0 /= qn; q1 /= qn; q2 /= qn; q3 /= qn;
+    V3<double> v{q1, q2, q3};
+    double sh = norm(v);
+    double coef = (sh < 1e-12) ? 2.0 : 2.0 * atan2(sh, q0) / sh;
+    return coef * v;
+}
+
+// T_s(th) and its inverse (_beam_cr.py:856-908), in the tangent precision.
+template <class T>
+__device__ __forceinline__ M3<T> tangent_op(V3<double> thd, bool inverse) {
+    double a = norm(thd);
+    V3<T> th = cvt<T>(thd);
+    M3<T> S = skew(th);
+    M3<T> S2 = mul(S, S);
+    T c1, c2;
+    if (!inverse) {
+        if (a < 1e-10) { c1 = T(0.5); c2 = T(1.0 / 6.0); }
+        else { c1 = T((1.0 - cos(a)) / (a * a)); c2 = T((1.0 - sin(a) / a) / (a * a)); }
+    } else {
+        c1 = T(-0.5);
+        if (a < 1e-10) c2 = T(1.0 / 12.0);
+        else c2 = T(1.0 / (a * a) - (1.0 + cos(a)) / (2.0 * a * sin(a)));
+    }
+    M3<T> r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.m[k] = ((k % 4 == 0) ? T(1) : T(0)) + c1 * S.m[k] + c2 * S2.m[k];
+    return r;
+}
+
+template <class TT>
+__global__ void __launch_bounds__(64)
+beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
+            const int* __restrict__ conn, BeamConst bc, double* __restrict__ fe_stage,
+            TT* __restrict__ Ke_stage, int* __restrict__ flags) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elems) return;
+    const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
+    V3<double> P1{X[3 * n1], X[3 * n1 + 1], X[3 * n1 + 2]};
+    V3<double> P2{X[3 * n2], X[3 * n2 + 1], X[3 * n2 + 2]};
+    const double* u1 = u + 6 * (size_t)n1;
+    const double* u2 = u + 6 * (size_t)n2;
+    V3<double> t1{u1[0], u1[1], u1[2]}, r1{u1[3], u1[4], u1[5]};
+    V3<double> t2{u2[0], u2[1], u2[2]}, r2{u2[3], u2[4], u2[5]};
+
+    // reference frame R_0 (rows e_x, e_y, e_z)
+    V3<double> d0 = P2 - P1;
+    double L0 = norm(d0);
+    V3<double> ex0 = (1.0 / 1.0) * V3<double>{d0.x / L0, d0.y / L0, d0.z / L0};
+    V3<double> vr;
+    if (bc.has_vref) {
+        vr = V3<double>{bc.vref[0], bc.vref[1], bc.vref[2]};
+    } else {
+        double ax = fabs(ex0.x), ay = fabs(ex0.y), az = fabs(ex0.z);
+        int k = (ax <= ay && ax <= az) ? 0 : ((ay <= az) ? 1 : 2);
+        vr = V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+    }
+    V3<double> ez = cross(ex0, vr);
+    double nz = norm(ez);
+    nz = nz > 1e-10 ? nz : 1e-10;
+    ez = V3<double>{ez.x / nz, ez.y / nz, ez.z / nz};
+    V3<double> ey = cross(ez, ex0);
+    M3<double> R0;
+    R0.m[0] = ex0.x; R0.m[1] = ex0.y; R0.m[2] = ex0.z;
+    R0.m[3] = ey.x;  R0.m[4] = ey.y;  R0.m[5] = ey.z;
+    R0.m[6] = ez.x;  R0.m[7] = ez.y;  R0.m[8] = ez.z;
+
+    // deformed chord
+    V3<double> dd = (P2 + t2) - (P1 + t1);
+    double Ld = norm(dd);
+    V3<double> exd{dd.x / Ld, dd.y / Ld, dd.z / Ld};
+    if (!(L0 >= 1e-15) || !(Ld >= 1e-15)) atomicOr(flags, FLAG_DEGENERATE);
+
+    // minimal rotation e_x0 -> e_xd (_beam_cr.py:790-828)
+    M3<double> Rrod;
+    {
+        V3<double> v = cross(ex0, exd);
+        double s = norm(v);
+        double c = dot(ex0, exd);
+        if (s < 1e-14) {
+            if (c > 0.0) {
+                Rrod = eye3<double>();
+            } else {
+                double ax = fabs(ex0.x), ay = fabs(ex0.y), az = fabs(ex0.z);
+                int k = (ax <= ay && ax <= az) ? 0 : ((ay <= az) ? 1 : 2);
+                V3<double> perp{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+                V3<double> axis = cross(ex0, perp);
+                double an = norm(axis);
+                axis = V3<double>{axis.x / an, axis.y / an, axis.z / an};
+                double aa[3] = {axis.x, axis.y, axis.z};
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) Rrod.m[3 * i + j] = 2.0 * aa[i] * aa[j] - (i == j ? 1.0 : 0.0);
+            }
+        } else {
+            M3<double> V = skew(v);
+            M3<double> V2 = mul(V, V);
+            double coef = (1.0 - c) / (s * s);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Rrod.m[k] = (((k % 4 == 0) ? 1.0 : 0.0) + V.m[k]) + coef * V2.m[k];
+        }
+    }
+    M3<double> Rcr = mulNT(R0, Rrod);  // R_0 R_rod^T
+
+    // nodal rotations relative to the corotated frame
+    M3<double> Rd1 = mulNT(mul(Rcr, rotvec_to_rotmat(r1)), R0);
+    M3<double> Rd2 = mulNT(mul(Rcr, rotvec_to_rotmat(r2)), R0);
+    V3<double> th1 = rotmat_to_rotvec(Rd1);
+    V3<double> th2 = rotmat_to_rotvec(Rd2);
+    double dax = Ld - L0;
+
+    // local Timoshenko stiffness coefficients (_beam_cr.py:911-998)
+    const double L = L0;
+    double phy = 12.0 * bc.E * bc.Iy / (bc.ky * bc.G * bc.A * L * L);
+    double phz = 12.0 * bc.E * bc.Iz / (bc.kz * bc.G * bc.A * L * L);
+    if (bc.scf > 0.0) {
+        phy = phy * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iy))));
+        phz = phz * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iz))));
+    }
+    const double dy = 1.0 + phy, dz = 1.0 + phz;
+    const double L2 = L * L, L3 = L2 * L;
+    const double ea = bc.E * bc.A / L;
+    const double gj = bc.G * bc.J / L;
+    const double k15 = 6.0 * (bc.E * bc.Iz / L2) / dz;          // Ke[1][5]
+    const double k55 = (4.0 + phz) * (bc.E * bc.Iz / L) / dz;   // Ke[5][5]
+    const double k511 = (2.0 - phz) * (bc.E * bc.Iz / L) / dz;  // Ke[5][11]
+    const double m15 = 6.0 * (bc.E * bc.Iy / L2) / dy;          // -Ke[2][4]
+    const double m44 = (4.0 + phy) * (bc.E * bc.Iy / L) / dy;   // Ke[4][4]
+    const double m410 = (2.0 - phy) * (bc.E * bc.Iy / L) / dy;  // Ke[4][10]
+    (void)L3;
+
+    // corotated element force f_cr = Ke d_cr (only nonzero d_cr entries used)
+    double fcr[12];
+    fcr[0] = -ea * dax;
+    fcr[6] = ea * dax;
+    fcr[3] = gj * th1.x - gj * th2.x;
+    fcr[9] = -gj * th1.x + gj * th2.x;
+    fcr[1] = k15 * th1.z + k15 * th2.z;
+    fcr[7] = -k15 * th1.z - k15 * th2.z;
+    fcr[5] = k55 * th1.z + k511 * th2.z;
+    fcr[11] = k511 * th1.z + k55 * th2.z;
+    fcr[2] = -m15 * th1.y - m15 * th2.y;
+    fcr[8] = m15 * th1.y + m15 * th2.y;
+    fcr[4] = m44 * th1.y + m410 * th2.y;
+    fcr[10] = m410 * th1.y + m44 * th2.y;
+
+    if (fe_stage) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
+            fe_stage[(size_t)(3 * b) * n_elems + e] = fb.x;
+            fe_stage[(size_t)(3 * b + 1) * n_elems + e] = fb.y;
+            fe_stage[(size_t)(3 * b + 2) * n_elems + e] = fb.z;
+        }
+    }
+    if (!Ke_stage) return;
+
+    // ---- analytical tangent in precision TT ----
+    using T = TT;
+    M3<T> Rc = cvt<T>(Rcr);
+    M3<T> dpsi = cvt<T>(mul(Rcr, skew(exd)));
+    {
+        T il = T(1.0 / Ld);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dpsi.m[k] = il * dpsi.m[k];
+    }
+    M3<T> Ti1 = tangent_op<T>(th1, true);
+    M3<T> Ti2 = tangent_op<T>(th2, true);
+    M3<T> Ts1 = tangent_op<T>(r1, false);
+    M3<T> Ts2 = tangent_op<T>(r2, false);
+    M3<T> P1m = mul(Ti1, dpsi);            // B[3:6, 0:3]  (B[3:6, 6:9] = -P1)
+    M3<T> Q1m = mul(mul(Ti1, Rc), Ts1);    // B[3:6, 3:6]
+    M3<T> P2m = mul(Ti2, dpsi);            // B[9:12, 0:3] (B[9:12, 6:9] = -P2)
+    M3<T> Q2m = mul(mul(Ti2, Rc), Ts2);    // B[9:12, 9:12]
+    V3<T> ex = cvt<T>(exd);
+
+    const T tea = T(ea), tgj = T(gj), tk15 = T(k15), tk55 = T(k55), tk511 = T(k511);
+    const T tm15 = T(m15), tm44 = T(m44), tm410 = T(m410);
+
+    // (Ke B) rows as x / y / z parts; KBx[r] etc. are 3-vectors
+    T KBx[12][3], KBy[12][3], KBz[12][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T exj = j == 0 ? ex.x : (j == 1 ? ex.y : ex.z);
+        // axial rows 0 / 6: B[6] = [-e_x, 0, e_x, 0]
+        KBx[0][j] = tea * exj;  KBy[0][j] = T(0); KBz[0][j] = T(0);
+        KBx[6][j] = -tea * exj; KBy[6][j] = T(0); KBz[6][j] = T(0);
+        // torsion rows 3 / 9
+        KBx[3][j] = tgj * P1m(0, j) - tgj * P2m(0, j);
+        KBy[3][j] = tgj * Q1m(0, j);
+        KBz[3][j] = -tgj * Q2m(0, j);
+        KBx[9][j] = -KBx[3][j]; KBy[9][j] = -KBy[3][j]; KBz[9][j] = -KBz[3][j];
+        // bending x-y plane rows 1, 5, 7, 11 (rotation component z = row 2 of P/Q)
+        KBx[1][j] = tk15 * P1m(2, j) + tk15 * P2m(2, j);
+        KBy[1][j] = tk15 * Q1m(2, j);
+        KBz[1][j] = tk15 * Q2m(2, j);
+        KBx[7][j] = -KBx[1][j]; KBy[7][j] = -KBy[1][j]; KBz[7][j] = -KBz[1][j];
+        KBx[5][j] = tk55 * P1m(2, j) + tk511 * P2m(2, j);
+        KBy[5][j] = tk55 * Q1m(2, j);
+        KBz[5][j] = tk511 * Q2m(2, j);
+        KBx[11][j] = tk511 * P1m(2, j) + tk55 * P2m(2, j);
+        KBy[11][j] = tk511 * Q1m(2, j);
+        KBz[11][j] = tk55 * Q2m(2, j);
+        // bending x-z plane rows 2, 4, 8, 10 (rotation component y = row 1)
+        KBx[2][j] = -tm15 * P1m(1, j) - tm15 * P2m(1, j);
+        KBy[2][j] = -tm15 * Q1m(1, j);
+        KBz[2][j] = -tm15 * Q2m(1, j);
+        KBx[8][j] = -KBx[2][j]; KBy[8][j] = -KBy[2][j]; KBz[8][j] = -KBz[2][j];
+        KBx[4][j] = tm44 * P1m(1, j) + tm410 * P2m(1, j);
+        KBy[4][j] = tm44 * Q1m(1, j);
+        KBz[4][j] = tm410 * Q2m(1, j);
+        KBx[10][j] = tm410 * P1m(1, j) + tm44 * P2m(1, j);
+        KBy[10][j] = tm410 * Q1m(1, j);
+        KBz[10][j] = tm44 * Q2m(1, j);
+    }
+
+    // block rows b: X_b = Rc^T KBx_b + Rc^T S(f_b) dpsi, Y_b = Rc^T KBy_b, Z_b = Rc^T KBz_b
+    M3<T> Xb[4], Yb[4], Zb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        M3<T> Ax, Ay, Az;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                Ax.m[3 * i + j] = KBx[3 * b + i][j];
+                Ay.m[3 * i + j] = KBy[3 * b + i][j];
+                Az.m[3 * i + j] = KBz[3 * b + i][j];
+            }
+        M3<T> W = mul(mulTN(Rc, skew(V3<T>{T(fcr[3 * b]), T(fcr[3 * b + 1]), T(fcr[3 * b + 2])})), dpsi);
+        Xb[b] = mulTN(Rc, Ax);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Xb[b].m[k] = Xb[b].m[k] + W.m[k];
+        Yb[b] = mulTN(Rc, Ay);
+        Zb[b] = mulTN(Rc, Az);
+    }
+
+    // symmetrised 3x3 group blocks Ks(b,c) = (K(b,c) + K(c,b)^T) / 2,
+    // K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b
+    const T h = T(0.5);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const M3<T>& Kbc = (c == 0 || c == 2) ? Xb[b] : (c == 1 ? Yb[b] : Zb[b]);
+            const T sbc = (c == 2) ? T(-1) : T(1);
+            const M3<T>& Kcb = (b == 0 || b == 2) ? Xb[c] : (b == 1 ? Yb[c] : Zb[c]);
+            const T scb = (b == 2) ? T(-1) : T(1);
+            const int lb = b * 4 + c;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    T v = h * (sbc * Kbc(i, j) + scb * Kcb(j, i));
+                    Ke_stage[(size_t)(lb * 9 + 3 * i + j) * n_elems + e] = v;
+                }
+        }
+    }
+}
+
+template __global__ void beam_kernel<double>(int, const double*, const double*, const int*, BeamConst,
+                                             double*, double*, int*);
+template __global__ void beam_kernel<float>(int, const double*, const double*, const int*, BeamConst,
+                                            double*, float*, int*);
+
+}  // namespace xk
+
+// host-side launchers
+namespace xk {
+hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, const int* conn,
+                       const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags,
+                       hipStream_t st) {
+    if (n_elems <= 0) return hipSuccess;
+    dim3 blk(64), grd((n_elems + 63) / 64);
+    if (prec == 0)
+        hipLaunchKernelGGL(beam_kernel<double>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
+                           (double*)Ke_stage, flags);
+    else
+        hipLaunchKernelGGL(beam_kernel<float>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
+                           (float*)Ke_stage, flags);
+    return hipGetLastError();
+}
+}  // namespace xk

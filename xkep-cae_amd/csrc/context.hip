@@ -1,0 +1,668 @@
+// Host side of the gfx950 assembly library: device context, pair list,
+// per-increment tangent pattern and the extern "C" entry points declared in
+// include/xkamd.h.
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "xk_common.h"
+#include "../../include/xkamd.h"
+#include "xk_internal.h"
+
+namespace xk {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+#define XK_TRY(expr)                                                                 \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+hipError_t DBuf::ensure(size_t nbytes) {
+    if (nbytes <= bytes && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return e;
+        p = nullptr;
+        bytes = 0;
+    }
+    size_t want = nbytes < 256 ? 256 : nbytes;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+}
+
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+template <class T>
+static hipError_t upload(DBuf& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// counting sort of (key, code) pairs into CSR lists, preserving insertion order
+static void bucket(int n_keys, const std::vector<int>& keys, const std::vector<int>& codes,
+                   std::vector<int>& ptr, std::vector<int>& src) {
+    ptr.assign(n_keys + 1, 0);
+    for (int k : keys) ptr[k + 1]++;
+    for (int i = 0; i < n_keys; ++i) ptr[i + 1] += ptr[i];
+    src.resize(keys.size());
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t q = 0; q < keys.size(); ++q) src[fill[keys[q]]++] = codes[q];
+}
+
+int Ctx::grow_pairs(int64_t need) {
+    if (need <= cap) return 0;
+    XK_TRY(hipDeviceSynchronize());
+    int64_t nc = std::max<int64_t>(need, std::max<int64_t>(1024, cap * 3 / 2));
+    PairSoA old = P;
+    std::vector<DBuf> nb(PB_COUNT);
+    const size_t sz[PB_COUNT] = {4, 4, 4, 4, 8, 8, 8, 8, 8, 24, 24, 8, 8, 8, 8, 8, 16, 1, 1, 4, 4, 4};
+    for (int k = 0; k < PB_COUNT; ++k) XK_TRY(nb[k].ensure(sz[k] * (size_t)nc));
+    // copy existing columns (component-major arrays are re-strided)
+    auto cp = [&](int k, int comps, size_t esz) -> hipError_t {
+        for (int c = 0; c < comps; ++c) {
+            hipError_t e = hipMemcpy((char*)nb[k].p + c * nc * esz, (char*)pb[k].p + c * cap * esz,
+                                     n_pairs * esz, hipMemcpyDeviceToDevice);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (n_pairs > 0) {
+        for (int k = 0; k < PB_COUNT; ++k) {
+            int comps = (k == PB_N || k == PB_T1) ? 3 : (k == PB_Z ? 2 : 1);
+            size_t esz = sz[k] / comps;
+            XK_TRY(cp(k, comps, esz));
+        }
+    }
+    for (int k = 0; k < PB_COUNT; ++k) {
+        pb[k].release();
+        pb[k] = nb[k];
+    }
+    (void)old;
+    cap = nc;
+    bind_pairs();
+    return 0;
+}
+
+void Ctx::bind_pairs() {
+    P.na0 = (int*)pb[PB_NA0].p; P.na1 = (int*)pb[PB_NA1].p;
+    P.nb0 = (int*)pb[PB_NB0].p; P.nb1 = (int*)pb[PB_NB1].p;
+    P.ra = (double*)pb[PB_RA].p; P.rb = (double*)pb[PB_RB].p;
+    P.s = (double*)pb[PB_S].p; P.t = (double*)pb[PB_T].p; P.gap = (double*)pb[PB_GAP].p;
+    P.n = (double*)pb[PB_N].p; P.t1 = (double*)pb[PB_T1].p;
+    P.p_n = (double*)pb[PB_PN].p; P.k_pen = (double*)pb[PB_KPEN].p; P.k_t = (double*)pb[PB_KT].p;
+    P.qtn = (double*)pb[PB_QTN].p; P.diss = (double*)pb[PB_DISS].p; P.z = (double*)pb[PB_Z].p;
+    P.status = (int8_t*)pb[PB_STATUS].p; P.stick = (int8_t*)pb[PB_STICK].p;
+    P.sinc = (int*)pb[PB_SINC].p;
+    P.cap = cap;
+    ea = (int*)pb[PB_EA].p; eb = (int*)pb[PB_EB].p;
+}
+
+int Ctx::build_pattern() {
+    XK_TRY(hipDeviceSynchronize());
+    const int64_t np = n_pairs;
+    std::vector<int8_t> st(np);
+    std::vector<double> pn(np);
+    std::vector<int> nd[4];
+    if (np) {
+        XK_TRY(hipMemcpy(st.data(), P.status, np, hipMemcpyDeviceToHost));
+        XK_TRY(hipMemcpy(pn.data(), P.p_n, np * 8, hipMemcpyDeviceToHost));
+        int* src[4] = {P.na0, P.na1, P.nb0, P.nb1};
+        for (int k = 0; k < 4; ++k) {
+            nd[k].resize(np);
+            XK_TRY(hipMemcpy(nd[k].data(), src[k], np * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    // contributing pairs of the increment: status != INACTIVE or p_n > 0
+    std::vector<int> sinc_of(np, -1);
+    std::vector<int> sinc_list;
+    for (int64_t i = 0; i < np; ++i)
+        if (st[i] != ST_INACTIVE || pn[i] > 0.0) {
+            sinc_of[i] = (int)sinc_list.size();
+            sinc_list.push_back((int)i);
+        }
+    const int ns = (int)sinc_list.size();
+    const int ng = 2 * n_nodes;
+
+    // block keys (row group, col group)
+    std::vector<uint64_t> keys;
+    keys.reserve(16 * ((size_t)n_elems + ns));
+    auto key = [](uint64_t r, uint64_t c) { return (r << 32) | c; };
+    for (int e = 0; e < n_elems; ++e) {
+        int g[4] = {2 * conn_h[2 * e], 2 * conn_h[2 * e] + 1, 2 * conn_h[2 * e + 1], 2 * conn_h[2 * e + 1] + 1};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) keys.push_back(key(g[a], g[b]));
+    }
+    for (int q = 0; q < ns; ++q) {
+        int i = sinc_list[q];
+        int g[4] = {2 * nd[0][i], 2 * nd[1][i], 2 * nd[2][i], 2 * nd[3][i]};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) keys.push_back(key(g[a], g[b]));
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int nblk = (int)keys.size();
+    gptr_h.assign(ng + 1, 0);
+    gcol_h.resize(nblk);
+    std::vector<int> brow(nblk);
+    for (int b = 0; b < nblk; ++b) {
+        int r = (int)(keys[b] >> 32);
+        gcol_h[b] = (int)(keys[b] & 0xffffffffu);
+        brow[b] = r;
+        gptr_h[r + 1]++;
+    }
+    for (int r = 0; r < ng; ++r) gptr_h[r + 1] += gptr_h[r];
+    auto find = [&](int r, int c) -> int {
+        auto lo = gcol_h.begin() + gptr_h[r], hi = gcol_h.begin() + gptr_h[r + 1];
+        auto it = std::lower_bound(lo, hi, c);
+        return (int)(it - gcol_h.begin());
+    };
+    // block contribution lists
+    std::vector<int> kb, cb, kp, cp;
+    kb.reserve(16 * (size_t)n_elems);
+    cb.reserve(16 * (size_t)n_elems);
+    for (int e = 0; e < n_elems; ++e) {
+        int g[4] = {2 * conn_h[2 * e], 2 * conn_h[2 * e] + 1, 2 * conn_h[2 * e + 1], 2 * conn_h[2 * e + 1] + 1};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                kb.push_back(find(g[a], g[b]));
+                cb.push_back(e * 16 + a * 4 + b);
+            }
+    }
+    std::vector<int> nk, nc;
+    kp.reserve(16 * (size_t)ns);
+    cp.reserve(16 * (size_t)ns);
+    nk.reserve(4 * (size_t)ns);
+    nc.reserve(4 * (size_t)ns);
+    for (int q = 0; q < ns; ++q) {
+        int i = sinc_list[q];
+        int nn[4] = {nd[0][i], nd[1][i], nd[2][i], nd[3][i]};
+        for (int a = 0; a < 4; ++a) {
+            for (int b = 0; b < 4; ++b) {
+                kp.push_back(find(2 * nn[a], 2 * nn[b]));
+                cp.push_back(q * 16 + a * 4 + b);
+            }
+            nk.push_back(nn[a]);
+            nc.push_back(q * 4 + a);
+        }
+    }
+    std::vector<int> bptr, bsrc, pptr, psrc, nptr, nsrc;
+    bucket(nblk, kb, cb, bptr, bsrc);
+    bucket(nblk, kp, cp, pptr, psrc);
+    bucket(n_nodes, nk, nc, nptr, nsrc);
+
+    XK_TRY(upload(d_gptr, gptr_h));
+    XK_TRY(upload(d_gcol, gcol_h));
+    XK_TRY(upload(d_brow, brow));
+    XK_TRY(upload(d_blk_bptr, bptr));
+    XK_TRY(upload(d_blk_bsrc, bsrc));
+    XK_TRY(upload(d_blk_pptr, pptr));
+    XK_TRY(upload(d_blk_psrc, psrc));
+    XK_TRY(upload(d_node_pptr, nptr));
+    XK_TRY(upload(d_node_psrc, nsrc));
+    if (np) XK_TRY(hipMemcpy(P.sinc, sinc_of.data(), np * 4, hipMemcpyHostToDevice));
+    XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * 7 * sizeof(double)));
+    XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * 9 * (prec == 0 ? 8 : 4)));
+    n_sinc = ns;
+    n_blocks = nblk;
+    pattern_valid = true;
+    return 0;
+}
+
+int Ctx::ensure_pattern() { return pattern_valid ? 0 : build_pattern(); }
+
+BlockLists Ctx::block_lists() const {
+    BlockLists B;
+    B.gptr = (const int*)d_gptr.p; B.gcol = (const int*)d_gcol.p; B.brow = (const int*)d_brow.p;
+    B.bptr = (const int*)d_blk_bptr.p; B.bsrc = (const int*)d_blk_bsrc.p;
+    B.pptr = (const int*)d_blk_pptr.p; B.psrc = (const int*)d_blk_psrc.p;
+    return B;
+}
+
+NodeLists Ctx::node_lists() const {
+    NodeLists L;
+    L.bptr = (const int*)d_node_bptr.p; L.bsrc = (const int*)d_node_bsrc.p;
+    L.pptr = (const int*)d_node_pptr.p; L.psrc = (const int*)d_node_psrc.p;
+    return L;
+}
+
+}  // namespace xk
+
+using namespace xk;
+
+struct xk_ctx : public Ctx {};
+
+extern "C" {
+
+const char* xk_last_error(void) { return g_last_error.c_str(); }
+
+int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coords, const int32_t* connectivity,
+                  const double* section8, double scf, const double* v_ref3, const double* elem_radius,
+                  int precision) {
+    if (!out || n_nodes <= 0 || n_elems < 0 || !node_coords || (n_elems && !connectivity) || !section8) {
+        set_error("xk_ctx_create: invalid arguments");
+        return -1;
+    }
+    if (precision != XK_PREC_FP64 && precision != XK_PREC_MIXED) {
+        set_error("xk_ctx_create: unknown precision");
+        return -1;
+    }
+    for (int i = 0; i < 2 * n_elems; ++i)
+        if (connectivity[i] < 0 || connectivity[i] >= n_nodes) {
+            set_error("xk_ctx_create: connectivity index out of range");
+            return -1;
+        }
+    xk_ctx* c = new xk_ctx();
+    c->n_nodes = n_nodes;
+    c->n_elems = n_elems;
+    c->prec = precision;
+    c->X_h.assign(node_coords, node_coords + 3 * (size_t)n_nodes);
+    c->conn_h.assign(connectivity, connectivity + 2 * (size_t)n_elems);
+    c->rad_h.assign(n_elems, 0.0);
+    if (elem_radius) c->rad_h.assign(elem_radius, elem_radius + n_elems);
+    BeamConst& b = c->bc;
+    b.E = section8[0]; b.G = section8[1]; b.A = section8[2]; b.Iy = section8[3];
+    b.Iz = section8[4]; b.J = section8[5]; b.ky = section8[6]; b.kz = section8[7];
+    b.scf = scf;
+    b.has_vref = v_ref3 ? 1 : 0;
+    for (int k = 0; k < 3; ++k) b.vref[k] = v_ref3 ? v_ref3[k] : 0.0;
+    int rc = 0;
+    auto fail = [&](const char* what) {
+        set_error(std::string("xk_ctx_create: ") + what);
+        rc = -1;
+    };
+    if (upload(c->d_X, c->X_h) != hipSuccess) fail("X");
+    if (!rc && upload(c->d_conn, c->conn_h) != hipSuccess) fail("conn");
+    if (!rc && upload(c->d_rad, c->rad_h) != hipSuccess) fail("radius");
+    // static element -> node lists (element order)
+    if (!rc) {
+        std::vector<int> k, cd, ptr, src;
+        for (int e = 0; e < n_elems; ++e)
+            for (int a = 0; a < 2; ++a) {
+                k.push_back(c->conn_h[2 * e + a]);
+                cd.push_back(2 * e + a);
+            }
+        bucket(n_nodes, k, cd, ptr, src);
+        if (upload(c->d_node_bptr, ptr) != hipSuccess || upload(c->d_node_bsrc, src) != hipSuccess)
+            fail("node lists");
+    }
+    if (!rc && c->d_fe.ensure((size_t)std::max(n_elems, 1) * 12 * 8) != hipSuccess) fail("fe stage");
+    if (!rc && c->d_Ke.ensure((size_t)std::max(n_elems, 1) * 144 * (precision == 0 ? 8 : 4)) != hipSuccess)
+        fail("Ke stage");
+    if (!rc && c->d_flags.ensure(64) != hipSuccess) fail("flags");
+    if (!rc && c->d_partial.ensure(sizeof(double) * (size_t)(node_gather_blocks(n_nodes) + 1)) != hipSuccess)
+        fail("partials");
+    if (!rc && hipMemset(c->d_flags.p, 0, 64) != hipSuccess) fail("flags init");
+    if (!rc) {
+        c->pb.resize(PB_COUNT);
+        if (c->grow_pairs(1024) != 0) rc = -1;
+    }
+    if (rc) {
+        c->release_all();
+        delete c;
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+int xk_ctx_destroy(xk_ctx* c) {
+    if (!c) return 0;
+    (void)hipDeviceSynchronize();
+    c->release_all();
+    delete c;
+    return 0;
+}
+
+int xk_ctx_precision(const xk_ctx* c) { return c ? c->prec : -1; }
+
+int xk_beam_assemble(xk_ctx* c, const double* u, double* f_int, void* K_vals, int64_t K_cap, void* stream) {
+    if (!c || !u) {
+        set_error("xk_beam_assemble: null argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (K_vals && c->ensure_pattern() != 0) return -1;
+    if (K_vals && K_cap < 9 * (int64_t)c->n_blocks) {
+        set_error("xk_beam_assemble: K_vals holds fewer entries than the current pattern");
+        return -1;
+    }
+    XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
+                       f_int ? (double*)c->d_fe.p : nullptr, K_vals ? c->d_Ke.p : nullptr, (int*)c->d_flags.p,
+                       st));
+    if (f_int) {
+        DynArgs dy{};
+        XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), (const double*)c->d_fe.p,
+                                  nullptr, 0, nullptr, nullptr, nullptr, dy, f_int, nullptr, (int*)c->d_flags.p,
+                                  st));
+    }
+    if (K_vals) {
+        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), c->d_Ke.p,
+                                   nullptr, nullptr, 1, 0, 0.0, nullptr, K_vals, st));
+    }
+    return 0;
+}
+
+int xk_pairs_count(const xk_ctx* c, int64_t* n) {
+    if (!c || !n) return -1;
+    *n = c->n_pairs;
+    return 0;
+}
+
+int xk_pairs_set(xk_ctx* c, int64_t n, const int32_t* ea, const int32_t* eb, const double* s, const double* t,
+                 const double* gap, const double* normal, const double* tangent1, const double* p_n,
+                 const double* k_pen, const double* k_t, const double* q_trial_norm, const double* dissipation,
+                 const double* z_t, const int8_t* status, const int8_t* stick) {
+    if (!c || n < 0 || (n && (!ea || !eb))) {
+        set_error("xk_pairs_set: invalid arguments");
+        return -1;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (ea[i] < 0 || ea[i] >= c->n_elems || eb[i] < 0 || eb[i] >= c->n_elems) {
+            set_error("xk_pairs_set: element index out of range");
+            return -1;
+        }
+    XK_TRY(hipDeviceSynchronize());
+    if (c->grow_pairs(std::max<int64_t>(n, 1)) != 0) return -1;
+    c->n_pairs = n;
+    c->keys_valid = false;
+    c->filtered_valid = false;
+    if (n == 0) {
+        c->pattern_valid = false;
+        return 0;
+    }
+    const int64_t cap = c->cap;
+    std::vector<int> na0(n), na1(n), nb0(n), nb1(n);
+    std::vector<double> ra(n), rb(n);
+    for (int64_t i = 0; i < n; ++i) {
+        na0[i] = c->conn_h[2 * ea[i]]; na1[i] = c->conn_h[2 * ea[i] + 1];
+        nb0[i] = c->conn_h[2 * eb[i]]; nb1[i] = c->conn_h[2 * eb[i] + 1];
+        ra[i] = c->rad_h[ea[i]]; rb[i] = c->rad_h[eb[i]];
+    }
+    auto put = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+    auto put_or_zero = [&](void* dst, const void* src, size_t bytes) {
+        return src ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipMemset(dst, 0, bytes);
+    };
+    XK_TRY(put(c->ea, ea, n * 4));
+    XK_TRY(put(c->eb, eb, n * 4));
+    XK_TRY(put(c->P.na0, na0.data(), n * 4));
+    XK_TRY(put(c->P.na1, na1.data(), n * 4));
+    XK_TRY(put(c->P.nb0, nb0.data(), n * 4));
+    XK_TRY(put(c->P.nb1, nb1.data(), n * 4));
+    XK_TRY(put(c->P.ra, ra.data(), n * 8));
+    XK_TRY(put(c->P.rb, rb.data(), n * 8));
+    XK_TRY(put_or_zero(c->P.s, s, n * 8));
+    XK_TRY(put_or_zero(c->P.t, t, n * 8));
+    XK_TRY(put_or_zero(c->P.gap, gap, n * 8));
+    for (int k = 0; k < 3; ++k) {
+        std::vector<double> col(n, 0.0);
+        if (normal) for (int64_t i = 0; i < n; ++i) col[i] = normal[3 * i + k];
+        XK_TRY(put(c->P.n + k * cap, col.data(), n * 8));
+        if (tangent1) for (int64_t i = 0; i < n; ++i) col[i] = tangent1[3 * i + k];
+        else std::fill(col.begin(), col.end(), 0.0);
+        XK_TRY(put(c->P.t1 + k * cap, col.data(), n * 8));
+    }
+    for (int k = 0; k < 2; ++k) {
+        std::vector<double> col(n, 0.0);
+        if (z_t) for (int64_t i = 0; i < n; ++i) col[i] = z_t[2 * i + k];
+        XK_TRY(put(c->P.z + k * cap, col.data(), n * 8));
+    }
+    XK_TRY(put_or_zero(c->P.p_n, p_n, n * 8));
+    XK_TRY(put_or_zero(c->P.k_pen, k_pen, n * 8));
+    XK_TRY(put_or_zero(c->P.k_t, k_t, n * 8));
+    XK_TRY(put_or_zero(c->P.qtn, q_trial_norm, n * 8));
+    XK_TRY(put_or_zero(c->P.diss, dissipation, n * 8));
+    XK_TRY(put_or_zero(c->P.status, status, n));
+    if (stick) XK_TRY(put(c->P.stick, stick, n));
+    else XK_TRY(hipMemset(c->P.stick, 1, n));
+    XK_TRY(hipMemset(c->P.sinc, 0xff, n * 4));  // -1: not contributing until the pattern is built
+    c->pattern_valid = false;
+    return 0;
+}
+
+int xk_pairs_get(xk_ctx* c, int32_t* ea, int32_t* eb, double* s, double* t, double* gap, double* normal,
+                 double* tangent1, double* p_n, double* k_pen, double* k_t, double* q_trial_norm,
+                 double* dissipation, double* z_t, int8_t* status, int8_t* stick) {
+    if (!c) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    const int64_t n = c->n_pairs, cap = c->cap;
+    if (n == 0) return 0;
+    auto get = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        return dst ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+    };
+    XK_TRY(get(ea, c->ea, n * 4));
+    XK_TRY(get(eb, c->eb, n * 4));
+    XK_TRY(get(s, c->P.s, n * 8));
+    XK_TRY(get(t, c->P.t, n * 8));
+    XK_TRY(get(gap, c->P.gap, n * 8));
+    XK_TRY(get(p_n, c->P.p_n, n * 8));
+    XK_TRY(get(k_pen, c->P.k_pen, n * 8));
+    XK_TRY(get(k_t, c->P.k_t, n * 8));
+    XK_TRY(get(q_trial_norm, c->P.qtn, n * 8));
+    XK_TRY(get(dissipation, c->P.diss, n * 8));
+    XK_TRY(get(status, c->P.status, n));
+    XK_TRY(get(stick, c->P.stick, n));
+    std::vector<double> col(n);
+    for (int k = 0; k < 3; ++k) {
+        if (normal) {
+            XK_TRY(hipMemcpy(col.data(), c->P.n + k * cap, n * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) normal[3 * i + k] = col[i];
+        }
+        if (tangent1) {
+            XK_TRY(hipMemcpy(col.data(), c->P.t1 + k * cap, n * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) tangent1[3 * i + k] = col[i];
+        }
+    }
+    if (z_t)
+        for (int k = 0; k < 2; ++k) {
+            XK_TRY(hipMemcpy(col.data(), c->P.z + k * cap, n * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) z_t[2 * i + k] = col[i];
+        }
+    return 0;
+}
+
+int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u, const double* u_ref,
+                 double k_pen, double smoothing_delta, double k_t_ratio, double mu_eff, double g_on, double g_off,
+                 int allow_deactivation, void* stream) {
+    if (!c) return -1;
+    const int phys = PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F;
+    if ((modes & PAIR_HYST) && (modes & phys)) {
+        set_error("xk_pairs_run: the hysteresis update cannot be combined with force/tangent modes");
+        return -1;
+    }
+    if ((modes & PAIR_HYST) && !(modes & PAIR_GEOM)) {
+        set_error("xk_pairs_run: the hysteresis update needs the geometry mode");
+        return -1;
+    }
+    if ((modes & PAIR_GEOM) && !coords_ref) {
+        set_error("xk_pairs_run: geometry needs coordinates");
+        return -1;
+    }
+    if ((modes & PAIR_FRIC) && !u) {
+        set_error("xk_pairs_run: friction needs u");
+        return -1;
+    }
+    if ((modes & phys) && c->ensure_pattern() != 0) return -1;
+    PairParams pp;
+    pp.k_pen = k_pen;
+    pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
+    pp.k_t_ratio = k_t_ratio;
+    pp.mu_eff = mu_eff;
+    pp.g_on = g_on;
+    pp.g_off = g_off;
+    pp.allow_deact = allow_deactivation;
+    pp.n_sinc = c->n_sinc;
+    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_sd.p,
+                        c->d_sg.p, nullptr, (hipStream_t)stream));
+    if (modes & PAIR_HYST) c->pattern_valid = false;
+    c->last_modes = modes;
+    return 0;
+}
+
+int xk_pairs_gather(xk_ctx* c, double* f, void* K_vals, int64_t K_cap, void* stream) {
+    if (!c) return -1;
+    if (c->ensure_pattern() != 0) return -1;
+    if (K_vals && K_cap < 9 * (int64_t)c->n_blocks) {
+        set_error("xk_pairs_gather: K_vals holds fewer entries than the current pattern");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (f) {
+        DynArgs dy{};
+        XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), nullptr,
+                                  (const double*)c->d_sd.p, 1, nullptr, nullptr, nullptr, dy, f, nullptr,
+                                  (int*)c->d_flags.p, st));
+    }
+    if (K_vals) {
+        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), nullptr,
+                                   (const double*)c->d_sd.p, c->d_sg.p, 0, 1, 0.0, nullptr, K_vals, st));
+    }
+    return 0;
+}
+
+int xk_pattern_build(xk_ctx* c) { return c ? c->build_pattern() : -1; }
+
+int xk_pattern_info(xk_ctx* c, int64_t* nnz, int64_t* n_blocks, int64_t* n_contrib) {
+    if (!c) return -1;
+    if (c->ensure_pattern() != 0) return -1;
+    if (nnz) *nnz = 9 * (int64_t)c->n_blocks;
+    if (n_blocks) *n_blocks = c->n_blocks;
+    if (n_contrib) *n_contrib = c->n_sinc;
+    return 0;
+}
+
+int xk_pattern_export(xk_ctx* c, int64_t* indptr, int32_t* indices) {
+    if (!c) return -1;
+    if (c->ensure_pattern() != 0) return -1;
+    const int ng = 2 * c->n_nodes;
+    int64_t pos = 0;
+    for (int gr = 0; gr < ng; ++gr) {
+        const int r0 = c->gptr_h[gr], r1 = c->gptr_h[gr + 1], deg = r1 - r0;
+        for (int a = 0; a < 3; ++a) {
+            const int64_t row = 3 * (int64_t)gr + a;
+            if (indptr) indptr[row] = 9 * (int64_t)r0 + (int64_t)a * 3 * deg;
+            if (indices)
+                for (int b = r0; b < r1; ++b)
+                    for (int k = 0; k < 3; ++k) indices[pos++] = 3 * c->gcol_h[b] + k;
+        }
+    }
+    if (indptr) indptr[3 * (int64_t)ng] = 9 * (int64_t)c->n_blocks;
+    return 0;
+}
+
+int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const double* f_ext,
+                          const uint8_t* fixed_mask, const xk_dyn* dyn, double k_pen, double smoothing_delta,
+                          double k_t_ratio, double mu_eff, double* R, void* K_vals, int64_t K_cap,
+                          double* res_norm2, int32_t* n_active, void* stream) {
+    if (!c || !u || !f_ext || !fixed_mask || !R) {
+        set_error("xk_assemble_iteration: null argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (c->ensure_pattern() != 0) return -1;
+    if (K_vals && K_cap < 9 * (int64_t)c->n_blocks) {
+        set_error("xk_assemble_iteration: K_vals holds fewer entries than the current pattern");
+        return -1;
+    }
+    if (n_active) XK_TRY(launch_fill_i32(1, n_active, 0, st));
+    PairParams pp;
+    pp.k_pen = k_pen;
+    pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
+    pp.k_t_ratio = k_t_ratio;
+    pp.mu_eff = mu_eff;
+    pp.g_on = 0.0;
+    pp.g_off = 0.0;
+    pp.allow_deact = 0;
+    pp.n_sinc = c->n_sinc;
+    int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    hipEvent_t* ev = nullptr;
+    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[5 * (size_t)c->prof_calls++];
+    if (ev) XK_TRY(hipEventRecord(ev[0], st));
+    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                        (double*)c->d_sd.p, c->d_sg.p, n_active, st));
+    if (ev) XK_TRY(hipEventRecord(ev[1], st));
+    XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
+                       (double*)c->d_fe.p, K_vals ? c->d_Ke.p : nullptr, (int*)c->d_flags.p, st));
+    if (ev) XK_TRY(hipEventRecord(ev[2], st));
+    DynArgs dy{};
+    double mass_coef = 0.0;
+    if (dyn && dyn->dt > 1e-30) {
+        dy.on = 1;
+        dy.M = dyn->M_diag; dy.u_pred = dyn->u_pred; dy.v_pred = dyn->v_pred; dy.acc_old = dyn->acc_old;
+        dy.acc = dyn->acc; dy.vel = dyn->vel;
+        dy.c0 = 1.0 / (dyn->beta * (dyn->dt * dyn->dt));
+        dy.dt = dyn->dt; dy.gamma = dyn->gamma; dy.alpha_m = dyn->alpha_m;
+        mass_coef = (1.0 - dyn->alpha_m) * dy.c0;
+    }
+    double* partial = res_norm2 ? (double*)c->d_partial.p : nullptr;
+    XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), (const double*)c->d_fe.p,
+                              (const double*)c->d_sd.p, 2, u, f_ext, fixed_mask, dy, R, partial,
+                              (int*)c->d_flags.p, st));
+    if (res_norm2) XK_TRY(launch_sum_partials(node_gather_blocks(c->n_nodes), partial, res_norm2, st));
+    if (ev) XK_TRY(hipEventRecord(ev[3], st));
+    if (K_vals) {
+        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), c->d_Ke.p,
+                                   (const double*)c->d_sd.p, c->d_sg.p, 1, 1, mass_coef, dy.on ? dy.M : nullptr,
+                                   K_vals, st));
+    }
+    if (ev) XK_TRY(hipEventRecord(ev[4], st));
+    c->last_modes = modes;
+    return 0;
+}
+
+int xk_profile_begin(xk_ctx* c, int max_calls) {
+    if (!c || max_calls < 0) return -1;
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.assign(5 * (size_t)max_calls, nullptr);
+    for (auto& e : c->prof_ev) XK_TRY(hipEventCreate(&e));
+    c->prof_max = max_calls;
+    c->prof_calls = 0;
+    return 0;
+}
+
+int xk_profile_end(xk_ctx* c, double* stage_ms, int* n_calls) {
+    if (!c) return -1;
+    double acc[4] = {0, 0, 0, 0};
+    const int n = c->prof_calls;
+    if (n > 0) XK_TRY(hipEventSynchronize(c->prof_ev[5 * (size_t)(n - 1) + 4]));
+    for (int k = 0; k < n; ++k)
+        for (int s = 0; s < 4; ++s) {
+            float ms = 0.f;
+            XK_TRY(hipEventElapsedTime(&ms, c->prof_ev[5 * (size_t)k + s], c->prof_ev[5 * (size_t)k + s + 1]));
+            acc[s] += ms;
+        }
+    if (stage_ms)
+        for (int s = 0; s < 4; ++s) stage_ms[s] = acc[s];
+    if (n_calls) *n_calls = n;
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+    c->prof_max = 0;
+    c->prof_calls = 0;
+    return 0;
+}
+
+int xk_status(xk_ctx* c, int* flags) {
+    if (!c || !flags) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    XK_TRY(hipMemcpy(flags, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost));
+    XK_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+// Beam-beam contact pairs: one pair per lane.
+//
+// Covers, per pair and in this order (all selectable by mode bits):
+//   narrowphase   clamped segment-segment closest point
+//                 (xkep_cae/contact/geometry/_compute.py:55-151)
+//   frame         parallel transport of t1 along the minimal rotation
+//                 n_old -> n_new, Gram-Schmidt and axis fallbacks (:154-261)
+//   hysteresis    INACTIVE -> ACTIVE at gap <= g_on, ACTIVE/SLIDING ->
+//                 INACTIVE at gap >= g_off (contact/_manager_process.py:475-494)
+//   Huber         p_n = huber(k_pen (-gap), k_pen / smoothing_delta) for
+//                 non-INACTIVE pairs (contact_force/strategy.py:104-170)
+//   friction      Coulomb return mapping for every pair with p_n > 0,
+//                 k_t initialised on first use, z_t <- q, status <- ACTIVE /
+//                 SLIDING (friction/_assembly.py:260-342, law_friction.py:89-153)
+//   tangents      Huber material + geometric, friction material
+//                 sum_ab D_ab t_a t_b^T and geometric M / dist
+//                 (contact_force/strategy.py:172-254, friction/_assembly.py:108-257)
+//
+// Every 3x3 node block (k, l) of a pair's contribution to K is c_k c_l G with
+// one 3x3 G per pair and c = [(1-s), s, -(1-t), -t]; every node force is
+// c_k F with one 3-vector F.  The kernel therefore stages per pair only
+// c[4], F[3] (FP64) and G[9] (FP64 or FP32), structure-of-arrays over the
+// increment's contributing pairs; the gather kernels expand them.
+//
+// This translation unit is compiled with FP contraction off so the
+// branch-deciding quantities (parallel test, clamps, gaps, hysteresis
+// thresholds) are formed with the same roundings as the NumPy reference.
+
+#include "xk_common.h"
+#include "xk_structs.h"
+
+namespace xk {
+
+__device__ __forceinline__ double clip01(double x) { return fmin(fmax(x, 0.0), 1.0); }
+
+__device__ __forceinline__ V3<double> node_x(const double* __restrict__ Xr, const double* __restrict__ u, int k) {
+    V3<double> x{Xr[3 * k], Xr[3 * k + 1], Xr[3 * k + 2]};
+    if (u) x = x + V3<double>{u[6 * (size_t)k], u[6 * (size_t)k + 1], u[6 * (size_t)k + 2]};
+    return x;
+}
+
+__device__ __forceinline__ V3<double> node_du(const double* __restrict__ u, const double* __restrict__ ur, int k) {
+    const size_t o = 6 * (size_t)k;
+    V3<double> a{u[o], u[o + 1], u[o + 2]};
+    if (ur) a = a - V3<double>{ur[o], ur[o + 1], ur[o + 2]};
+    return a;
+}
+
+__device__ __forceinline__ V3<double> min_axis(V3<double> v) {
+    double ax = fabs(v.x), ay = fabs(v.y), az = fabs(v.z);
+    int k = (ax <= ay && ax <= az) ? 0 : ((ay <= az) ? 1 : 2);
+    return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+}
+
+template <class GT>
+__global__ void __launch_bounds__(256)
+pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
+            const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
+            GT* __restrict__ sg, int* __restrict__ n_active) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int active_here = 0;
+    if (i < n_pairs) {
+        const int64_t cap = P.cap;
+        int8_t st = P.status[i];
+        double s, t, gap;
+        V3<double> n, t1;
+        const int a0 = P.na0[i], a1 = P.na1[i], b0 = P.nb0[i], b1 = P.nb1[i];
+        const double ra = P.ra[i], rb = P.rb[i];
+        if (modes & PAIR_GEOM) {
+            V3<double> xA0 = node_x(Xr, u, a0), xA1 = node_x(Xr, u, a1);
+            V3<double> xB0 = node_x(Xr, u, b0), xB1 = node_x(Xr, u, b1);
+            V3<double> dA = xA1 - xA0, dB = xB1 - xB0, w0 = xA0 - xB0;
+            double a = dot(dA, dA), b = dot(dA, dB), c = dot(dB, dB), d = dot(dA, w0), e = dot(dB, w0);
+            double det = a * c - b * b;
+            double ac = a * c;
+            bool par = det < 1e-10 * (ac > 1e-30 ? ac : 1e-30);
+            double s_unc = par ? 0.0 : (b * e - c * d) / det;
+            double t_unc = par ? 0.0 : (a * e - b * d) / det;
+            bool c_ok = c > 1e-10, a_ok = a > 1e-10;
+            double sc = c_ok ? c : 1.0, sa = a_ok ? a : 1.0;
+            s = par ? 0.0 : clip01(s_unc);
+            t = par ? (c_ok ? clip01(e / sc) : 0.0) : clip01(t_unc);
+            bool s_cl = (s_unc < 0.0) || (s_unc > 1.0);
+            bool t_cl = (t_unc < 0.0) || (t_unc > 1.0);
+            if (s_cl && !par) t = c_ok ? clip01((b * s + e) / sc) : 0.0;
+            if (t_cl && !par) {
+                s = a_ok ? clip01((b * t - d) / sa) : 0.0;
+                t = c_ok ? clip01((b * s + e) / sc) : 0.0;
+            }
+            V3<double> diff = (xA0 + s * dA) - (xB0 + t * dB);
+            double dist = sqrt(dot(diff, diff));
+            double sd_ = dist > 1e-30 ? dist : 1e-30;
+            V3<double> nrm{diff.x / sd_, diff.y / sd_, diff.z / sd_};
+            if (dist < 1e-30) nrm = V3<double>{0.0, 0.0, 1.0};
+            gap = dist - (ra + rb);
+
+            // frame (n, t1); t2 = n x t1 is never stored
+            double nn_ = sqrt(dot(nrm, nrm));
+            nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
+            n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
+            V3<double> pt1{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
+            V3<double> pn{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
+            bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
+            bool has_n = sqrt(dot(pn, pn)) > 1e-10;
+            bool done = false;
+            if (has_t && has_n) {
+                double on = sqrt(dot(pn, pn));
+                on = on > 1e-30 ? on : 1e-30;
+                V3<double> no{pn.x / on, pn.y / on, pn.z / on};
+                V3<double> v = cross(no, n);
+                double sv = sqrt(dot(v, v));
+                double cv = dot(no, n);
+                V3<double> tt = pt1;
+                if (sv >= 1e-12) {
+                    V3<double> vxt = cross(v, pt1);
+                    V3<double> vxvxt = cross(v, vxt);
+                    double fac = (1.0 - cv) / (sv * sv);
+                    tt = (pt1 + vxt) + fac * vxvxt;
+                }
+                V3<double> raw = tt - dot(tt, n) * n;
+                double rn = sqrt(dot(raw, raw));
+                if (rn > 1e-10) {
+                    t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+                    done = true;
+                }
+            }
+            if (!done && has_t) {
+                V3<double> raw = pt1 - dot(pt1, n) * n;
+                double rn = sqrt(dot(raw, raw));
+                if (rn > 1e-10) {
+                    t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+                    done = true;
+                }
+            }
+            if (!done) {
+                V3<double> ref = min_axis(n);
+                V3<double> raw = ref - dot(ref, n) * n;
+                double rn = sqrt(dot(raw, raw));
+                rn = rn > 1e-30 ? rn : 1e-30;
+                t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+            }
+            P.s[i] = s; P.t[i] = t; P.gap[i] = gap;
+            P.n[i] = n.x; P.n[cap + i] = n.y; P.n[2 * cap + i] = n.z;
+            P.t1[i] = t1.x; P.t1[cap + i] = t1.y; P.t1[2 * cap + i] = t1.z;
+            if (modes & PAIR_HYST) {
+                int8_t ns = st;
+                if (st == ST_INACTIVE) {
+                    if (gap <= pp.g_on) ns = ST_ACTIVE;
+                } else if (pp.allow_deact && gap >= pp.g_off) {
+                    ns = ST_INACTIVE;
+                }
+                if (ns != st) { P.status[i] = ns; st = ns; }
+            }
+        } else {
+            s = P.s[i]; t = P.t[i]; gap = P.gap[i];
+            n = V3<double>{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
+            t1 = V3<double>{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
+        }
+
+        const int si = P.sinc[i];
+        if (si >= 0 && (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F))) {
+            const int ns = pp.n_sinc;
+            const V3<double> t2 = cross(n, t1);
+            double p_n = P.p_n[i];
+            V3<double> F{0.0, 0.0, 0.0};
+            const double x_pen = pp.k_pen * (-gap);
+            const double dh = pp.delta_h;
+            if ((modes & PAIR_HUBER) && st != ST_INACTIVE) {
+                if (dh <= 0.0) p_n = x_pen > 0.0 ? x_pen : 0.0;
+                else if (x_pen < -dh) p_n = 0.0;
+                else if (x_pen > dh) p_n = x_pen;
+                else p_n = ((x_pen + dh) * (x_pen + dh)) / (4.0 * dh);
+                P.p_n[i] = p_n;
+                if (p_n > 1e-30) F = p_n * n;
+            }
+            if ((modes & PAIR_FRIC) && p_n > 0.0 && pp.mu_eff > 0.0) {
+                double kp = P.k_pen[i], kt = P.k_t[i];
+                if (kp <= 0.0) {
+                    kp = pp.k_pen;
+                    kt = pp.k_pen * pp.k_t_ratio;
+                    P.k_pen[i] = kp;
+                    P.k_t[i] = kt;
+                }
+                V3<double> duA = (1.0 - s) * node_du(u, u_ref, a0) + s * node_du(u, u_ref, a1);
+                V3<double> duB = (1.0 - t) * node_du(u, u_ref, b0) + t * node_du(u, u_ref, b1);
+                V3<double> dur = duB - duA;
+                double du1 = dot(dur, t1), du2 = dot(dur, t2);
+                double q1 = P.z[i] + kt * du1;
+                double q2 = P.z[cap + i] + kt * du2;
+                double qtn = sqrt(q1 * q1 + q2 * q2);
+                double fy = pp.mu_eff * p_n;
+                bool stick = qtn <= fy;
+                if (!stick) {
+                    q1 = (fy * q1) / qtn;
+                    q2 = (fy * q2) / qtn;
+                }
+                P.z[i] = q1; P.z[cap + i] = q2;
+                P.stick[i] = stick ? 1 : 0;
+                P.qtn[i] = qtn;
+                P.diss[i] = q1 * du1 + q2 * du2;
+                st = stick ? ST_ACTIVE : ST_SLIDING;
+                P.status[i] = st;
+                if (sqrt(q1 * q1 + q2 * q2) >= 1e-30) {
+                    double f1 = fabs(q1) < 1e-30 ? 0.0 : q1;
+                    double f2 = fabs(q2) < 1e-30 ? 0.0 : q2;
+                    F = F + (f1 * t1 + f2 * t2);
+                }
+            }
+            if (modes & (PAIR_HUBER | PAIR_FRIC)) {
+                sd[0 * ns + si] = 1.0 - s;
+                sd[1 * ns + si] = s;
+                sd[2 * ns + si] = -(1.0 - t);
+                sd[3 * ns + si] = -t;
+                sd[4 * ns + si] = F.x;
+                sd[5 * ns + si] = F.y;
+                sd[6 * ns + si] = F.z;
+            }
+            if (modes & (PAIR_TAN_C | PAIR_TAN_F)) {
+                double G[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) G[k] = 0.0;
+                const double nv[3] = {n.x, n.y, n.z};
+                const double dist = (gap + ra) + rb;
+                if ((modes & PAIR_TAN_C) && st != ST_INACTIVE) {
+                    double hd;
+                    if (dh <= 0.0) hd = x_pen > 0.0 ? 1.0 : 0.0;
+                    else if (x_pen < -dh) hd = 0.0;
+                    else if (x_pen > dh) hd = 1.0;
+                    else hd = (x_pen + dh) / (2.0 * dh);
+                    if (!(hd < 1e-30 && p_n < 1e-30)) {
+                        double wm = hd * pp.k_pen;
+                        double wg = dist > 1e-15 ? p_n / dist : 0.0;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) {
+                                double nab = nv[a] * nv[b];
+                                G[3 * a + b] = wm * nab - wg * ((a == b ? 1.0 : 0.0) - nab);
+                            }
+                    }
+                }
+                if ((modes & PAIR_TAN_F) && st != ST_INACTIVE && p_n > 0.0 && pp.mu_eff > 0.0) {
+                    double q1 = P.z[i], q2 = P.z[cap + i];
+                    double zn = sqrt(q1 * q1 + q2 * q2);
+                    if (zn >= 1e-30) {
+                        const double sgn = (modes & PAIR_TAN_C) ? -1.0 : 1.0;
+                        double kt = P.k_t[i], qtn = P.qtn[i];
+                        double D00, D01, D10, D11;
+                        if (P.stick[i] || zn < 1e-30 || qtn < 1e-30) {
+                            D00 = kt; D01 = 0.0; D10 = 0.0; D11 = kt;
+                        } else {
+                            double h1 = q1 / zn, h2 = q2 / zn;
+                            double rk = ((pp.mu_eff * p_n) / qtn) * kt;
+                            D00 = rk * (1.0 - h1 * h1); D01 = rk * (0.0 - h1 * h2);
+                            D10 = rk * (0.0 - h2 * h1); D11 = rk * (1.0 - h2 * h2);
+                        }
+                        const double tv[2][3] = {{t1.x, t1.y, t1.z}, {t2.x, t2.y, t2.z}};
+                        const double Dm[2][2] = {{D00, D01}, {D10, D11}};
+                        double H[9];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) H[k] = 0.0;
+#pragma unroll
+                        for (int al = 0; al < 2; ++al)
+#pragma unroll
+                            for (int be = 0; be < 2; ++be) {
+                                double dv = Dm[al][be];
+                                if (fabs(dv) < 1e-30) continue;
+#pragma unroll
+                                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                                    for (int b = 0; b < 3; ++b) H[3 * a + b] += dv * tv[al][a] * tv[be][b];
+                            }
+                        if (!(fabs(q1) < 1e-30 && fabs(q2) < 1e-30) && dist >= 1e-15) {
+                            const double id = 1.0 / dist;
+                            const double t2v[3] = {t2.x, t2.y, t2.z};
+                            const double t1v[3] = {t1.x, t1.y, t1.z};
+                            double M[9];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                                for (int b = 0; b < 3; ++b)
+                                    M[3 * a + b] = -q1 * nv[a] * t1v[b] - q2 * t2v[a] * nv[b];
+                            M[1] += q2 * t1.z;  M[2] += q2 * (-t1.y);
+                            M[3] += q2 * (-t1.z); M[5] += q2 * t1.x;
+                            M[6] += q2 * t1.y;  M[7] += q2 * (-t1.x);
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) H[k] += id * M[k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) G[k] += sgn * H[k];
+                    }
+                }
+                sd[0 * ns + si] = 1.0 - s;
+                sd[1 * ns + si] = s;
+                sd[2 * ns + si] = -(1.0 - t);
+                sd[3 * ns + si] = -t;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) sg[(size_t)k * ns + si] = GT(G[k]);
+            }
+            active_here = p_n > 0.0 ? 1 : 0;
+        }
+    }
+    if (n_active) {
+        // wavefront reduction, one atomic per wave (integer: order-independent)
+        unsigned long long m = __ballot(active_here);
+        if ((threadIdx.x & 63) == 0) {
+            int c = __popcll(m);
+            if (c) atomicAdd(n_active, c);
+        }
+    }
+}
+
+template __global__ void pair_kernel<double>(int64_t, PairSoA, const double*, const double*,
+                                             const double*, int, PairParams, double*, double*, int*);
+template __global__ void pair_kernel<float>(int64_t, PairSoA, const double*, const double*,
+                                            const double*, int, PairParams, double*, float*, int*);
+
+hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
+                        const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
+                        int* n_active, hipStream_t st) {
+    if (n_pairs <= 0) return hipSuccess;
+    dim3 blk(256), grd((unsigned)((n_pairs + 255) / 256));
+    if (prec == 0)
+        hipLaunchKernelGGL(pair_kernel<double>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
+                           (double*)sg, n_active);
+    else
+        hipLaunchKernelGGL(pair_kernel<float>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
+                           (float*)sg, n_active);
+    return hipGetLastError();
+}
+
+}  // namespace xk

@@ -1,0 +1,105 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "xk_common.h"
+
+namespace xk {
+
+void set_error(const std::string& s);
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t nbytes);
+    void release();
+};
+
+struct PairSoA;
+struct PairParams;
+struct NodeLists;
+struct BlockLists;
+struct DynArgs;
+
+}  // namespace xk
+
+// complete definitions needed by Ctx
+#include "xk_structs.h"
+
+namespace xk {
+
+enum PairBuf {
+    PB_NA0, PB_NA1, PB_NB0, PB_NB1, PB_RA, PB_RB, PB_S, PB_T, PB_GAP, PB_N, PB_T1,
+    PB_PN, PB_KPEN, PB_KT, PB_QTN, PB_DISS, PB_Z, PB_STATUS, PB_STICK, PB_SINC, PB_EA, PB_EB,
+    PB_COUNT
+};
+
+struct Ctx {
+    int n_nodes = 0, n_elems = 0, prec = 0;
+    std::vector<double> X_h, rad_h;
+    std::vector<int> conn_h;
+    BeamConst bc{};
+    DBuf d_X, d_conn, d_rad, d_node_bptr, d_node_bsrc, d_fe, d_Ke, d_flags, d_partial;
+    // pairs
+    std::vector<DBuf> pb;
+    PairSoA P{};
+    int* ea = nullptr;
+    int* eb = nullptr;
+    int64_t n_pairs = 0, cap = 0;
+    int last_modes = 0;
+    // tangent pattern of the current increment
+    bool pattern_valid = false;
+    int n_sinc = 0, n_blocks = 0;
+    std::vector<int> gptr_h, gcol_h;
+    DBuf d_gptr, d_gcol, d_brow, d_blk_bptr, d_blk_bsrc, d_blk_pptr, d_blk_psrc, d_node_pptr, d_node_psrc;
+    DBuf d_sd, d_sg;
+    // broadphase scratch and candidate bookkeeping
+    DBuf bp[16];
+    int64_t last_cand = 0;
+    int64_t raw_n = 0, raw_cap = 1;
+    double last_cell = 0.0;
+    double cell_factor = 1.5;       // grid cell = factor x mean box extent
+    bool filtered_valid = false;    // filtered list of the last xk_prescreen_radii still held
+    bool keys_valid = false;        // bp[14] holds the sorted keys of the pair list
+    const int* cand_I = nullptr;
+    const int* cand_J = nullptr;
+    // optional per-stage event timing of the fused assembly
+    std::vector<hipEvent_t> prof_ev;
+    int prof_max = 0, prof_calls = 0;
+
+    int grow_pairs(int64_t need);
+    void bind_pairs();
+    int build_pattern();
+    int ensure_pattern();
+    BlockLists block_lists() const;
+    NodeLists node_lists() const;
+    void release_all() {
+        DBuf* all[] = {&d_X, &d_conn, &d_rad, &d_node_bptr, &d_node_bsrc, &d_fe, &d_Ke, &d_flags, &d_partial,
+                       &d_gptr, &d_gcol, &d_brow, &d_blk_bptr, &d_blk_bsrc, &d_blk_pptr, &d_blk_psrc,
+                       &d_node_pptr, &d_node_psrc, &d_sd, &d_sg};
+        for (DBuf* b : all) b->release();
+        for (auto& b : pb) b.release();
+        for (auto& b : bp) b.release();
+    }
+};
+
+// launchers (one per translation unit)
+hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, const int* conn,
+                       const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags, hipStream_t st);
+hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
+                        const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
+                        int* n_active, hipStream_t st);
+hipError_t launch_node_gather(int n_nodes, int n_elems, int n_sinc, const NodeLists& L, const double* fe,
+                              const double* sd, int mode, const double* u, const double* f_ext,
+                              const uint8_t* fixed, const DynArgs& dy, double* out, double* partial,
+                              int* flags, hipStream_t st);
+int node_gather_blocks(int n_nodes);
+hipError_t launch_sum_partials(int n, const double* partial, double* out, hipStream_t st);
+hipError_t launch_block_gather(int prec, int n_blocks, int n_elems, int n_sinc, const BlockLists& B,
+                               const void* Ke, const double* sd, const void* sg, int use_beam, int use_pair,
+                               double mass_coef, const double* M, void* K, hipStream_t st);
+hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st);
+
+}  // namespace xk
