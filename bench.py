@@ -5,11 +5,14 @@ Workload: the 91-wire strand (layers 1+6+12+18+24+30, r = 0.4 mm, L = 80 mm,
 dynamic step, Huber contact k = 1e4 E r, Coulomb mu = 0.3, seeded
 perturbation u ~ N(0, (1e-3 r)^2).  One step = one complete fused assembly
 (narrowphase over every candidate pair, Huber, friction, element f_int and K,
-R and K composition with inertia terms, ||R||^2) replayed from a HIP graph,
-inputs resident in HBM.  Metric: (N_elem + N_pair) / t_step in the FP32-mixed
-form; the FP64 plain form is timed beside it.
+R and K composition with inertia terms, ||R||^2, #(p_n > 0)) replayed from a
+HIP graph, inputs resident in HBM.
 
-Prints ONE JSON line (rank 0, one GPU).
+value: the FP32-mixed form over a batch of B independent 91-wire instances
+(different seeds) assembled by the same launches, (sum N_elem + N_pair) per
+second.  Beside it: the single-instance FP64 plain and FP32-mixed forms.
+
+Prints ONE JSON line (one GPU).
 """
 
 from __future__ import annotations
@@ -18,6 +21,7 @@ import argparse
 import importlib
 import json
 import os
+import platform
 import sys
 import time
 
@@ -33,24 +37,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(case, st, precision, torch):
+def setup(case, states, precision, torch):
+    """Context + assembly for one instance or a concatenated batch of instances."""
+    wl = importlib.import_module("xkep-cae_amd.workloads")
     C = importlib.import_module("xkep-cae_amd.context")
     A = importlib.import_module("xkep-cae_amd.assembly")
     T = importlib.import_module("xkep-cae_amd.time_integration")
     E = importlib.import_module("xkep-cae_amd.elements")
-    m = st.mesh
+    if len(states) == 1:
+        st = states[0]
+        X, conn, radii, u, f_ext, fixed = (st.mesh.node_coords, st.mesh.connectivity, st.mesh.radii, st.u,
+                                           st.f_ext, st.fixed_dofs)
+    else:
+        b = wl.concat_states(states)
+        X, conn, radii, u, f_ext, fixed = b.node_coords, b.connectivity, b.radii, b.u, b.f_ext, b.fixed_dofs
     sec = C.circle_section(case.wire_radius, case.E, case.nu)
-    ctx = C.DeviceContext(m.node_coords, m.connectivity, sec, radii=m.radii, precision=precision)
-    Md = E.ULCRBeamAssembler(m.node_coords, m.connectivity, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J,
-                             sec.kappa_y, sec.kappa_z).lumped_mass_diag(case.rho)
+    ctx = C.DeviceContext(X, conn, sec, radii=radii, precision=precision)
+    Md = E.ULCRBeamAssembler(X, conn, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
+                             sec.kappa_z).lumped_mass_diag(case.rho)
     dt = case.dt()
     ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
     ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
     asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
-                           f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, time=ga, dt=dt)
-    t0 = time.perf_counter()
+                           f_ext=f_ext, fixed_dofs=fixed, time=ga, dt=dt)
     asm.begin_increment(np.zeros(ctx.ndof))   # set-up broadphase on the reference configuration
-    asm.begin_increment(st.u)                  # increment: broadphase + hysteresis + pattern
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    asm.begin_increment(u)                     # increment: broadphase + hysteresis + pattern
     torch.cuda.synchronize()
     t_inc = time.perf_counter() - t0
     return ctx, asm, t_inc
@@ -81,16 +94,20 @@ def profile_stages(asm, ctx, torch, calls):
     return ms / max(n, 1)
 
 
+STAGES = ["pairs", "elements", "gather"]
+
+
 def stage_bytes(ctx, asm, n_pair, precision):
-    """Algorithmic (compulsory) bytes per launch of each stage."""
+    """Algorithmic (compulsory) bytes per launch of each stage: pairs N_pair x 216;
+    elements X + u per node and conn per element; gather u, f_ext, R per node
+    plus the K values."""
     nn, ne = ctx.n_nodes, ctx.n_elems
     s_k = 8 if precision == "fp64" else 4
     nnz = asm.K.numel()
     return {
         "pairs": n_pair * 216,
         "elements": nn * (24 + 48) + ne * 8,
-        "residual": nn * (48 + 48 + 48),
-        "tangent": nnz * s_k,
+        "gather": nn * (48 + 48 + 48) + nnz * s_k,
     }
 
 
@@ -119,7 +136,7 @@ def cpu_baseline(case, st, budget_s=15.0):
         t_start = time.perf_counter()
         while True:
             t0 = time.perf_counter()
-            res = oasm.assemble_iteration(prob, op, st.u, st.u_ref, st.f_ext, st.fixed_dofs, ga, dt)
+            oasm.assemble_iteration(prob, op, st.u, st.u_ref, st.f_ext, st.fixed_dofs, ga, dt)
             times.append(time.perf_counter() - t0)
             if time.perf_counter() - t_start > budget_s and len(times) >= 3:
                 break
@@ -127,17 +144,35 @@ def cpu_baseline(case, st, budget_s=15.0):
     return (m.n_elems + len(op)) / t, len(times), t
 
 
+def run(label, case, states, prec, torch, steps, warmup):
+    log(f"[bench] {label}: setting up {len(states)} x {case.name} precision={prec}")
+    ctx, asm, t_inc = setup(case, states, prec, torch)
+    n_pair = ctx.n_pairs
+    asm.assemble()
+    torch.cuda.synchronize()
+    stages = profile_stages(asm, ctx, torch, 20)
+    ms, ms_wall = time_graph(asm, torch, steps, warmup)
+    n_eval = ctx.n_elems + n_pair
+    log(f"[bench] {label}: elems={ctx.n_elems} pairs={n_pair} nnzK={asm.K.numel()} incr {t_inc * 1e3:.1f} ms | "
+        f"{ms * 1e3:.1f} us/step (graph) wall {ms_wall * 1e3:.1f} | stages us {np.round(stages * 1e3, 1).tolist()} | "
+        f"{n_eval / (ms * 1e-3):.3e} evals/s")
+    out = dict(ctx=ctx, asm=asm, n_pair=n_pair, n_eval=n_eval, ms=ms, ms_wall=ms_wall, stages=stages,
+               t_inc=t_inc, value=n_eval / (ms * 1e-3))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=16, help="91-wire instances per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding by wire ownership is a design, not an implementation).", file=sys.stderr)
+              "(sharding by wire ownership is designed, not implemented).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -145,40 +180,28 @@ def main():
     wl = importlib.import_module("xkep-cae_amd.workloads")
     case = wl.C3
     st = wl.make_state(case)
-    results = {}
-    for prec in ("fp64", "mixed"):
-        log(f"[bench] setting up {case.name} precision={prec}")
-        ctx, asm, t_inc = setup(case, st, prec, torch)
-        n_pair = ctx.n_pairs
-        log(f"[bench] {prec}: nodes={ctx.n_nodes} elems={ctx.n_elems} pairs={n_pair} "
-            f"nnzK={asm.K.numel()} increment setup {t_inc * 1e3:.1f} ms")
-        asm.assemble()
-        torch.cuda.synchronize()
-        stages = profile_stages(asm, ctx, torch, 20)
-        ms_graph, ms_wall = time_graph(asm, torch, args.steps, args.warmup)
-        log(f"[bench] {prec}: {ms_graph * 1e3:.1f} us/step (graph, events), wall {ms_wall * 1e3:.1f} us/step, "
-            f"stages us {np.round(stages * 1e3, 1).tolist()}")
-        results[prec] = dict(ctx=ctx, asm=asm, n_pair=n_pair, ms=ms_graph, ms_wall=ms_wall, stages=stages,
-                             t_inc=t_inc)
-    mixed = results["mixed"]
-    plain = results["fp64"]
-    n_eval = mixed["ctx"].n_elems + mixed["n_pair"]
-    value = n_eval / (mixed["ms"] * 1e-3)
-    # roofline of the dominant stage of the mixed step
-    names = ["pairs", "elements", "residual", "tangent"]
-    sb = stage_bytes(mixed["ctx"], mixed["asm"], mixed["n_pair"], "mixed")
-    dom = int(np.argmax(mixed["stages"]))
-    dom_ms = float(mixed["stages"][dom])
-    achieved = sb[names[dom]] / (dom_ms * 1e-3) / 1e9
+    plain = run("fp64 x1", case, [st], "fp64", torch, args.steps, args.warmup)
+    single = run("mixed x1", case, [st], "mixed", torch, args.steps, args.warmup)
+    del plain["asm"], plain["ctx"], single["asm"], single["ctx"]
+    torch.cuda.empty_cache()
+    states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
+    batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup)
+
+    sb = stage_bytes(batched["ctx"], batched["asm"], batched["n_pair"], "mixed")
+    dom = int(np.argmax(batched["stages"]))
+    dom_ms = float(batched["stages"][dom])
+    achieved = sb[STAGES[dom]] / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": names[dom],
-            "kernel_bytes": sb[names[dom]], "kernel_us": round(dom_ms * 1e3, 2)}
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": STAGES[dom],
+            "kernel_bytes": int(sb[STAGES[dom]]), "kernel_us": round(dom_ms * 1e3, 2)}
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (FP64 NumPy oracle, 1 thread)")
         cv, n_runs, t_one = cpu_baseline(case, st, args.cpu_budget)
         cpu = {"value": round(cv, 1), "unit": "evals/s", "cores": 1, "kind": "port",
-               "sample": f"{n_runs} full FP64 assemblies of the 91-wire step ({t_one:.2f} s each, median)"}
+               "sample": f"{n_runs} full FP64 assemblies of one 91-wire step ({t_one:.3f} s each, median), "
+                         f"host {platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
+    value = batched["value"]
     out = {
         "metric": "assembled elem+contact-pair evals/s, FP32-mixed vs FP64 plain, 91 wires",
         "value": round(value, 1),
@@ -186,25 +209,30 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms"], 5),
+        "ms_per_step": round(batched["ms"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (fp64 kinematics/forces/residual, fp32 tangent)",
+        "dtype": "fp32-mixed (fp64 kinematics, forces, residual, norm; fp32 tangent)",
         "data": "synthetic",
-        "config": {"workload": "91-wire strand dynamic Newton assembly (C3)", "n_elem": mixed["ctx"].n_elems,
-                   "n_pair": mixed["n_pair"], "n_nodes": mixed["ctx"].n_nodes, "nnz_K": int(mixed["asm"].K.numel()),
-                   "graph": True},
-        "fp64_plain": {"value": round(n_eval / (plain["ms"] * 1e-3), 1), "ms_per_step": round(plain["ms"], 5),
-                       "stages_us": np.round(plain["stages"] * 1e3, 2).tolist()},
-        "mixed_stages_us": np.round(mixed["stages"] * 1e3, 2).tolist(),
-        "increment_setup_ms": round(mixed["t_inc"] * 1e3, 1),
+        "config": {"workload": f"91-wire strand dynamic Newton assembly, batch of {args.batch} instances",
+                   "instances": args.batch, "n_elem": batched["ctx"].n_elems, "n_pair": batched["n_pair"],
+                   "n_nodes": batched["ctx"].n_nodes, "nnz_K": int(batched["asm"].K.numel()), "graph": True},
+        "single_instance": {
+            "fp64_plain": {"value": round(plain["value"], 1), "us_per_step": round(plain["ms"] * 1e3, 2),
+                           "stages_us": np.round(plain["stages"] * 1e3, 2).tolist(),
+                           "n_elem": 13650, "n_pair": plain["n_pair"]},
+            "fp32_mixed": {"value": round(single["value"], 1), "us_per_step": round(single["ms"] * 1e3, 2),
+                           "stages_us": np.round(single["stages"] * 1e3, 2).tolist()},
+        },
+        "batched_stages_us": np.round(batched["stages"] * 1e3, 2).tolist(),
+        "increment_setup_ms": round(batched["t_inc"] * 1e3, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "speedup_vs_fp64_gpu": round(value / plain["value"], 3),
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
-    out["speedup_vs_fp64_gpu"] = round(out["value"] / out["fp64_plain"]["value"], 3)
     print(json.dumps(out), flush=True)
 
 

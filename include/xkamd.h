@@ -177,11 +177,13 @@ int xk_status(xk_ctx* ctx, int* flags);
 
 /* Per-stage timing of xk_assemble_iteration with HIP events recorded on the
  * call's stream: begin allocates events for up to max_calls calls; end
- * synchronises and returns the summed milliseconds of the four stages
- * [pairs, elements, residual gather + norm, tangent gather] and the number of
- * calls recorded.  Not for use under graph capture. */
+ * synchronises and returns the summed milliseconds of the three stages
+ * [pair kernel, element kernel, gather (residual + tangent + norm)] and the
+ * number of calls recorded.  Profiled calls run the three stages in sequence
+ * (unprofiled calls overlap the element and pair kernels on two streams).
+ * Not for use under graph capture. */
 int xk_profile_begin(xk_ctx* ctx, int max_calls);
-int xk_profile_end(xk_ctx* ctx, double* stage_ms4, int* n_calls);
+int xk_profile_end(xk_ctx* ctx, double* stage_ms3, int* n_calls);
 
 #ifdef __cplusplus
 }

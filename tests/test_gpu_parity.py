@@ -177,7 +177,11 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu):
     fr = K.CoulombReturnMappingProcess(ndof, k_pen=case.k_pen, k_t_ratio=0.5)
     ff, fres = fr.evaluate(st.u, mgr.pairs, case.mu, u_ref=st.u_ref)
     ofr = ocon.friction_evaluate(op, st.u, st.u_ref, case.k_pen, 0.5, case.mu, ndof)
-    assert rel(ff, ofr.f_fric) < FP64_TOL and rel(fres, ofr.residual) < FP64_TOL
+    assert rel(ff, ofr.f_fric) < FP64_TOL
+    # residual entries max(0, |q| - mu p_n) are rounding-level for slipping pairs:
+    # compare on the scale of the friction bound mu p_n
+    assert fres.shape == ofr.residual.shape
+    assert np.abs(fres - ofr.residual).max() <= 1e-12 * case.mu * ofr.pairs.p_n.max()
     assert len(fr.friction_tangents) == int(ofr.mask.sum())
     Kc = hub.tangent(st.u, mgr, case.k_pen)
     assert srel(Kc, ocon.huber_tangent(ofr.pairs, case.k_pen, case.smoothing_delta, ndof)) < FP64_TOL
@@ -230,6 +234,41 @@ def test_mixed_precision_residual_and_active_set_at_91_wires(gpu):
     assert rel(Rm, R64) < MIXED_R_TOL
     assert srel(Km, K64) < MIXED_K_TOL
     assert nm == n64 and np.array_equal(sm, s64)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_instances_match_oracle_and_never_pair_across(gpu, precision):
+    """Three 7-wire instances of different seeds concatenated into one context."""
+    wl = pkg("workloads")
+    C, A = pkg("context"), pkg("assembly")
+    case = wl.C1
+    b = wl.concat_states([wl.make_state(case, seed=s) for s in (7, 8, 9)])
+    ctx = C.DeviceContext(b.node_coords, b.connectivity, section(case.wire_radius), radii=b.radii,
+                          precision=precision)
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                           f_ext=b.f_ext, fixed_dofs=b.fixed_dofs)
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(b.u)
+    R, Kv, res2, nact = asm.assemble(b.u)
+    d = ctx.get_pairs()
+    inst = np.searchsorted([e1 for _, e1 in b.elem_ranges], d["elem_a"], side="right")
+    inst_b = np.searchsorted([e1 for _, e1 in b.elem_ranges], d["elem_b"], side="right")
+    assert np.array_equal(inst, inst_b) and len(set(inst.tolist())) == 3
+    osec = obeam.circle_section(case.wire_radius, 200e3, 0.3)
+    cfg = ocon.ContactConfig(mu=case.mu)
+    prob = oasm.Problem(b.node_coords, b.connectivity, osec, case.k_pen, case.smoothing_delta, 0.5, case.mu, cfg)
+    op, _ = ocon.detect_candidates(ocon.Pairs(), b.node_coords, b.connectivity, b.radii, cfg)
+    x = b.node_coords + b.u.reshape(-1, 6)[:, :3]
+    op, _ = ocon.detect_candidates(op, x, b.connectivity, b.radii, cfg)
+    op = ocon.update_geometry(op, x, cfg)
+    ref = oasm.assemble_iteration(prob, op, b.u, b.u_ref, b.f_ext, b.fixed_dofs, None, 0.0)
+    Rh = R.cpu().numpy()
+    for n0, n1 in b.node_ranges:
+        sl = slice(6 * n0, 6 * n1)
+        assert rel(Rh[sl], ref.R[sl]) < (FP64_TOL if precision == "fp64" else MIXED_R_TOL)
+    assert srel(asm.K_csr(), ref.K) < (FP64_TOL if precision == "fp64" else MIXED_K_TOL)
+    assert int(nact.item()) == ref.n_active
+    assert np.array_equal(ctx.get_pairs()["status"], ref.pairs.status)
 
 
 def test_assembly_is_bitwise_deterministic_and_graph_replay_matches(gpu):

@@ -294,8 +294,8 @@ class DeviceContext:
         L.check(L.lib().xk_profile_begin(self.h, int(max_calls)), "xk_profile_begin")
 
     def profile_end(self):
-        """(stage_ms[4] summed over calls, n_calls): pairs, elements, residual, tangent."""
-        ms = np.zeros(4)
+        """(stage_ms[3] summed over calls, n_calls): pair kernel, element kernel, gather."""
+        ms = np.zeros(3)
         n = C.c_int()
         L.check(L.lib().xk_profile_end(self.h, ms.ctypes.data, C.byref(n)), "xk_profile_end")
         return ms, n.value

@@ -1,153 +1,169 @@
 // Deterministic gathers of staged element / pair contributions into the
 // global residual and the block-CSR tangent, plus the Generalized-alpha
-// effective terms and the FP64 residual norm.
+// effective terms, the FP64 residual norm and the active-pair count -- one
+// launch.
 //
 // Every output entry is summed by one lane over a contribution list fixed at
 // pattern-build time (elements in element order, then pairs in pair order), so
 // results are bitwise reproducible run to run (no floating-point atomics).
+// Lanes [0, 6 n_nodes) own one residual DOF each; the following lanes own one
+// row of one 3x3 tangent block each.  The residual partial sums and the pair
+// kernel's per-workgroup active counts are reduced by a one-workgroup finish
+// kernel (no atomics, no memset, capturable in a HIP graph).
 //
-// Residual (per node, 6 DOFs) follows ContactForceAssemblyProcess
+// Residual follows ContactForceAssemblyProcess
 // (xkep_cae/contact/solver/_newton_steps.py:124-129) and the dynamic steps
 // of _newton_dynamic.py:189-192 with GeneralizedAlphaProcess.correct /
 // effective_residual (time_integration/strategy.py:178-216):
 //   R = f_int - (f_c + f_fric) - f_ext ; R[fixed] = 0
 //   acc = c0 (u - u_pred) ; vel = v_pred + dt gamma acc
 //   R += M ((1 - alpha_m) acc + alpha_m acc_old) ; R[fixed] = 0
-// Tangent (per 3x3 block) follows TangentAssemblyProcess (:289-313) and
-// effective_stiffness (time_integration/strategy.py:190-203):
+// Tangent follows TangentAssemblyProcess (:289-313) and effective_stiffness
+// (time_integration/strategy.py:190-203):
 //   K = K_beam + K_c - K_fric + (1 - alpha_m) c0 M.
 
 #include "xk_common.h"
 #include "xk_structs.h"
+#include "xk_internal.h"
 #include <type_traits>
 
 namespace xk {
 
-// mode: 0 = f_int only, 1 = pair force only, 2 = full residual
-__global__ void __launch_bounds__(256)
-node_gather_kernel(int n_nodes, int n_elems, int n_sinc, NodeLists L, const double* __restrict__ fe,
-                   const double* __restrict__ sd, int mode, const double* __restrict__ u,
-                   const double* __restrict__ f_ext, const uint8_t* __restrict__ fixed, DynArgs dy,
-                   double* __restrict__ out, double* __restrict__ partial, int* __restrict__ flags) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int GB = 256;  // lanes per workgroup
+
+// node_mode: -1 no residual work, 0 = f_int only, 1 = pair force only, 2 = full residual
+template <class KT, class ST, class GT>
+__global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __restrict__ Ke,
+                                                     const GT* __restrict__ sg, KT* __restrict__ K) {
+    __shared__ double red[GB];
+    const int ne = g.n_elems, ns = g.n_sinc;
     double ss = 0.0;
-    if (i < n_nodes) {
-        double fb[6] = {0, 0, 0, 0, 0, 0};
-        double fp[3] = {0, 0, 0};
-        if (mode != 1) {
-            for (int q = L.bptr[i]; q < L.bptr[i + 1]; ++q) {
-                int code = L.bsrc[q];
-                int e = code >> 1, kn = code & 1;
-#pragma unroll
-                for (int d = 0; d < 6; ++d) fb[d] += fe[(size_t)(6 * kn + d) * n_elems + e];
-            }
-        }
-        if (mode != 0 && L.pptr) {
-            for (int q = L.pptr[i]; q < L.pptr[i + 1]; ++q) {
-                int code = L.psrc[q];
-                int p = code >> 2, k = code & 3;
-                double ck = sd[(size_t)k * n_sinc + p];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) fp[d] += ck * sd[(size_t)(4 + d) * n_sinc + p];
-            }
-        }
-        const size_t o = 6 * (size_t)i;
-        if (mode == 0) {
-#pragma unroll
-            for (int d = 0; d < 6; ++d) out[o + d] = fb[d];
-        } else if (mode == 1) {
-#pragma unroll
-            for (int d = 0; d < 6; ++d) out[o + d] = d < 3 ? fp[d] : 0.0;
-        } else {
-            int bad = 0;
-#pragma unroll
-            for (int d = 0; d < 6; ++d) {
-                double r = (fb[d] + (d < 3 ? -fp[d] : -0.0)) - f_ext[o + d];
-                if (fixed[o + d]) r = 0.0;
-                if (dy.on) {
-                    double ac = dy.c0 * (u[o + d] - dy.u_pred[o + d]);
-                    dy.acc[o + d] = ac;
-                    dy.vel[o + d] = dy.v_pred[o + d] + dy.dt * dy.gamma * ac;
-                    double amid = (1.0 - dy.alpha_m) * ac + dy.alpha_m * dy.acc_old[o + d];
-                    r = r + dy.M[o + d] * amid;
-                    if (fixed[o + d]) r = 0.0;
+    const bool node_block = (int)blockIdx.x < g.node_blocks;
+    if (node_block) {
+        const int tid = blockIdx.x * GB + threadIdx.x;
+        if (tid < 6 * g.n_nodes) {
+            const int i = tid / 6, d = tid - 6 * (tid / 6);
+            double fb = 0.0, fp = 0.0;
+            if (g.node_mode != 1) {
+                const int q0 = g.L.bptr[i], q1 = g.L.bptr[i + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int code = g.L.bsrc[q];
+                    fb += g.fe[(size_t)(6 * (code & 1) + d) * ne + (code >> 1)];
                 }
-                out[o + d] = r;
-                ss += r * r;
-                bad |= !isfinite(r);
             }
-            if (bad) atomicOr(flags, FLAG_NONFINITE);
+            if (g.node_mode != 0 && d < 3 && g.L.pptr) {
+                const int q0 = g.L.pptr[i], q1 = g.L.pptr[i + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int code = g.L.psrc[q];
+                    const int p = code >> 2;
+                    fp += g.sd[(size_t)(code & 3) * ns + p] * g.sd[(size_t)(4 + d) * ns + p];
+                }
+            }
+            double r;
+            if (g.node_mode == 0) {
+                r = fb;
+            } else if (g.node_mode == 1) {
+                r = d < 3 ? fp : 0.0;
+            } else {
+                r = (fb + (d < 3 ? -fp : -0.0)) - g.f_ext[tid];
+                const bool fx = g.fixed[tid] != 0;
+                if (fx) r = 0.0;
+                if (g.dy.on) {
+                    const double ac = g.dy.c0 * (g.u[tid] - g.dy.u_pred[tid]);
+                    g.dy.acc[tid] = ac;
+                    g.dy.vel[tid] = g.dy.v_pred[tid] + g.dy.dt * g.dy.gamma * ac;
+                    const double amid = (1.0 - g.dy.alpha_m) * ac + g.dy.alpha_m * g.dy.acc_old[tid];
+                    r = r + g.dy.M[tid] * amid;
+                    if (fx) r = 0.0;
+                }
+                ss = r * r;
+                if (!isfinite(r)) atomicOr(g.flags, FLAG_NONFINITE);
+            }
+            g.out[tid] = r;
+        }
+    } else if (K) {
+        const int tid = (blockIdx.x - g.node_blocks) * GB + threadIdx.x;
+        if (tid < 3 * g.n_blocks) {
+            const int blk = tid / 3, a = tid - 3 * (tid / 3);
+            using A = typename std::conditional<sizeof(KT) == 8, double, float>::type;
+            A v0 = A(0), v1 = A(0), v2 = A(0);
+            if (g.use_beam) {
+                const int q0 = g.B.bptr[blk], q1 = g.B.bptr[blk + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int code = g.B.bsrc[q];
+                    const int e = code >> 4, lb = code & 15;
+                    const size_t o = (size_t)(lb * 9 + 3 * a) * ne + e;
+                    v0 += A(Ke[o]);
+                    v1 += A(Ke[o + ne]);
+                    v2 += A(Ke[o + 2 * (size_t)ne]);
+                }
+            }
+            if (g.use_pair && g.B.pptr) {
+                const int q0 = g.B.pptr[blk], q1 = g.B.pptr[blk + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int code = g.B.psrc[q];
+                    const int p = code >> 4, k = (code >> 2) & 3, l = code & 3;
+                    const A cc = A(g.sd[(size_t)k * ns + p] * g.sd[(size_t)l * ns + p]);
+                    const size_t o = (size_t)(3 * a) * ns + p;
+                    v0 += cc * A(sg[o]);
+                    v1 += cc * A(sg[o + ns]);
+                    v2 += cc * A(sg[o + 2 * (size_t)ns]);
+                }
+            }
+            const int gr = g.B.brow[blk];
+            if (g.mass_coef != 0.0 && gr == g.B.gcol[blk]) {
+                const A m = A(g.mass_coef * g.M[3 * gr + a]);
+                if (a == 0) v0 += m;
+                else if (a == 1) v1 += m;
+                else v2 += m;
+            }
+            const int r0 = g.B.gptr[gr];
+            const int deg = g.B.gptr[gr + 1] - r0;
+            const size_t base = 9 * (size_t)r0 + 3 * (size_t)(blk - r0) + (size_t)a * 3 * deg;
+            K[base] = KT(v0);
+            K[base + 1] = KT(v1);
+            K[base + 2] = KT(v2);
         }
     }
-    if (partial) {
-        // deterministic block partial: fixed-shape tree in LDS
-        __shared__ double red[256];
+    // residual partial of this workgroup (fixed-shape tree: deterministic)
+    if (g.partial && node_block) {
         red[threadIdx.x] = ss;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = GB / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+        if (threadIdx.x == 0) g.partial[blockIdx.x] = red[0];
     }
 }
 
-__global__ void __launch_bounds__(256)
-sum_partials_kernel(int n, const double* __restrict__ partial, double* __restrict__ out) {
-    __shared__ double red[256];
+// ||R||^2 from the gather's workgroup partials and #(p_n > 0) from the pair
+// kernel's workgroup counts, one workgroup, fixed summation order.  (A
+// last-workgroup reduction inside the gather would need a device-scope release
+// fence per workgroup, which on a multi-XCD part writes back L2 each time.)
+__global__ void __launch_bounds__(GB) finish_kernel(int n_part, const double* __restrict__ partial,
+                                                    int n_act, const int* __restrict__ act_part,
+                                                    double* __restrict__ res_norm2, int* __restrict__ n_active) {
+    __shared__ double red[GB];
+    __shared__ int ired[GB];
     double s = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) s += partial[k];
+    int a = 0;
+    for (int k = threadIdx.x; k < n_part; k += GB) s += partial[k];
+    for (int k = threadIdx.x; k < n_act; k += GB) a += act_part[k];
     red[threadIdx.x] = s;
+    ired[threadIdx.x] = a;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    for (int w = GB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x] += red[threadIdx.x + w];
+            ired[threadIdx.x] += ired[threadIdx.x + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
-}
-
-// beam_sign / pair_sign: 0 excludes the term; mass_coef multiplies M on the diagonal.
-template <class KT, class ST, class GT>
-__global__ void __launch_bounds__(256)
-block_gather_kernel(int n_blocks, int n_elems, int n_sinc, BlockLists B, const ST* __restrict__ Ke,
-                    const double* __restrict__ sd, const GT* __restrict__ sg, int use_beam, int use_pair,
-                    double mass_coef, const double* __restrict__ M, KT* __restrict__ K) {
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blk >= n_blocks) return;
-    using A = typename std::conditional<sizeof(KT) == 8, double, float>::type;
-    A v[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = A(0);
-    if (use_beam) {
-        for (int q = B.bptr[blk]; q < B.bptr[blk + 1]; ++q) {
-            int code = B.bsrc[q];
-            int e = code >> 4, lb = code & 15;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) v[k] += A(Ke[(size_t)(lb * 9 + k) * n_elems + e]);
-        }
+    if (threadIdx.x == 0) {
+        if (res_norm2) *res_norm2 = red[0];
+        if (n_active) *n_active = ired[0];
     }
-    if (use_pair && B.pptr) {
-        for (int q = B.pptr[blk]; q < B.pptr[blk + 1]; ++q) {
-            int code = B.psrc[q];
-            int p = code >> 4, k = (code >> 2) & 3, l = code & 3;
-            A cc = A(sd[(size_t)k * n_sinc + p] * sd[(size_t)l * n_sinc + p]);
-#pragma unroll
-            for (int m = 0; m < 9; ++m) v[m] += cc * A(sg[(size_t)m * n_sinc + p]);
-        }
-    }
-    const int gr = B.brow[blk];
-    const int gc = B.gcol[blk];
-    if (mass_coef != 0.0 && gr == gc) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) v[4 * a] += A(mass_coef * M[3 * gr + a]);
-    }
-    const int r0 = B.gptr[gr];
-    const int deg = B.gptr[gr + 1] - r0;
-    const size_t base = 9 * (size_t)r0 + 3 * (size_t)(blk - r0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) K[base + (size_t)a * 3 * deg + b] = KT(v[3 * a + b]);
 }
 
 template <class T>
@@ -158,37 +174,24 @@ __global__ void fill_kernel(size_t n, T* p, T v) {
 
 // ---------------------------------------------------------------- launchers
 
-hipError_t launch_node_gather(int n_nodes, int n_elems, int n_sinc, const NodeLists& L, const double* fe,
-                              const double* sd, int mode, const double* u, const double* f_ext,
-                              const uint8_t* fixed, const DynArgs& dy, double* out, double* partial,
-                              int* flags, hipStream_t st) {
-    if (n_nodes <= 0) return hipSuccess;
-    dim3 blk(256), grd((n_nodes + 255) / 256);
-    hipLaunchKernelGGL(node_gather_kernel, grd, blk, 0, st, n_nodes, n_elems, n_sinc, L, fe, sd, mode, u,
-                       f_ext, fixed, dy, out, partial, flags);
-    return hipGetLastError();
-}
+int gather_node_blocks(int n_nodes) { return (6 * n_nodes + GB - 1) / GB; }
 
-int node_gather_blocks(int n_nodes) { return (n_nodes + 255) / 256; }
-
-hipError_t launch_sum_partials(int n, const double* partial, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, n, partial, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_block_gather(int prec, int n_blocks, int n_elems, int n_sinc, const BlockLists& B,
-                               const void* Ke, const double* sd, const void* sg, int use_beam, int use_pair,
-                               double mass_coef, const double* M, void* K, hipStream_t st) {
-    if (n_blocks <= 0) return hipSuccess;
-    dim3 blk(256), grd((n_blocks + 255) / 256);
+hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st) {
+    g.node_blocks = g.node_mode >= 0 ? gather_node_blocks(g.n_nodes) : 0;
+    const int kb = K ? (3 * g.n_blocks + GB - 1) / GB : 0;
+    const int nb = g.node_blocks + kb;
+    if (nb <= 0) return hipSuccess;
     if (prec == 0)
-        hipLaunchKernelGGL((block_gather_kernel<double, double, double>), grd, blk, 0, st, n_blocks, n_elems,
-                           n_sinc, B, (const double*)Ke, sd, (const double*)sg, use_beam, use_pair, mass_coef,
-                           M, (double*)K);
+        hipLaunchKernelGGL((gather_kernel<double, double, double>), dim3(nb), dim3(GB), 0, st, g,
+                           (const double*)Ke, (const double*)sg, (double*)K);
     else
-        hipLaunchKernelGGL((block_gather_kernel<float, float, float>), grd, blk, 0, st, n_blocks, n_elems,
-                           n_sinc, B, (const float*)Ke, sd, (const float*)sg, use_beam, use_pair, mass_coef, M,
-                           (float*)K);
+        hipLaunchKernelGGL((gather_kernel<float, float, float>), dim3(nb), dim3(GB), 0, st, g, (const float*)Ke,
+                           (const float*)sg, (float*)K);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (g.partial && (g.res_norm2 || g.n_active))
+        hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(GB), 0, st, g.node_blocks, (const double*)g.partial,
+                           g.n_act_part, g.act_part, g.res_norm2, g.n_active);
     return hipGetLastError();
 }
 

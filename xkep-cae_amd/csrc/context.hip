@@ -242,6 +242,21 @@ NodeLists Ctx::node_lists() const {
     return L;
 }
 
+GatherArgs Ctx::gather_args(int node_mode) const {
+    GatherArgs g{};
+    g.n_nodes = n_nodes;
+    g.n_elems = n_elems;
+    g.n_sinc = n_sinc;
+    g.n_blocks = n_blocks;
+    g.node_mode = node_mode;
+    g.L = node_lists();
+    g.B = block_lists();
+    g.fe = (const double*)d_fe.p;
+    g.sd = (const double*)d_sd.p;
+    g.flags = (int*)d_flags.p;
+    return g;
+}
+
 }  // namespace xk
 
 using namespace xk;
@@ -306,9 +321,13 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
     if (!rc && c->d_Ke.ensure((size_t)std::max(n_elems, 1) * 144 * (precision == 0 ? 8 : 4)) != hipSuccess)
         fail("Ke stage");
     if (!rc && c->d_flags.ensure(64) != hipSuccess) fail("flags");
-    if (!rc && c->d_partial.ensure(sizeof(double) * (size_t)(node_gather_blocks(n_nodes) + 1)) != hipSuccess)
+    if (!rc && c->d_partial.ensure(sizeof(double) * (size_t)(gather_node_blocks(n_nodes) + 1)) != hipSuccess)
         fail("partials");
     if (!rc && hipMemset(c->d_flags.p, 0, 64) != hipSuccess) fail("flags init");
+    if (!rc && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) fail("side stream");
+    if (!rc && (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess))
+        fail("fork/join events");
     if (!rc) {
         c->pb.resize(PB_COUNT);
         if (c->grow_pairs(1024) != 0) rc = -1;
@@ -346,16 +365,10 @@ int xk_beam_assemble(xk_ctx* c, const double* u, double* f_int, void* K_vals, in
     XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
                        f_int ? (double*)c->d_fe.p : nullptr, K_vals ? c->d_Ke.p : nullptr, (int*)c->d_flags.p,
                        st));
-    if (f_int) {
-        DynArgs dy{};
-        XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), (const double*)c->d_fe.p,
-                                  nullptr, 0, nullptr, nullptr, nullptr, dy, f_int, nullptr, (int*)c->d_flags.p,
-                                  st));
-    }
-    if (K_vals) {
-        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), c->d_Ke.p,
-                                   nullptr, nullptr, 1, 0, 0.0, nullptr, K_vals, st));
-    }
+    GatherArgs g = c->gather_args(f_int ? 0 : -1);
+    g.out = f_int;
+    g.use_beam = 1;
+    XK_TRY(launch_gather(c->prec, g, c->d_Ke.p, nullptr, K_vals, st));
     return 0;
 }
 
@@ -523,16 +536,10 @@ int xk_pairs_gather(xk_ctx* c, double* f, void* K_vals, int64_t K_cap, void* str
         return -1;
     }
     hipStream_t st = (hipStream_t)stream;
-    if (f) {
-        DynArgs dy{};
-        XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), nullptr,
-                                  (const double*)c->d_sd.p, 1, nullptr, nullptr, nullptr, dy, f, nullptr,
-                                  (int*)c->d_flags.p, st));
-    }
-    if (K_vals) {
-        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), nullptr,
-                                   (const double*)c->d_sd.p, c->d_sg.p, 0, 1, 0.0, nullptr, K_vals, st));
-    }
+    GatherArgs g = c->gather_args(f ? 1 : -1);
+    g.out = f;
+    g.use_pair = 1;
+    XK_TRY(launch_gather(c->prec, g, nullptr, c->d_sg.p, K_vals, st));
     return 0;
 }
 
@@ -580,7 +587,6 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         set_error("xk_assemble_iteration: K_vals holds fewer entries than the current pattern");
         return -1;
     }
-    if (n_active) XK_TRY(launch_fill_i32(1, n_active, 0, st));
     PairParams pp;
     pp.k_pen = k_pen;
     pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
@@ -590,38 +596,57 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.g_off = 0.0;
     pp.allow_deact = 0;
     pp.n_sinc = c->n_sinc;
-    int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    const int npb = pair_blocks(c->n_pairs);
+    XK_TRY(c->d_act.ensure(sizeof(int) * (size_t)std::max(npb, 1)));
+    int* act = (int*)c->d_act.p;
+    void* Ke = K_vals ? c->d_Ke.p : nullptr;
     hipEvent_t* ev = nullptr;
-    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[5 * (size_t)c->prof_calls++];
-    if (ev) XK_TRY(hipEventRecord(ev[0], st));
-    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                        (double*)c->d_sd.p, c->d_sg.p, n_active, st));
-    if (ev) XK_TRY(hipEventRecord(ev[1], st));
-    XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
-                       (double*)c->d_fe.p, K_vals ? c->d_Ke.p : nullptr, (int*)c->d_flags.p, st));
-    if (ev) XK_TRY(hipEventRecord(ev[2], st));
-    DynArgs dy{};
-    double mass_coef = 0.0;
+    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[4 * (size_t)c->prof_calls++];
+    if (ev) {
+        // profiled: the three stages in sequence on the caller's stream
+        XK_TRY(hipEventRecord(ev[0], st));
+        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(hipEventRecord(ev[1], st));
+        XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
+                           (double*)c->d_fe.p, Ke, (int*)c->d_flags.p, st));
+        XK_TRY(hipEventRecord(ev[2], st));
+    } else {
+        // element kernel on the side stream, concurrent with the pair kernel
+        XK_TRY(hipEventRecord(c->ev_fork, st));
+        XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
+                           (double*)c->d_fe.p, Ke, (int*)c->d_flags.p, c->side));
+        XK_TRY(hipEventRecord(c->ev_join, c->side));
+        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+    }
+    GatherArgs g = c->gather_args(2);
+    g.u = u;
+    g.f_ext = f_ext;
+    g.fixed = fixed_mask;
+    g.out = R;
+    g.use_beam = 1;
+    g.use_pair = 1;
     if (dyn && dyn->dt > 1e-30) {
+        DynArgs& dy = g.dy;
         dy.on = 1;
         dy.M = dyn->M_diag; dy.u_pred = dyn->u_pred; dy.v_pred = dyn->v_pred; dy.acc_old = dyn->acc_old;
         dy.acc = dyn->acc; dy.vel = dyn->vel;
         dy.c0 = 1.0 / (dyn->beta * (dyn->dt * dyn->dt));
         dy.dt = dyn->dt; dy.gamma = dyn->gamma; dy.alpha_m = dyn->alpha_m;
-        mass_coef = (1.0 - dyn->alpha_m) * dy.c0;
+        g.mass_coef = (1.0 - dyn->alpha_m) * dy.c0;
+        g.M = dyn->M_diag;
     }
-    double* partial = res_norm2 ? (double*)c->d_partial.p : nullptr;
-    XK_TRY(launch_node_gather(c->n_nodes, c->n_elems, c->n_sinc, c->node_lists(), (const double*)c->d_fe.p,
-                              (const double*)c->d_sd.p, 2, u, f_ext, fixed_mask, dy, R, partial,
-                              (int*)c->d_flags.p, st));
-    if (res_norm2) XK_TRY(launch_sum_partials(node_gather_blocks(c->n_nodes), partial, res_norm2, st));
+    g.partial = (double*)c->d_partial.p;
+    g.res_norm2 = res_norm2;
+    g.act_part = act;
+    g.n_act_part = c->n_pairs > 0 ? npb : 0;
+    g.n_active = n_active;
+    XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st));
     if (ev) XK_TRY(hipEventRecord(ev[3], st));
-    if (K_vals) {
-        XK_TRY(launch_block_gather(c->prec, c->n_blocks, c->n_elems, c->n_sinc, c->block_lists(), c->d_Ke.p,
-                                   (const double*)c->d_sd.p, c->d_sg.p, 1, 1, mass_coef, dy.on ? dy.M : nullptr,
-                                   K_vals, st));
-    }
-    if (ev) XK_TRY(hipEventRecord(ev[4], st));
     c->last_modes = modes;
     return 0;
 }
@@ -629,7 +654,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
 int xk_profile_begin(xk_ctx* c, int max_calls) {
     if (!c || max_calls < 0) return -1;
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
-    c->prof_ev.assign(5 * (size_t)max_calls, nullptr);
+    c->prof_ev.assign(4 * (size_t)max_calls, nullptr);
     for (auto& e : c->prof_ev) XK_TRY(hipEventCreate(&e));
     c->prof_max = max_calls;
     c->prof_calls = 0;
@@ -638,17 +663,17 @@ int xk_profile_begin(xk_ctx* c, int max_calls) {
 
 int xk_profile_end(xk_ctx* c, double* stage_ms, int* n_calls) {
     if (!c) return -1;
-    double acc[4] = {0, 0, 0, 0};
+    double acc[3] = {0, 0, 0};
     const int n = c->prof_calls;
-    if (n > 0) XK_TRY(hipEventSynchronize(c->prof_ev[5 * (size_t)(n - 1) + 4]));
+    if (n > 0) XK_TRY(hipEventSynchronize(c->prof_ev[4 * (size_t)(n - 1) + 3]));
     for (int k = 0; k < n; ++k)
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < 3; ++s) {
             float ms = 0.f;
-            XK_TRY(hipEventElapsedTime(&ms, c->prof_ev[5 * (size_t)k + s], c->prof_ev[5 * (size_t)k + s + 1]));
+            XK_TRY(hipEventElapsedTime(&ms, c->prof_ev[4 * (size_t)k + s], c->prof_ev[4 * (size_t)k + s + 1]));
             acc[s] += ms;
         }
     if (stage_ms)
-        for (int s = 0; s < 4; ++s) stage_ms[s] = acc[s];
+        for (int s = 0; s < 3; ++s) stage_ms[s] = acc[s];
     if (n_calls) *n_calls = n;
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();

@@ -52,11 +52,13 @@ __device__ __forceinline__ V3<double> min_axis(V3<double> v) {
     return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
 }
 
+constexpr int PAIR_BLOCK = 256;
+
 template <class GT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(PAIR_BLOCK)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
-            GT* __restrict__ sg, int* __restrict__ n_active) {
+            GT* __restrict__ sg, int* __restrict__ act_part) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int active_here = 0;
     if (i < n_pairs) {
@@ -300,12 +302,16 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             active_here = p_n > 0.0 ? 1 : 0;
         }
     }
-    if (n_active) {
-        // wavefront reduction, one atomic per wave (integer: order-independent)
+    if (act_part) {
+        // #(p_n > 0) of this workgroup: wave ballots, then one store per workgroup
+        __shared__ int wc[PAIR_BLOCK / 64];
         unsigned long long m = __ballot(active_here);
-        if ((threadIdx.x & 63) == 0) {
-            int c = __popcll(m);
-            if (c) atomicAdd(n_active, c);
+        if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int c = 0;
+            for (int w = 0; w < PAIR_BLOCK / 64; ++w) c += wc[w];
+            act_part[blockIdx.x] = c;
         }
     }
 }
@@ -315,17 +321,19 @@ template __global__ void pair_kernel<double>(int64_t, PairSoA, const double*, co
 template __global__ void pair_kernel<float>(int64_t, PairSoA, const double*, const double*,
                                             const double*, int, PairParams, double*, float*, int*);
 
+int pair_blocks(int64_t n_pairs) { return (int)((n_pairs + PAIR_BLOCK - 1) / PAIR_BLOCK); }
+
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
-                        int* n_active, hipStream_t st) {
+                        int* act_part, hipStream_t st) {
     if (n_pairs <= 0) return hipSuccess;
-    dim3 blk(256), grd((unsigned)((n_pairs + 255) / 256));
+    dim3 blk(PAIR_BLOCK), grd((unsigned)pair_blocks(n_pairs));
     if (prec == 0)
         hipLaunchKernelGGL(pair_kernel<double>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
-                           (double*)sg, n_active);
+                           (double*)sg, act_part);
     else
         hipLaunchKernelGGL(pair_kernel<float>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
-                           (float*)sg, n_active);
+                           (float*)sg, act_part);
     return hipGetLastError();
 }
 

@@ -68,6 +68,10 @@ struct Ctx {
     // optional per-stage event timing of the fused assembly
     std::vector<hipEvent_t> prof_ev;
     int prof_max = 0, prof_calls = 0;
+    // second stream (default priority) for the element kernel, joined by events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    DBuf d_act;   // per-workgroup active counts of the pair kernel
 
     int grow_pairs(int64_t need);
     void bind_pairs();
@@ -82,7 +86,16 @@ struct Ctx {
         for (DBuf* b : all) b->release();
         for (auto& b : pb) b.release();
         for (auto& b : bp) b.release();
+        d_act.release();
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+        prof_ev.clear();
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+        ev_fork = ev_join = nullptr;
+        side = nullptr;
     }
+    GatherArgs gather_args(int node_mode) const;
 };
 
 // launchers (one per translation unit)
@@ -90,16 +103,10 @@ hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, 
                        const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags, hipStream_t st);
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
-                        int* n_active, hipStream_t st);
-hipError_t launch_node_gather(int n_nodes, int n_elems, int n_sinc, const NodeLists& L, const double* fe,
-                              const double* sd, int mode, const double* u, const double* f_ext,
-                              const uint8_t* fixed, const DynArgs& dy, double* out, double* partial,
-                              int* flags, hipStream_t st);
-int node_gather_blocks(int n_nodes);
-hipError_t launch_sum_partials(int n, const double* partial, double* out, hipStream_t st);
-hipError_t launch_block_gather(int prec, int n_blocks, int n_elems, int n_sinc, const BlockLists& B,
-                               const void* Ke, const double* sd, const void* sg, int use_beam, int use_pair,
-                               double mass_coef, const double* M, void* K, hipStream_t st);
+                        int* act_part, hipStream_t st);
+int pair_blocks(int64_t n_pairs);
+int gather_node_blocks(int n_nodes);
+hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st);
 hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st);
 
 }  // namespace xk

@@ -46,4 +46,30 @@ struct DynArgs {
     int on;
 };
 
+// Arguments of the one-launch residual / tangent gather (compose.hip).
+struct GatherArgs {
+    int n_nodes, n_elems, n_sinc, n_blocks;
+    int node_mode;     // -1 none, 0 f_int, 1 pair force, 2 full residual
+    int node_blocks;   // set by the launcher
+    NodeLists L;
+    BlockLists B;
+    const double* fe;  // element force staging
+    const double* sd;  // pair c[4], F[3] staging
+    const double* u;
+    const double* f_ext;
+    const uint8_t* fixed;
+    DynArgs dy;
+    double* out;       // residual / force output (6 n_nodes)
+    int* flags;
+    int use_beam, use_pair;
+    double mass_coef;
+    const double* M;
+    // norm / active-count reduction (partial == nullptr: none)
+    double* partial;       // node_blocks partial sums of R^2
+    double* res_norm2;
+    const int* act_part;   // per-workgroup active counts of the pair kernel
+    int n_act_part;
+    int* n_active;
+};
+
 }  // namespace xk

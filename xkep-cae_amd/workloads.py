@@ -98,6 +98,62 @@ def make_state(case: StrandCase, *, seed: int | None = None) -> StrandState:
                        fixed_dofs=fixed)
 
 
+@dataclass
+class Batch:
+    """Several strand instances concatenated into one mesh.  Instance b is
+    translated by ``offsets[b]`` along x so that no box of one instance meets a
+    box of another (the broadphase then never pairs across instances); node,
+    element and DOF ranges of instance b are ``node_ranges[b]``,
+    ``elem_ranges[b]`` and 6x the node range."""
+
+    states: list
+    node_coords: np.ndarray
+    connectivity: np.ndarray
+    radii: np.ndarray
+    u: np.ndarray
+    u_ref: np.ndarray
+    f_ext: np.ndarray
+    fixed_dofs: np.ndarray
+    node_ranges: list
+    elem_ranges: list
+    offsets: np.ndarray
+
+    @property
+    def n_instances(self) -> int:
+        return len(self.states)
+
+
+def concat_states(states: list, gap_factor: float = 4.0) -> Batch:
+    xs, cs, rs, us, urs, fs, fx = [], [], [], [], [], [], []
+    nr, er, offs = [], [], []
+    n0 = e0 = 0
+    shift = 0.0
+    for st in states:
+        m = st.mesh
+        X = m.node_coords.copy()
+        lo, hi = X.min(axis=0), X.max(axis=0)
+        pad = gap_factor * (2.0 * m.wire_radius + (hi[2] - lo[2]) / m.n_elems_per_wire)
+        dx = shift - lo[0] + pad
+        X[:, 0] += dx
+        shift = X[:, 0].max() + pad
+        offs.append(dx)
+        xs.append(X)
+        cs.append(m.connectivity + n0)
+        rs.append(m.radii)
+        us.append(st.u)
+        urs.append(st.u_ref)
+        fs.append(st.f_ext)
+        fx.append(st.fixed_dofs + 6 * n0)
+        nr.append((n0, n0 + m.n_nodes))
+        er.append((e0, e0 + m.n_elems))
+        n0 += m.n_nodes
+        e0 += m.n_elems
+    return Batch(states=states, node_coords=np.concatenate(xs), connectivity=np.concatenate(cs),
+                 radii=np.concatenate(rs), u=np.concatenate(us), u_ref=np.concatenate(urs),
+                 f_ext=np.concatenate(fs), fixed_dofs=np.concatenate(fx), node_ranges=nr, elem_ranges=er,
+                 offsets=np.array(offs))
+
+
 def batched_cases(n_instances: int = 64, seed: int = 64) -> list[StrandCase]:
     """The uneven batched workload: wires in {7, 19, 37, 61}, 50-150 elements per
     wire, lay angle U(8, 15) deg, mu U(0.1, 0.5), random end moment, r = 0.5,
