@@ -118,13 +118,18 @@ __device__ __forceinline__ M3<T> tangent_op(V3<double> thd, bool inverse) {
     return r;
 }
 
+constexpr int BEAM_BLOCK = 64;
+
 template <class TT>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(BEAM_BLOCK)
 beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
             const int* __restrict__ conn, BeamConst bc, double* __restrict__ fe_stage,
             TT* __restrict__ Ke_stage, int* __restrict__ flags) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_elems) return;
+    // tail lanes recompute the last element (nothing of theirs is stored) so
+    // that every lane reaches the LDS barriers of the tangent write-out
+    const int e_lane = blockIdx.x * BEAM_BLOCK + threadIdx.x;
+    const bool live = e_lane < n_elems;
+    const int e = live ? e_lane : n_elems - 1;
     const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
     V3<double> P1{X[3 * n1], X[3 * n1 + 1], X[3 * n1 + 2]};
     V3<double> P2{X[3 * n2], X[3 * n2 + 1], X[3 * n2 + 2]};
@@ -136,7 +141,7 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     // reference frame R_0 (rows e_x, e_y, e_z)
     V3<double> d0 = P2 - P1;
     double L0 = norm(d0);
-    V3<double> ex0 = (1.0 / 1.0) * V3<double>{d0.x / L0, d0.y / L0, d0.z / L0};
+    V3<double> ex0{d0.x / L0, d0.y / L0, d0.z / L0};
     V3<double> vr;
     if (bc.has_vref) {
         vr = V3<double>{bc.vref[0], bc.vref[1], bc.vref[2]};
@@ -235,16 +240,17 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     fcr[4] = m44 * th1.y + m410 * th2.y;
     fcr[10] = m410 * th1.y + m44 * th2.y;
 
-    if (fe_stage) {
+    if (fe_stage && live) {
+        double* fo = fe_stage + 12 * (size_t)e;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
-            fe_stage[(size_t)(3 * b) * n_elems + e] = fb.x;
-            fe_stage[(size_t)(3 * b + 1) * n_elems + e] = fb.y;
-            fe_stage[(size_t)(3 * b + 2) * n_elems + e] = fb.z;
+            fo[3 * b] = fb.x;
+            fo[3 * b + 1] = fb.y;
+            fo[3 * b + 2] = fb.z;
         }
     }
-    if (!Ke_stage) return;
+    if (!Ke_stage) return;  // uniform across the launch
 
     // ---- analytical tangent in precision TT ----
     using T = TT;
@@ -327,8 +333,14 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     }
 
     // symmetrised 3x3 group blocks Ks(b,c) = (K(b,c) + K(c,b)^T) / 2,
-    // K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b
+    // K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b.
+    // Each element's 144 values form one contiguous record (block-row b = one
+    // 36-value chunk); the chunk of all lanes goes through LDS so the global
+    // stores are contiguous runs.
+    __shared__ T tile[BEAM_BLOCK * 36];
     const T h = T(0.5);
+    const int e0 = blockIdx.x * BEAM_BLOCK;
+    const int cnt = min(BEAM_BLOCK, n_elems - e0);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
 #pragma unroll
@@ -337,15 +349,19 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
             const T sbc = (c == 2) ? T(-1) : T(1);
             const M3<T>& Kcb = (b == 0 || b == 2) ? Xb[c] : (b == 1 ? Yb[c] : Zb[c]);
             const T scb = (b == 2) ? T(-1) : T(1);
-            const int lb = b * 4 + c;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    T v = h * (sbc * Kbc(i, j) + scb * Kcb(j, i));
-                    Ke_stage[(size_t)(lb * 9 + 3 * i + j) * n_elems + e] = v;
-                }
+                for (int j = 0; j < 3; ++j)
+                    tile[threadIdx.x * 36 + c * 9 + 3 * i + j] = h * (sbc * Kbc(i, j) + scb * Kcb(j, i));
         }
+        __syncthreads();
+        T* dst = Ke_stage + (size_t)e0 * 144 + 36 * b;
+        for (int k = threadIdx.x; k < cnt * 36; k += BEAM_BLOCK) {
+            const int el = k / 36, j = k - 36 * (k / 36);
+            dst[(size_t)el * 144 + j] = tile[k];
+        }
+        __syncthreads();
     }
 }
 
@@ -362,7 +378,7 @@ hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, 
                        const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags,
                        hipStream_t st) {
     if (n_elems <= 0) return hipSuccess;
-    dim3 blk(64), grd((n_elems + 63) / 64);
+    dim3 blk(BEAM_BLOCK), grd((n_elems + BEAM_BLOCK - 1) / BEAM_BLOCK);
     if (prec == 0)
         hipLaunchKernelGGL(beam_kernel<double>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
                            (double*)Ke_stage, flags);

@@ -36,7 +36,6 @@ template <class KT, class ST, class GT>
 __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __restrict__ Ke,
                                                      const GT* __restrict__ sg, KT* __restrict__ K) {
     __shared__ double red[GB];
-    const int ne = g.n_elems, ns = g.n_sinc;
     double ss = 0.0;
     const bool node_block = (int)blockIdx.x < g.node_blocks;
     if (node_block) {
@@ -48,15 +47,15 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                 const int q0 = g.L.bptr[i], q1 = g.L.bptr[i + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.L.bsrc[q];
-                    fb += g.fe[(size_t)(6 * (code & 1) + d) * ne + (code >> 1)];
+                    fb += g.fe[12 * (size_t)(code >> 1) + 6 * (code & 1) + d];
                 }
             }
             if (g.node_mode != 0 && d < 3 && g.L.pptr) {
                 const int q0 = g.L.pptr[i], q1 = g.L.pptr[i + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.L.psrc[q];
-                    const int p = code >> 2;
-                    fp += g.sd[(size_t)(code & 3) * ns + p] * g.sd[(size_t)(4 + d) * ns + p];
+                    const double* rec = g.sd + SD_REC * (size_t)(code >> 2);
+                    fp += rec[code & 3] * rec[4 + d];
                 }
             }
             double r;
@@ -91,23 +90,24 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                 const int q0 = g.B.bptr[blk], q1 = g.B.bptr[blk + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.B.bsrc[q];
-                    const int e = code >> 4, lb = code & 15;
-                    const size_t o = (size_t)(lb * 9 + 3 * a) * ne + e;
-                    v0 += A(Ke[o]);
-                    v1 += A(Ke[o + ne]);
-                    v2 += A(Ke[o + 2 * (size_t)ne]);
+                    // element record: 144 values, 3x3 group block (code & 15) at 9 (code & 15)
+                    const ST* r = Ke + 144 * (size_t)(code >> 4) + 9 * (code & 15) + 3 * a;
+                    v0 += A(r[0]);
+                    v1 += A(r[1]);
+                    v2 += A(r[2]);
                 }
             }
             if (g.use_pair && g.B.pptr) {
                 const int q0 = g.B.pptr[blk], q1 = g.B.pptr[blk + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.B.psrc[q];
-                    const int p = code >> 4, k = (code >> 2) & 3, l = code & 3;
-                    const A cc = A(g.sd[(size_t)k * ns + p] * g.sd[(size_t)l * ns + p]);
-                    const size_t o = (size_t)(3 * a) * ns + p;
-                    v0 += cc * A(sg[o]);
-                    v1 += cc * A(sg[o + ns]);
-                    v2 += cc * A(sg[o + 2 * (size_t)ns]);
+                    const size_t p = (size_t)(code >> 4);
+                    const double* rec = g.sd + SD_REC * p;
+                    const A cc = A(rec[(code >> 2) & 3] * rec[code & 3]);
+                    const GT* gr_ = sg + SG_REC * p + 3 * a;
+                    v0 += cc * A(gr_[0]);
+                    v1 += cc * A(gr_[1]);
+                    v2 += cc * A(gr_[2]);
                 }
             }
             const int gr = g.B.brow[blk];

@@ -217,8 +217,8 @@ int Ctx::build_pattern() {
     XK_TRY(upload(d_node_pptr, nptr));
     XK_TRY(upload(d_node_psrc, nsrc));
     if (np) XK_TRY(hipMemcpy(P.sinc, sinc_of.data(), np * 4, hipMemcpyHostToDevice));
-    XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * 7 * sizeof(double)));
-    XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * 9 * (prec == 0 ? 8 : 4)));
+    XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * SD_REC * sizeof(double)));
+    XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * (prec == 0 ? 8 : 4)));
     n_sinc = ns;
     n_blocks = nblk;
     pattern_valid = true;

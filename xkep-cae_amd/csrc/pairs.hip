@@ -161,7 +161,6 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
 
         const int si = P.sinc[i];
         if (si >= 0 && (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F))) {
-            const int ns = pp.n_sinc;
             const V3<double> t2 = cross(n, t1);
             double p_n = P.p_n[i];
             V3<double> F{0.0, 0.0, 0.0};
@@ -209,13 +208,14 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 }
             }
             if (modes & (PAIR_HUBER | PAIR_FRIC)) {
-                sd[0 * ns + si] = 1.0 - s;
-                sd[1 * ns + si] = s;
-                sd[2 * ns + si] = -(1.0 - t);
-                sd[3 * ns + si] = -t;
-                sd[4 * ns + si] = F.x;
-                sd[5 * ns + si] = F.y;
-                sd[6 * ns + si] = F.z;
+                double* rec = sd + SD_REC * (size_t)si;
+                rec[0] = 1.0 - s;
+                rec[1] = s;
+                rec[2] = -(1.0 - t);
+                rec[3] = -t;
+                rec[4] = F.x;
+                rec[5] = F.y;
+                rec[6] = F.z;
             }
             if (modes & (PAIR_TAN_C | PAIR_TAN_F)) {
                 double G[9];
@@ -292,12 +292,14 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                         for (int k = 0; k < 9; ++k) G[k] += sgn * H[k];
                     }
                 }
-                sd[0 * ns + si] = 1.0 - s;
-                sd[1 * ns + si] = s;
-                sd[2 * ns + si] = -(1.0 - t);
-                sd[3 * ns + si] = -t;
+                double* rec = sd + SD_REC * (size_t)si;
+                rec[0] = 1.0 - s;
+                rec[1] = s;
+                rec[2] = -(1.0 - t);
+                rec[3] = -t;
+                GT* grec = sg + SG_REC * (size_t)si;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) sg[(size_t)k * ns + si] = GT(G[k]);
+                for (int k = 0; k < 9; ++k) grec[k] = GT(G[k]);
             }
             active_here = p_n > 0.0 ? 1 : 0;
         }

@@ -135,7 +135,10 @@ struct BeamConst {
     double vref[3];
 };
 
-// Pair-staging row count: c[4], F[3], G[9] (see pairs.hip).
-constexpr int PSTAGE = 16;
+// Per contributing pair, the pair kernel stages one FP64 record
+// [c0, c1, c2, c3, Fx, Fy, Fz, pad] and one tangent record [G00 .. G22, pad x3]
+// (FP64 or FP32); element records are 12 FP64 forces and 144 tangent values.
+constexpr int SD_REC = 8;
+constexpr int SG_REC = 12;
 
 }  // namespace xk
