@@ -33,6 +33,12 @@ typedef struct xk_ctx xk_ctx;
 
 enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1 };
 
+/* OR-ed into xk_ctx_create's precision: use the staged element path even when
+ * the connectivity is made of element chains (the direct path is chosen
+ * automatically for chains: every node shared by two elements e and e+1 as
+ * node 2 of e and node 1 of e+1, no node in more than two elements). */
+enum { XK_OPT_STAGED = 256 };
+
 /* status flags returned (bitwise OR) by assembly calls */
 enum {
     XK_FLAG_NONFINITE = 1,     /* a non-finite residual entry was produced */
@@ -62,6 +68,8 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
 int xk_ctx_destroy(xk_ctx* ctx);
 const char* xk_last_error(void);
 int xk_ctx_precision(const xk_ctx* ctx);
+/* 1 if the context uses the direct element path (element chains), 0 staged. */
+int xk_ctx_direct(const xk_ctx* ctx);
 
 /* Element assembly.  Replaces ULCRBeamAssembler.assemble_internal_force /
  * assemble_tangent (_beam_assembler.py:73-115) via assemble_cr_beam3d

@@ -105,18 +105,19 @@ def compare_pairs(d, a, pre, tol):
         assert err < tol, (pre, f, err)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision):
+@pytest.mark.parametrize("precision,staged", [("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)])
+def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision, staged):
     """Two increments x two Newton attempts of a 7-wire strand with Huber contact
     and Coulomb friction, against the reference's ContactForceAssembly /
-    TangentAssembly / GeneralizedAlpha outputs."""
+    TangentAssembly / GeneralizedAlpha outputs, on both element paths."""
     a = golden("assembly_7wire.npz")
     C, K, T, A = pkg("context"), pkg("contact"), pkg("time_integration"), pkg("assembly")
     k_pen, sd, mu, rho_inf, dt, rho, ktr = a["params"]
     r = float(a["radius"])
     X, conn = a["X"], a["conn"]
     radii = np.full(len(conn), r)
-    ctx = C.DeviceContext(X, conn, section(r), radii=radii, precision=precision)
+    ctx = C.DeviceContext(X, conn, section(r), radii=radii, precision=precision, staged=staged)
+    assert ctx.direct == (not staged)
     mgr = K.ContactManager(ctx, K.ContactConfig(mu=mu, k_t_ratio=ktr))
     K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, X, conn, radii))
     E = pkg("elements")
@@ -146,7 +147,8 @@ def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision):
         asm.set_u_ref(u)
 
 
-def test_strategy_api_matches_oracle_on_37_wires(gpu):
+@pytest.mark.parametrize("staged", [False, True])
+def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     """Drop-in Processes / Strategies (DetectCandidates, UpdateGeometry, Huber,
     Coulomb evaluate + tangent) against the oracle, one call each."""
     wl = pkg("workloads")
@@ -154,7 +156,11 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu):
     case = wl.C2
     st = wl.make_state(case)
     m = st.mesh
-    ctx = C.DeviceContext(m.node_coords, m.connectivity, section(m.wire_radius), radii=m.radii)
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, section(m.wire_radius), radii=m.radii, staged=staged)
+    E = pkg("elements")
+    s = section(m.wire_radius)
+    beam = E.ULCRBeamAssembler(m.node_coords, m.connectivity, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y,
+                               s.kappa_z, context=ctx)
     cfg = K.ContactConfig(mu=case.mu)
     mgr = K.ContactManager(ctx, cfg)
     ocfg = ocon.ContactConfig(mu=case.mu)
@@ -189,6 +195,43 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu):
     assert srel(Kf, ocon.friction_tangent(ofr.pairs, ofr, ndof)) < FP64_TOL
     d = ctx.get_pairs()
     assert np.array_equal(d["status"], ofr.pairs.status)
+    # beam callbacks sharing the context (pattern now holds contact blocks too)
+    osec = obeam.circle_section(case.wire_radius, 200e3, 0.3)
+    Kr, fr_ = obeam.assemble_cr_beam3d(m.node_coords, m.connectivity, st.u, osec)
+    assert rel(beam.assemble_internal_force(st.u), fr_) < FP64_TOL
+    assert srel(beam.assemble_tangent(st.u), Kr) < FP64_TOL
+
+
+def test_non_chain_connectivity_takes_the_staged_path(gpu):
+    """Reversed and reordered elements: the library detects that the mesh is
+    not made of element chains and assembles through the staged path."""
+    wl = pkg("workloads")
+    C, A = pkg("context"), pkg("assembly")
+    case = wl.StrandCase("shuffled_7wire", 7, 1.0, 20.0, 20, 10.0, seed=11)
+    st = wl.make_state(case)
+    m = st.mesh
+    conn = m.connectivity.copy()
+    conn[20:40] = conn[20:40, ::-1]  # wire 1 runs backwards
+    perm = np.random.default_rng(3).permutation(len(conn))
+    conn = conn[perm]
+    ctx = C.DeviceContext(m.node_coords, conn, section(case.wire_radius), radii=m.radii)
+    assert not ctx.direct
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                           f_ext=st.f_ext, fixed_dofs=st.fixed_dofs)
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(st.u)
+    R, Kv, _, nact = asm.assemble(st.u)
+    osec = obeam.circle_section(case.wire_radius, 200e3, 0.3)
+    cfg = ocon.ContactConfig(mu=case.mu)
+    prob = oasm.Problem(m.node_coords, conn, osec, case.k_pen, case.smoothing_delta, 0.5, case.mu, cfg)
+    op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, conn, m.radii, cfg)
+    x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
+    op, _ = ocon.detect_candidates(op, x, conn, m.radii, cfg)
+    op = ocon.update_geometry(op, x, cfg)
+    ref = oasm.assemble_iteration(prob, op, st.u, st.u_ref, st.f_ext, st.fixed_dofs, None, 0.0)
+    assert rel(R.cpu().numpy(), ref.R) < FP64_TOL
+    assert srel(asm.K_csr(), ref.K) < FP64_TOL
+    assert int(nact.item()) == ref.n_active
 
 
 def test_mixed_precision_residual_and_active_set_at_91_wires(gpu):

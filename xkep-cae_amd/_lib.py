@@ -22,6 +22,7 @@ vp = C.c_void_p
 
 PAIR_GEOM, PAIR_HYST, PAIR_HUBER, PAIR_FRIC, PAIR_TAN_C, PAIR_TAN_F = 1, 2, 4, 8, 16, 32
 PREC = {"fp64": 0, "mixed": 1}
+OPT_STAGED = 256
 
 
 class XkDyn(C.Structure):
@@ -35,6 +36,7 @@ SIGNATURES = {
     "xk_ctx_destroy": (C.c_int, [vp]),
     "xk_last_error": (C.c_char_p, []),
     "xk_ctx_precision": (C.c_int, [vp]),
+    "xk_ctx_direct": (C.c_int, [vp]),
     "xk_beam_assemble": (C.c_int, [vp, vp, vp, vp, C.c_int64, vp]),
     "xk_pairs_count": (C.c_int, [vp, c_int64_p]),
     "xk_pairs_set": (C.c_int, [vp, C.c_int64] + [vp] * 15),

@@ -64,7 +64,9 @@ class DeviceContext:
     """Mesh, section, contact radii and pair list resident on the GPU."""
 
     def __init__(self, node_coords, connectivity, section: Section, *, radii=None, scf=None, v_ref=None,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", staged: bool = False):
+        """``staged`` forces the staged element path; otherwise meshes made of
+        element chains use the direct path (see ``direct``)."""
         torch = require_gpu()
         self.torch = torch
         self.precision = precision
@@ -84,8 +86,9 @@ class DeviceContext:
         L.check(L.lib().xk_ctx_create(C.byref(h), self.n_nodes, self.n_elems, X.ctypes.data, conn.ctypes.data,
                                       sec.ctypes.data, float(scf) if scf else 0.0,
                                       vr.ctypes.data if vr is not None else None, self.radii_np.ctypes.data,
-                                      L.PREC[precision]), "xk_ctx_create")
+                                      L.PREC[precision] | (L.OPT_STAGED if staged else 0)), "xk_ctx_create")
         self.h = h
+        self.direct = bool(L.lib().xk_ctx_direct(h))
         self.dev = torch.device("cuda")
         self.X = torch.from_numpy(X).to(self.dev)
         self.kdtype = torch.float64 if precision == "fp64" else torch.float32

@@ -15,12 +15,20 @@
 // rotations, z: node-2 rotations), so each 3-row block b of the unsymmetrised
 // tangent is carried as three 3x3 matrices X_b, Y_b, Z_b.
 //
-// Outputs are staged structure-of-arrays so that every store instruction is
-// coalesced across the wavefront:
-//   fe_stage[i * n_elems + e], i = 0..11   (global-frame element force)
-//   Ke_stage[(lb * 9 + a * 3 + b) * n_elems + e]
-// with lb = g_row * 4 + g_col over the element's DOF groups
-// {node1 trans, node1 rot, node2 trans, node2 rot}.
+// Two output paths share the element math (beam_core):
+//  * direct (meshes made of element chains, every strand): a workgroup takes
+//    64 consecutive elements and owns the first 63.  Each owner writes its
+//    off-diagonal node blocks straight into K, and the diagonal block and the
+//    internal force of its node 2 summed with the next element's node-1 part
+//    (exchanged through LDS, element order), plus the lumped-mass term; the
+//    64th lane only supplies that node-1 part (it is owned by the next
+//    workgroup).  Nothing is staged.
+//  * staged (any connectivity): per-element records
+//    fe_stage[12 e + i] (global-frame force) and
+//    Ke_stage[144 e + 9 lb + 3 a + b] (symmetrised 3x3 group blocks,
+//    lb = g_row * 4 + g_col over {node1 trans, node1 rot, node2 trans,
+//    node2 rot}), written through LDS as contiguous runs; the gather kernel
+//    sums them per node / per block.
 
 #include "xk_common.h"
 
@@ -120,16 +128,13 @@ __device__ __forceinline__ M3<T> tangent_op(V3<double> thd, bool inverse) {
 
 constexpr int BEAM_BLOCK = 64;
 
+// Element e: global-frame force fe[12] (FP64) and, when want_K, the three 3x3
+// matrices per 3-row block of the unsymmetrised tangent (precision TT).
 template <class TT>
-__global__ void __launch_bounds__(BEAM_BLOCK)
-beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
-            const int* __restrict__ conn, BeamConst bc, double* __restrict__ fe_stage,
-            TT* __restrict__ Ke_stage, int* __restrict__ flags) {
-    // tail lanes recompute the last element (nothing of theirs is stored) so
-    // that every lane reaches the LDS barriers of the tangent write-out
-    const int e_lane = blockIdx.x * BEAM_BLOCK + threadIdx.x;
-    const bool live = e_lane < n_elems;
-    const int e = live ? e_lane : n_elems - 1;
+__device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, const double* __restrict__ u,
+                                          const int* __restrict__ conn, const BeamConst& bc, double* fe,
+                                          M3<TT>* Xb, M3<TT>* Yb, M3<TT>* Zb, bool want_K,
+                                          int* __restrict__ flags) {
     const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
     V3<double> P1{X[3 * n1], X[3 * n1 + 1], X[3 * n1 + 2]};
     V3<double> P2{X[3 * n2], X[3 * n2 + 1], X[3 * n2 + 2]};
@@ -240,17 +245,14 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     fcr[4] = m44 * th1.y + m410 * th2.y;
     fcr[10] = m410 * th1.y + m44 * th2.y;
 
-    if (fe_stage && live) {
-        double* fo = fe_stage + 12 * (size_t)e;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
-            fo[3 * b] = fb.x;
-            fo[3 * b + 1] = fb.y;
-            fo[3 * b + 2] = fb.z;
-        }
+    for (int b = 0; b < 4; ++b) {
+        V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
+        fe[3 * b] = fb.x;
+        fe[3 * b + 1] = fb.y;
+        fe[3 * b + 2] = fb.z;
     }
-    if (!Ke_stage) return;  // uniform across the launch
+    if (!want_K) return;
 
     // ---- analytical tangent in precision TT ----
     using T = TT;
@@ -312,7 +314,6 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     }
 
     // block rows b: X_b = Rc^T KBx_b + Rc^T S(f_b) dpsi, Y_b = Rc^T KBy_b, Z_b = Rc^T KBz_b
-    M3<T> Xb[4], Yb[4], Zb[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         M3<T> Ax, Ay, Az;
@@ -332,31 +333,52 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
         Zb[b] = mulTN(Rc, Az);
     }
 
-    // symmetrised 3x3 group blocks Ks(b,c) = (K(b,c) + K(c,b)^T) / 2,
-    // K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b.
-    // Each element's 144 values form one contiguous record (block-row b = one
-    // 36-value chunk); the chunk of all lanes goes through LDS so the global
-    // stores are contiguous runs.
-    __shared__ T tile[BEAM_BLOCK * 36];
-    const T h = T(0.5);
+}
+
+// Symmetrised 3x3 group block Ks(b,c) = (K(b,c) + K(c,b)^T) / 2 of the element
+// tangent, K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b.
+template <class T>
+__device__ __forceinline__ T ks(const M3<T>* Xb, const M3<T>* Yb, const M3<T>* Zb, int b, int c, int i, int j) {
+    const M3<T>& Kbc = (c == 0 || c == 2) ? Xb[b] : (c == 1 ? Yb[b] : Zb[b]);
+    const T sbc = (c == 2) ? T(-1) : T(1);
+    const M3<T>& Kcb = (b == 0 || b == 2) ? Xb[c] : (b == 1 ? Yb[c] : Zb[c]);
+    const T scb = (b == 2) ? T(-1) : T(1);
+    return T(0.5) * (sbc * Kbc(i, j) + scb * Kcb(j, i));
+}
+
+// Staged path: per-element records (any connectivity).
+template <class TT>
+__global__ void __launch_bounds__(BEAM_BLOCK)
+beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
+            const int* __restrict__ conn, BeamConst bc, double* __restrict__ fe_stage,
+            TT* __restrict__ Ke_stage, int* __restrict__ flags) {
+    // tail lanes recompute the last element (nothing of theirs is stored) so
+    // that every lane reaches the LDS barriers of the tangent write-out
+    const int e_lane = blockIdx.x * BEAM_BLOCK + threadIdx.x;
+    const bool live = e_lane < n_elems;
+    const int e = live ? e_lane : n_elems - 1;
+    double fe[12];
+    M3<TT> Xb[4], Yb[4], Zb[4];
+    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, Ke_stage != nullptr, flags);
+    if (fe_stage && live) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) fe_stage[12 * (size_t)e + k] = fe[k];
+    }
+    if (!Ke_stage) return;  // uniform across the launch
+    // block row b of the 144-value record = one 36-value chunk, through LDS
+    __shared__ TT tile[BEAM_BLOCK * 36];
     const int e0 = blockIdx.x * BEAM_BLOCK;
     const int cnt = min(BEAM_BLOCK, n_elems - e0);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const M3<T>& Kbc = (c == 0 || c == 2) ? Xb[b] : (c == 1 ? Yb[b] : Zb[b]);
-            const T sbc = (c == 2) ? T(-1) : T(1);
-            const M3<T>& Kcb = (b == 0 || b == 2) ? Xb[c] : (b == 1 ? Yb[c] : Zb[c]);
-            const T scb = (b == 2) ? T(-1) : T(1);
+        for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    tile[threadIdx.x * 36 + c * 9 + 3 * i + j] = h * (sbc * Kbc(i, j) + scb * Kcb(j, i));
-        }
+                for (int j = 0; j < 3; ++j) tile[threadIdx.x * 36 + c * 9 + 3 * i + j] = ks(Xb, Yb, Zb, b, c, i, j);
         __syncthreads();
-        T* dst = Ke_stage + (size_t)e0 * 144 + 36 * b;
+        TT* dst = Ke_stage + (size_t)e0 * 144 + 36 * b;
         for (int k = threadIdx.x; k < cnt * 36; k += BEAM_BLOCK) {
             const int el = k / 36, j = k - 36 * (k / 36);
             dst[(size_t)el * 144 + j] = tile[k];
@@ -365,10 +387,80 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     }
 }
 
-template __global__ void beam_kernel<double>(int, const double*, const double*, const int*, BeamConst,
-                                             double*, double*, int*);
-template __global__ void beam_kernel<float>(int, const double*, const double*, const int*, BeamConst,
-                                            double*, float*, int*);
+// Direct path (chains): eflag bit0 = node 1 shared with element e-1, bit1 =
+// node 2 shared with element e+1 (each such node has exactly these two
+// elements).  eslot[16 e + lb] = global block index of group block lb.
+constexpr int DIRECT_OWN = BEAM_BLOCK - 1;
+
+template <class TT>
+__global__ void __launch_bounds__(BEAM_BLOCK)
+beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
+                   const int* __restrict__ conn, BeamConst bc, const int8_t* __restrict__ eflag,
+                   const int* __restrict__ eslot, const int* __restrict__ gptr, double mass_coef,
+                   const double* __restrict__ M, double* __restrict__ fint, TT* __restrict__ K,
+                   int* __restrict__ flags) {
+    const int l = threadIdx.x;
+    const int e_lane = blockIdx.x * DIRECT_OWN + l;
+    const bool live = e_lane < n_elems;
+    const int e = live ? e_lane : n_elems - 1;
+    double fe[12];
+    M3<TT> Xb[4], Yb[4], Zb[4];
+    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, K != nullptr, flags);
+    // publish the node-1 part for the previous lane
+    __shared__ double fsh[BEAM_BLOCK * 6];
+    __shared__ TT ksh[BEAM_BLOCK * 36];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) fsh[l * 6 + d] = fe[d];
+    if (K) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) ksh[l * 36 + (2 * b + c) * 9 + 3 * i + j] = ks(Xb, Yb, Zb, b, c, i, j);
+    }
+    __syncthreads();
+    if (!live || l >= DIRECT_OWN) return;
+    const int fl = eflag[e];
+    const bool prv = fl & 1, nxt = fl & 2;
+    const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
+    if (fint) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) fint[6 * (size_t)n2 + d] = nxt ? fe[6 + d] + fsh[(l + 1) * 6 + d] : fe[6 + d];
+        if (!prv) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) fint[6 * (size_t)n1 + d] = fe[d];
+        }
+    }
+    if (!K) return;
+    const int grp[4] = {2 * n1, 2 * n1 + 1, 2 * n2, 2 * n2 + 1};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int gr = grp[b];
+        const int r0 = gptr[gr];
+        const int deg = gptr[gr + 1] - r0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool diag1 = b < 2 && c < 2;
+            const bool diag2 = b >= 2 && c >= 2;
+            if (diag1 && prv) continue;  // owned by the previous element
+            const int blk = eslot[16 * (size_t)e + 4 * b + c];
+            const size_t base = 9 * (size_t)r0 + 3 * (size_t)(blk - r0);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    TT v = ks(Xb, Yb, Zb, b, c, i, j);
+                    if (diag2 && nxt) v += ksh[(l + 1) * 36 + (2 * (b - 2) + (c - 2)) * 9 + 3 * i + j];
+                    if ((diag1 || diag2) && b == c && i == j && mass_coef != 0.0)
+                        v += TT(mass_coef * M[3 * (size_t)gr + i]);
+                    K[base + (size_t)i * 3 * deg + j] = v;
+                }
+        }
+    }
+}
 
 }  // namespace xk
 
@@ -385,6 +477,21 @@ hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, 
     else
         hipLaunchKernelGGL(beam_kernel<float>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
                            (float*)Ke_stage, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const double* u, const int* conn,
+                              const BeamConst& bc, const int8_t* eflag, const int* eslot, const int* gptr,
+                              double mass_coef, const double* M, double* f_int, void* K, int* flags,
+                              hipStream_t st) {
+    if (n_elems <= 0) return hipSuccess;
+    dim3 blk(BEAM_BLOCK), grd((n_elems + DIRECT_OWN - 1) / DIRECT_OWN);
+    if (prec == 0)
+        hipLaunchKernelGGL(beam_direct_kernel<double>, grd, blk, 0, st, n_elems, X, u, conn, bc, eflag, eslot,
+                           gptr, mass_coef, M, f_int, (double*)K, flags);
+    else
+        hipLaunchKernelGGL(beam_direct_kernel<float>, grd, blk, 0, st, n_elems, X, u, conn, bc, eflag, eslot,
+                           gptr, mass_coef, M, f_int, (float*)K, flags);
     return hipGetLastError();
 }
 }  // namespace xk

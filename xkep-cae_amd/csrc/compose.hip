@@ -43,7 +43,9 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
         if (tid < 6 * g.n_nodes) {
             const int i = tid / 6, d = tid - 6 * (tid / 6);
             double fb = 0.0, fp = 0.0;
-            if (g.node_mode != 1) {
+            if (g.node_mode != 1 && g.fint) {
+                fb = g.fint[tid];
+            } else if (g.node_mode != 1) {
                 const int q0 = g.L.bptr[i], q1 = g.L.bptr[i + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.L.bsrc[q];
@@ -82,11 +84,27 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
         }
     } else if (K) {
         const int tid = (blockIdx.x - g.node_blocks) * GB + threadIdx.x;
-        if (tid < 3 * g.n_blocks) {
-            const int blk = tid / 3, a = tid - 3 * (tid / 3);
+        const int n_lane_blocks = g.cblk ? g.n_cblk : g.n_blocks;
+        if (tid < 3 * n_lane_blocks) {
+            const int kk = tid / 3, a = tid - 3 * (tid / 3);
+            int blk = kk;
+            bool beam_done = false;  // direct path: the element kernel wrote the beam value
+            if (g.cblk) {
+                const int code = g.cblk[kk];
+                blk = code & 0x7fffffff;
+                beam_done = code < 0;
+            }
             using A = typename std::conditional<sizeof(KT) == 8, double, float>::type;
             A v0 = A(0), v1 = A(0), v2 = A(0);
-            if (g.use_beam) {
+            const int gr0 = g.B.brow[blk];
+            const int r00 = g.B.gptr[gr0];
+            const int deg0 = g.B.gptr[gr0 + 1] - r00;
+            const size_t base0 = 9 * (size_t)r00 + 3 * (size_t)(blk - r00) + (size_t)a * 3 * deg0;
+            if (beam_done && g.use_beam) {
+                v0 = A(K[base0]);
+                v1 = A(K[base0 + 1]);
+                v2 = A(K[base0 + 2]);
+            } else if (g.use_beam && !g.cblk) {
                 const int q0 = g.B.bptr[blk], q1 = g.B.bptr[blk + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.B.bsrc[q];
@@ -110,19 +128,16 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                     v2 += cc * A(gr_[2]);
                 }
             }
-            const int gr = g.B.brow[blk];
-            if (g.mass_coef != 0.0 && gr == g.B.gcol[blk]) {
-                const A m = A(g.mass_coef * g.M[3 * gr + a]);
+            // the direct path adds the mass term in the element kernel
+            if (!g.cblk && g.mass_coef != 0.0 && gr0 == g.B.gcol[blk]) {
+                const A m = A(g.mass_coef * g.M[3 * gr0 + a]);
                 if (a == 0) v0 += m;
                 else if (a == 1) v1 += m;
                 else v2 += m;
             }
-            const int r0 = g.B.gptr[gr];
-            const int deg = g.B.gptr[gr + 1] - r0;
-            const size_t base = 9 * (size_t)r0 + 3 * (size_t)(blk - r0) + (size_t)a * 3 * deg;
-            K[base] = KT(v0);
-            K[base + 1] = KT(v1);
-            K[base + 2] = KT(v2);
+            K[base0] = KT(v0);
+            K[base0 + 1] = KT(v1);
+            K[base0 + 2] = KT(v2);
         }
     }
     // residual partial of this workgroup (fixed-shape tree: deterministic)
@@ -178,7 +193,7 @@ int gather_node_blocks(int n_nodes) { return (6 * n_nodes + GB - 1) / GB; }
 
 hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st) {
     g.node_blocks = g.node_mode >= 0 ? gather_node_blocks(g.n_nodes) : 0;
-    const int kb = K ? (3 * g.n_blocks + GB - 1) / GB : 0;
+    const int kb = K ? (3 * (g.cblk ? g.n_cblk : g.n_blocks) + GB - 1) / GB : 0;
     const int nb = g.node_blocks + kb;
     if (nb <= 0) return hipSuccess;
     if (prec == 0)
@@ -195,10 +210,15 @@ hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg,
     return hipGetLastError();
 }
 
-hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st) {
+template <class T>
+static hipError_t launch_fill(size_t n, T* p, T v, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(fill_kernel<int>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, p, v);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, p, v);
     return hipGetLastError();
 }
+
+hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st) { return launch_fill<int>(n, p, v, st); }
+hipError_t launch_fill_f64(size_t n, double* p, double v, hipStream_t st) { return launch_fill<double>(n, p, v, st); }
+hipError_t launch_fill_f32(size_t n, float* p, float v, hipStream_t st) { return launch_fill<float>(n, p, v, st); }
 
 }  // namespace xk

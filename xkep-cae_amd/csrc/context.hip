@@ -217,6 +217,15 @@ int Ctx::build_pattern() {
     XK_TRY(upload(d_node_pptr, nptr));
     XK_TRY(upload(d_node_psrc, nsrc));
     if (np) XK_TRY(hipMemcpy(P.sinc, sinc_of.data(), np * 4, hipMemcpyHostToDevice));
+    if (direct) {
+        // element -> block slots, and the blocks the gather must still visit
+        XK_TRY(upload(d_eslot, kb));
+        std::vector<int> cb_list;
+        for (int b = 0; b < nblk; ++b)
+            if (pptr[b + 1] > pptr[b]) cb_list.push_back(b | (bptr[b + 1] > bptr[b] ? (int)0x80000000u : 0));
+        XK_TRY(upload(d_cblk, cb_list));
+        n_cblk = (int)cb_list.size();
+    }
     XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * SD_REC * sizeof(double)));
     XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * (prec == 0 ? 8 : 4)));
     n_sinc = ns;
@@ -254,7 +263,22 @@ GatherArgs Ctx::gather_args(int node_mode) const {
     g.fe = (const double*)d_fe.p;
     g.sd = (const double*)d_sd.p;
     g.flags = (int*)d_flags.p;
+    if (direct) {
+        g.fint = (const double*)d_fint.p;
+        g.cblk = (const int*)d_cblk.p;
+        g.n_cblk = n_cblk;
+    }
     return g;
+}
+
+hipError_t Ctx::launch_elements(const double* u, bool want_f, void* K, double mass_coef, const double* M,
+                                hipStream_t st) {
+    if (direct)
+        return launch_beam_direct(prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
+                                  (const int8_t*)d_eflag.p, (const int*)d_eslot.p, (const int*)d_gptr.p,
+                                  mass_coef, M, want_f ? (double*)d_fint.p : nullptr, K, (int*)d_flags.p, st);
+    return launch_beam(prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
+                       want_f ? (double*)d_fe.p : nullptr, K ? d_Ke.p : nullptr, (int*)d_flags.p, st);
 }
 
 }  // namespace xk
@@ -274,6 +298,8 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
         set_error("xk_ctx_create: invalid arguments");
         return -1;
     }
+    const bool force_staged = (precision & XK_OPT_STAGED) != 0;
+    precision &= ~XK_OPT_STAGED;
     if (precision != XK_PREC_FP64 && precision != XK_PREC_MIXED) {
         set_error("xk_ctx_create: unknown precision");
         return -1;
@@ -317,6 +343,38 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
         if (upload(c->d_node_bptr, ptr) != hipSuccess || upload(c->d_node_bsrc, src) != hipSuccess)
             fail("node lists");
     }
+    // element chains -> direct path
+    if (!rc) {
+        std::vector<int> deg(n_nodes, 0);
+        bool chain = n_elems > 0;
+        for (int e = 0; e < n_elems; ++e) {
+            deg[c->conn_h[2 * e]]++;
+            deg[c->conn_h[2 * e + 1]]++;
+            if (c->conn_h[2 * e] == c->conn_h[2 * e + 1]) chain = false;
+        }
+        std::vector<int8_t> ef(std::max(n_elems, 1), 0);
+        int shared = 0, deg2 = 0;
+        for (int n = 0; n < n_nodes; ++n) {
+            if (deg[n] > 2) chain = false;
+            deg2 += deg[n] == 2;
+        }
+        for (int e = 0; e < n_elems; ++e) {
+            const int n1 = c->conn_h[2 * e], n2 = c->conn_h[2 * e + 1];
+            if (e > 0 && c->conn_h[2 * (e - 1) + 1] == n1 && deg[n1] == 2) ef[e] |= 1;
+            if (e + 1 < n_elems && c->conn_h[2 * (e + 1)] == n2 && deg[n2] == 2) {
+                ef[e] |= 2;
+                ++shared;
+            }
+        }
+        if (shared != deg2) chain = false;
+        c->direct = chain && !force_staged;
+        if (c->direct) {
+            if (upload(c->d_eflag, ef) != hipSuccess) fail("element flags");
+            if (!rc && (c->d_fint.ensure(sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess ||
+                        hipMemset(c->d_fint.p, 0, sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess))
+                fail("internal force buffer");
+        }
+    }
     if (!rc && c->d_fe.ensure((size_t)std::max(n_elems, 1) * 12 * 8) != hipSuccess) fail("fe stage");
     if (!rc && c->d_Ke.ensure((size_t)std::max(n_elems, 1) * 144 * (precision == 0 ? 8 : 4)) != hipSuccess)
         fail("Ke stage");
@@ -351,6 +409,8 @@ int xk_ctx_destroy(xk_ctx* c) {
 
 int xk_ctx_precision(const xk_ctx* c) { return c ? c->prec : -1; }
 
+int xk_ctx_direct(const xk_ctx* c) { return c ? (c->direct ? 1 : 0) : -1; }
+
 int xk_beam_assemble(xk_ctx* c, const double* u, double* f_int, void* K_vals, int64_t K_cap, void* stream) {
     if (!c || !u) {
         set_error("xk_beam_assemble: null argument");
@@ -362,9 +422,8 @@ int xk_beam_assemble(xk_ctx* c, const double* u, double* f_int, void* K_vals, in
         set_error("xk_beam_assemble: K_vals holds fewer entries than the current pattern");
         return -1;
     }
-    XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
-                       f_int ? (double*)c->d_fe.p : nullptr, K_vals ? c->d_Ke.p : nullptr, (int*)c->d_flags.p,
-                       st));
+    XK_TRY(c->launch_elements(u, f_int != nullptr, K_vals, 0.0, nullptr, st));
+    // node lanes copy / sum f_int; K lanes (direct path: contact blocks only) add nothing of the pairs
     GatherArgs g = c->gather_args(f_int ? 0 : -1);
     g.out = f_int;
     g.use_beam = 1;
@@ -536,6 +595,12 @@ int xk_pairs_gather(xk_ctx* c, double* f, void* K_vals, int64_t K_cap, void* str
         return -1;
     }
     hipStream_t st = (hipStream_t)stream;
+    if (K_vals && c->direct) {
+        // the direct gather visits contact blocks only: clear the rest first
+        const size_t nnz = 9 * (size_t)c->n_blocks;
+        XK_TRY(c->prec == 0 ? launch_fill_f64(nnz, (double*)K_vals, 0.0, st)
+                            : launch_fill_f32(nnz, (float*)K_vals, 0.0f, st));
+    }
     GatherArgs g = c->gather_args(f ? 1 : -1);
     g.out = f;
     g.use_pair = 1;
@@ -601,28 +666,6 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     XK_TRY(c->d_act.ensure(sizeof(int) * (size_t)std::max(npb, 1)));
     int* act = (int*)c->d_act.p;
     void* Ke = K_vals ? c->d_Ke.p : nullptr;
-    hipEvent_t* ev = nullptr;
-    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[4 * (size_t)c->prof_calls++];
-    if (ev) {
-        // profiled: the three stages in sequence on the caller's stream
-        XK_TRY(hipEventRecord(ev[0], st));
-        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st));
-        XK_TRY(hipEventRecord(ev[1], st));
-        XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
-                           (double*)c->d_fe.p, Ke, (int*)c->d_flags.p, st));
-        XK_TRY(hipEventRecord(ev[2], st));
-    } else {
-        // element kernel on the side stream, concurrent with the pair kernel
-        XK_TRY(hipEventRecord(c->ev_fork, st));
-        XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        XK_TRY(launch_beam(c->prec, c->n_elems, (const double*)c->d_X.p, u, (const int*)c->d_conn.p, c->bc,
-                           (double*)c->d_fe.p, Ke, (int*)c->d_flags.p, c->side));
-        XK_TRY(hipEventRecord(c->ev_join, c->side));
-        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st));
-        XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
-    }
     GatherArgs g = c->gather_args(2);
     g.u = u;
     g.f_ext = f_ext;
@@ -645,6 +688,28 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     g.act_part = act;
     g.n_act_part = c->n_pairs > 0 ? npb : 0;
     g.n_active = n_active;
+    // the direct element kernel writes the beam blocks and the mass term into K
+    const double emass = c->direct ? g.mass_coef : 0.0;
+    hipEvent_t* ev = nullptr;
+    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[4 * (size_t)c->prof_calls++];
+    if (ev) {
+        // profiled: the three stages in sequence on the caller's stream
+        XK_TRY(hipEventRecord(ev[0], st));
+        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(hipEventRecord(ev[1], st));
+        XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
+        XK_TRY(hipEventRecord(ev[2], st));
+    } else {
+        // element kernel on the side stream, concurrent with the pair kernel
+        XK_TRY(hipEventRecord(c->ev_fork, st));
+        XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, c->side));
+        XK_TRY(hipEventRecord(c->ev_join, c->side));
+        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+    }
     XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st));
     if (ev) XK_TRY(hipEventRecord(ev[3], st));
     c->last_modes = modes;

@@ -72,6 +72,13 @@ struct Ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DBuf d_act;   // per-workgroup active counts of the pair kernel
+    // direct path (chain-structured meshes): element kernel writes K / f_int itself
+    bool direct = false;
+    DBuf d_eflag;   // per element: bit0 shares node1 with element e-1, bit1 shares node2 with e+1
+    DBuf d_eslot;   // per element: global block index of its 16 group blocks
+    DBuf d_cblk;    // blocks with pair contributions (bit 31: also a beam block)
+    DBuf d_fint;    // internal force written by the element kernel (6 n_nodes)
+    int n_cblk = 0;
 
     int grow_pairs(int64_t need);
     void bind_pairs();
@@ -87,6 +94,10 @@ struct Ctx {
         for (auto& b : pb) b.release();
         for (auto& b : bp) b.release();
         d_act.release();
+        d_eflag.release();
+        d_eslot.release();
+        d_cblk.release();
+        d_fint.release();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         prof_ev.clear();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -96,11 +107,21 @@ struct Ctx {
         side = nullptr;
     }
     GatherArgs gather_args(int node_mode) const;
+    // element kernel of the context's path (direct: K gets the beam blocks
+    // and the mass term; staged: element records)
+    hipError_t launch_elements(const double* u, bool want_f, void* K, double mass_coef, const double* M,
+                               hipStream_t st);
 };
 
 // launchers (one per translation unit)
 hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, const int* conn,
                        const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags, hipStream_t st);
+// direct variant: writes f_int (nullable) and the beam blocks of K (nullable,
+// pattern order, mass_coef * M added on diagonal blocks) without staging
+hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const double* u, const int* conn,
+                              const BeamConst& bc, const int8_t* eflag, const int* eslot, const int* gptr,
+                              double mass_coef, const double* M, double* f_int, void* K, int* flags,
+                              hipStream_t st);
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
                         int* act_part, hipStream_t st);
@@ -108,5 +129,7 @@ int pair_blocks(int64_t n_pairs);
 int gather_node_blocks(int n_nodes);
 hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st);
 hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st);
+hipError_t launch_fill_f64(size_t n, double* p, double v, hipStream_t st);
+hipError_t launch_fill_f32(size_t n, float* p, float v, hipStream_t st);
 
 }  // namespace xk

@@ -53,7 +53,10 @@ struct GatherArgs {
     int node_blocks;   // set by the launcher
     NodeLists L;
     BlockLists B;
-    const double* fe;  // element force staging
+    const double* fe;  // element force staging (general path)
+    const double* fint;  // element-kernel internal force (direct path), replaces fe
+    const int* cblk;   // direct path: K lanes visit only these blocks (bit 31: beam block)
+    int n_cblk;
     const double* sd;  // pair c[4], F[3] staging
     const double* u;
     const double* f_ext;
