@@ -3,7 +3,8 @@
 Workload: the 91-wire strand (layers 1+6+12+18+24+30, r = 0.4 mm, L = 80 mm,
 150 elements per wire, lay angle 12 deg), Generalized-alpha (rho_inf = 0.8)
 dynamic step, Huber contact k = 1e4 E r, Coulomb mu = 0.3, seeded
-perturbation u ~ N(0, (1e-3 r)^2).  One step = one complete fused assembly
+perturbation u ~ N(0, (1e-3 r)^2), candidates between layers (same-layer
+exclusion by layer index, so the rest state has no initial penetration).  One step = one complete fused assembly
 (narrowphase over every candidate pair, Huber, friction, element f_int and K,
 R and K composition with inertia terms, ||R||^2, #(p_n > 0)) replayed from a
 HIP graph, inputs resident in HBM.
@@ -44,13 +45,17 @@ def setup(case, states, precision, torch):
     A = importlib.import_module("xkep-cae_amd.assembly")
     T = importlib.import_module("xkep-cae_amd.time_integration")
     E = importlib.import_module("xkep-cae_amd.elements")
+    K = importlib.import_module("xkep-cae_amd.contact")
     if len(states) == 1:
         st = states[0]
         X, conn, radii, u, f_ext, fixed = (st.mesh.node_coords, st.mesh.connectivity, st.mesh.radii, st.u,
                                            st.f_ext, st.fixed_dofs)
+        layer = st.elem_layer
     else:
         b = wl.concat_states(states)
         X, conn, radii, u, f_ext, fixed = b.node_coords, b.connectivity, b.radii, b.u, b.f_ext, b.fixed_dofs
+        layer = b.elem_layer
+    cfg = K.ContactConfig(mu=case.mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer))
     sec = C.circle_section(case.wire_radius, case.E, case.nu)
     ctx = C.DeviceContext(X, conn, sec, radii=radii, precision=precision)
     Md = E.ULCRBeamAssembler(X, conn, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
@@ -59,7 +64,7 @@ def setup(case, states, precision, torch):
     ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
     ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
     asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
-                           f_ext=f_ext, fixed_dofs=fixed, time=ga, dt=dt)
+                           f_ext=f_ext, fixed_dofs=fixed, config=cfg, time=ga, dt=dt)
     asm.begin_increment(np.zeros(ctx.ndof))   # set-up broadphase on the reference configuration
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -94,21 +99,56 @@ def profile_stages(asm, ctx, torch, calls):
     return ms / max(n, 1)
 
 
-STAGES = ["pairs", "elements", "gather"]
+STAGES = ["pairs", "elements", "gather", "finish"]
+KERNELS = {"pairs": "xk::pair_kernel<float>", "elements": "xk::beam_direct_kernel<float>",
+           "gather": "xk::gather_kernel<float, float, float>", "finish": "xk::finish_kernel"}
+TRAFFIC_TABLE = os.path.join(HERE, "profiles", "assembly_kernels_91wire_hbm_traffic.md")
 
 
-def stage_bytes(ctx, asm, n_pair, precision):
-    """Algorithmic (compulsory) bytes per launch of each stage: pairs N_pair x 216;
-    elements X + u per node and conn per element; gather u, f_ext, R per node
-    plus the K values."""
+def beam_nnz(conn) -> int:
+    """Entries of the beam part of the tangent pattern: distinct 3x3 blocks over
+    the DOF groups (2n, 2n + 1) of every element's two nodes, times 9."""
+    conn = np.asarray(conn, np.int64)
+    g = np.stack([2 * conn[:, 0], 2 * conn[:, 0] + 1, 2 * conn[:, 1], 2 * conn[:, 1] + 1], axis=1)
+    keys = (g[:, :, None] << 32) | g[:, None, :]
+    return 9 * int(np.unique(keys.ravel()).size)
+
+
+def stage_bytes(ctx, asm, n_pair, precision, nnz_beam):
+    """Algorithmic bytes per launch of each kernel (compulsory inputs and outputs,
+    nothing counted for staging records or re-reads):
+    pairs     N_pair x 216 (read ids, previous frame, history, status 84 B;
+              write s, t, gap, frame, p_n, history, status 132 B);
+    elements  per node X 24 + u 48 read, f_int 48 written; per element conn 8;
+              the beam values of K written (nnz_beam x s_K);
+    gather    per node u, f_ext, f_int, M, u_pred, v_pred, acc_old read and R,
+              acc, vel written (10 x 48 B); the contact-only values of K written;
+    finish    one partial sum per gather workgroup and one count per pair workgroup."""
     nn, ne = ctx.n_nodes, ctx.n_elems
     s_k = 8 if precision == "fp64" else 4
     nnz = asm.K.numel()
     return {
         "pairs": n_pair * 216,
-        "elements": nn * (24 + 48) + ne * 8,
-        "gather": nn * (48 + 48 + 48) + nnz * s_k,
+        "elements": nn * (24 + 48 + 48) + ne * 8 + nnz_beam * s_k,
+        "gather": nn * 10 * 48 + (nnz - nnz_beam) * s_k,
+        "finish": (6 * nn + 255) // 256 * 8 + (n_pair + 255) // 256 * 4,
     }
+
+
+def traffic_bytes(kernel: str):
+    """HBM bytes per dispatch of `kernel` at the largest grid in the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy kernel),
+    or None if the summary has no row for it."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    best = None
+    for line in open(TRAFFIC_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 8 and cells[0] == kernel and cells[1].isdigit():
+            grid, mb = int(cells[1]), float(cells[7])
+            if best is None or grid > best[0]:
+                best = (grid, mb)
+    return None if best is None else int(best[1] * 1e6)
 
 
 def cpu_baseline(case, st, budget_s=15.0):
@@ -119,9 +159,10 @@ def cpu_baseline(case, st, budget_s=15.0):
     from oracle import beam as obeam
     from oracle import contact as ocon
 
+    wl = importlib.import_module("xkep-cae_amd.workloads")
     m = st.mesh
     osec = obeam.circle_section(case.wire_radius, case.E, case.nu)
-    cfg = ocon.ContactConfig(mu=case.mu)
+    cfg = ocon.ContactConfig(mu=case.mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(st.elem_layer))
     prob = oasm.Problem(m.node_coords, m.connectivity, osec, case.k_pen, case.smoothing_delta, 0.5, case.mu, cfg)
     with threadpool_limits(1):
         op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, cfg)
@@ -156,7 +197,7 @@ def run(label, case, states, prec, torch, steps, warmup):
     log(f"[bench] {label}: elems={ctx.n_elems} pairs={n_pair} nnzK={asm.K.numel()} incr {t_inc * 1e3:.1f} ms | "
         f"{ms * 1e3:.1f} us/step (graph) wall {ms_wall * 1e3:.1f} | stages us {np.round(stages * 1e3, 1).tolist()} | "
         f"{n_eval / (ms * 1e-3):.3e} evals/s")
-    out = dict(ctx=ctx, asm=asm, n_pair=n_pair, n_eval=n_eval, ms=ms, ms_wall=ms_wall, stages=stages,
+    out = dict(ctx=ctx, asm=asm, n_pair=n_pair, n_elem=ctx.n_elems, n_eval=n_eval, ms=ms, ms_wall=ms_wall, stages=stages,
                t_inc=t_inc, value=n_eval / (ms * 1e-3))
     return out
 
@@ -200,13 +241,18 @@ def main():
     states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
     batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup)
 
-    sb = stage_bytes(batched["ctx"], batched["asm"], batched["n_pair"], "mixed")
+    nnz_beam = beam_nnz(batched["ctx"].conn_np)
+    sb = stage_bytes(batched["ctx"], batched["asm"], batched["n_pair"], "mixed", nnz_beam)
     dom = int(np.argmax(batched["stages"]))
     dom_ms = float(batched["stages"][dom])
     achieved = sb[STAGES[dom]] / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": STAGES[dom],
-            "kernel_bytes": int(sb[STAGES[dom]]), "kernel_us": round(dom_ms * 1e3, 2)}
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes(KERNELS[STAGES[dom]]),
+            "kernel": KERNELS[STAGES[dom]], "algorithmic_bytes": int(sb[STAGES[dom]]),
+            "kernel_us": round(dom_ms * 1e3, 2),
+            "all_kernels": {k: {"us": round(float(t) * 1e3, 2), "algorithmic_bytes": int(sb[k]),
+                                "GB_s": round(sb[k] / (float(t) * 1e-3) / 1e9, 1)}
+                            for k, t in zip(STAGES, batched["stages"])}}
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (FP64 NumPy oracle, 1 thread)")
@@ -234,7 +280,7 @@ def main():
         "single_instance": {
             "fp64_plain": {"value": round(plain["value"], 1), "us_per_step": round(plain["ms"] * 1e3, 2),
                            "stages_us": np.round(plain["stages"] * 1e3, 2).tolist(),
-                           "n_elem": 13650, "n_pair": plain["n_pair"]},
+                           "n_elem": plain["n_elem"], "n_pair": plain["n_pair"]},
             "fp32_mixed": {"value": round(single["value"], 1), "us_per_step": round(single["ms"] * 1e3, 2),
                            "stages_us": np.round(single["stages"] * 1e3, 2).tolist()},
         },

@@ -35,8 +35,9 @@ enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1 };
 
 /* OR-ed into xk_ctx_create's precision: use the staged element path even when
  * the connectivity is made of element chains (the direct path is chosen
- * automatically for chains: every node shared by two elements e and e+1 as
- * node 2 of e and node 1 of e+1, no node in more than two elements). */
+ * automatically for chains: every element e joins nodes n and n + 1, every
+ * node shared by two elements e and e+1 as node 2 of e and node 1 of e+1, no
+ * node in more than two elements). */
 enum { XK_OPT_STAGED = 256 };
 
 /* status flags returned (bitwise OR) by assembly calls */
@@ -185,13 +186,13 @@ int xk_status(xk_ctx* ctx, int* flags);
 
 /* Per-stage timing of xk_assemble_iteration with HIP events recorded on the
  * call's stream: begin allocates events for up to max_calls calls; end
- * synchronises and returns the summed milliseconds of the three stages
- * [pair kernel, element kernel, gather (residual + tangent + norm)] and the
- * number of calls recorded.  Profiled calls run the three stages in sequence
+ * synchronises and returns in stage_ms4 the summed milliseconds of the four
+ * stages [pair kernel, element kernel, gather kernel (residual + tangent),
+ * finish kernel (norm + active count)] and the number of calls recorded.  Profiled calls run the three stages in sequence
  * (unprofiled calls overlap the element and pair kernels on two streams).
  * Not for use under graph capture. */
 int xk_profile_begin(xk_ctx* ctx, int max_calls);
-int xk_profile_end(xk_ctx* ctx, double* stage_ms3, int* n_calls);
+int xk_profile_end(xk_ctx* ctx, double* stage_ms4, int* n_calls);
 
 #ifdef __cplusplus
 }

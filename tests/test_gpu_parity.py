@@ -234,14 +234,19 @@ def test_non_chain_connectivity_takes_the_staged_path(gpu):
     assert int(nact.item()) == ref.n_active
 
 
-def test_mixed_precision_residual_and_active_set_at_91_wires(gpu):
+@pytest.mark.parametrize("layers", [True, False], ids=["between_layers", "all_pairs"])
+def test_mixed_precision_residual_and_active_set_at_91_wires(gpu, layers):
     """Full-size 91-wire workload: FP64 kernels against the oracle (1e-10) and
-    the mixed form against FP64 (residual 1e-6, identical active set)."""
+    the mixed form against FP64 (residual 1e-6, identical active set).  With
+    the same-layer filter (the benchmark's candidate set) and without it (every
+    non-adjacent pair, including overlapping elements of one wire: many more
+    active pairs)."""
     wl = pkg("workloads")
-    C, A, T, E = pkg("context"), pkg("assembly"), pkg("time_integration"), pkg("elements")
+    C, A, T, E, Kc = pkg("context"), pkg("assembly"), pkg("time_integration"), pkg("elements"), pkg("contact")
     case = wl.C3
     st = wl.make_state(case)
     m = st.mesh
+    lmap = wl.layer_map(st.elem_layer) if layers else None
     s = section(case.wire_radius)
     Md = E.ULCRBeamAssembler(m.node_coords, m.connectivity, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y,
                              s.kappa_z).lumped_mass_diag(case.rho)
@@ -252,14 +257,15 @@ def test_mixed_precision_residual_and_active_set_at_91_wires(gpu):
         ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
         ga.predict_device(ctx.torch.zeros(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev), dt)
         asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
-                               f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, time=ga, dt=dt)
+                               f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, time=ga, dt=dt,
+                               config=Kc.ContactConfig(mu=case.mu, exclude_same_layer=layers, elem_layer_map=lmap))
         asm.begin_increment(np.zeros(ctx.ndof))  # candidates on the reference configuration
         asm.begin_increment(st.u)
         R, Kv, res2, nact = asm.assemble(st.u)
         outs[prec] = (R.cpu().numpy(), asm.K_csr(), int(nact.item()), ctx.get_pairs()["status"].copy())
     # oracle
     osec = obeam.circle_section(case.wire_radius, 200e3, 0.3)
-    cfg = ocon.ContactConfig(mu=case.mu)
+    cfg = ocon.ContactConfig(mu=case.mu, exclude_same_layer=layers, elem_layer_map=lmap)
     prob = oasm.Problem(m.node_coords, m.connectivity, osec, case.k_pen, case.smoothing_delta, 0.5, case.mu, cfg)
     op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, cfg)
     x = m.node_coords + st.u.reshape(-1, 6)[:, :3]

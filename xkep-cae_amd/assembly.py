@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib as L
-from .contact import ContactConfig
+from .contact import ContactConfig, layer_array
 from .context import DeviceContext
 from .time_integration import GeneralizedAlphaProcess
 
@@ -37,6 +37,7 @@ class StrandAssembly:
         self.k_t_ratio = float(k_t_ratio)
         self.mu = float(mu)
         self.cfg = config or ContactConfig(mu=mu, k_t_ratio=k_t_ratio)
+        self._layer = layer_array(self.cfg, ctx.n_elems)
         n = ctx.ndof
         dev = ctx.dev
         self.f_ext = ctx.to_dev(np.zeros(n) if f_ext is None else f_ext)
@@ -74,10 +75,7 @@ class StrandAssembly:
         x = self.deformed_coords()
         cfg = self.cfg
         if detect:
-            layer = None
-            if cfg.exclude_same_layer and cfg.elem_layer_map is not None:
-                layer = np.array([cfg.elem_layer_map.get(e, -1) for e in range(self.ctx.n_elems)], np.int32)
-            self.ctx.detect_candidates(x, margin=margin, cell_size=cell_size, elem_layer=layer,
+            self.ctx.detect_candidates(x, margin=margin, cell_size=cell_size, elem_layer=self._layer,
                                        midpoint_prescreening=cfg.midpoint_prescreening,
                                        prescreening_margin=cfg.prescreening_margin)
         self.ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, coords=x, g_on=cfg.g_on, g_off=cfg.g_off)

@@ -54,6 +54,28 @@ class ContactConfig:
     smoothing_delta: float = 0.0
 
 
+def layer_array(cfg: ContactConfig, n_elems: int):
+    """Per-element layer ids (int32, -1 = none) for the same-layer filter of
+    DetectCandidatesProcess (contact/_manager_process.py:201-212), or None when
+    the filter is off.  ``elem_layer_map`` is the reference's dict; an integer
+    array of length n_elems is accepted as well."""
+    if not (cfg.exclude_same_layer and cfg.elem_layer_map is not None):
+        return None
+    m = cfg.elem_layer_map
+    if isinstance(m, dict):
+        out = np.full(n_elems, -1, np.int32)
+        if m:
+            k = np.fromiter(m.keys(), np.int64, len(m))
+            v = np.fromiter(m.values(), np.int64, len(m))
+            ok = (k >= 0) & (k < n_elems)
+            out[k[ok]] = v[ok]
+        return out
+    a = np.asarray(m)
+    if a.shape != (n_elems,):
+        raise ValueError("elem_layer_map array must have one entry per element")
+    return a.astype(np.int32)
+
+
 @dataclass(frozen=True)
 class ContactState:
     """Read-only snapshot of one pair's ``_ContactStateOutput``."""
@@ -196,9 +218,7 @@ class DetectCandidatesProcess:
             if not np.array_equal(r, ctx.radii_np):
                 raise ValueError("radii differ from the device context's")
         coords = ctx.to_dev(inp.node_coords)
-        layer = None
-        if cfg.exclude_same_layer and cfg.elem_layer_map is not None:
-            layer = np.array([cfg.elem_layer_map.get(e, -1) for e in range(ctx.n_elems)], np.int32)
+        layer = layer_array(cfg, ctx.n_elems)
         ctx.detect_candidates(coords, margin=inp.margin, cell_size=inp.cell_size, elem_layer=layer,
                               midpoint_prescreening=cfg.midpoint_prescreening,
                               prescreening_margin=cfg.prescreening_margin)

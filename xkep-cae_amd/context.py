@@ -297,8 +297,9 @@ class DeviceContext:
         L.check(L.lib().xk_profile_begin(self.h, int(max_calls)), "xk_profile_begin")
 
     def profile_end(self):
-        """(stage_ms[3] summed over calls, n_calls): pair kernel, element kernel, gather."""
-        ms = np.zeros(3)
+        """(stage_ms[4] summed over calls, n_calls): pair kernel, element kernel,
+        gather kernel, finish kernel."""
+        ms = np.zeros(4)
         n = C.c_int()
         L.check(L.lib().xk_profile_end(self.h, ms.ctypes.data, C.byref(n)), "xk_profile_end")
         return ms, n.value

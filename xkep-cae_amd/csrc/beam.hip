@@ -16,13 +16,14 @@
 // tangent is carried as three 3x3 matrices X_b, Y_b, Z_b.
 //
 // Two output paths share the element math (beam_core):
-//  * direct (meshes made of element chains, every strand): a workgroup takes
-//    64 consecutive elements and owns the first 63.  Each owner writes its
-//    off-diagonal node blocks straight into K, and the diagonal block and the
-//    internal force of its node 2 summed with the next element's node-1 part
-//    (exchanged through LDS, element order), plus the lumped-mass term; the
-//    64th lane only supplies that node-1 part (it is owned by the next
-//    workgroup).  Nothing is staged.
+//  * direct (meshes made of element chains with consecutive node numbers,
+//    every strand): a workgroup takes 64 consecutive elements and owns the
+//    first 63, i.e. the node 2 of each.  The internal force of an owned node
+//    and its tangent rows (the element's node-2 blocks plus the next
+//    element's node-1 blocks, exchanged through LDS in element order, plus
+//    the lumped-mass term) go straight into f_int and K, the rows as
+//    contiguous runs stored by consecutive lanes; the 64th lane only supplies
+//    its node-1 part (it is owned by the next workgroup).  Nothing is staged.
 //  * staged (any connectivity): per-element records
 //    fe_stage[12 e + i] (global-frame force) and
 //    Ke_stage[144 e + 9 lb + 3 a + b] (symmetrised 3x3 group blocks,
@@ -387,10 +388,21 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     }
 }
 
-// Direct path (chains): eflag bit0 = node 1 shared with element e-1, bit1 =
-// node 2 shared with element e+1 (each such node has exactly these two
-// elements).  eslot[16 e + lb] = global block index of group block lb.
+// Direct path (chains with consecutive node numbers: element e joins n and
+// n + 1): eflag bit0 = node 1 shared with element e-1, bit1 = node 2 shared
+// with element e+1 (each such node has exactly these two elements).
+// eslot[16 e + lb] = global block index of group block lb.
+//
+// The tangent rows of a node n of such a chain hold its beam entries as one
+// contiguous run of the scalar CSR row: column groups 2(n-1) .. 2(n+1)+1, 18
+// entries (12 at a chain end).  A workgroup builds the runs of the rows of
+// its owned nodes in LDS (one row group -- translations, then rotations -- at
+// a time): each lane writes its element's node-2 rows, then adds its node-1
+// rows into the previous lane's image (element order, so the sums are fixed),
+// and the workgroup stores the runs with consecutive lanes on consecutive
+// addresses.  The first node of a chain is written by its element directly.
 constexpr int DIRECT_OWN = BEAM_BLOCK - 1;
+constexpr int RUN = 18;   // entries of an interior row run
 
 template <class TT>
 __global__ void __launch_bounds__(BEAM_BLOCK)
@@ -406,27 +418,16 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
     double fe[12];
     M3<TT> Xb[4], Yb[4], Zb[4];
     beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, K != nullptr, flags);
-    // publish the node-1 part for the previous lane
+    const int fl = eflag[e];
+    const bool prv = live && (fl & 1), nxt = (fl & 2) != 0;
+    const bool own = live && l < DIRECT_OWN;
+    const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
+    // node-1 force part for the previous lane
     __shared__ double fsh[BEAM_BLOCK * 6];
-    __shared__ TT ksh[BEAM_BLOCK * 36];
 #pragma unroll
     for (int d = 0; d < 6; ++d) fsh[l * 6 + d] = fe[d];
-    if (K) {
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) ksh[l * 36 + (2 * b + c) * 9 + 3 * i + j] = ks(Xb, Yb, Zb, b, c, i, j);
-    }
     __syncthreads();
-    if (!live || l >= DIRECT_OWN) return;
-    const int fl = eflag[e];
-    const bool prv = fl & 1, nxt = fl & 2;
-    const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
-    if (fint) {
+    if (fint && own) {
 #pragma unroll
         for (int d = 0; d < 6; ++d) fint[6 * (size_t)n2 + d] = nxt ? fe[6 + d] + fsh[(l + 1) * 6 + d] : fe[6 + d];
         if (!prv) {
@@ -434,30 +435,83 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
             for (int d = 0; d < 6; ++d) fint[6 * (size_t)n1 + d] = fe[d];
         }
     }
-    if (!K) return;
-    const int grp[4] = {2 * n1, 2 * n1 + 1, 2 * n2, 2 * n2 + 1};
+    if (!K) return;  // uniform across the launch
+
+    __shared__ TT img[BEAM_BLOCK * 3 * RUN];   // lane l: 3 rows of its node-2 row group
+    __shared__ int2 dst[BEAM_BLOCK * 3];       // per image row: CSR offset (-1: not owned), run length
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int gr = grp[b];
-        const int r0 = gptr[gr];
-        const int deg = gptr[gr + 1] - r0;
+    for (int p = 0; p < 2; ++p) {
+        // own node-2 rows: columns [node 1 | node 2 | next element's node 2]
+        {
+            const int gr = 2 * n2 + p;
+            const int r0 = gptr[gr], deg = gptr[gr + 1] - r0;
+            const int b0 = eslot[16 * (size_t)e + 4 * (2 + p)];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool diag1 = b < 2 && c < 2;
-            const bool diag2 = b >= 2 && c >= 2;
-            if (diag1 && prv) continue;  // owned by the previous element
-            const int blk = eslot[16 * (size_t)e + 4 * b + c];
-            const size_t base = 9 * (size_t)r0 + 3 * (size_t)(blk - r0);
+            for (int i = 0; i < 3; ++i) {
+                TT* row = img + (l * 3 + i) * RUN;
+                const TT m = mass_coef != 0.0 ? TT(mass_coef * M[3 * (size_t)gr + i]) : TT(0);
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    TT v = ks(Xb, Yb, Zb, b, c, i, j);
-                    if (diag2 && nxt) v += ksh[(l + 1) * 36 + (2 * (b - 2) + (c - 2)) * 9 + 3 * i + j];
-                    if ((diag1 || diag2) && b == c && i == j && mass_coef != 0.0)
-                        v += TT(mass_coef * M[3 * (size_t)gr + i]);
-                    K[base + (size_t)i * 3 * deg + j] = v;
-                }
+                    for (int j = 0; j < 3; ++j) {
+                        TT v = ks(Xb, Yb, Zb, 2 + p, c, i, j);
+                        if (c == 2 + p && j == i) v += m;
+                        row[3 * c + j] = v;
+                    }
+#pragma unroll
+                for (int k = 12; k < RUN; ++k) row[k] = TT(0);
+                dst[l * 3 + i] = int2{own ? 9 * r0 + i * 3 * deg + 3 * (b0 - r0) : -1, nxt ? RUN : 12};
+            }
+        }
+        __syncthreads();
+        // node-1 rows into the previous lane's node-2 rows (same node)
+        if (prv && l > 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                TT* row = img + ((l - 1) * 3 + i) * RUN + 6;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) row[3 * c + j] += ks(Xb, Yb, Zb, p, c, i, j);
+            }
+        }
+        __syncthreads();
+        // entry t = l + 64 it of the owned image: row rr = t / RUN, column k = t % RUN
+        {
+            constexpr int TOTAL = DIRECT_OWN * 3 * RUN;
+            constexpr int STEP_R = BEAM_BLOCK / RUN, STEP_K = BEAM_BLOCK % RUN;
+            int rr = l / RUN, k = l % RUN;
+#pragma unroll 6
+            for (int t = l; t < TOTAL; t += BEAM_BLOCK) {
+                const int2 d = dst[rr];
+                if (d.x >= 0 && k < d.y) K[(size_t)d.x + k] = img[t];
+                rr += STEP_R;
+                k += STEP_K;
+                if (k >= RUN) { k -= RUN; ++rr; }
+            }
+        }
+        __syncthreads();
+    }
+    // first node of a chain: its rows hold this element's node-1 blocks only
+    if (own && !prv) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int gr = 2 * n1 + p;
+            const int r0 = gptr[gr], deg = gptr[gr + 1] - r0;
+            const int b0 = eslot[16 * (size_t)e + 4 * p];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const size_t d = 9 * (size_t)r0 + (size_t)i * 3 * deg + 3 * (size_t)(b0 - r0);
+                const TT m = mass_coef != 0.0 ? TT(mass_coef * M[3 * (size_t)gr + i]) : TT(0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        TT v = ks(Xb, Yb, Zb, p, c, i, j);
+                        if (c == p && j == i) v += m;
+                        K[d + 3 * c + j] = v;
+                    }
+            }
         }
     }
 }

@@ -6,10 +6,14 @@
 // Every output entry is summed by one lane over a contribution list fixed at
 // pattern-build time (elements in element order, then pairs in pair order), so
 // results are bitwise reproducible run to run (no floating-point atomics).
-// Lanes [0, 6 n_nodes) own one residual DOF each; the following lanes own one
-// row of one 3x3 tangent block each.  The residual partial sums and the pair
-// kernel's per-workgroup active counts are reduced by a one-workgroup finish
-// kernel (no atomics, no memset, capturable in a HIP graph).
+// Lanes [0, 6 n_nodes) own one residual DOF each.  The following lanes own
+// tangent entries: on the direct path (element kernel already wrote the beam
+// blocks) one lane per 3x3 block with pair contributions, driven by a 16-byte
+// descriptor {CSR offset, row stride, list begin, list end | beam bit}; on the
+// staged path one row of one 3x3 block of the whole pattern.  The residual
+// partial sums and the pair kernel's per-workgroup active counts are reduced by
+// a one-workgroup finish kernel (no atomics, no memset, capturable in a HIP
+// graph).
 //
 // Residual follows ContactForceAssemblyProcess
 // (xkep_cae/contact/solver/_newton_steps.py:124-129) and the dynamic steps
@@ -82,29 +86,52 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
             }
             g.out[tid] = r;
         }
-    } else if (K) {
-        const int tid = (blockIdx.x - g.node_blocks) * GB + threadIdx.x;
-        const int n_lane_blocks = g.cblk ? g.n_cblk : g.n_blocks;
-        if (tid < 3 * n_lane_blocks) {
-            const int kk = tid / 3, a = tid - 3 * (tid / 3);
-            int blk = kk;
-            bool beam_done = false;  // direct path: the element kernel wrote the beam value
-            if (g.cblk) {
-                const int code = g.cblk[kk];
-                blk = code & 0x7fffffff;
-                beam_done = code < 0;
+    } else if (K && g.direct) {
+        // direct path: one lane per block with pair contributions
+        const int kk = (blockIdx.x - g.node_blocks) * GB + threadIdx.x;
+        const int4 dsc = kk < g.n_cdesc ? g.cdesc[kk] : int4{0, 0, 0, 0};
+        const bool beam = dsc.w < 0 && g.use_beam;
+        // without pair terms a beam block already holds its final value;
+        // contact-only blocks are written (zero) either way
+        if (kk < g.n_cdesc && (g.use_pair || !beam)) {
+            using A = typename std::conditional<sizeof(KT) == 8, double, float>::type;
+            const size_t base = (size_t)(unsigned)dsc.x;
+            const size_t rs = (size_t)dsc.y;
+            const int q0 = dsc.z, q1 = g.use_pair ? (dsc.w & 0x7fffffff) : q0;
+            A v[9];
+            if (beam) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) v[3 * a + b] = A(K[base + a * rs + b]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[k] = A(0);
             }
+            for (int q = q0; q < q1; ++q) {
+                const int code = g.B.psrc[q];
+                const GT* r = sg + SG_REC * (size_t)(code >> 4);
+                const A cc = A(r[SG_C + ((code >> 2) & 3)]) * A(r[SG_C + (code & 3)]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[k] += cc * A(r[k]);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) K[base + a * rs + b] = KT(v[3 * a + b]);
+        }
+    } else if (K) {
+        // staged path: one lane per row of every block of the pattern
+        const int tid = (blockIdx.x - g.node_blocks) * GB + threadIdx.x;
+        if (tid < 3 * g.n_blocks) {
+            const int blk = tid / 3, a = tid - 3 * (tid / 3);
             using A = typename std::conditional<sizeof(KT) == 8, double, float>::type;
             A v0 = A(0), v1 = A(0), v2 = A(0);
             const int gr0 = g.B.brow[blk];
             const int r00 = g.B.gptr[gr0];
             const int deg0 = g.B.gptr[gr0 + 1] - r00;
             const size_t base0 = 9 * (size_t)r00 + 3 * (size_t)(blk - r00) + (size_t)a * 3 * deg0;
-            if (beam_done && g.use_beam) {
-                v0 = A(K[base0]);
-                v1 = A(K[base0 + 1]);
-                v2 = A(K[base0 + 2]);
-            } else if (g.use_beam && !g.cblk) {
+            if (g.use_beam) {
                 const int q0 = g.B.bptr[blk], q1 = g.B.bptr[blk + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.B.bsrc[q];
@@ -119,17 +146,14 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                 const int q0 = g.B.pptr[blk], q1 = g.B.pptr[blk + 1];
                 for (int q = q0; q < q1; ++q) {
                     const int code = g.B.psrc[q];
-                    const size_t p = (size_t)(code >> 4);
-                    const double* rec = g.sd + SD_REC * p;
-                    const A cc = A(rec[(code >> 2) & 3] * rec[code & 3]);
-                    const GT* gr_ = sg + SG_REC * p + 3 * a;
-                    v0 += cc * A(gr_[0]);
-                    v1 += cc * A(gr_[1]);
-                    v2 += cc * A(gr_[2]);
+                    const GT* r = sg + SG_REC * (size_t)(code >> 4);
+                    const A cc = A(r[SG_C + ((code >> 2) & 3)]) * A(r[SG_C + (code & 3)]);
+                    v0 += cc * A(r[3 * a]);
+                    v1 += cc * A(r[3 * a + 1]);
+                    v2 += cc * A(r[3 * a + 2]);
                 }
             }
-            // the direct path adds the mass term in the element kernel
-            if (!g.cblk && g.mass_coef != 0.0 && gr0 == g.B.gcol[blk]) {
+            if (g.mass_coef != 0.0 && gr0 == g.B.gcol[blk]) {
                 const A m = A(g.mass_coef * g.M[3 * gr0 + a]);
                 if (a == 0) v0 += m;
                 else if (a == 1) v1 += m;
@@ -156,19 +180,27 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
 // kernel's workgroup counts, one workgroup, fixed summation order.  (A
 // last-workgroup reduction inside the gather would need a device-scope release
 // fence per workgroup, which on a multi-XCD part writes back L2 each time.)
-__global__ void __launch_bounds__(GB) finish_kernel(int n_part, const double* __restrict__ partial,
+constexpr int FB = 1024;  // finish kernel lanes
+
+__global__ void __launch_bounds__(FB) finish_kernel(int n_part, const double* __restrict__ partial,
                                                     int n_act, const int* __restrict__ act_part,
                                                     double* __restrict__ res_norm2, int* __restrict__ n_active) {
-    __shared__ double red[GB];
-    __shared__ int ired[GB];
+    __shared__ double red[FB];
+    __shared__ int ired[FB];
+    // each lane sums its strided entries in index order, four loads in flight
     double s = 0.0;
     int a = 0;
-    for (int k = threadIdx.x; k < n_part; k += GB) s += partial[k];
-    for (int k = threadIdx.x; k < n_act; k += GB) a += act_part[k];
+    int k = threadIdx.x;
+    for (; k + 3 * FB < n_part; k += 4 * FB) {
+        const double p0 = partial[k], p1 = partial[k + FB], p2 = partial[k + 2 * FB], p3 = partial[k + 3 * FB];
+        s = (((s + p0) + p1) + p2) + p3;
+    }
+    for (; k < n_part; k += FB) s += partial[k];
+    for (k = threadIdx.x; k < n_act; k += FB) a += act_part[k];
     red[threadIdx.x] = s;
     ired[threadIdx.x] = a;
     __syncthreads();
-    for (int w = GB / 2; w > 0; w >>= 1) {
+    for (int w = FB / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             red[threadIdx.x] += red[threadIdx.x + w];
             ired[threadIdx.x] += ired[threadIdx.x + w];
@@ -191,9 +223,11 @@ __global__ void fill_kernel(size_t n, T* p, T v) {
 
 int gather_node_blocks(int n_nodes) { return (6 * n_nodes + GB - 1) / GB; }
 
-hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st) {
+hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st,
+                         hipEvent_t between) {
+    if (K && g.use_pair && !sg) return hipErrorInvalidValue;  // pair terms need the tangent records
     g.node_blocks = g.node_mode >= 0 ? gather_node_blocks(g.n_nodes) : 0;
-    const int kb = K ? (3 * (g.cblk ? g.n_cblk : g.n_blocks) + GB - 1) / GB : 0;
+    const int kb = K ? ((g.direct ? g.n_cdesc : 3 * g.n_blocks) + GB - 1) / GB : 0;
     const int nb = g.node_blocks + kb;
     if (nb <= 0) return hipSuccess;
     if (prec == 0)
@@ -204,8 +238,9 @@ hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg,
                            (const float*)sg, (float*)K);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
     if (g.partial && (g.res_norm2 || g.n_active))
-        hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(GB), 0, st, g.node_blocks, (const double*)g.partial,
+        hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(FB), 0, st, g.node_blocks, (const double*)g.partial,
                            g.n_act_part, g.act_part, g.res_norm2, g.n_active);
     return hipGetLastError();
 }

@@ -3,6 +3,7 @@
 // include/xkamd.h.
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -220,11 +221,22 @@ int Ctx::build_pattern() {
     if (direct) {
         // element -> block slots, and the blocks the gather must still visit
         XK_TRY(upload(d_eslot, kb));
-        std::vector<int> cb_list;
-        for (int b = 0; b < nblk; ++b)
-            if (pptr[b + 1] > pptr[b]) cb_list.push_back(b | (bptr[b + 1] > bptr[b] ? (int)0x80000000u : 0));
-        XK_TRY(upload(d_cblk, cb_list));
-        n_cblk = (int)cb_list.size();
+        std::vector<int> desc;
+        for (int b = 0; b < nblk; ++b) {
+            if (pptr[b + 1] == pptr[b]) continue;
+            const int r = brow[b], r0 = gptr_h[r], deg = gptr_h[r + 1] - r0;
+            const int64_t base = 9 * (int64_t)r0 + 3 * (int64_t)(b - r0);
+            if (base + 2 * 3 * (int64_t)deg + 3 > INT32_MAX) {
+                set_error("tangent pattern exceeds 2^31 entries");
+                return -1;
+            }
+            desc.push_back((int)base);
+            desc.push_back(3 * deg);
+            desc.push_back(pptr[b]);
+            desc.push_back(pptr[b + 1] | (bptr[b + 1] > bptr[b] ? (int)0x80000000u : 0));
+        }
+        XK_TRY(upload(d_cdesc, desc));
+        n_cdesc = (int)(desc.size() / 4);
     }
     XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * SD_REC * sizeof(double)));
     XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * (prec == 0 ? 8 : 4)));
@@ -265,8 +277,9 @@ GatherArgs Ctx::gather_args(int node_mode) const {
     g.flags = (int*)d_flags.p;
     if (direct) {
         g.fint = (const double*)d_fint.p;
-        g.cblk = (const int*)d_cblk.p;
-        g.n_cblk = n_cblk;
+        g.direct = 1;
+        g.cdesc = (const int4*)d_cdesc.p;
+        g.n_cdesc = n_cdesc;
     }
     return g;
 }
@@ -350,7 +363,9 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
         for (int e = 0; e < n_elems; ++e) {
             deg[c->conn_h[2 * e]]++;
             deg[c->conn_h[2 * e + 1]]++;
-            if (c->conn_h[2 * e] == c->conn_h[2 * e + 1]) chain = false;
+            // the direct element kernel stores each node's row run contiguously,
+            // which needs element e to join nodes n and n + 1
+            if (c->conn_h[2 * e + 1] != c->conn_h[2 * e] + 1) chain = false;
         }
         std::vector<int8_t> ef(std::max(n_elems, 1), 0);
         int shared = 0, deg2 = 0;
@@ -691,7 +706,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     // the direct element kernel writes the beam blocks and the mass term into K
     const double emass = c->direct ? g.mass_coef : 0.0;
     hipEvent_t* ev = nullptr;
-    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[4 * (size_t)c->prof_calls++];
+    if (c->prof_calls < c->prof_max) ev = &c->prof_ev[PROF_EV * (size_t)c->prof_calls++];
     if (ev) {
         // profiled: the three stages in sequence on the caller's stream
         XK_TRY(hipEventRecord(ev[0], st));
@@ -710,8 +725,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
                             (double*)c->d_sd.p, c->d_sg.p, act, st));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
-    XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st));
-    if (ev) XK_TRY(hipEventRecord(ev[3], st));
+    XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st, ev ? ev[3] : nullptr));
+    if (ev) XK_TRY(hipEventRecord(ev[4], st));
     c->last_modes = modes;
     return 0;
 }
@@ -719,7 +734,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
 int xk_profile_begin(xk_ctx* c, int max_calls) {
     if (!c || max_calls < 0) return -1;
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
-    c->prof_ev.assign(4 * (size_t)max_calls, nullptr);
+    c->prof_ev.assign(PROF_EV * (size_t)max_calls, nullptr);
     for (auto& e : c->prof_ev) XK_TRY(hipEventCreate(&e));
     c->prof_max = max_calls;
     c->prof_calls = 0;
@@ -728,17 +743,18 @@ int xk_profile_begin(xk_ctx* c, int max_calls) {
 
 int xk_profile_end(xk_ctx* c, double* stage_ms, int* n_calls) {
     if (!c) return -1;
-    double acc[3] = {0, 0, 0};
+    double acc[PROF_EV - 1] = {};
     const int n = c->prof_calls;
-    if (n > 0) XK_TRY(hipEventSynchronize(c->prof_ev[4 * (size_t)(n - 1) + 3]));
+    if (n > 0) XK_TRY(hipEventSynchronize(c->prof_ev[PROF_EV * (size_t)(n - 1) + PROF_EV - 1]));
     for (int k = 0; k < n; ++k)
-        for (int s = 0; s < 3; ++s) {
+        for (int s = 0; s < PROF_EV - 1; ++s) {
             float ms = 0.f;
-            XK_TRY(hipEventElapsedTime(&ms, c->prof_ev[4 * (size_t)k + s], c->prof_ev[4 * (size_t)k + s + 1]));
+            XK_TRY(hipEventElapsedTime(&ms, c->prof_ev[PROF_EV * (size_t)k + s],
+                                       c->prof_ev[PROF_EV * (size_t)k + s + 1]));
             acc[s] += ms;
         }
     if (stage_ms)
-        for (int s = 0; s < 3; ++s) stage_ms[s] = acc[s];
+        for (int s = 0; s < PROF_EV - 1; ++s) stage_ms[s] = acc[s];
     if (n_calls) *n_calls = n;
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();

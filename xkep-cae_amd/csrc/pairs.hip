@@ -18,9 +18,11 @@
 //
 // Every 3x3 node block (k, l) of a pair's contribution to K is c_k c_l G with
 // one 3x3 G per pair and c = [(1-s), s, -(1-t), -t]; every node force is
-// c_k F with one 3-vector F.  The kernel therefore stages per pair only
-// c[4], F[3] (FP64) and G[9] (FP64 or FP32), structure-of-arrays over the
-// increment's contributing pairs; the gather kernels expand them.
+// c_k F with one 3-vector F.  The kernel therefore stages per contributing
+// pair only a force record c[4], F[3] (FP64) and a tangent record G[9], c[4]
+// (FP64 or FP32), at the pair's contributing slot of the increment; the
+// gather kernel expands them.  Pair blocks are mapped XCD-contiguously, so
+// the node coordinates of nearby pairs are fetched into one L2.
 //
 // This translation unit is compiled with FP contraction off so the
 // branch-deciding quantities (parallel test, clamps, gaps, hysteresis
@@ -59,7 +61,10 @@ __global__ void __launch_bounds__(PAIR_BLOCK)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
             GT* __restrict__ sg, int* __restrict__ act_part) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_logical_block();
+    const int n_blk = (int)((n_pairs + PAIR_BLOCK - 1) / PAIR_BLOCK);
+    if (lb >= n_blk) return;  // padding workgroup (uniform over the workgroup)
+    const int64_t i = (int64_t)lb * PAIR_BLOCK + threadIdx.x;
     int active_here = 0;
     if (i < n_pairs) {
         const int64_t cap = P.cap;
@@ -300,6 +305,10 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 GT* grec = sg + SG_REC * (size_t)si;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) grec[k] = GT(G[k]);
+                grec[SG_C + 0] = GT(1.0 - s);
+                grec[SG_C + 1] = GT(s);
+                grec[SG_C + 2] = GT(-(1.0 - t));
+                grec[SG_C + 3] = GT(-t);
             }
             active_here = p_n > 0.0 ? 1 : 0;
         }
@@ -313,7 +322,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
         if (threadIdx.x == 0) {
             int c = 0;
             for (int w = 0; w < PAIR_BLOCK / 64; ++w) c += wc[w];
-            act_part[blockIdx.x] = c;
+            act_part[lb] = c;
         }
     }
 }
@@ -329,7 +338,7 @@ hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const doubl
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
                         int* act_part, hipStream_t st) {
     if (n_pairs <= 0) return hipSuccess;
-    dim3 blk(PAIR_BLOCK), grd((unsigned)pair_blocks(n_pairs));
+    dim3 blk(PAIR_BLOCK), grd((unsigned)xcd_padded(pair_blocks(n_pairs)));
     if (prec == 0)
         hipLaunchKernelGGL(pair_kernel<double>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
                            (double*)sg, act_part);

@@ -135,10 +135,25 @@ struct BeamConst {
     double vref[3];
 };
 
-// Per contributing pair, the pair kernel stages one FP64 record
-// [c0, c1, c2, c3, Fx, Fy, Fz, pad] and one tangent record [G00 .. G22, pad x3]
-// (FP64 or FP32); element records are 12 FP64 forces and 144 tangent values.
+// Per contributing pair, the pair kernel stages one FP64 force record
+// [c0, c1, c2, c3, Fx, Fy, Fz, pad] and one tangent record
+// [G00 .. G22, c0, c1, c2, c3, pad x3] (FP64 or FP32; 16 entries, so a record
+// is one or two aligned 64-byte segments); element records are 12 FP64 forces
+// and 144 tangent values.
 constexpr int SD_REC = 8;
-constexpr int SG_REC = 12;
+constexpr int SG_REC = 16;
+constexpr int SG_C = 9;   // offset of c[4] in the tangent record
+
+// Workgroups of one launch are dispatched round-robin over the 8 XCDs of the
+// device (workgroup b on XCD b % 8), each XCD with its own L2.  For a grid
+// padded to a multiple of 8, logical block (b % 8) * (grid / 8) + b / 8 gives
+// every XCD one contiguous range of logical blocks, so work items with nearby
+// indices (pairs of nearby elements, rows of nearby nodes) share an L2.
+constexpr int N_XCD = 8;
+__host__ __device__ inline int xcd_padded(int n_blocks) { return (n_blocks + N_XCD - 1) / N_XCD * N_XCD; }
+__device__ __forceinline__ int xcd_logical_block() {
+    const int per = (int)gridDim.x / N_XCD;
+    return ((int)blockIdx.x % N_XCD) * per + (int)blockIdx.x / N_XCD;
+}
 
 }  // namespace xk

@@ -36,6 +36,8 @@ enum PairBuf {
     PB_COUNT
 };
 
+constexpr int PROF_EV = 5;   // events per profiled assembly call: 4 stages
+
 struct Ctx {
     int n_nodes = 0, n_elems = 0, prec = 0;
     std::vector<double> X_h, rad_h;
@@ -66,7 +68,7 @@ struct Ctx {
     const int* cand_I = nullptr;
     const int* cand_J = nullptr;
     // optional per-stage event timing of the fused assembly
-    std::vector<hipEvent_t> prof_ev;
+    std::vector<hipEvent_t> prof_ev;   // PROF_EV per profiled call
     int prof_max = 0, prof_calls = 0;
     // second stream (default priority) for the element kernel, joined by events
     hipStream_t side = nullptr;
@@ -76,9 +78,9 @@ struct Ctx {
     bool direct = false;
     DBuf d_eflag;   // per element: bit0 shares node1 with element e-1, bit1 shares node2 with e+1
     DBuf d_eslot;   // per element: global block index of its 16 group blocks
-    DBuf d_cblk;    // blocks with pair contributions (bit 31: also a beam block)
+    DBuf d_cdesc;   // int4 descriptors of the blocks with pair contributions
     DBuf d_fint;    // internal force written by the element kernel (6 n_nodes)
-    int n_cblk = 0;
+    int n_cdesc = 0;
 
     int grow_pairs(int64_t need);
     void bind_pairs();
@@ -96,7 +98,7 @@ struct Ctx {
         d_act.release();
         d_eflag.release();
         d_eslot.release();
-        d_cblk.release();
+        d_cdesc.release();
         d_fint.release();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         prof_ev.clear();
@@ -127,7 +129,9 @@ hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const doubl
                         int* act_part, hipStream_t st);
 int pair_blocks(int64_t n_pairs);
 int gather_node_blocks(int n_nodes);
-hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st);
+// `between` (nullable) is recorded after the gather kernel, before the finish kernel
+hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st,
+                         hipEvent_t between = nullptr);
 hipError_t launch_fill_i32(size_t n, int* p, int v, hipStream_t st);
 hipError_t launch_fill_f64(size_t n, double* p, double v, hipStream_t st);
 hipError_t launch_fill_f32(size_t n, float* p, float v, hipStream_t st);

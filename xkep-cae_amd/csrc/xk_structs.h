@@ -55,8 +55,12 @@ struct GatherArgs {
     BlockLists B;
     const double* fe;  // element force staging (general path)
     const double* fint;  // element-kernel internal force (direct path), replaces fe
-    const int* cblk;   // direct path: K lanes visit only these blocks (bit 31: beam block)
-    int n_cblk;
+    // direct path: descriptors of the blocks with pair contributions,
+    // {CSR offset of the block's first entry, scalar row stride, psrc begin,
+    // psrc end | bit 31 if the element kernel wrote a beam value there}
+    int direct;
+    const int4* cdesc;
+    int n_cdesc;
     const double* sd;  // pair c[4], F[3] staging
     const double* u;
     const double* f_ext;

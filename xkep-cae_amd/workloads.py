@@ -3,7 +3,13 @@
 Units are mm-ton-MPa.  Material E = 200e3 MPa, nu = 0.3, rho = 7.85e-9.
 Contact: k_pen = 1e4 E r, mu = 0.3, Huber smoothing_delta = 5000 / r
 (the reference's rule, numerical_tests/three_point_bend_jig.py:1121),
-k_t_ratio = 0.5, g_on = 0, g_off = 1e-6.
+k_t_ratio = 0.5, g_on = 0, g_off = 1e-6.  Candidate filter: exclude_same_layer
+with elem_layer_map = layer index of each element's wire (the map the
+reference's MeshData.layer_ids carries, core/data.py:26), so only pairs between
+layers remain; with it the rest configuration has no initial penetration,
+which the reference's solver requires (contact/solver/process.py:312-324),
+while wires of one layer (helices at one lay radius) and non-adjacent elements
+of one wire would overlap.
 
 State: u ~ N(0, (1e-3 r)^2) i.i.d. on all six DOFs of every node, from
 ``numpy.random.default_rng(seed)``; u_ref = 0; Generalized-alpha predicted
@@ -71,6 +77,11 @@ C5 = StrandCase("C5_1000wire", 1000, 0.3, 100.0, 100, 12.0, seed=1000)
 CASES = {c.name: c for c in (C1, C2, C3, C5)}
 
 
+def layer_map(elem_layer) -> dict:
+    """elem_layer_map of the contact config: element -> layer index."""
+    return {int(e): int(k) for e, k in enumerate(np.asarray(elem_layer))}
+
+
 @dataclass
 class StrandState:
     case: StrandCase
@@ -80,6 +91,10 @@ class StrandState:
     f_ext: np.ndarray
     fixed_dofs: np.ndarray
     extra: dict = field(default_factory=dict)
+
+    @property
+    def elem_layer(self) -> np.ndarray:
+        return self.mesh.elem_layer()
 
 
 def make_state(case: StrandCase, *, seed: int | None = None) -> StrandState:
@@ -114,6 +129,7 @@ class Batch:
     u_ref: np.ndarray
     f_ext: np.ndarray
     fixed_dofs: np.ndarray
+    elem_layer: np.ndarray
     node_ranges: list
     elem_ranges: list
     offsets: np.ndarray
@@ -124,7 +140,7 @@ class Batch:
 
 
 def concat_states(states: list, gap_factor: float = 4.0) -> Batch:
-    xs, cs, rs, us, urs, fs, fx = [], [], [], [], [], [], []
+    xs, cs, rs, us, urs, fs, fx, ly = [], [], [], [], [], [], [], []
     nr, er, offs = [], [], []
     n0 = e0 = 0
     shift = 0.0
@@ -144,13 +160,15 @@ def concat_states(states: list, gap_factor: float = 4.0) -> Batch:
         urs.append(st.u_ref)
         fs.append(st.f_ext)
         fx.append(st.fixed_dofs + 6 * n0)
+        ly.append(m.elem_layer())
         nr.append((n0, n0 + m.n_nodes))
         er.append((e0, e0 + m.n_elems))
         n0 += m.n_nodes
         e0 += m.n_elems
     return Batch(states=states, node_coords=np.concatenate(xs), connectivity=np.concatenate(cs),
                  radii=np.concatenate(rs), u=np.concatenate(us), u_ref=np.concatenate(urs),
-                 f_ext=np.concatenate(fs), fixed_dofs=np.concatenate(fx), node_ranges=nr, elem_ranges=er,
+                 f_ext=np.concatenate(fs), fixed_dofs=np.concatenate(fx),
+                 elem_layer=np.concatenate(ly), node_ranges=nr, elem_ranges=er,
                  offsets=np.array(offs))
 
 
