@@ -67,6 +67,13 @@ class GeneralizedAlpha:
         a_mid = (1.0 - self.alpha_m) * self.acc + self.alpha_m * self._acc_old
         return R.copy() + self.M @ a_mid
 
+    def checkpoint(self):
+        self._ckpt = (self.vel.copy(), self.acc.copy())
+
+    def restore_checkpoint(self):
+        if getattr(self, "_ckpt", None) is not None:
+            self.vel, self.acc = self._ckpt[0].copy(), self._ckpt[1].copy()
+
 
 @dataclass
 class Problem:

@@ -8,9 +8,12 @@ this script.  The fixtures pin what closed forms leave open at the 1e-10 bar:
 the exact algebra of the symmetrised CR analytical tangent, the friction
 tangents (sign convention of the material block, geometric block), the
 friction-loop state quirks (stale p_n, status rewrite, z_t overwrite) and the
-composed R / K_T of whole Newton attempts across two increments.
+composed R / K_T of whole Newton attempts across two increments, and the
+trajectory of the reference's own dynamic Newton solver (attempt counts,
+cutbacks, active counts, residual histories) on a small strand, which only
+running it can pin.
 
-Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [element|assembly|newton ...]
 """
 
 from __future__ import annotations
@@ -212,9 +215,109 @@ def assembly_fixture():
     np.savez_compressed(os.path.join(HERE, "assembly_7wire.npz"), **out)
 
 
+def newton_case():
+    """Inputs of the Newton-trajectory fixture: 7-wire strand (r = 1, L = 20, 20
+    elements per wire, lay angle 10 deg), core and outer layer in contact only
+    (same-layer exclusion by layer index), first node of every wire clamped,
+    radial inward force at the mid node of every outer wire, lumped mass,
+    Generalized-alpha rho_inf = 0.8, frictionless, penalty 0.5 c0 m_ii (the
+    dynamic estimate of contact/penalty/strategy.py:216-262 at the nominal
+    step), increments capped at a quarter of the total time."""
+    nw, r, L, m, ang = 7, 1.0, 20.0, 20, 10.0
+    mesh = mesh_mod.make_strand(nw, r, L, m, lay_angle_deg=ang)
+    X, conn = mesh.node_coords, mesh.connectivity.astype(np.int64)
+    s = section(r)
+    ndof = 6 * len(X)
+    nn = m + 1
+    first = np.arange(nw) * nn
+    fixed = (first[:, None] * 6 + np.arange(6)[None, :]).ravel()
+    mid = first[1:] + nn // 2
+    nrm = X[mid, :2] / np.linalg.norm(X[mid, :2], axis=1)[:, None]
+    f_ext = np.zeros(ndof)
+    f_ext[mid * 6 + 0] = -100.0 * nrm[:, 0]
+    f_ext[mid * 6 + 1] = -100.0 * nrm[:, 1]
+    # t_total = 4 T1/40 of one wire (first bending mode, numerical_tests/three_point_bend_jig.py:841-858)
+    Iv = s["Iy"]
+    f1 = (np.pi / (2.0 * L**2)) * np.sqrt(E * Iv / (RHO * s["A"]))
+    t_total = 4.0 * (1.0 / f1 / 40.0)
+    rho_inf, dt_max_fraction = 0.8, 0.25
+    am = (2 * rho_inf - 1) / (rho_inf + 1)
+    af = rho_inf / (rho_inf + 1)
+    beta = 0.25 * (1 - am + af) ** 2
+    dt_nom = t_total * dt_max_fraction
+    k_pen = 0.5 * (1.0 / (beta * dt_nom**2)) * RHO * s["A"] * (L / m) / 2.0
+    return dict(mesh=mesh, X=X, conn=conn, s=s, fixed=fixed, f_ext=f_ext, t_total=t_total, rho_inf=rho_inf,
+                dt_max_fraction=dt_max_fraction, k_pen=k_pen, mu=0.0, smoothing_delta=5000.0 / r, tol_force=1e-6,
+                tol_disp=1e-8, max_nr_attempts=30, radius=r, layer=mesh.elem_layer())
+
+
+def newton_fixture():
+    """The reference's own dynamic solver (ContactFrictionProcess) on newton_case():
+    every Newton call of every increment attempt (cutbacks included) with its
+    time window, attempt count, active count, convergence flags and residual
+    history, plus the final displacement."""
+    from xkep_cae.contact.solver import _newton_dynamic as nd
+    from xkep_cae.contact.solver.process import ContactFrictionProcess
+    from xkep_cae.core.data import (AssembleCallbacks, BoundaryData, ContactFrictionInputData,
+                                    ContactSetupData, MeshData)
+
+    c = newton_case()
+    s = c["s"]
+    X, conn = c["X"], c["conn"]
+    asm = ULCRBeamAssembler(X, conn, s["E"], s["G"], s["A"], s["Iy"], s["Iz"], s["J"], s["kappa_y"], s["kappa_z"])
+    M = asm.assemble_mass(RHO, lumped=True)
+    lay = c["layer"]
+    cfg = _ContactConfigInput(smoothing_delta=c["smoothing_delta"], exclude_same_layer=True,
+                              elem_layer_map={int(e): int(k) for e, k in enumerate(lay)},
+                              adjust_initial_penetration=False, dt_max_fraction=c["dt_max_fraction"],
+                              k_t_ratio=0.5)
+    setup = ContactSetupData(manager=_ContactManagerInput(config=cfg), k_pen=c["k_pen"], mu=c["mu"])
+    inp = ContactFrictionInputData(
+        mesh=MeshData(node_coords=X, connectivity=conn, radii=np.full(len(conn), c["radius"]), n_strands=7,
+                      layer_ids=lay),
+        boundary=BoundaryData(fixed_dofs=c["fixed"], f_ext_total=c["f_ext"]),
+        contact=setup,
+        callbacks=AssembleCallbacks(assemble_tangent=asm.assemble_tangent,
+                                    assemble_internal_force=asm.assemble_internal_force, ul_assembler=asm),
+        mass_matrix=M, dt_physical=c["t_total"], rho_inf=c["rho_inf"], tol_force=c["tol_force"],
+        tol_disp=c["tol_disp"], max_nr_attempts=c["max_nr_attempts"])
+    calls = []
+    orig = nd.NewtonDynamicProcess.process
+
+    def logged(self, step_input):
+        out = orig(self, step_input)
+        calls.append((step_input.load_frac, step_input.load_frac_prev, step_input.dt_sub, out.converged,
+                      out.n_attempts, out.n_active, out.diverged, list(out.diagnostics.res_history),
+                      list(out.diagnostics.n_active_history)))
+        return out
+
+    nd.NewtonDynamicProcess.process = logged
+    try:
+        res = ContactFrictionProcess().process(inp)
+    finally:
+        nd.NewtonDynamicProcess.process = orig
+    n = len(calls)
+    hist_len = np.array([len(cl[7]) for cl in calls])
+    out = dict(X=X, conn=conn, radii=np.full(len(conn), c["radius"]), layer=lay, fixed=c["fixed"],
+               f_ext=c["f_ext"], M_diag=M.diagonal(),
+               params=np.array([c["k_pen"], c["mu"], c["t_total"], c["rho_inf"], c["dt_max_fraction"],
+                                c["smoothing_delta"], c["tol_force"], c["tol_disp"], c["max_nr_attempts"], 0.5]),
+               call_load_frac=np.array([cl[0] for cl in calls]), call_load_frac_prev=np.array([cl[1] for cl in calls]),
+               call_dt=np.array([cl[2] for cl in calls]), call_converged=np.array([cl[3] for cl in calls]),
+               call_attempts=np.array([cl[4] for cl in calls]), call_active=np.array([cl[5] for cl in calls]),
+               call_diverged=np.array([cl[6] for cl in calls]), call_hist_len=hist_len,
+               res_history=np.concatenate([np.asarray(cl[7], float) for cl in calls]) if n else np.zeros(0),
+               active_history=np.concatenate([np.asarray(cl[8], np.int64) for cl in calls]) if n else np.zeros(0),
+               converged=np.array(res.converged), n_increments=np.array(res.n_increments),
+               n_cutbacks=np.array(res.n_cutbacks), u_final=res.u)
+    np.savez_compressed(os.path.join(HERE, "newton_7wire.npz"), **out)
+
+
+FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture}
+
 if __name__ == "__main__":
-    element_fixture()
-    assembly_fixture()
+    for name in sys.argv[1:] or list(FIXTURES):
+        FIXTURES[name]()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -62,14 +62,24 @@ class StrandAssembly:
     def set_u_ref(self, u_ref):
         self.u_ref.copy_(self.ctx.to_dev(u_ref))
 
+    def set_f_ext(self, f_ext):
+        self.f_ext.copy_(self.ctx.to_dev(f_ext))
+
+    def set_dt(self, dt: float):
+        """Time step of the inertia terms (a captured graph is dropped)."""
+        self.dt = float(dt)
+        self._dyn = self.time.dyn_struct(self.dt) if self.time is not None else None
+        self.graph = None
+
     def deformed_coords(self, u=None):
         """x = X + u[0:3] per node (``_deformed_coords``, contact/solver/_utils.py:16-28)."""
         u = self.u if u is None else self.ctx.to_dev(u)
         return (self.ctx.X + u.view(-1, 6)[:, :3]).contiguous()
 
-    def begin_increment(self, u=None, *, detect: bool = True, margin: float = 0.0, cell_size=None):
-        """Once-per-increment: GPU DetectCandidates + UpdateGeometry (unfrozen) on
-        x = X + u, then the tangent pattern of the increment."""
+    def begin_increment(self, u=None, *, detect: bool = True, geometry: bool = True, margin: float = 0.0,
+                        cell_size=None):
+        """Once-per-increment: GPU DetectCandidates and/or UpdateGeometry
+        (unfrozen) on x = X + u, then the tangent pattern of the increment."""
         if u is not None:
             self.set_u(u)
         x = self.deformed_coords()
@@ -78,10 +88,26 @@ class StrandAssembly:
             self.ctx.detect_candidates(x, margin=margin, cell_size=cell_size, elem_layer=self._layer,
                                        midpoint_prescreening=cfg.midpoint_prescreening,
                                        prescreening_margin=cfg.prescreening_margin)
-        self.ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, coords=x, g_on=cfg.g_on, g_off=cfg.g_off)
-        self.K = self.ctx.new_K()  # builds the pattern (synchronous)
+        if geometry:
+            self.ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, coords=x, g_on=cfg.g_on, g_off=cfg.g_off)
+            self.K = self.ctx.new_K()  # builds the pattern (synchronous)
+        else:
+            self.K = None
         self.graph = None
         return self
+
+    def count_penetrations(self, u=None) -> int:
+        """#(gap < 0) over the pair list at x = X + u without changing any pair
+        state (the initial-penetration check, contact/solver/_initial_penetration.py:117-142)."""
+        if u is not None:
+            self.set_u(u)
+        snap = self.ctx.get_pairs()
+        if len(snap["elem_a"]) == 0:
+            return 0
+        self.ctx.pairs_run(L.PAIR_GEOM, coords=self.deformed_coords())
+        gap = self.ctx.get_pairs()["gap"]
+        self.ctx.set_pairs(snap["elem_a"], snap["elem_b"], state=snap)
+        return int(np.sum(gap < 0.0))
 
     def assemble(self, u=None, *, tangent: bool = True):
         """Launch one fused assembly; returns device tensors (R, K, ||R||^2, n_active)."""

@@ -1,0 +1,431 @@
+"""Dynamic frictional-contact Newton driver around the device assembly.
+
+Restates, for the strand path (smooth-penalty Huber contact, Coulomb friction,
+Generalized-alpha dynamics, no coating, no prescribed displacements), the
+reference's solver loop so that the assembly on the GPU can be driven through
+whole solves and its Newton trajectory compared with the reference's:
+
+  ContactFrictionProcess.process   contact/solver/process.py:139-674
+      increment loop, predictor, per-increment DetectCandidates +
+      UpdateGeometry, cutback with checkpoint restore, accept / checkpoint
+  NewtonDynamicProcess.process     contact/solver/_newton_dynamic.py:115-389
+      attempts, force / displacement / energy convergence, divergence
+      detection, line search, update
+  ConvergenceCheckProcess          contact/solver/_newton_steps.py:182-243
+  LinearSolveProcess               contact/solver/_newton_steps.py:338-366
+  LineSearchUpdateProcess          contact/solver/_newton_steps.py:396-434
+  _ncp_line_search                 contact/solver/_utils.py:31-61
+  UnifiedTimeStepProcess           contact/solver/_unified_time_controller.py:103-215
+  AdaptiveSteppingProcess          contact/solver/_adaptive_stepping.py
+
+The per-attempt work (narrowphase, forces, tangent, effective terms, norm) is
+one call of the backend's ``assemble``; the linear solve is SciPy's SuperLU on
+the host, as in the reference.  The backend protocol is small so that the same
+driver runs on the device (``DeviceBackend``) or on any other assembly with
+the reference's semantics.
+"""
+
+from __future__ import annotations
+
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+
+@dataclass(frozen=True)
+class SolverConfig:
+    """Solver knobs with the reference's names and defaults
+    (ContactFrictionInputData, core/data.py:156-193; _ContactConfigInput
+    adaptive-stepping fields, contact/_contact_pair.py:146-153;
+    NewtonDynamicInput, _newton_dynamic.py:36-47)."""
+
+    max_nr_attempts: int = 50
+    tol_force: float = 1e-8
+    tol_disp: float = 1e-8
+    divergence_window: int = 5
+    du_norm_cap: float = 0.0
+    max_increments: int = 10000
+    use_line_search: bool = True
+    line_search_max_steps: int = 5
+    dt_grow_factor: float = 1.5
+    dt_shrink_factor: float = 0.5
+    dt_grow_attempt_threshold: int = 5
+    dt_shrink_attempt_threshold: int = 15
+    dt_contact_change_threshold: float = 0.3
+    dt_min_fraction: float = 0.0
+    dt_max_fraction: float = 0.0
+
+
+# ---------------------------------------------------------------- stepping
+class AdaptiveStepping:
+    """Load-fraction queue with growth / shrink / cutback
+    (AdaptiveSteppingProcess, _adaptive_stepping.py) behind the physical-time
+    interface of UnifiedTimeStepProcess (_unified_time_controller.py:103-215),
+    configured as ContactFrictionProcess does (process.py:332-354)."""
+
+    def __init__(self, t_total: float, cfg: SolverConfig):
+        if t_total <= 0.0:
+            raise ValueError("t_total must be positive")
+        self.t_total = t_total
+        dt_initial = t_total
+        dt_min = cfg.dt_min_fraction * t_total if cfg.dt_min_fraction > 0 else t_total / 32.0
+        dt_max = cfg.dt_max_fraction * t_total if cfg.dt_max_fraction > 0 else t_total
+        self.grow = cfg.dt_grow_factor
+        self.shrink = cfg.dt_shrink_factor
+        self.grow_thr = cfg.dt_grow_attempt_threshold if cfg.dt_grow_attempt_threshold > 0 else 5
+        self.shrink_thr = cfg.dt_shrink_attempt_threshold
+        self.contact_thr = cfg.dt_contact_change_threshold
+        f_init = dt_initial / t_total
+        self.f_min = dt_min / t_total
+        self.f_max = dt_max / t_total
+        n_eff = max(1, int(1.0 / f_init) if f_init > 0 else 1)
+        if self.f_min <= 0.0:
+            self.f_min = 1.0 / (n_eff * 32)
+        if self.f_max <= 0.0:
+            self.f_max = min(8.0 / n_eff, 1.0)
+        base = f_init if f_init > 0.0 else 1.0
+        if self.f_max > 0.0:
+            base = min(base, self.f_max)
+        self.queue: deque[float] = deque([min(base, 1.0)])
+        self.consecutive_good = 0
+        self.n_cutbacks = 0
+        self.last_prev = 0.0
+
+    def query(self, load_frac_prev: float):
+        """(has_more, load_frac, dt_sub)."""
+        while self.queue:
+            nxt = self.queue[0]
+            if nxt <= load_frac_prev + 1e-15:
+                self.queue.popleft()
+                continue
+            dt_sub = max((nxt - self.last_prev) * self.t_total, 0.0)
+            return True, nxt, dt_sub
+        return False, 0.0, 0.0
+
+    def success(self, load_frac, load_frac_prev, n_attempts, n_active, prev_n_active):
+        self.last_prev = load_frac
+        if self.queue and self.queue[0] <= load_frac + 1e-15:
+            self.queue.popleft()
+        if load_frac >= 1.0 - 1e-12:
+            return
+        cur = load_frac - load_frac_prev
+        nxt = cur
+        if n_attempts <= self.grow_thr:
+            self.consecutive_good += 1
+            if self.consecutive_good <= 2:
+                nxt = cur * self.grow
+            else:
+                damp = max(0.1, 1.0 / self.consecutive_good)
+                nxt = cur * (1.0 + (self.grow - 1.0) * damp)
+        elif n_attempts >= self.shrink_thr:
+            nxt = cur * self.shrink
+            self.consecutive_good = 0
+        else:
+            self.consecutive_good = 0
+        changed = abs(n_active - prev_n_active)
+        if changed > 0:
+            base = max(prev_n_active, n_active, 1)
+            if changed / base > self.contact_thr:
+                nxt = min(nxt, cur * self.shrink)
+                self.consecutive_good = 0
+        nxt = max(nxt, self.f_min)
+        nxt = min(nxt, self.f_max)
+        nf = min(load_frac + nxt, 1.0)
+        if 1.0 - nf < self.f_min * 0.5:
+            nf = 1.0
+        self.queue.appendleft(nf)
+
+    def failure(self, load_frac, load_frac_prev, diverged) -> bool:
+        """True if the increment can be retried with a smaller step."""
+        delta = load_frac - load_frac_prev
+        if delta <= self.f_min + 1e-15:
+            return False
+        self.consecutive_good = 0
+        self.n_cutbacks += 1
+        shrink = self.shrink * self.shrink if diverged else self.shrink
+        mid = load_frac_prev + delta * shrink
+        self.queue.appendleft(load_frac)
+        self.queue.appendleft(mid)
+        return True
+
+
+# ---------------------------------------------------------------- Newton pieces
+def linear_solve(K: sp.spmatrix, R: np.ndarray, fixed_dofs: np.ndarray):
+    """du solving K du = -R with fixed rows / columns replaced by identity
+    (LinearSolveProcess, _newton_steps.py:351-366); None if SuperLU fails."""
+    from scipy.sparse.linalg import spsolve
+
+    n = K.shape[0]
+    free = np.ones(n)
+    free[fixed_dofs] = 0.0
+    D = sp.diags(free)
+    K_eff = (D @ K.astype(np.float64) @ D + sp.diags(1.0 - free)).tocsc()
+    K_eff.eliminate_zeros()
+    rhs = -np.asarray(R, float).copy()
+    rhs[fixed_dofs] = 0.0
+    try:
+        du = spsolve(K_eff, rhs)
+    except Exception:
+        return None
+    return du
+
+
+def line_search(u, du, f_ext, fixed_dofs, internal_force, res_u_norm, max_steps, f_c, diverge_factor=3.0) -> float:
+    """Step length: halve until the frozen-contact static residual
+    f_int(u + a du) + f_c - f_ext is finite and below diverge_factor ||R||
+    (_ncp_line_search, contact/solver/_utils.py:31-61)."""
+    alpha = 1.0
+    for _ in range(max_steps):
+        try:
+            f_try = internal_force(u + alpha * du)
+        except Exception:
+            alpha *= 0.5
+            continue
+        R_try = f_try + f_c - f_ext
+        R_try[fixed_dofs] = 0.0
+        if not np.all(np.isfinite(R_try)):
+            alpha *= 0.5
+            continue
+        if float(np.linalg.norm(R_try)) < diverge_factor * max(res_u_norm, 1e-30):
+            return alpha
+        alpha *= 0.5
+    return alpha
+
+
+@dataclass
+class NewtonOutcome:
+    converged: bool
+    n_attempts: int
+    n_active: int
+    diverged: bool
+    f_c: np.ndarray
+    res_history: list = field(default_factory=list)
+    active_history: list = field(default_factory=list)
+
+
+def newton_dynamic(backend, u: np.ndarray, u_ref: np.ndarray, f_ext: np.ndarray, fixed_dofs: np.ndarray,
+                   dt: float, f_ext_ref_norm: float, dynamic_ref: bool, cfg: SolverConfig) -> NewtonOutcome:
+    """One increment's Newton iterations (NewtonDynamicProcess.process,
+    _newton_dynamic.py:115-389).  ``u`` is updated in place."""
+    energy_ref = None
+    incr_f_ref = 0.0
+    consecutive_increase = 0
+    prev_ratio = float("inf")
+    out = NewtonOutcome(False, 0, 0, False, np.zeros_like(u))
+    for att in range(cfg.max_nr_attempts):
+        out.n_attempts += 1
+        R, K, n_active, f_c = backend.assemble(u, u_ref, f_ext, dt)
+        out.f_c = f_c
+        # ConvergenceCheckProcess, force part (_newton_steps.py:194-213)
+        res = float(np.linalg.norm(R))
+        f_ref = incr_f_ref if incr_f_ref > 1e-30 else f_ext_ref_norm
+        if dynamic_ref and att == 0 and res > 1e-30:
+            f_ref = res
+        if att == 0 and dynamic_ref:
+            incr_f_ref = f_ref
+        out.n_active = n_active
+        ratio = res / f_ref
+        out.res_history.append(ratio)
+        out.active_history.append(n_active)
+        if ratio < cfg.tol_force:
+            out.converged = True
+            break
+        # divergence (_newton_dynamic.py:258-285)
+        if att > 0 and ratio > prev_ratio * 1.01:
+            consecutive_increase += 1
+        else:
+            consecutive_increase = 0
+        prev_ratio = ratio
+        if (consecutive_increase >= cfg.divergence_window and not ratio < 0.01) or (att >= 10 and ratio > 100.0):
+            out.diverged = True
+            break
+        du = linear_solve(K, R, fixed_dofs)
+        if du is None:
+            break
+        scale = 1.0
+        if cfg.use_line_search:
+            scale *= line_search(u, du, f_ext, fixed_dofs, backend.internal_force, res,
+                                 cfg.line_search_max_steps, f_c)
+        if cfg.du_norm_cap > 0.0:
+            dn = float(np.linalg.norm(scale * du))
+            un = max(float(np.linalg.norm(u)), 1.0)
+            if dn > cfg.du_norm_cap * un:
+                scale *= cfg.du_norm_cap * un / dn
+        du = scale * du
+        u += du
+        # ConvergenceCheckProcess, displacement / energy part (_newton_steps.py:215-243)
+        du_norm = float(np.linalg.norm(du))
+        u_norm = float(np.linalg.norm(u))
+        if u_norm > 1e-30 and du_norm / u_norm < cfg.tol_disp:
+            out.converged = True
+            break
+        energy = abs(float(np.dot(du, R)))
+        if energy_ref is None:
+            energy_ref = energy if energy > 1e-30 else 1.0
+        if energy_ref > 1e-30 and energy / energy_ref < 1e-10:
+            out.converged = True
+            break
+    return out
+
+
+# ---------------------------------------------------------------- driver
+@dataclass
+class NewtonCall:
+    """One Newton call (an increment attempt, cutbacks included)."""
+
+    load_frac: float
+    load_frac_prev: float
+    dt: float
+    converged: bool
+    n_attempts: int
+    n_active: int
+    diverged: bool
+    res_history: list
+    active_history: list
+
+
+@dataclass
+class SolverResult:
+    u: np.ndarray
+    converged: bool
+    n_increments: int
+    total_attempts: int
+    n_cutbacks: int
+    calls: list
+
+
+class ContactFrictionSolver:
+    """The dynamic increment loop of ContactFrictionProcess
+    (contact/solver/process.py:139-674) over a backend that assembles.
+
+    backend protocol:
+      detect(u)                 DetectCandidatesProcess on X + u
+      update_geometry(u)        UpdateGeometryProcess (hysteresis on) on X + u
+      initial_penetrations(u)   #(gap < 0) over the pairs, no state change
+      predict(u, dt) -> u_pred  Generalized-alpha predictor
+      assemble(u, u_ref, f_ext, dt) -> (R_eff, K_eff, n_active, f_c)
+      internal_force(u) -> f_int
+      correct(u, dt); checkpoint(); restore_checkpoint()
+    """
+
+    def __init__(self, backend, *, fixed_dofs, f_ext_total, t_total: float, f_ext_base=None,
+                 config: SolverConfig | None = None):
+        self.b = backend
+        self.cfg = config or SolverConfig()
+        self.fixed = np.asarray(fixed_dofs, np.int64)
+        self.f_total = np.asarray(f_ext_total, float)
+        self.f_base = np.zeros_like(self.f_total) if f_ext_base is None else np.asarray(f_ext_base, float)
+        self.t_total = float(t_total)
+
+    def solve(self, u0=None) -> SolverResult:
+        cfg = self.cfg
+        ndof = len(self.f_total)
+        u = np.zeros(ndof) if u0 is None else np.asarray(u0, float).copy()
+        u_ref = u.copy()
+        f_ref_norm = float(np.linalg.norm(self.f_total))
+        dynamic_ref = f_ref_norm < 1e-30
+        if dynamic_ref:
+            f_ref_norm = 1.0
+        b = self.b
+        # set-up candidates and the initial-penetration check (process.py:258-324)
+        b.detect(u)
+        n_pen = b.initial_penetrations(u)
+        if n_pen > 0:
+            raise ValueError(f"initial penetration detected in {n_pen} pairs")
+        ckpt = (u.copy(), u_ref.copy())
+        b.checkpoint()
+        stepping = AdaptiveStepping(self.t_total, cfg)
+        calls = []
+        load_frac_prev, prev_n_active, n_incr, incr_count, total = 0.0, 0, 0, 0, 0
+        while True:
+            if cfg.max_increments > 0 and incr_count >= cfg.max_increments:
+                break
+            more, load_frac, dt = stepping.query(load_frac_prev)
+            if not more:
+                break
+            incr_count += 1
+            n_incr += 1
+            f_ext = self.f_base + load_frac * self.f_total
+            u = b.predict(u, dt)
+            b.detect(u)
+            b.update_geometry(u)
+            step = newton_dynamic(b, u, u_ref, f_ext, self.fixed, dt, f_ref_norm, dynamic_ref, cfg)
+            total += step.n_attempts
+            calls.append(NewtonCall(load_frac, load_frac_prev, dt, step.converged, step.n_attempts, step.n_active,
+                                    step.diverged, step.res_history, step.active_history))
+            if not step.converged:
+                if stepping.failure(load_frac, load_frac_prev, step.diverged):
+                    u, u_ref = ckpt[0].copy(), ckpt[1].copy()
+                    b.restore_checkpoint()
+                    n_incr -= 1
+                    continue
+                return SolverResult(u, False, n_incr, total, stepping.n_cutbacks, calls)
+            b.correct(u, dt)
+            stepping.success(load_frac, load_frac_prev, step.n_attempts, step.n_active, prev_n_active)
+            prev_n_active = step.n_active
+            load_frac_prev = load_frac
+            u_ref = u.copy()
+            ckpt = (u.copy(), u_ref.copy())
+            b.checkpoint()
+        return SolverResult(u, True, n_incr, total, stepping.n_cutbacks, calls)
+
+
+# ---------------------------------------------------------------- device backend
+class DeviceBackend:
+    """The solver backend on one GPU: candidates, geometry, the fused assembly
+    (R_eff, K_eff, ||R||^2, #active in one graph-capturable launch sequence),
+    the element-only internal force of the line search and the
+    Generalized-alpha state all live on the device; R, K and f_c are copied to
+    the host for the SuperLU solve."""
+
+    def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
+                 k_t_ratio: float = 0.5, fixed_dofs=None, config=None):
+        from .assembly import StrandAssembly
+        from .time_integration import GeneralizedAlphaProcess
+
+        self.ctx = ctx
+        self.torch = ctx.torch
+        self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
+        self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
+                                  fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0)
+        self._f = self.torch.zeros(ctx.ndof, dtype=self.torch.float64, device=ctx.dev)
+        self._dt = None
+
+    def detect(self, u):
+        self.asm.begin_increment(u, geometry=False)
+
+    def update_geometry(self, u):
+        self.asm.begin_increment(u, detect=False)
+
+    def initial_penetrations(self, u) -> int:
+        return self.asm.count_penetrations(u)
+
+    def predict(self, u, dt):
+        return self.ga.predict(u, dt)
+
+    def assemble(self, u, u_ref, f_ext, dt):
+        asm = self.asm
+        if dt != self._dt:
+            asm.set_dt(dt)
+            self._dt = dt
+        asm.set_u(u)
+        asm.set_u_ref(u_ref)
+        asm.set_f_ext(f_ext)
+        R, K, _, n_act = asm.assemble()
+        self.ctx.pairs_gather(f=self._f)
+        return (R.cpu().numpy(), asm.K_csr(), int(n_act.item()), -self._f.cpu().numpy())
+
+    def internal_force(self, u):
+        self.ctx.beam(self.ctx.to_dev(u), f_int=self._f)
+        return self._f.cpu().numpy()
+
+    def correct(self, u, dt):
+        self.ga.correct(u, None, dt)
+
+    def checkpoint(self):
+        self.ga.checkpoint()
+
+    def restore_checkpoint(self):
+        self.ga.restore_checkpoint()
