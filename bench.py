@@ -100,8 +100,10 @@ def profile_stages(asm, ctx, torch, calls):
 
 
 STAGES = ["pairs", "elements", "gather", "finish"]
-KERNELS = {"pairs": "xk::pair_kernel<float>", "elements": "xk::beam_direct_kernel<float>",
-           "gather": "xk::gather_kernel<float, float, float>", "finish": "xk::finish_kernel"}
+# kernel-name prefixes of each stage in the FP32-mixed form (the template
+# arguments after the precision are the tuned workgroup sizes)
+KERNELS = {"pairs": "xk::pair_kernel<float", "elements": "xk::beam_direct_kernel<float",
+           "gather": "xk::gather_kernel<float, float, float", "finish": "xk::finish_kernel"}
 TRAFFIC_TABLE = os.path.join(HERE, "profiles", "assembly_kernels_91wire_hbm_traffic.md")
 
 
@@ -136,7 +138,7 @@ def stage_bytes(ctx, asm, n_pair, precision, nnz_beam):
 
 
 def traffic_bytes(kernel: str):
-    """HBM bytes per dispatch of `kernel` at the largest grid in the committed
+    """HBM bytes per dispatch of the kernel named `kernel`... at the largest grid in the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy kernel),
     or None if the summary has no row for it."""
     if not os.path.exists(TRAFFIC_TABLE):
@@ -144,7 +146,7 @@ def traffic_bytes(kernel: str):
     best = None
     for line in open(TRAFFIC_TABLE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 8 and cells[0] == kernel and cells[1].isdigit():
+        if len(cells) == 8 and cells[0].startswith(kernel) and cells[1].isdigit():
             grid, mb = int(cells[1]), float(cells[7])
             if best is None or grid > best[0]:
                 best = (grid, mb)
@@ -185,9 +187,15 @@ def cpu_baseline(case, st, budget_s=15.0):
     return (m.n_elems + len(op)) / t, len(times), t
 
 
-def run(label, case, states, prec, torch, steps, warmup):
+def run(label, case, states, prec, torch, steps, warmup, tune=False):
     log(f"[bench] {label}: setting up {len(states)} x {case.name} precision={prec}")
     ctx, asm, t_inc = setup(case, states, prec, torch)
+    tuning = None
+    if tune:
+        T = importlib.import_module("xkep-cae_amd.autotune")
+        tuning = T.autotune(asm, use_cache=False, calls=5)
+        log(f"[bench] {label}: autotuned {{'cell_factor': {tuning['cell_factor']}, 'pair_block': "
+            f"{tuning['pair_block']}, 'gather_block': {tuning['gather_block']}}}")
     n_pair = ctx.n_pairs
     asm.assemble()
     torch.cuda.synchronize()
@@ -197,7 +205,7 @@ def run(label, case, states, prec, torch, steps, warmup):
     log(f"[bench] {label}: elems={ctx.n_elems} pairs={n_pair} nnzK={asm.K.numel()} incr {t_inc * 1e3:.1f} ms | "
         f"{ms * 1e3:.1f} us/step (graph) wall {ms_wall * 1e3:.1f} | stages us {np.round(stages * 1e3, 1).tolist()} | "
         f"{n_eval / (ms * 1e-3):.3e} evals/s")
-    out = dict(ctx=ctx, asm=asm, n_pair=n_pair, n_elem=ctx.n_elems, n_eval=n_eval, ms=ms, ms_wall=ms_wall, stages=stages,
+    out = dict(ctx=ctx, asm=asm, n_pair=n_pair, n_elem=ctx.n_elems, n_eval=n_eval, tuning=tuning, ms=ms, ms_wall=ms_wall, stages=stages,
                t_inc=t_inc, value=n_eval / (ms * 1e-3))
     return out
 
@@ -209,6 +217,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16, help="91-wire instances per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes and grid cell")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -239,7 +248,8 @@ def main():
     del plain["asm"], plain["ctx"], single["asm"], single["ctx"]
     torch.cuda.empty_cache()
     states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
-    batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup)
+    batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup,
+                  tune=not args.no_autotune)
 
     nnz_beam = beam_nnz(batched["ctx"].conn_np)
     sb = stage_bytes(batched["ctx"], batched["asm"], batched["n_pair"], "mixed", nnz_beam)
@@ -286,6 +296,8 @@ def main():
         },
         "batched_stages_us": np.round(batched["stages"] * 1e3, 2).tolist(),
         "increment_setup_ms": round(batched["t_inc"] * 1e3, 1),
+        "tuning": None if batched["tuning"] is None else {k: batched["tuning"][k] for k in
+                                                          ("cell_factor", "pair_block", "gather_block")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_fp64_gpu": round(value / plain["value"], 3),

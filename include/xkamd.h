@@ -181,6 +181,62 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
                           double* R, void* K_vals, int64_t K_cap, double* res_norm2, int32_t* n_active,
                           void* stream);
 
+/* ---- Learned candidate prescreen.  No counterpart in the reference: an
+ * 8-64-64-1 BF16 MLP (inference on the matrix cores) chooses which candidate
+ * pairs the fused assembly's pair kernel visits in the Newton iterations of an
+ * increment; every contributing pair (status != INACTIVE or p_n > 0) is always
+ * visited, so R, K_T and n_active equal the full path's.  Parameters: FP32,
+ * W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3 (xk_mlp_n_params()). */
+int xk_lp_set_params(xk_ctx* ctx, const float* params_host, int n_params, float threshold);
+/* Features of every pair at x = coords_ref + u (device), elem_layer (device
+ * int32 per element, nullable) scaled by layer_scale; features_out (device
+ * float, n_pairs x 8, nullable) receives a copy for logging / training.  Call
+ * after the increment's geometry update; remembers each pair's gap for the
+ * next increment's gap-change feature. */
+int xk_lp_features(xk_ctx* ctx, const double* coords_ref, const double* u, const int32_t* elem_layer,
+                   float layer_scale, float* features_out, void* stream);
+/* Inference + watch / prune lists of the current pattern; switches the fused
+ * assembly to the watch list until the next xk_lp_judge / xk_lp_off, pattern
+ * rebuild or pair-list change.  Synchronous. */
+int xk_lp_select(xk_ctx* ctx, int64_t* n_watch, void* stream);
+/* The judge: exact narrowphase (geometry, active set frozen) over the pruned
+ * pairs at x = coords_ref + u, n_miss = #(pruned pairs with gap <= g_on);
+ * switches the watch list off.  Synchronous. */
+int xk_lp_judge(xk_ctx* ctx, const double* coords_ref, const double* u, double g_on, int64_t* n_miss,
+                void* stream);
+int xk_lp_off(xk_ctx* ctx);
+/* Copy of the whole pair state on the device, and its restore (the pair list
+ * must not have changed in between); used to re-run a Newton call on the full
+ * path after a prescreen miss.  Synchronous. */
+int xk_pairs_snapshot(xk_ctx* ctx);
+int xk_pairs_restore(xk_ctx* ctx);
+/* Stand-alone network kernels (device pointers; no context).  Forward: logits
+ * of n feature rows (FP32, rounded to BF16 on load).  Gradient: mean weighted
+ * logistic loss w+ y softplus(-z) + (1 - y) softplus(z) and its gradient with
+ * respect to the FP32 parameters (grad_and_loss: n_params gradient entries,
+ * then the loss), deterministic; work holds xk_mlp_grad_work(n) floats.  Adam:
+ * one bias-corrected step (step counts from 1). */
+int xk_mlp_n_params(void);
+int64_t xk_mlp_grad_work(int64_t n);
+int xk_mlp_forward(const float* X, int64_t n, const float* params, float* logits, void* stream);
+int xk_mlp_grad(const float* X, const float* y, int64_t n, const float* params, float pos_weight,
+                float* grad_and_loss, float* work, int64_t work_len, void* stream);
+int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr, float beta1,
+                float beta2, float eps, void* stream);
+
+/* Launch shapes and the broadphase grid (no counterpart in the reference;
+ * set by the autotuner): grid cell = cell_factor x mean box extent when a call
+ * passes cell_size <= 0 (the candidate set does not depend on it), pair-kernel
+ * workgroup 64/128/256/512 lanes, gather workgroup 128/256/512 lanes.  A HIP
+ * graph captured before a change keeps the old shapes. */
+typedef struct xk_tuning {
+    double cell_factor;
+    int pair_block;
+    int gather_block;
+} xk_tuning;
+int xk_ctx_set_tuning(xk_ctx* ctx, const xk_tuning* tuning);
+int xk_ctx_get_tuning(const xk_ctx* ctx, xk_tuning* tuning);
+
 /* Device status word of the last calls (XK_FLAG_* bits); synchronous. */
 int xk_status(xk_ctx* ctx, int* flags);
 

@@ -1,0 +1,160 @@
+"""Learned candidate prescreen: the network kernels against the oracle
+(oracle/mlp.py), exactness of the prescreened assembly, and what the trained
+network gains over the method without it."""
+
+import numpy as np
+import pytest
+
+from conftest import golden, pkg
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_case(seed=1, n=3000):
+    rng = np.random.default_rng(seed)
+    p = mlp.pack(dict(W1=rng.normal(0, 0.6, (8, 64)), b1=rng.normal(0, 0.2, 64), W2=rng.normal(0, 0.2, (64, 64)),
+                      b2=rng.normal(0, 0.1, 64), w3=rng.normal(0, 0.3, 64), b3=[0.2])).astype(np.float32)
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    y = (rng.random(n) < 0.1).astype(np.float32)
+    return p, X, y
+
+
+def test_mfma_forward_matches_oracle(gpu):
+    P = pkg("prescreen")
+    for seed, n in ((1, 3000), (2, 33), (3, 1)):
+        p, X, _ = rand_case(seed, n)
+        net = P.PrescreenNet(p)
+        z = net.logits(X).cpu().numpy()
+        zo, _ = mlp.forward(p, X, inference=True)
+        assert z.shape == zo.shape
+        assert np.abs(z - zo).max() <= 1e-2 * max(1.0, np.abs(zo).max()), (seed, np.abs(z - zo).max())
+
+
+def test_gradient_matches_oracle(gpu):
+    P = pkg("prescreen")
+    for seed, n in ((4, 5000), (5, 70)):
+        p, X, y = rand_case(seed, n)
+        net = P.PrescreenNet(p)
+        loss, g = net.loss_and_grad(X, y, 6.0)
+        lo, go = mlp.loss_and_grad(p, X, y, 6.0)
+        assert loss == pytest.approx(lo, rel=1e-5)
+        g = g.cpu().numpy()
+        assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+        # bitwise reproducible
+        assert np.array_equal(net.loss_and_grad(X, y, 6.0)[1].cpu().numpy(), g)
+
+
+def test_adam_matches_oracle(gpu):
+    P = pkg("prescreen")
+    p, X, y = rand_case(6, 500)
+    net = P.PrescreenNet(p)
+    _, g = net.loss_and_grad(X, y, 3.0)
+    net.adam_step(g, 1e-2)
+    pn, _, _ = mlp.adam_step(p.astype(np.float64), g.cpu().numpy().astype(np.float64), np.zeros(p.size),
+                             np.zeros(p.size), 1, 1e-2)
+    np.testing.assert_allclose(net.host_params(), pn, rtol=1e-5, atol=1e-7)
+
+
+def test_training_separates_a_learnable_rule(gpu):
+    """Label = first feature below a cut: the network trained by the library's
+    kernels reaches high recall and precision."""
+    P = pkg("prescreen")
+    rng = np.random.default_rng(7)
+    X = rng.normal(0, 1, (20000, 8)).astype(np.float32)
+    y = (X[:, 0] + 0.5 * X[:, 3] < -1.2).astype(np.float32)
+    net = P.PrescreenNet(seed=3)
+    hist = net.fit(X, y, epochs=400, lr=3e-3)
+    assert hist[-1] < 0.2 * hist[0]
+    z = net.logits(X).cpu().numpy()
+    pred = z >= 0.0
+    recall = (pred & (y > 0.5)).sum() / (y > 0.5).sum()
+    precision = (pred & (y > 0.5)).sum() / max(pred.sum(), 1)
+    assert recall > 0.97 and precision > 0.7
+
+
+def squeeze_case(force, n_wires=7, n_elems=20, length=20.0):
+    """Dynamic radial squeeze of a strand, frictionless (the configuration of
+    tests/golden/newton_7wire.npz, with another force / size)."""
+    g = golden("newton_7wire.npz")
+    k_pen, mu, t_total, rho_inf, dtmax, sd, tolf, told, maxa, ktr = g["params"]
+    M = pkg("mesh")
+    E_ = pkg("elements")
+    C = pkg("context")
+    m = M.make_strand(n_wires, 1.0, length, n_elems, lay_angle_deg=10.0)
+    X, conn = m.node_coords, m.connectivity
+    nn = n_elems + 1
+    first = np.arange(n_wires) * nn
+    fixed = (first[:, None] * 6 + np.arange(6)[None, :]).ravel()
+    mid = first[1:] + nn // 2
+    nrm = X[mid, :2] / np.linalg.norm(X[mid, :2], axis=1)[:, None]
+    f = np.zeros(6 * len(X))
+    f[mid * 6] = -force * nrm[:, 0]
+    f[mid * 6 + 1] = -force * nrm[:, 1]
+    s = C.circle_section(1.0, 200e3, 0.3)
+    Md = E_.ULCRBeamAssembler(X, conn, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z).lumped_mass_diag(7.85e-9)
+    return dict(X=X, conn=conn, radii=m.radii, layer=m.elem_layer(), fixed=fixed, f=f, M=Md, k_pen=k_pen, mu=mu,
+                t_total=t_total, rho_inf=rho_inf, sd=sd, ktr=ktr,
+                cfg=dict(max_nr_attempts=int(maxa), tol_force=tolf, tol_disp=told, dt_max_fraction=dtmax))
+
+
+def run(case, precision="fp64", net=None, log=False):
+    S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
+    ctx = C.DeviceContext(case["X"], case["conn"], C.circle_section(1.0, 200e3, 0.3), radii=case["radii"],
+                          precision=precision)
+    kcfg = Kc.ContactConfig(mu=case["mu"], k_t_ratio=case["ktr"], exclude_same_layer=True,
+                            elem_layer_map=wl.layer_map(case["layer"]))
+    b = S.DeviceBackend(ctx, M_diag=case["M"], rho_inf=case["rho_inf"], k_pen=case["k_pen"], smoothing_delta=case["sd"],
+                        mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg)
+    if net is not None:
+        b.use_prescreen(net, elem_layer=case["layer"])
+    if log:
+        b.log_features(True, elem_layer=case["layer"])
+    res = S.ContactFrictionSolver(b, fixed_dofs=case["fixed"], f_ext_total=case["f"], t_total=case["t_total"],
+                                  config=S.SolverConfig(**case["cfg"])).solve()
+    return res, b
+
+
+def training_set(runs):
+    P = pkg("prescreen")
+    Xs, ys = [], []
+    for _, b in runs:
+        for feat, gap in b.feature_log:
+            x, y = P.labels_of(feat, gap)
+            Xs.append(x)
+            ys.append(y)
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def same_trajectory(a, b):
+    assert len(a.calls) == len(b.calls)
+    for p, q in zip(a.calls, b.calls):
+        assert (p.load_frac, p.n_attempts, p.n_active, p.converged) == (q.load_frac, q.n_attempts, q.n_active,
+                                                                         q.converged)
+        assert p.res_history == q.res_history     # bitwise: the contributing pairs are always evaluated
+    assert np.array_equal(a.u, b.u)
+
+
+def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
+    """Train on features logged from full-path runs of three squeezes, then
+    solve held-out squeezes with the prescreen: identical trajectory bit for
+    bit (attempts, active counts, residual histories, final u), and the share of
+    pairs the narrowphase visits per Newton attempt against the method without
+    it (all candidates)."""
+    P = pkg("prescreen")
+    train = [run(squeeze_case(f), log=True) for f in (80.0, 100.0, 120.0)]
+    X, y = training_set(train)
+    assert y.sum() > 0
+    net = P.PrescreenNet(seed=0)
+    hist = net.fit(X, y, epochs=300, lr=3e-3)
+    assert hist[-1] < hist[0]
+    net.choose_threshold(X, y, margin=1.0)
+    for case in (squeeze_case(110.0), squeeze_case(100.0, n_wires=19, n_elems=20)):
+        full, _ = run(case)
+        lp, _ = run(case, net=net)
+        same_trajectory(full, lp)
+        watched = sum(w for w, _ in lp.watch) / sum(n for _, n in lp.watch)
+        print(f"watched {watched:.3f} of candidate pairs per attempt, rechecks {lp.n_rechecks} "
+              f"of {len(lp.calls)} calls")
+        assert watched < 0.5
+        assert lp.n_rechecks <= len(lp.calls) // 4

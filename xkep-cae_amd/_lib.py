@@ -25,6 +25,10 @@ PREC = {"fp64": 0, "mixed": 1}
 OPT_STAGED = 256
 
 
+class XkTuning(C.Structure):
+    _fields_ = [("cell_factor", C.c_double), ("pair_block", C.c_int), ("gather_block", C.c_int)]
+
+
 class XkDyn(C.Structure):
     _fields_ = [("M_diag", vp), ("u_pred", vp), ("v_pred", vp), ("acc_old", vp), ("acc", vp), ("vel", vp),
                 ("dt", C.c_double), ("alpha_m", C.c_double), ("beta", C.c_double), ("gamma", C.c_double)]
@@ -56,6 +60,20 @@ SIGNATURES = {
     "xk_status": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "xk_profile_begin": (C.c_int, [vp, C.c_int]),
     "xk_profile_end": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
+    "xk_ctx_set_tuning": (C.c_int, [vp, C.POINTER(XkTuning)]),
+    "xk_ctx_get_tuning": (C.c_int, [vp, C.POINTER(XkTuning)]),
+    "xk_lp_set_params": (C.c_int, [vp, vp, C.c_int, C.c_float]),
+    "xk_lp_features": (C.c_int, [vp, vp, vp, vp, C.c_float, vp, vp]),
+    "xk_lp_select": (C.c_int, [vp, c_int64_p, vp]),
+    "xk_lp_judge": (C.c_int, [vp, vp, vp, C.c_double, c_int64_p, vp]),
+    "xk_lp_off": (C.c_int, [vp]),
+    "xk_pairs_snapshot": (C.c_int, [vp]),
+    "xk_pairs_restore": (C.c_int, [vp]),
+    "xk_mlp_n_params": (C.c_int, []),
+    "xk_mlp_grad_work": (C.c_int64, [C.c_int64]),
+    "xk_mlp_forward": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
+    "xk_mlp_grad": (C.c_int, [vp, vp, C.c_int64, vp, C.c_float, vp, vp, C.c_int64, vp]),
+    "xk_mlp_adam": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
 }
 
 

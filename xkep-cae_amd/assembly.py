@@ -55,6 +55,29 @@ class StrandAssembly:
         self.dt = float(dt)
         self._dyn = time.dyn_struct(self.dt) if time is not None else None
         self.graph = None
+        # learned prescreen (set_prescreen): network, feature layer ids, logging
+        self.prescreen = None
+        self.lp_layer = None
+        self.lp_scale = 0.125
+        self.lp_log = None
+        self.n_watch = None
+        self._lp_feat = None
+
+    def set_prescreen(self, net, *, elem_layer=None, layer_scale: float = 0.125):
+        """Run the per-iteration pair kernel over the learned watch list (None: all pairs)."""
+        self.prescreen = net
+        self.lp_layer = elem_layer if elem_layer is not None else self._layer
+        self.lp_scale = layer_scale
+        if net is not None:
+            self.ctx.lp_set(net.host_params(), net.threshold)
+
+    def log_features(self, on: bool = True, *, elem_layer=None, layer_scale: float = 0.125):
+        """Record (features at increment start, pair gaps at the end of the
+        Newton call) for every call, as training data."""
+        self.lp_log = [] if on else None
+        if elem_layer is not None or self.lp_layer is None:
+            self.lp_layer = elem_layer if elem_layer is not None else self._layer
+        self.lp_scale = layer_scale
 
     def set_u(self, u):
         self.u.copy_(self.ctx.to_dev(u))
@@ -91,10 +114,29 @@ class StrandAssembly:
         if geometry:
             self.ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, coords=x, g_on=cfg.g_on, g_off=cfg.g_off)
             self.K = self.ctx.new_K()  # builds the pattern (synchronous)
+            self.n_watch = None
+            if self.prescreen is not None or self.lp_log is not None:
+                f = self.ctx.lp_features(self.u, self.lp_layer, self.lp_scale, log=self.lp_log is not None)
+                self._lp_feat = None if f is None else f.cpu().numpy()
+                if self.prescreen is not None:
+                    self.n_watch = self.ctx.lp_select()
         else:
             self.K = None
         self.graph = None
         return self
+
+    def end_call(self) -> int:
+        """After a Newton call: the judge (exact narrowphase over the pairs the
+        prescreen left out, at the last assembled u) -> #misses; logs the
+        training rows when logging."""
+        miss = 0
+        if self.n_watch is not None:
+            miss = self.ctx.lp_judge(self.u, self.cfg.g_on)
+            self.n_watch = None
+        if self.lp_log is not None and self._lp_feat is not None:
+            self.lp_log.append((self._lp_feat, self.ctx.get_pairs()["gap"].copy()))
+            self._lp_feat = None
+        return miss
 
     def count_penetrations(self, u=None) -> int:
         """#(gap < 0) over the pair list at x = X + u without changing any pair

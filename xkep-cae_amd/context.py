@@ -304,6 +304,55 @@ class DeviceContext:
         L.check(L.lib().xk_profile_end(self.h, ms.ctypes.data, C.byref(n)), "xk_profile_end")
         return ms, n.value
 
+    # ------------------------------------------------------------ launch shapes
+    def get_tuning(self) -> dict:
+        t = L.XkTuning()
+        L.check(L.lib().xk_ctx_get_tuning(self.h, C.byref(t)), "xk_ctx_get_tuning")
+        return {"cell_factor": t.cell_factor, "pair_block": t.pair_block, "gather_block": t.gather_block}
+
+    def set_tuning(self, *, cell_factor=None, pair_block=None, gather_block=None):
+        cur = self.get_tuning()
+        t = L.XkTuning(cell_factor=float(cell_factor if cell_factor is not None else cur["cell_factor"]),
+                       pair_block=int(pair_block if pair_block is not None else cur["pair_block"]),
+                       gather_block=int(gather_block if gather_block is not None else cur["gather_block"]))
+        L.check(L.lib().xk_ctx_set_tuning(self.h, C.byref(t)), "xk_ctx_set_tuning")
+
+    # ------------------------------------------------------------ learned prescreen
+    def lp_set(self, params, threshold: float):
+        p = np.ascontiguousarray(np.asarray(params, np.float32))
+        L.check(L.lib().xk_lp_set_params(self.h, p.ctypes.data, int(p.size), float(threshold)), "xk_lp_set_params")
+
+    def lp_features(self, u=None, elem_layer=None, layer_scale: float = 0.125, log: bool = False):
+        """Features of every pair at X + u (after the increment's geometry update);
+        returns them as an (n_pairs, 8) float32 device tensor when ``log``."""
+        torch = self.torch
+        u = None if u is None else self.to_dev(u)
+        layer = None if elem_layer is None else self.to_dev(elem_layer, torch.int32)
+        out = torch.empty((max(self.n_pairs, 1), 8), dtype=torch.float32, device=self.dev) if log else None
+        L.check(L.lib().xk_lp_features(self.h, L.ptr(self.X), L.ptr(u), L.ptr(layer), float(layer_scale),
+                                       L.ptr(out), self.stream), "xk_lp_features")
+        return None if out is None else out[: self.n_pairs]
+
+    def lp_select(self) -> int:
+        n = C.c_int64()
+        L.check(L.lib().xk_lp_select(self.h, C.byref(n), self.stream), "xk_lp_select")
+        return n.value
+
+    def lp_judge(self, u, g_on: float = 0.0) -> int:
+        n = C.c_int64()
+        L.check(L.lib().xk_lp_judge(self.h, L.ptr(self.X), L.ptr(self.to_dev(u)), float(g_on), C.byref(n),
+                                    self.stream), "xk_lp_judge")
+        return n.value
+
+    def lp_off(self):
+        L.check(L.lib().xk_lp_off(self.h), "xk_lp_off")
+
+    def pairs_snapshot(self):
+        L.check(L.lib().xk_pairs_snapshot(self.h), "xk_pairs_snapshot")
+
+    def pairs_restore(self):
+        L.check(L.lib().xk_pairs_restore(self.h), "xk_pairs_restore")
+
     def status_flags(self) -> int:
         f = C.c_int()
         L.check(L.lib().xk_status(self.h, C.byref(f)))

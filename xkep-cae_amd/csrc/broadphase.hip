@@ -77,10 +77,6 @@ __global__ void scan_add_kernel(int64_t n, int64_t* __restrict__ out, const int6
     if (i < n) out[i] += offs[i / SCAN_TILE];
 }
 
-// out = exclusive scan of in (n entries); returns the total in *total_host
-// (synchronous).  Uses ctx scratch slots [s0, s0 + 4).
-static int scan_exclusive(Ctx* c, int s0, const int64_t* in, int64_t* out, int64_t n, int64_t* total_host,
-                          hipStream_t st);
 
 static int scan_rec(Ctx* c, int slot, const int64_t* in, int64_t* out, int64_t n, hipStream_t st) {
     if (n <= 0) return 0;
@@ -103,8 +99,10 @@ static int scan_rec(Ctx* c, int slot, const int64_t* in, int64_t* out, int64_t n
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-static int scan_exclusive(Ctx* c, int s0, const int64_t* in, int64_t* out, int64_t n, int64_t* total_host,
-                          hipStream_t st) {
+// out = exclusive scan of in (n entries); returns the total in *total_host
+// (synchronous).  Uses ctx scratch slots [s0, s0 + 4).
+int scan_exclusive_i64(Ctx* c, int s0, const int64_t* in, int64_t* out, int64_t n, int64_t* total_host,
+                       hipStream_t st) {
     if (total_host) *total_host = 0;
     if (n <= 0) return 0;
     if (scan_rec(c, s0, in, out, n, st) != 0) return -1;
@@ -497,7 +495,7 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
     hipLaunchKernelGGL(i32_to_i64_kernel, dim3(nblk(ncell)), dim3(256), 0, st, ncell, gcnt, gcnt64);
     BP_TRY(hipGetLastError());
     int64_t total = 0;
-    if (scan_exclusive(c, 0, gcnt64, gstart, ncell, &total, st) != 0) return -1;
+    if (scan_exclusive_i64(c, 0, gcnt64, gstart, ncell, &total, st) != 0) return -1;
     BP_TRY(hipMemcpyAsync(gstart + ncell, &total, 8, hipMemcpyHostToDevice, st));
     BP_TRY(c->bp[S_MEM].ensure(sizeof(int) * (size_t)std::max<int64_t>(total, 1)));
     int* members = (int*)c->bp[S_MEM].p;
@@ -513,7 +511,7 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
                        (int*)nullptr, (int*)nullptr);
     BP_TRY(hipGetLastError());
     int64_t npairs = 0;
-    if (scan_exclusive(c, 0, cnt, off, n, &npairs, st) != 0) return -1;
+    if (scan_exclusive_i64(c, 0, cnt, off, n, &npairs, st) != 0) return -1;
     BP_TRY(c->bp[S_RAW].ensure(sizeof(int) * 2 * (size_t)std::max<int64_t>(npairs, 1)));
     int* I = (int*)c->bp[S_RAW].p;
     int* J = I + std::max<int64_t>(npairs, 1);
@@ -613,7 +611,7 @@ int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double 
         hipLaunchKernelGGL(prescreen_kernel, dim3(nblk(n1)), dim3(256), 0, st, n1, I, J, coords,
                            (const int*)c->d_conn.p, (const double*)c->d_rad.p, prescreen_extra, keep);
         BP_TRY(hipGetLastError());
-        if (scan_exclusive(c, 0, keep, pos, n1, &n2, st) != 0) return -1;
+        if (scan_exclusive_i64(c, 0, keep, pos, n1, &n2, st) != 0) return -1;
         BP_TRY(c->bp[S_FILT].ensure(sizeof(int) * 2 * (size_t)std::max<int64_t>(n2, 1)));
         int* If = (int*)c->bp[S_FILT].p;
         int* Jf = If + std::max<int64_t>(n2, 1);
@@ -663,7 +661,7 @@ int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double 
         hipLaunchKernelGGL(newflag_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, I2, J2, np,
                            (const int64_t*)c->bp[S_KEYS].p, isnew);
         BP_TRY(hipGetLastError());
-        if (scan_exclusive(c, 0, isnew, pos, n2, &n_new, st) != 0) return -1;
+        if (scan_exclusive_i64(c, 0, isnew, pos, n2, &n_new, st) != 0) return -1;
         if (n_new > 0) {
             if (c->grow_pairs(np + n_new) != 0) return -1;
             hipLaunchKernelGGL(append_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, isnew, pos, I2, J2,
@@ -684,6 +682,7 @@ int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double 
     }
     BP_TRY(hipStreamSynchronize(st));
     c->pattern_valid = false;
+    c->lp_on = false;
     return 0;
 }
 

@@ -33,10 +33,8 @@
 
 namespace xk {
 
-constexpr int GB = 256;  // lanes per workgroup
-
 // node_mode: -1 no residual work, 0 = f_int only, 1 = pair force only, 2 = full residual
-template <class KT, class ST, class GT>
+template <class KT, class ST, class GT, int GB>
 __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __restrict__ Ke,
                                                      const GT* __restrict__ sg, KT* __restrict__ K) {
     __shared__ double red[GB];
@@ -221,22 +219,37 @@ __global__ void fill_kernel(size_t n, T* p, T v) {
 
 // ---------------------------------------------------------------- launchers
 
-int gather_node_blocks(int n_nodes) { return (6 * n_nodes + GB - 1) / GB; }
+int gather_node_blocks(int n_nodes, int bs) { return (6 * n_nodes + bs - 1) / bs; }
+
+bool gather_block_ok(int bs) { return bs == 128 || bs == 256 || bs == 512; }
+
+template <class KT, class GT, int BS>
+static void launch_gather_bs(int nb, const GatherArgs& g, const void* Ke, const void* sg, void* K, hipStream_t st) {
+    hipLaunchKernelGGL((gather_kernel<KT, KT, GT, BS>), dim3(nb), dim3(BS), 0, st, g, (const KT*)Ke, (const GT*)sg,
+                       (KT*)K);
+}
+
+template <class T>
+static hipError_t launch_gather_t(int nb, const GatherArgs& g, const void* Ke, const void* sg, void* K,
+                                  hipStream_t st) {
+    switch (g.block) {
+        case 128: launch_gather_bs<T, T, 128>(nb, g, Ke, sg, K, st); break;
+        case 256: launch_gather_bs<T, T, 256>(nb, g, Ke, sg, K, st); break;
+        case 512: launch_gather_bs<T, T, 512>(nb, g, Ke, sg, K, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st,
                          hipEvent_t between) {
     if (K && g.use_pair && !sg) return hipErrorInvalidValue;  // pair terms need the tangent records
-    g.node_blocks = g.node_mode >= 0 ? gather_node_blocks(g.n_nodes) : 0;
-    const int kb = K ? ((g.direct ? g.n_cdesc : 3 * g.n_blocks) + GB - 1) / GB : 0;
+    if (!g.block) g.block = 256;
+    g.node_blocks = g.node_mode >= 0 ? gather_node_blocks(g.n_nodes, g.block) : 0;
+    const int kb = K ? ((g.direct ? g.n_cdesc : 3 * g.n_blocks) + g.block - 1) / g.block : 0;
     const int nb = g.node_blocks + kb;
     if (nb <= 0) return hipSuccess;
-    if (prec == 0)
-        hipLaunchKernelGGL((gather_kernel<double, double, double>), dim3(nb), dim3(GB), 0, st, g,
-                           (const double*)Ke, (const double*)sg, (double*)K);
-    else
-        hipLaunchKernelGGL((gather_kernel<float, float, float>), dim3(nb), dim3(GB), 0, st, g, (const float*)Ke,
-                           (const float*)sg, (float*)K);
-    hipError_t e = hipGetLastError();
+    hipError_t e = prec == 0 ? launch_gather_t<double>(nb, g, Ke, sg, K, st) : launch_gather_t<float>(nb, g, Ke, sg, K, st);
     if (e != hipSuccess) return e;
     if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
     if (g.partial && (g.res_norm2 || g.n_active))

@@ -117,6 +117,7 @@ void Ctx::bind_pairs() {
 
 int Ctx::build_pattern() {
     XK_TRY(hipDeviceSynchronize());
+    lp_on = false;  // a watch list belongs to the pattern it was selected on
     const int64_t np = n_pairs;
     std::vector<int8_t> st(np);
     std::vector<double> pn(np);
@@ -265,6 +266,7 @@ NodeLists Ctx::node_lists() const {
 
 GatherArgs Ctx::gather_args(int node_mode) const {
     GatherArgs g{};
+    g.block = gather_block;
     g.n_nodes = n_nodes;
     g.n_elems = n_elems;
     g.n_sinc = n_sinc;
@@ -394,7 +396,7 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
     if (!rc && c->d_Ke.ensure((size_t)std::max(n_elems, 1) * 144 * (precision == 0 ? 8 : 4)) != hipSuccess)
         fail("Ke stage");
     if (!rc && c->d_flags.ensure(64) != hipSuccess) fail("flags");
-    if (!rc && c->d_partial.ensure(sizeof(double) * (size_t)(gather_node_blocks(n_nodes) + 1)) != hipSuccess)
+    if (!rc && c->d_partial.ensure(sizeof(double) * (size_t)(gather_node_blocks(n_nodes, 128) + 1)) != hipSuccess)
         fail("partials");
     if (!rc && hipMemset(c->d_flags.p, 0, 64) != hipSuccess) fail("flags init");
     if (!rc && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) fail("side stream");
@@ -468,6 +470,8 @@ int xk_pairs_set(xk_ctx* c, int64_t n, const int32_t* ea, const int32_t* eb, con
     XK_TRY(hipDeviceSynchronize());
     if (c->grow_pairs(std::max<int64_t>(n, 1)) != 0) return -1;
     c->n_pairs = n;
+    c->lp_gprev_n = 0;
+    c->lp_on = false;
     c->keys_valid = false;
     c->filtered_valid = false;
     if (n == 0) {
@@ -596,7 +600,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     pp.allow_deact = allow_deactivation;
     pp.n_sinc = c->n_sinc;
     XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_sd.p,
-                        c->d_sg.p, nullptr, (hipStream_t)stream));
+                        c->d_sg.p, nullptr, (hipStream_t)stream, nullptr, c->pair_block));
     if (modes & PAIR_HYST) c->pattern_valid = false;
     c->last_modes = modes;
     return 0;
@@ -677,7 +681,12 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.allow_deact = 0;
     pp.n_sinc = c->n_sinc;
     const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
-    const int npb = pair_blocks(c->n_pairs);
+    // with the prescreen on, the pair kernel visits the watch list (every
+    // contributing pair and the ones the network keeps) instead of all pairs
+    const bool lp = c->lp_on;
+    const int64_t n_work = lp ? c->n_watch : c->n_pairs;
+    const int* work_idx = lp ? (const int*)c->lp_watch.p : nullptr;
+    const int npb = pair_blocks(n_work, c->pair_block);
     XK_TRY(c->d_act.ensure(sizeof(int) * (size_t)std::max(npb, 1)));
     int* act = (int*)c->d_act.p;
     void* Ke = K_vals ? c->d_Ke.p : nullptr;
@@ -701,7 +710,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     g.partial = (double*)c->d_partial.p;
     g.res_norm2 = res_norm2;
     g.act_part = act;
-    g.n_act_part = c->n_pairs > 0 ? npb : 0;
+    g.n_act_part = n_work > 0 ? npb : 0;
     g.n_active = n_active;
     // the direct element kernel writes the beam blocks and the mass term into K
     const double emass = c->direct ? g.mass_coef : 0.0;
@@ -710,8 +719,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     if (ev) {
         // profiled: the three stages in sequence on the caller's stream
         XK_TRY(hipEventRecord(ev[0], st));
-        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(launch_pairs(c->prec, n_work, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st, work_idx, c->pair_block));
         XK_TRY(hipEventRecord(ev[1], st));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipEventRecord(ev[2], st));
@@ -721,8 +730,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, c->side));
         XK_TRY(hipEventRecord(c->ev_join, c->side));
-        XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st));
+        XK_TRY(launch_pairs(c->prec, n_work, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st, work_idx, c->pair_block));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
     XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st, ev ? ev[3] : nullptr));
@@ -760,6 +769,196 @@ int xk_profile_end(xk_ctx* c, double* stage_ms, int* n_calls) {
     c->prof_ev.clear();
     c->prof_max = 0;
     c->prof_calls = 0;
+    return 0;
+}
+
+// ------------------------------------------------------------ learned prescreen
+int xk_lp_set_params(xk_ctx* c, const float* params, int n_params, float threshold) {
+    if (!c || !params || n_params != lp_n_params()) {
+        set_error("xk_lp_set_params: expected " + std::to_string(lp_n_params()) + " parameters");
+        return -1;
+    }
+    XK_TRY(c->lp_prm.ensure(sizeof(float) * (size_t)n_params));
+    XK_TRY(hipMemcpy(c->lp_prm.p, params, sizeof(float) * (size_t)n_params, hipMemcpyHostToDevice));
+    c->lp_thr = threshold;
+    c->lp_ready = true;
+    return 0;
+}
+
+int xk_lp_features(xk_ctx* c, const double* coords_ref, const double* u, const int32_t* elem_layer,
+                   float layer_scale, float* features_out, void* stream) {
+    if (!c || !coords_ref) {
+        set_error("xk_lp_features: null argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = c->n_pairs;
+    if (n > c->lp_gprev_cap) {
+        // grow the previous-gap column, keeping the entries of the pairs it already holds
+        const int64_t nc = std::max<int64_t>(n, c->lp_gprev_cap * 3 / 2);
+        DBuf nb;
+        XK_TRY(nb.ensure(sizeof(double) * (size_t)nc));
+        if (c->lp_gprev_n > 0)
+            XK_TRY(hipMemcpyAsync(nb.p, c->lp_gprev.p, sizeof(double) * (size_t)c->lp_gprev_n,
+                                  hipMemcpyDeviceToDevice, st));
+        XK_TRY(hipStreamSynchronize(st));
+        c->lp_gprev.release();
+        c->lp_gprev = nb;
+        c->lp_gprev_cap = nc;
+    }
+    XK_TRY(c->lp_feat.ensure(sizeof(uint16_t) * 8 * (size_t)std::max<int64_t>(n, 1)));
+    XK_TRY(launch_lp_features(n, c->P, c->ea, c->eb, coords_ref, u, elem_layer, layer_scale,
+                              (double*)c->lp_gprev.p, std::min(c->lp_gprev_n, n), (uint16_t*)c->lp_feat.p,
+                              features_out, st));
+    c->lp_gprev_n = n;
+    return 0;
+}
+
+int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
+    if (!c) return -1;
+    if (!c->lp_ready) {
+        set_error("xk_lp_select: no parameters (xk_lp_set_params)");
+        return -1;
+    }
+    if (c->lp_gprev_n != c->n_pairs) {
+        set_error("xk_lp_select: features are not those of the current pair list (xk_lp_features)");
+        return -1;
+    }
+    if (c->ensure_pattern() != 0) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = c->n_pairs;
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    XK_TRY(c->lp_keep.ensure(sizeof(int64_t) * nn));
+    XK_TRY(c->lp_pos.ensure(sizeof(int64_t) * nn));
+    XK_TRY(c->lp_watch.ensure(sizeof(int) * nn));
+    XK_TRY(c->lp_prune.ensure(sizeof(int) * nn));
+    XK_TRY(launch_lp_infer_bf16(n, (const uint16_t*)c->lp_feat.p, (const float*)c->lp_prm.p, c->lp_thr, c->P.sinc,
+                                (int64_t*)c->lp_keep.p, nullptr, st));
+    int64_t nw = 0;
+    if (scan_exclusive_i64(c, 0, (const int64_t*)c->lp_keep.p, (int64_t*)c->lp_pos.p, n, &nw, st) != 0) return -1;
+    XK_TRY(launch_lp_split(n, (const int64_t*)c->lp_keep.p, (const int64_t*)c->lp_pos.p, (int*)c->lp_watch.p,
+                           (int*)c->lp_prune.p, st));
+    XK_TRY(hipStreamSynchronize(st));
+    c->n_watch = nw;
+    c->n_prune = n - nw;
+    c->lp_on = true;
+    if (n_watch) *n_watch = nw;
+    return 0;
+}
+
+int xk_lp_judge(xk_ctx* c, const double* coords_ref, const double* u, double g_on, int64_t* n_miss, void* stream) {
+    if (!c || !coords_ref) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    int64_t miss = 0;
+    if (c->lp_on && c->n_prune > 0) {
+        // exact narrowphase (geometry only, active set frozen) over the pruned pairs
+        PairParams pp{};
+        pp.n_sinc = c->n_sinc;
+        XK_TRY(launch_pairs(c->prec, c->n_prune, c->P, coords_ref, u, nullptr, PAIR_GEOM, pp, (double*)c->d_sd.p,
+                            c->d_sg.p, nullptr, st, (const int*)c->lp_prune.p, c->pair_block));
+        XK_TRY(c->lp_count.ensure(sizeof(int)));
+        XK_TRY(launch_fill_i32(1, (int*)c->lp_count.p, 0, st));
+        XK_TRY(launch_lp_miss(c->n_prune, (const int*)c->lp_prune.p, c->P.gap, g_on, (int*)c->lp_count.p, st));
+        int h = 0;
+        XK_TRY(hipMemcpyAsync(&h, c->lp_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        XK_TRY(hipStreamSynchronize(st));
+        miss = h;
+    }
+    c->lp_on = false;
+    if (n_miss) *n_miss = miss;
+    return 0;
+}
+
+int xk_lp_off(xk_ctx* c) {
+    if (!c) return -1;
+    c->lp_on = false;
+    return 0;
+}
+
+int xk_pairs_snapshot(xk_ctx* c) {
+    if (!c) return -1;
+    c->snap.resize(PB_COUNT);
+    for (int k = 0; k < PB_COUNT; ++k) {
+        XK_TRY(c->snap[k].ensure(std::max<size_t>(c->pb[k].bytes, 1)));
+        if (c->pb[k].bytes)
+            XK_TRY(hipMemcpy(c->snap[k].p, c->pb[k].p, c->pb[k].bytes, hipMemcpyDeviceToDevice));
+    }
+    c->snap_n = c->n_pairs;
+    return 0;
+}
+
+int xk_pairs_restore(xk_ctx* c) {
+    if (!c) return -1;
+    if (c->snap_n != c->n_pairs || (int)c->snap.size() != PB_COUNT) {
+        set_error("xk_pairs_restore: the pair list changed since the snapshot");
+        return -1;
+    }
+    for (int k = 0; k < PB_COUNT; ++k) {
+        if (c->snap[k].bytes < c->pb[k].bytes) {
+            set_error("xk_pairs_restore: the pair list grew since the snapshot");
+            return -1;
+        }
+        if (c->pb[k].bytes)
+            XK_TRY(hipMemcpy(c->pb[k].p, c->snap[k].p, c->pb[k].bytes, hipMemcpyDeviceToDevice));
+    }
+    return 0;
+}
+
+int xk_mlp_n_params(void) { return lp_n_params(); }
+
+int64_t xk_mlp_grad_work(int64_t n) { return (int64_t)lp_grad_blocks(n) * (lp_n_params() + 1); }
+
+int xk_mlp_forward(const float* X, int64_t n, const float* params, float* logits, void* stream) {
+    if (n < 0 || (n && (!X || !params || !logits))) {
+        set_error("xk_mlp_forward: invalid arguments");
+        return -1;
+    }
+    XK_TRY(launch_lp_infer_f32(n, X, params, 0.f, nullptr, logits, (hipStream_t)stream));
+    return 0;
+}
+
+int xk_mlp_grad(const float* X, const float* y, int64_t n, const float* params, float pos_weight,
+                float* grad_and_loss, float* work, int64_t work_len, void* stream) {
+    if (n <= 0 || !X || !y || !params || !grad_and_loss || !work) {
+        set_error("xk_mlp_grad: invalid arguments");
+        return -1;
+    }
+    if (work_len < xk_mlp_grad_work(n)) {
+        set_error("xk_mlp_grad: workspace too small (xk_mlp_grad_work)");
+        return -1;
+    }
+    XK_TRY(launch_lp_grad(n, X, y, params, pos_weight, work, grad_and_loss, (hipStream_t)stream));
+    return 0;
+}
+
+int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr, float beta1, float beta2,
+                float eps, void* stream) {
+    if (!params || !grad || !m || !v || step < 1) {
+        set_error("xk_mlp_adam: invalid arguments");
+        return -1;
+    }
+    XK_TRY(launch_lp_adam(params, grad, m, v, step, lr, beta1, beta2, eps, (hipStream_t)stream));
+    return 0;
+}
+
+int xk_ctx_set_tuning(xk_ctx* c, const xk_tuning* t) {
+    if (!c || !t) return -1;
+    if (!(t->cell_factor > 0.0) || !pair_block_ok(t->pair_block) || !gather_block_ok(t->gather_block)) {
+        set_error("xk_ctx_set_tuning: cell_factor > 0, pair_block in {64,128,256,512}, "
+                  "gather_block in {128,256,512}");
+        return -1;
+    }
+    c->cell_factor = t->cell_factor;
+    c->pair_block = t->pair_block;
+    c->gather_block = t->gather_block;
+    return 0;
+}
+
+int xk_ctx_get_tuning(const xk_ctx* c, xk_tuning* t) {
+    if (!c || !t) return -1;
+    t->cell_factor = c->cell_factor;
+    t->pair_block = c->pair_block;
+    t->gather_block = c->gather_block;
     return 0;
 }
 

@@ -54,19 +54,18 @@ __device__ __forceinline__ V3<double> min_axis(V3<double> v) {
     return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
 }
 
-constexpr int PAIR_BLOCK = 256;
-
-template <class GT>
-__global__ void __launch_bounds__(PAIR_BLOCK)
+template <class GT, int BS>
+__global__ void __launch_bounds__(BS)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
-            GT* __restrict__ sg, int* __restrict__ act_part) {
+            GT* __restrict__ sg, int* __restrict__ act_part, const int* __restrict__ idx) {
     const int lb = xcd_logical_block();
-    const int n_blk = (int)((n_pairs + PAIR_BLOCK - 1) / PAIR_BLOCK);
+    const int n_blk = (int)((n_pairs + BS - 1) / BS);
     if (lb >= n_blk) return;  // padding workgroup (uniform over the workgroup)
-    const int64_t i = (int64_t)lb * PAIR_BLOCK + threadIdx.x;
+    const int64_t w = (int64_t)lb * BS + threadIdx.x;
     int active_here = 0;
-    if (i < n_pairs) {
+    if (w < n_pairs) {
+        const int64_t i = idx ? (int64_t)idx[w] : w;
         const int64_t cap = P.cap;
         int8_t st = P.status[i];
         double s, t, gap;
@@ -315,37 +314,52 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
     }
     if (act_part) {
         // #(p_n > 0) of this workgroup: wave ballots, then one store per workgroup
-        __shared__ int wc[PAIR_BLOCK / 64];
+        __shared__ int wc[BS / 64];
         unsigned long long m = __ballot(active_here);
         if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
         __syncthreads();
         if (threadIdx.x == 0) {
             int c = 0;
-            for (int w = 0; w < PAIR_BLOCK / 64; ++w) c += wc[w];
+            for (int w = 0; w < BS / 64; ++w) c += wc[w];
             act_part[lb] = c;
         }
     }
 }
 
-template __global__ void pair_kernel<double>(int64_t, PairSoA, const double*, const double*,
-                                             const double*, int, PairParams, double*, double*, int*);
-template __global__ void pair_kernel<float>(int64_t, PairSoA, const double*, const double*,
-                                            const double*, int, PairParams, double*, float*, int*);
+int pair_blocks(int64_t n_pairs, int bs) { return (int)((n_pairs + bs - 1) / bs); }
 
-int pair_blocks(int64_t n_pairs) { return (int)((n_pairs + PAIR_BLOCK - 1) / PAIR_BLOCK); }
+bool pair_block_ok(int bs) { return bs == 64 || bs == 128 || bs == 256 || bs == 512; }
+
+template <class GT, int BS>
+static void launch_pairs_bs(int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u, const double* u_ref,
+                            int modes, const PairParams& pp, double* sd, GT* sg, int* act_part, hipStream_t st,
+                            const int* idx) {
+    dim3 blk(BS), grd((unsigned)xcd_padded(pair_blocks(n_pairs, BS)));
+    hipLaunchKernelGGL((pair_kernel<GT, BS>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part,
+                       idx);
+}
+
+template <class GT>
+static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
+                                 const double* u_ref, int modes, const PairParams& pp, double* sd, GT* sg,
+                                 int* act_part, hipStream_t st, const int* idx) {
+    switch (bs) {
+        case 64: launch_pairs_bs<GT, 64>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
+        case 128: launch_pairs_bs<GT, 128>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
+        case 256: launch_pairs_bs<GT, 256>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
+        case 512: launch_pairs_bs<GT, 512>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
-                        int* act_part, hipStream_t st) {
+                        int* act_part, hipStream_t st, const int* idx, int bs) {
     if (n_pairs <= 0) return hipSuccess;
-    dim3 blk(PAIR_BLOCK), grd((unsigned)xcd_padded(pair_blocks(n_pairs)));
     if (prec == 0)
-        hipLaunchKernelGGL(pair_kernel<double>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
-                           (double*)sg, act_part);
-    else
-        hipLaunchKernelGGL(pair_kernel<float>, grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd,
-                           (float*)sg, act_part);
-    return hipGetLastError();
+        return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (double*)sg, act_part, st, idx);
+    return launch_pairs_t<float>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (float*)sg, act_part, st, idx);
 }
 
 }  // namespace xk

@@ -81,6 +81,19 @@ struct Ctx {
     DBuf d_cdesc;   // int4 descriptors of the blocks with pair contributions
     DBuf d_fint;    // internal force written by the element kernel (6 n_nodes)
     int n_cdesc = 0;
+    // launch shapes (xk_ctx_set_tuning)
+    int pair_block = 256, gather_block = 256;
+    // learned prescreen: parameters, features, watch / prune lists of the increment
+    DBuf lp_prm;        // FP32 parameters
+    float lp_thr = 0.f;
+    bool lp_ready = false;
+    DBuf lp_feat, lp_gprev, lp_keep, lp_pos, lp_watch, lp_prune, lp_count;
+    int64_t lp_gprev_n = 0, lp_gprev_cap = 0;
+    int64_t n_watch = 0, n_prune = 0;
+    bool lp_on = false;          // the fused assembly runs the pair kernel over the watch list
+    // pair-state snapshot (xk_pairs_snapshot / xk_pairs_restore)
+    std::vector<DBuf> snap;
+    int64_t snap_n = -1;
 
     int grow_pairs(int64_t need);
     void bind_pairs();
@@ -100,6 +113,8 @@ struct Ctx {
         d_eslot.release();
         d_cdesc.release();
         d_fint.release();
+        for (DBuf* b : {&lp_prm, &lp_feat, &lp_gprev, &lp_keep, &lp_pos, &lp_watch, &lp_prune, &lp_count}) b->release();
+        for (auto& b : snap) b.release();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         prof_ev.clear();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -124,11 +139,35 @@ hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const doub
                               const BeamConst& bc, const int8_t* eflag, const int* eslot, const int* gptr,
                               double mass_coef, const double* M, double* f_int, void* K, int* flags,
                               hipStream_t st);
+// idx (nullable): the kernel visits pairs idx[0 .. n_pairs) instead of 0 .. n_pairs;
+// bs: workgroup size (64, 128, 256 or 512), act_part holds pair_blocks(n, bs) counts
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
-                        int* act_part, hipStream_t st);
-int pair_blocks(int64_t n_pairs);
-int gather_node_blocks(int n_nodes);
+                        int* act_part, hipStream_t st, const int* idx = nullptr, int bs = 256);
+bool pair_block_ok(int bs);
+bool gather_block_ok(int bs);
+// exclusive scan of n int64 values (scratch slots bp[s0..]); total to *total_host (synchronous)
+int scan_exclusive_i64(Ctx* c, int s0, const int64_t* in, int64_t* out, int64_t n, int64_t* total_host,
+                       hipStream_t st);
+// learned prescreen (prescreen.hip)
+int lp_n_params();
+int lp_grad_blocks(int64_t n);
+hipError_t launch_lp_features(int64_t n, const PairSoA& P, const int* ea, const int* eb, const double* Xr,
+                              const double* u, const int* elem_layer, float layer_scale, double* gprev,
+                              int64_t n_prev, uint16_t* feat, float* featf, hipStream_t st);
+hipError_t launch_lp_infer_bf16(int64_t n, const uint16_t* feat, const float* prm, float thr, const int* sinc,
+                                int64_t* keep, float* logits, hipStream_t st);
+hipError_t launch_lp_infer_f32(int64_t n, const float* feat, const float* prm, float thr, int64_t* keep,
+                               float* logits, hipStream_t st);
+hipError_t launch_lp_split(int64_t n, const int64_t* keep, const int64_t* pos, int* watch, int* prune,
+                           hipStream_t st);
+hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g_on, int* count, hipStream_t st);
+hipError_t launch_lp_grad(int64_t n, const float* X, const float* y, const float* prm, float pos_w, float* partial,
+                          float* grad, hipStream_t st);
+hipError_t launch_lp_adam(float* prm, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
+                          float eps, hipStream_t st);
+int pair_blocks(int64_t n_pairs, int bs);
+int gather_node_blocks(int n_nodes, int bs);
 // `between` (nullable) is recorded after the gather kernel, before the finish kernel
 hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg, void* K, hipStream_t st,
                          hipEvent_t between = nullptr);

@@ -51,6 +51,7 @@ struct GatherArgs {
     int n_nodes, n_elems, n_sinc, n_blocks;
     int node_mode;     // -1 none, 0 f_int, 1 pair force, 2 full residual
     int node_blocks;   // set by the launcher
+    int block;         // workgroup size of the gather (128, 256 or 512)
     NodeLists L;
     BlockLists B;
     const double* fe;  // element force staging (general path)

@@ -294,13 +294,16 @@ class SolverResult:
     total_attempts: int
     n_cutbacks: int
     calls: list
+    n_rechecks: int = 0          # Newton calls re-run on the full path after a prescreen miss
+    watch: list = field(default_factory=list)   # (pairs watched, pairs) per call with the prescreen on
 
 
 class ContactFrictionSolver:
     """The dynamic increment loop of ContactFrictionProcess
     (contact/solver/process.py:139-674) over a backend that assembles.
 
-    backend protocol:
+    backend protocol (the prescreen hooks watch_count / snapshot_pairs /
+    restore_pairs / end_call are optional):
       detect(u)                 DetectCandidatesProcess on X + u
       update_geometry(u)        UpdateGeometryProcess (hysteresis on) on X + u
       initial_penetrations(u)   #(gap < 0) over the pairs, no state change
@@ -337,7 +340,7 @@ class ContactFrictionSolver:
         ckpt = (u.copy(), u_ref.copy())
         b.checkpoint()
         stepping = AdaptiveStepping(self.t_total, cfg)
-        calls = []
+        calls, watches, n_rechecks = [], [], 0
         load_frac_prev, prev_n_active, n_incr, incr_count, total = 0.0, 0, 0, 0, 0
         while True:
             if cfg.max_increments > 0 and incr_count >= cfg.max_increments:
@@ -351,7 +354,21 @@ class ContactFrictionSolver:
             u = b.predict(u, dt)
             b.detect(u)
             b.update_geometry(u)
+            watch = b.watch_count() if hasattr(b, "watch_count") else None
+            if watch is not None:
+                u_start = u.copy()
+                b.snapshot_pairs()
+                watches.append(watch)
             step = newton_dynamic(b, u, u_ref, f_ext, self.fixed, dt, f_ref_norm, dynamic_ref, cfg)
+            if hasattr(b, "end_call") and b.end_call() > 0:
+                # a pruned pair ended the call within contact range: re-run the
+                # call on the full path from the same start
+                n_rechecks += 1
+                total += step.n_attempts
+                u = u_start
+                b.restore_pairs()
+                step = newton_dynamic(b, u, u_ref, f_ext, self.fixed, dt, f_ref_norm, dynamic_ref, cfg)
+                b.end_call()
             total += step.n_attempts
             calls.append(NewtonCall(load_frac, load_frac_prev, dt, step.converged, step.n_attempts, step.n_active,
                                     step.diverged, step.res_history, step.active_history))
@@ -361,7 +378,7 @@ class ContactFrictionSolver:
                     b.restore_checkpoint()
                     n_incr -= 1
                     continue
-                return SolverResult(u, False, n_incr, total, stepping.n_cutbacks, calls)
+                return SolverResult(u, False, n_incr, total, stepping.n_cutbacks, calls, n_rechecks, watches)
             b.correct(u, dt)
             stepping.success(load_frac, load_frac_prev, step.n_attempts, step.n_active, prev_n_active)
             prev_n_active = step.n_active
@@ -369,7 +386,7 @@ class ContactFrictionSolver:
             u_ref = u.copy()
             ckpt = (u.copy(), u_ref.copy())
             b.checkpoint()
-        return SolverResult(u, True, n_incr, total, stepping.n_cutbacks, calls)
+        return SolverResult(u, True, n_incr, total, stepping.n_cutbacks, calls, n_rechecks, watches)
 
 
 # ---------------------------------------------------------------- device backend
@@ -429,3 +446,31 @@ class DeviceBackend:
 
     def restore_checkpoint(self):
         self.ga.restore_checkpoint()
+
+    # ---- learned prescreen hooks (optional part of the backend protocol)
+    def use_prescreen(self, net, *, elem_layer=None, layer_scale: float = 0.125):
+        self.asm.set_prescreen(net, elem_layer=elem_layer, layer_scale=layer_scale)
+
+    def log_features(self, on: bool = True, **kw):
+        self.asm.log_features(on, **kw)
+
+    @property
+    def feature_log(self):
+        return self.asm.lp_log
+
+    def watch_count(self):
+        """(watched pairs, pairs) of the increment when the prescreen is on, else None."""
+        if self.asm.n_watch is None:
+            return None
+        return self.asm.n_watch, self.ctx.n_pairs
+
+    def snapshot_pairs(self):
+        self.ctx.pairs_snapshot()
+
+    def restore_pairs(self):
+        self.ctx.pairs_restore()
+        self.ctx.lp_off()
+        self.asm.n_watch = None
+
+    def end_call(self) -> int:
+        return self.asm.end_call()
