@@ -224,6 +224,12 @@ int xk_mlp_grad(const float* X, const float* y, int64_t n, const float* params, 
 int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr, float beta1,
                 float beta2, float eps, void* stream);
 
+/* Per-element friction coefficients (host, n_elems, >= 0; NULL = off): a pair
+ * uses mu_eff x mu_elem[elem_a] instead of mu_eff.  No counterpart in the
+ * reference, whose solve has one coefficient: it lets independent solves with
+ * their own coefficients (mu_eff = 1) share one context and one launch. */
+int xk_ctx_set_friction(xk_ctx* ctx, const double* mu_elem);
+
 /* Launch shapes and the broadphase grid (no counterpart in the reference;
  * set by the autotuner): grid cell = cell_factor x mean box extent when a call
  * passes cell_size <= 0 (the candidate set does not depend on it), pair-kernel

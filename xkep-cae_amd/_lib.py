@@ -60,6 +60,7 @@ SIGNATURES = {
     "xk_status": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "xk_profile_begin": (C.c_int, [vp, C.c_int]),
     "xk_profile_end": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
+    "xk_ctx_set_friction": (C.c_int, [vp, vp]),
     "xk_ctx_set_tuning": (C.c_int, [vp, C.POINTER(XkTuning)]),
     "xk_ctx_get_tuning": (C.c_int, [vp, C.POINTER(XkTuning)]),
     "xk_lp_set_params": (C.c_int, [vp, vp, C.c_int, C.c_float]),

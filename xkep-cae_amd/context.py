@@ -304,6 +304,16 @@ class DeviceContext:
         L.check(L.lib().xk_profile_end(self.h, ms.ctypes.data, C.byref(n)), "xk_profile_end")
         return ms, n.value
 
+    def set_friction(self, mu_elem=None):
+        """Per-element friction coefficients (pair: mu_eff x mu_elem[elem_a]); None: off."""
+        if mu_elem is None:
+            L.check(L.lib().xk_ctx_set_friction(self.h, None), "xk_ctx_set_friction")
+            return
+        m = np.ascontiguousarray(np.asarray(mu_elem, float))
+        if m.shape != (self.n_elems,):
+            raise ValueError("one coefficient per element")
+        L.check(L.lib().xk_ctx_set_friction(self.h, m.ctypes.data), "xk_ctx_set_friction")
+
     # ------------------------------------------------------------ launch shapes
     def get_tuning(self) -> dict:
         t = L.XkTuning()

@@ -595,6 +595,8 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
     pp.k_t_ratio = k_t_ratio;
     pp.mu_eff = mu_eff;
+    pp.mu_elem = c->has_mu_elem ? (const double*)c->d_mu_elem.p : nullptr;
+    pp.ea = c->ea;
     pp.g_on = g_on;
     pp.g_off = g_off;
     pp.allow_deact = allow_deactivation;
@@ -676,6 +678,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
     pp.k_t_ratio = k_t_ratio;
     pp.mu_eff = mu_eff;
+    pp.mu_elem = c->has_mu_elem ? (const double*)c->d_mu_elem.p : nullptr;
+    pp.ea = c->ea;
     pp.g_on = 0.0;
     pp.g_off = 0.0;
     pp.allow_deact = 0;
@@ -938,6 +942,23 @@ int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, 
         return -1;
     }
     XK_TRY(launch_lp_adam(params, grad, m, v, step, lr, beta1, beta2, eps, (hipStream_t)stream));
+    return 0;
+}
+
+int xk_ctx_set_friction(xk_ctx* c, const double* mu_elem) {
+    if (!c) return -1;
+    if (!mu_elem) {
+        c->has_mu_elem = false;
+        return 0;
+    }
+    for (int e = 0; e < c->n_elems; ++e)
+        if (!(mu_elem[e] >= 0.0)) {
+            set_error("xk_ctx_set_friction: coefficients must be >= 0");
+            return -1;
+        }
+    XK_TRY(c->d_mu_elem.ensure(sizeof(double) * (size_t)std::max(c->n_elems, 1)));
+    XK_TRY(hipMemcpy(c->d_mu_elem.p, mu_elem, sizeof(double) * (size_t)c->n_elems, hipMemcpyHostToDevice));
+    c->has_mu_elem = true;
     return 0;
 }
 

@@ -178,7 +178,8 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 P.p_n[i] = p_n;
                 if (p_n > 1e-30) F = p_n * n;
             }
-            if ((modes & PAIR_FRIC) && p_n > 0.0 && pp.mu_eff > 0.0) {
+            const double mu = pp.mu_elem ? pp.mu_eff * pp.mu_elem[pp.ea[i]] : pp.mu_eff;
+            if ((modes & PAIR_FRIC) && p_n > 0.0 && mu > 0.0) {
                 double kp = P.k_pen[i], kt = P.k_t[i];
                 if (kp <= 0.0) {
                     kp = pp.k_pen;
@@ -193,7 +194,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 double q1 = P.z[i] + kt * du1;
                 double q2 = P.z[cap + i] + kt * du2;
                 double qtn = sqrt(q1 * q1 + q2 * q2);
-                double fy = pp.mu_eff * p_n;
+                double fy = mu * p_n;
                 bool stick = qtn <= fy;
                 if (!stick) {
                     q1 = (fy * q1) / qtn;
@@ -245,7 +246,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                             }
                     }
                 }
-                if ((modes & PAIR_TAN_F) && st != ST_INACTIVE && p_n > 0.0 && pp.mu_eff > 0.0) {
+                if ((modes & PAIR_TAN_F) && st != ST_INACTIVE && p_n > 0.0 && mu > 0.0) {
                     double q1 = P.z[i], q2 = P.z[cap + i];
                     double zn = sqrt(q1 * q1 + q2 * q2);
                     if (zn >= 1e-30) {
@@ -256,7 +257,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                             D00 = kt; D01 = 0.0; D10 = 0.0; D11 = kt;
                         } else {
                             double h1 = q1 / zn, h2 = q2 / zn;
-                            double rk = ((pp.mu_eff * p_n) / qtn) * kt;
+                            double rk = ((mu * p_n) / qtn) * kt;
                             D00 = rk * (1.0 - h1 * h1); D01 = rk * (0.0 - h1 * h2);
                             D10 = rk * (0.0 - h2 * h1); D11 = rk * (1.0 - h2 * h2);
                         }

@@ -81,6 +81,9 @@ struct Ctx {
     DBuf d_cdesc;   // int4 descriptors of the blocks with pair contributions
     DBuf d_fint;    // internal force written by the element kernel (6 n_nodes)
     int n_cdesc = 0;
+    // per-element friction coefficients (xk_ctx_set_friction)
+    DBuf d_mu_elem;
+    bool has_mu_elem = false;
     // launch shapes (xk_ctx_set_tuning)
     int pair_block = 256, gather_block = 256;
     // learned prescreen: parameters, features, watch / prune lists of the increment
@@ -113,6 +116,7 @@ struct Ctx {
         d_eslot.release();
         d_cdesc.release();
         d_fint.release();
+        d_mu_elem.release();
         for (DBuf* b : {&lp_prm, &lp_feat, &lp_gprev, &lp_keep, &lp_pos, &lp_watch, &lp_prune, &lp_count}) b->release();
         for (auto& b : snap) b.release();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);

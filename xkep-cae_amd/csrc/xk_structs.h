@@ -24,6 +24,10 @@ struct PairParams {
     double k_pen, delta_h, k_t_ratio, mu_eff, g_on, g_off;
     int allow_deact;
     int n_sinc;
+    // per-element friction coefficient (nullable): pair i uses mu_elem[ea[i]]
+    // scaled by mu_eff instead of mu_eff (batches of independent solves)
+    const double* mu_elem;
+    const int* ea;
 };
 
 struct NodeLists {

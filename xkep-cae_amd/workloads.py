@@ -172,6 +172,17 @@ def concat_states(states: list, gap_factor: float = 4.0) -> Batch:
                  offsets=np.array(offs))
 
 
+def uneven_batch(n_instances: int = 64, seed: int = 64):
+    """The uneven batched workload assembled as one: the cases, their states
+    (end moment on the last node of every wire), the concatenated batch and
+    the per-element friction coefficient of each element's instance."""
+    cases = batched_cases(n_instances, seed)
+    states = [make_state(c) for c in cases]
+    batch = concat_states(states)
+    mu_elem = np.concatenate([np.full(st.mesh.n_elems, c.mu) for c, st in zip(cases, states)])
+    return cases, states, batch, mu_elem
+
+
 def batched_cases(n_instances: int = 64, seed: int = 64) -> list[StrandCase]:
     """The uneven batched workload: wires in {7, 19, 37, 61}, 50-150 elements per
     wire, lay angle U(8, 15) deg, mu U(0.1, 0.5), random end moment, r = 0.5,
