@@ -38,15 +38,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(case, states, precision, torch):
-    """Context + assembly for one instance or a concatenated batch of instances."""
+def setup(case, states, precision, torch, batch=None, mu_elem=None):
+    """Context + assembly for one instance, a concatenated batch of instances
+    of one case, or a prepared uneven batch (per-element friction)."""
     wl = importlib.import_module("xkep-cae_amd.workloads")
     C = importlib.import_module("xkep-cae_amd.context")
     A = importlib.import_module("xkep-cae_amd.assembly")
     T = importlib.import_module("xkep-cae_amd.time_integration")
     E = importlib.import_module("xkep-cae_amd.elements")
     K = importlib.import_module("xkep-cae_amd.contact")
-    if len(states) == 1:
+    if batch is not None:
+        b = batch
+        X, conn, radii, u, f_ext, fixed = b.node_coords, b.connectivity, b.radii, b.u, b.f_ext, b.fixed_dofs
+        layer = b.elem_layer
+    elif len(states) == 1:
         st = states[0]
         X, conn, radii, u, f_ext, fixed = (st.mesh.node_coords, st.mesh.connectivity, st.mesh.radii, st.u,
                                            st.f_ext, st.fixed_dofs)
@@ -55,16 +60,20 @@ def setup(case, states, precision, torch):
         b = wl.concat_states(states)
         X, conn, radii, u, f_ext, fixed = b.node_coords, b.connectivity, b.radii, b.u, b.f_ext, b.fixed_dofs
         layer = b.elem_layer
-    cfg = K.ContactConfig(mu=case.mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer))
+    mu = case.mu if mu_elem is None else 1.0
+    cfg = K.ContactConfig(mu=mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer))
     sec = C.circle_section(case.wire_radius, case.E, case.nu)
     ctx = C.DeviceContext(X, conn, sec, radii=radii, precision=precision)
+    if mu_elem is not None:
+        ctx.set_friction(mu_elem)
     Md = E.ULCRBeamAssembler(X, conn, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
                              sec.kappa_z).lumped_mass_diag(case.rho)
     dt = case.dt()
     ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
     ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
-    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=mu,
                            f_ext=f_ext, fixed_dofs=fixed, config=cfg, time=ga, dt=dt)
+    asm.bench_layer = layer
     asm.begin_increment(np.zeros(ctx.ndof))   # set-up broadphase on the reference configuration
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -187,9 +196,9 @@ def cpu_baseline(case, st, budget_s=15.0):
     return (m.n_elems + len(op)) / t, len(times), t
 
 
-def run(label, case, states, prec, torch, steps, warmup, tune=False):
-    log(f"[bench] {label}: setting up {len(states)} x {case.name} precision={prec}")
-    ctx, asm, t_inc = setup(case, states, prec, torch)
+def run(label, case, states, prec, torch, steps, warmup, tune=False, batch=None, mu_elem=None):
+    log(f"[bench] {label}: setting up precision={prec}")
+    ctx, asm, t_inc = setup(case, states, prec, torch, batch=batch, mu_elem=mu_elem)
     tuning = None
     if tune:
         T = importlib.import_module("xkep-cae_amd.autotune")
@@ -210,6 +219,45 @@ def run(label, case, states, prec, torch, steps, warmup, tune=False):
     return out
 
 
+def side_measurements(batched, torch, wl, args):
+    """Beside the headline: the learned prescreen on the batched 91-wire
+    workload, the uneven 64-instance batch, and the 1000-wire strand (FP32-mixed,
+    HIP-graph replay, per-step times)."""
+    out = {}
+    steps = max(20, args.steps // 4)
+    P = importlib.import_module("xkep-cae_amd.prescreen")
+    if os.path.exists(P.WEIGHTS):
+        asm, ctx = batched["asm"], batched["ctx"]
+        net = P.PrescreenNet.load()
+        asm.set_prescreen(net, elem_layer=asm.bench_layer)
+        asm.begin_increment(detect=False)          # features + selection of the increment
+        watched = asm.n_watch / max(ctx.n_pairs, 1)
+        stages = profile_stages(asm, ctx, torch, 20)
+        ms, _ = time_graph(asm, torch, args.steps, args.warmup)
+        log(f"[bench] learned prescreen on mixed x{args.batch}: watched {watched:.3f} of {ctx.n_pairs} pairs | "
+            f"{ms * 1e3:.1f} us/step (graph) | stages us {np.round(stages * 1e3, 1).tolist()}")
+        out["learned_prescreen"] = {
+            "workload": f"same batch of {args.batch} x 91 wires, pair kernel over the learned watch list",
+            "watched_fraction": round(watched, 4), "us_per_step": round(ms * 1e3, 2),
+            "stages_us": np.round(stages * 1e3, 2).tolist(), "speedup_vs_full_path": round(batched["ms"] / ms, 3),
+            "value": round(batched["n_eval"] / (ms * 1e-3), 1),
+            "exactness": "contributing pairs always visited: R, K_T, n_active identical to the full path"}
+        asm.set_prescreen(None)
+        ctx.lp_off()
+    cases4, _, b4, mu4 = wl.uneven_batch()
+    r4 = run("mixed uneven x64", cases4[0], None, "mixed", torch, steps, args.warmup, batch=b4, mu_elem=mu4)
+    out["uneven_batch_64"] = {"n_elem": r4["n_elem"], "n_pair": r4["n_pair"], "us_per_step": round(r4["ms"] * 1e3, 2),
+                              "value": round(r4["value"], 1), "stages_us": np.round(r4["stages"] * 1e3, 2).tolist()}
+    del r4
+    torch.cuda.empty_cache()
+    st5 = wl.make_state(wl.C5)
+    r5 = run("mixed 1000 wires", wl.C5, [st5], "mixed", torch, steps, args.warmup)
+    out["wires_1000"] = {"n_elem": r5["n_elem"], "n_pair": r5["n_pair"], "us_per_step": round(r5["ms"] * 1e3, 2),
+                         "value": round(r5["value"], 1), "stages_us": np.round(r5["stages"] * 1e3, 2).tolist(),
+                         "increment_setup_ms": round(r5["t_inc"] * 1e3, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +266,8 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="91-wire instances per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes and grid cell")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the side measurements (uneven 64-instance batch, 1000 wires, learned prescreen)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -251,6 +301,7 @@ def main():
     batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup,
                   tune=not args.no_autotune)
 
+    extra = {} if args.no_extra else side_measurements(batched, torch, wl, args)
     nnz_beam = beam_nnz(batched["ctx"].conn_np)
     sb = stage_bytes(batched["ctx"], batched["asm"], batched["n_pair"], "mixed", nnz_beam)
     dom = int(np.argmax(batched["stages"]))
@@ -304,6 +355,7 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -45,6 +45,19 @@ def check_trajectory(g, res, kind):
     assert res.n_cutbacks == int(g["n_cutbacks"])
 
 
+def test_squeeze_workload_is_the_fixture_case():
+    """workloads.squeeze_case (used for training data and held-out solves) is
+    the configuration the reference solved for the fixture."""
+    g = golden("newton_7wire.npz")
+    c = pkg("workloads").squeeze_case(100.0)
+    k_pen, mu, t_total, rho_inf, dtmax = g["params"][:5]
+    assert c["k_pen"] == pytest.approx(k_pen, rel=1e-14) and c["t_total"] == pytest.approx(t_total, rel=1e-14)
+    assert (c["mu"], c["rho_inf"], c["cfg"]["dt_max_fraction"]) == (mu, rho_inf, dtmax)
+    assert np.array_equal(c["X"], g["X"]) and np.array_equal(c["fixed"], g["fixed"])
+    assert np.allclose(c["f"], g["f_ext"], rtol=0, atol=1e-13)
+    assert np.array_equal(c["layer"], g["layer"])
+
+
 def test_adaptive_stepping_schedule():
     """The load-fraction queue alone: growth capped by dt_max, cutback to the
     midpoint, retry of the failed fraction, contact-change shrink."""

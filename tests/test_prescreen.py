@@ -5,7 +5,7 @@ network gains over the method without it."""
 import numpy as np
 import pytest
 
-from conftest import golden, pkg
+from conftest import pkg
 from oracle import mlp
 
 pytestmark = pytest.mark.gpu
@@ -73,46 +73,8 @@ def test_training_separates_a_learnable_rule(gpu):
     assert recall > 0.97 and precision > 0.7
 
 
-def squeeze_case(force, n_wires=7, n_elems=20, length=20.0):
-    """Dynamic radial squeeze of a strand, frictionless (the configuration of
-    tests/golden/newton_7wire.npz, with another force / size)."""
-    g = golden("newton_7wire.npz")
-    k_pen, mu, t_total, rho_inf, dtmax, sd, tolf, told, maxa, ktr = g["params"]
-    M = pkg("mesh")
-    E_ = pkg("elements")
-    C = pkg("context")
-    m = M.make_strand(n_wires, 1.0, length, n_elems, lay_angle_deg=10.0)
-    X, conn = m.node_coords, m.connectivity
-    nn = n_elems + 1
-    first = np.arange(n_wires) * nn
-    fixed = (first[:, None] * 6 + np.arange(6)[None, :]).ravel()
-    mid = first[1:] + nn // 2
-    nrm = X[mid, :2] / np.linalg.norm(X[mid, :2], axis=1)[:, None]
-    f = np.zeros(6 * len(X))
-    f[mid * 6] = -force * nrm[:, 0]
-    f[mid * 6 + 1] = -force * nrm[:, 1]
-    s = C.circle_section(1.0, 200e3, 0.3)
-    Md = E_.ULCRBeamAssembler(X, conn, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z).lumped_mass_diag(7.85e-9)
-    return dict(X=X, conn=conn, radii=m.radii, layer=m.elem_layer(), fixed=fixed, f=f, M=Md, k_pen=k_pen, mu=mu,
-                t_total=t_total, rho_inf=rho_inf, sd=sd, ktr=ktr,
-                cfg=dict(max_nr_attempts=int(maxa), tol_force=tolf, tol_disp=told, dt_max_fraction=dtmax))
-
-
 def run(case, precision="fp64", net=None, log=False):
-    S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
-    ctx = C.DeviceContext(case["X"], case["conn"], C.circle_section(1.0, 200e3, 0.3), radii=case["radii"],
-                          precision=precision)
-    kcfg = Kc.ContactConfig(mu=case["mu"], k_t_ratio=case["ktr"], exclude_same_layer=True,
-                            elem_layer_map=wl.layer_map(case["layer"]))
-    b = S.DeviceBackend(ctx, M_diag=case["M"], rho_inf=case["rho_inf"], k_pen=case["k_pen"], smoothing_delta=case["sd"],
-                        mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg)
-    if net is not None:
-        b.use_prescreen(net, elem_layer=case["layer"])
-    if log:
-        b.log_features(True, elem_layer=case["layer"])
-    res = S.ContactFrictionSolver(b, fixed_dofs=case["fixed"], f_ext_total=case["f"], t_total=case["t_total"],
-                                  config=S.SolverConfig(**case["cfg"])).solve()
-    return res, b
+    return pkg("train_prescreen").run_case(case, precision=precision, net=net, log=log)
 
 
 def training_set(runs):
@@ -142,14 +104,15 @@ def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
     pairs the narrowphase visits per Newton attempt against the method without
     it (all candidates)."""
     P = pkg("prescreen")
-    train = [run(squeeze_case(f), log=True) for f in (80.0, 100.0, 120.0)]
+    W = pkg("workloads")
+    train = [run(W.squeeze_case(f), log=True) for f in (80.0, 100.0, 120.0)]
     X, y = training_set(train)
     assert y.sum() > 0
     net = P.PrescreenNet(seed=0)
     hist = net.fit(X, y, epochs=300, lr=3e-3)
     assert hist[-1] < hist[0]
     net.choose_threshold(X, y, margin=1.0)
-    for case in (squeeze_case(110.0), squeeze_case(100.0, n_wires=19, n_elems=20)):
+    for case in (W.squeeze_case(110.0), W.squeeze_case(100.0, n_wires=19)):
         full, _ = run(case)
         lp, _ = run(case, net=net)
         same_trajectory(full, lp)

@@ -172,6 +172,39 @@ def concat_states(states: list, gap_factor: float = 4.0) -> Batch:
                  offsets=np.array(offs))
 
 
+def squeeze_case(force: float, *, n_wires: int = 7, n_elems: int = 20, length: float = 20.0, radius: float = 1.0,
+                 lay_angle_deg: float = 10.0, mu: float = 0.0) -> dict:
+    """Dynamic radial squeeze of a strand: first node of every wire clamped, a
+    radial inward force at the mid node of every outer wire, lumped mass,
+    Generalized-alpha rho_inf = 0.8 over t_total = 4 T1/40 of one wire, steps
+    capped at a quarter of it, penalty 0.5 c0 m_ii at that step (the dynamic
+    estimate of contact/penalty/strategy.py:216-262), Huber delta 5000/r,
+    same-layer exclusion by layer index, frictionless by default."""
+    m = make_strand(n_wires, radius, length, n_elems, lay_angle_deg=lay_angle_deg)
+    X, conn = m.node_coords, m.connectivity
+    nn = n_elems + 1
+    first = np.arange(n_wires) * nn
+    fixed = (first[:, None] * 6 + np.arange(6)[None, :]).ravel()
+    mid = first[1:] + nn // 2
+    nrm = X[mid, :2] / np.linalg.norm(X[mid, :2], axis=1)[:, None]
+    f = np.zeros(6 * len(X))
+    f[mid * 6] = -force * nrm[:, 0]
+    f[mid * 6 + 1] = -force * nrm[:, 1]
+    A = math.pi * radius * radius
+    Iv = math.pi * (2 * radius) ** 4 / 64.0
+    f1 = (math.pi / (2.0 * length**2)) * math.sqrt(E_STEEL * Iv / (RHO_STEEL * A))
+    t_total = 4.0 * (1.0 / f1 / 40.0)
+    rho_inf, dt_max_fraction = 0.8, 0.25
+    am = (2 * rho_inf - 1) / (rho_inf + 1)
+    af = rho_inf / (rho_inf + 1)
+    beta = 0.25 * (1 - am + af) ** 2
+    dt_nom = t_total * dt_max_fraction
+    k_pen = 0.5 * (1.0 / (beta * dt_nom**2)) * RHO_STEEL * A * (length / n_elems) / 2.0
+    return dict(mesh=m, X=X, conn=conn, radii=m.radii, layer=m.elem_layer(), fixed=fixed, f=f, t_total=t_total,
+                rho_inf=rho_inf, k_pen=k_pen, mu=mu, sd=5000.0 / radius, ktr=0.5,
+                cfg=dict(max_nr_attempts=30, tol_force=1e-6, tol_disp=1e-8, dt_max_fraction=dt_max_fraction))
+
+
 def uneven_batch(n_instances: int = 64, seed: int = 64):
     """The uneven batched workload assembled as one: the cases, their states
     (end moment on the last node of every wire), the concatenated batch and
