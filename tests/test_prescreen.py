@@ -121,3 +121,36 @@ def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
               f"of {len(lp.calls)} calls")
         assert watched < 0.5
         assert lp.n_rechecks <= len(lp.calls) // 4
+
+
+def test_prescreened_assembly_is_bitwise_the_full_path_at_91_wires(gpu):
+    """With the committed weights on the 91-wire workload: R, K_T and the active
+    count of one assembly over the compacted watch list equal the full path's
+    bit for bit, and so do the watched pairs' states once written back."""
+    wl, C, A, Kc, P = pkg("workloads"), pkg("context"), pkg("assembly"), pkg("contact"), pkg("prescreen")
+    case = wl.C3
+    st = wl.make_state(case)
+    m = st.mesh
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, C.circle_section(case.wire_radius, case.E, case.nu),
+                          radii=m.radii, precision="mixed")
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu, f_ext=st.f_ext,
+                           fixed_dofs=st.fixed_dofs,
+                           config=Kc.ContactConfig(mu=case.mu, exclude_same_layer=True,
+                                                   elem_layer_map=wl.layer_map(st.elem_layer)))
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.set_prescreen(P.PrescreenNet.load(), elem_layer=st.elem_layer)
+    asm.begin_increment(st.u)
+    n_watch = asm.n_watch
+    assert 0 < n_watch < ctx.n_pairs
+    ctx.pairs_snapshot()
+    R1, K1, _, n1 = (t.clone() for t in asm.assemble())
+    ctx.lp_off()
+    lp_state = ctx.get_pairs()
+    ctx.pairs_restore()
+    R0, K0, _, n0 = (t.clone() for t in asm.assemble())     # full path from the same state
+    full_state = ctx.get_pairs()
+    assert gpu.equal(R0, R1) and gpu.equal(K0, K1) and int(n0.item()) == int(n1.item())
+    sinc = full_state["status"] != 0
+    for k in ("p_n", "z_t", "status", "gap", "s", "t"):
+        assert np.array_equal(lp_state[k][sinc], full_state[k][sinc]), k
+    print(f"91 wires: watched {n_watch} of {ctx.n_pairs} pairs")

@@ -585,6 +585,7 @@ int xk_prescreen_radii(xk_ctx* c, const double* coords, double margin, double ce
 
 int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double cell_size, const int32_t* elem_layer,
                          double prescreen_extra, int64_t* n_raw, int64_t* n_candidates, void* stream) {
+    if (c && c->lp_sync((hipStream_t)stream, false) != 0) return -1;
     if (!c || !coords) {
         set_error("xk_detect_candidates: null argument");
         return -1;

@@ -67,13 +67,62 @@ static void bucket(int n_keys, const std::vector<int>& keys, const std::vector<i
     for (size_t q = 0; q < keys.size(); ++q) src[fill[keys[q]]++] = codes[q];
 }
 
+// bytes per pair of each pair-state array, and its component count
+static const size_t PB_SZ[PB_COUNT] = {4, 4, 4, 4, 8, 8, 8, 8, 8, 24, 24, 8, 8, 8, 8, 8, 16, 1, 1, 4, 4, 4};
+static int pb_comps(int k) { return (k == PB_N || k == PB_T1) ? 3 : (k == PB_Z ? 2 : 1); }
+// arrays the pair kernel writes (the others are the pair's identity)
+static bool pb_written(int k) {
+    return !(k == PB_NA0 || k == PB_NA1 || k == PB_NB0 || k == PB_NB1 || k == PB_RA || k == PB_RB ||
+             k == PB_SINC || k == PB_EA || k == PB_EB);
+}
+
+static PairSoA soa_of(std::vector<DBuf>& b, int64_t cap) {
+    PairSoA P{};
+    P.na0 = (int*)b[PB_NA0].p; P.na1 = (int*)b[PB_NA1].p;
+    P.nb0 = (int*)b[PB_NB0].p; P.nb1 = (int*)b[PB_NB1].p;
+    P.ra = (double*)b[PB_RA].p; P.rb = (double*)b[PB_RB].p;
+    P.s = (double*)b[PB_S].p; P.t = (double*)b[PB_T].p; P.gap = (double*)b[PB_GAP].p;
+    P.n = (double*)b[PB_N].p; P.t1 = (double*)b[PB_T1].p;
+    P.p_n = (double*)b[PB_PN].p; P.k_pen = (double*)b[PB_KPEN].p; P.k_t = (double*)b[PB_KT].p;
+    P.qtn = (double*)b[PB_QTN].p; P.diss = (double*)b[PB_DISS].p; P.z = (double*)b[PB_Z].p;
+    P.status = (int8_t*)b[PB_STATUS].p; P.stick = (int8_t*)b[PB_STICK].p;
+    P.sinc = (int*)b[PB_SINC].p;
+    P.cap = cap;
+    return P;
+}
+
+int Ctx::lp_compact(hipStream_t st) {
+    wb.resize(PB_COUNT);
+    const int64_t n = n_watch;
+    for (int k = 0; k < PB_COUNT; ++k) {
+        XK_TRY(wb[k].ensure(PB_SZ[k] * (size_t)std::max<int64_t>(n, 1)));
+        const int comps = pb_comps(k);
+        XK_TRY(launch_permute((int)(PB_SZ[k] / comps), n, comps, pb[k].p, cap, wb[k].p, n,
+                              (const int*)lp_watch.p, 0, st));
+    }
+    PW = soa_of(wb, std::max<int64_t>(n, 1));
+    return 0;
+}
+
+int Ctx::lp_sync(hipStream_t st, bool keep_on) {
+    if (!lp_on) return 0;
+    for (int k = 0; k < PB_COUNT; ++k) {
+        if (!pb_written(k)) continue;
+        const int comps = pb_comps(k);
+        XK_TRY(launch_permute((int)(PB_SZ[k] / comps), n_watch, comps, wb[k].p, std::max<int64_t>(n_watch, 1),
+                              pb[k].p, cap, (const int*)lp_watch.p, 1, st));
+    }
+    if (!keep_on) lp_on = false;
+    return 0;
+}
+
 int Ctx::grow_pairs(int64_t need) {
     if (need <= cap) return 0;
     XK_TRY(hipDeviceSynchronize());
     int64_t nc = std::max<int64_t>(need, std::max<int64_t>(1024, cap * 3 / 2));
     PairSoA old = P;
     std::vector<DBuf> nb(PB_COUNT);
-    const size_t sz[PB_COUNT] = {4, 4, 4, 4, 8, 8, 8, 8, 8, 24, 24, 8, 8, 8, 8, 8, 16, 1, 1, 4, 4, 4};
+    const size_t* sz = PB_SZ;
     for (int k = 0; k < PB_COUNT; ++k) XK_TRY(nb[k].ensure(sz[k] * (size_t)nc));
     // copy existing columns (component-major arrays are re-strided)
     auto cp = [&](int k, int comps, size_t esz) -> hipError_t {
@@ -86,7 +135,7 @@ int Ctx::grow_pairs(int64_t need) {
     };
     if (n_pairs > 0) {
         for (int k = 0; k < PB_COUNT; ++k) {
-            int comps = (k == PB_N || k == PB_T1) ? 3 : (k == PB_Z ? 2 : 1);
+            int comps = pb_comps(k);
             size_t esz = sz[k] / comps;
             XK_TRY(cp(k, comps, esz));
         }
@@ -117,7 +166,9 @@ void Ctx::bind_pairs() {
 
 int Ctx::build_pattern() {
     XK_TRY(hipDeviceSynchronize());
-    lp_on = false;  // a watch list belongs to the pattern it was selected on
+    // a watch list belongs to the pattern it was selected on; its pairs' state goes back first
+    if (lp_sync(nullptr, false) != 0) return -1;
+    XK_TRY(hipDeviceSynchronize());
     const int64_t np = n_pairs;
     std::vector<int8_t> st(np);
     std::vector<double> pn(np);
@@ -532,6 +583,8 @@ int xk_pairs_get(xk_ctx* c, int32_t* ea, int32_t* eb, double* s, double* t, doub
                  double* dissipation, double* z_t, int8_t* status, int8_t* stick) {
     if (!c) return -1;
     XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, false) != 0) return -1;
+    XK_TRY(hipDeviceSynchronize());
     const int64_t n = c->n_pairs, cap = c->cap;
     if (n == 0) return 0;
     auto get = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
@@ -572,6 +625,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
                  double k_pen, double smoothing_delta, double k_t_ratio, double mu_eff, double g_on, double g_off,
                  int allow_deactivation, void* stream) {
     if (!c) return -1;
+    if (c->lp_sync((hipStream_t)stream, false) != 0) return -1;
     const int phys = PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F;
     if ((modes & PAIR_HYST) && (modes & phys)) {
         set_error("xk_pairs_run: the hysteresis update cannot be combined with force/tangent modes");
@@ -686,10 +740,11 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.n_sinc = c->n_sinc;
     const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
     // with the prescreen on, the pair kernel visits the watch list (every
-    // contributing pair and the ones the network keeps) instead of all pairs
+    // contributing pair and the ones the network keeps), compacted
     const bool lp = c->lp_on;
     const int64_t n_work = lp ? c->n_watch : c->n_pairs;
-    const int* work_idx = lp ? (const int*)c->lp_watch.p : nullptr;
+    const PairSoA& PS = lp ? c->PW : c->P;
+    if (lp) pp.ea = (const int*)c->wb[PB_EA].p;
     const int npb = pair_blocks(n_work, c->pair_block);
     XK_TRY(c->d_act.ensure(sizeof(int) * (size_t)std::max(npb, 1)));
     int* act = (int*)c->d_act.p;
@@ -723,8 +778,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     if (ev) {
         // profiled: the three stages in sequence on the caller's stream
         XK_TRY(hipEventRecord(ev[0], st));
-        XK_TRY(launch_pairs(c->prec, n_work, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st, work_idx, c->pair_block));
+        XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st, nullptr, c->pair_block));
         XK_TRY(hipEventRecord(ev[1], st));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipEventRecord(ev[2], st));
@@ -734,8 +789,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, c->side));
         XK_TRY(hipEventRecord(c->ev_join, c->side));
-        XK_TRY(launch_pairs(c->prec, n_work, c->P, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st, work_idx, c->pair_block));
+        XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
+                            (double*)c->d_sd.p, c->d_sg.p, act, st, nullptr, c->pair_block));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
     XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st, ev ? ev[3] : nullptr));
@@ -842,9 +897,10 @@ int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
     if (scan_exclusive_i64(c, 0, (const int64_t*)c->lp_keep.p, (int64_t*)c->lp_pos.p, n, &nw, st) != 0) return -1;
     XK_TRY(launch_lp_split(n, (const int64_t*)c->lp_keep.p, (const int64_t*)c->lp_pos.p, (int*)c->lp_watch.p,
                            (int*)c->lp_prune.p, st));
-    XK_TRY(hipStreamSynchronize(st));
     c->n_watch = nw;
     c->n_prune = n - nw;
+    if (c->lp_compact(st) != 0) return -1;
+    XK_TRY(hipStreamSynchronize(st));
     c->lp_on = true;
     if (n_watch) *n_watch = nw;
     return 0;
@@ -854,7 +910,9 @@ int xk_lp_judge(xk_ctx* c, const double* coords_ref, const double* u, double g_o
     if (!c || !coords_ref) return -1;
     hipStream_t st = (hipStream_t)stream;
     int64_t miss = 0;
-    if (c->lp_on && c->n_prune > 0) {
+    const bool was_on = c->lp_on;
+    if (c->lp_sync(st, false) != 0) return -1;   // the watched pairs' state back in the pair list
+    if (was_on && c->n_prune > 0) {
         // exact narrowphase (geometry only, active set frozen) over the pruned pairs
         PairParams pp{};
         pp.n_sinc = c->n_sinc;
@@ -875,12 +933,17 @@ int xk_lp_judge(xk_ctx* c, const double* coords_ref, const double* u, double g_o
 
 int xk_lp_off(xk_ctx* c) {
     if (!c) return -1;
-    c->lp_on = false;
+    XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, false) != 0) return -1;
+    XK_TRY(hipDeviceSynchronize());
     return 0;
 }
 
 int xk_pairs_snapshot(xk_ctx* c) {
     if (!c) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, true) != 0) return -1;   // current state, watch list kept
+    XK_TRY(hipDeviceSynchronize());
     c->snap.resize(PB_COUNT);
     for (int k = 0; k < PB_COUNT; ++k) {
         XK_TRY(c->snap[k].ensure(std::max<size_t>(c->pb[k].bytes, 1)));
@@ -897,6 +960,7 @@ int xk_pairs_restore(xk_ctx* c) {
         set_error("xk_pairs_restore: the pair list changed since the snapshot");
         return -1;
     }
+    c->lp_on = false;   // the compact copy of the watched pairs is older than the restored state
     for (int k = 0; k < PB_COUNT; ++k) {
         if (c->snap[k].bytes < c->pb[k].bytes) {
             set_error("xk_pairs_restore: the pair list grew since the snapshot");

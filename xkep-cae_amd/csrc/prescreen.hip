@@ -202,6 +202,36 @@ __global__ void lp_miss_kernel(int64_t n, const int* __restrict__ idx, const dou
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, __popcll(b));
 }
 
+// ------------------------------------------------------------ compaction
+// gather (scatter == 0): b[c][j] = a[c][idx[j]]; scatter: b[c][idx[j]] = a[c][j]
+template <class T>
+__global__ void permute_kernel(int64_t n, int comps, const T* __restrict__ a, int64_t sa, T* __restrict__ b,
+                               int64_t sb, const int* __restrict__ idx, int scatter) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t k = idx[j];
+    for (int c = 0; c < comps; ++c) {
+        if (scatter) b[c * sb + k] = a[c * sa + j];
+        else b[c * sb + j] = a[c * sa + k];
+    }
+}
+
+hipError_t launch_permute(int esz, int64_t n, int comps, const void* a, int64_t sa, void* b, int64_t sb,
+                          const int* idx, int scatter, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const dim3 g((unsigned)((n + 255) / 256)), blk(256);
+    switch (esz) {
+        case 1: hipLaunchKernelGGL(permute_kernel<int8_t>, g, blk, 0, st, n, comps, (const int8_t*)a, sa, (int8_t*)b,
+                                   sb, idx, scatter); break;
+        case 4: hipLaunchKernelGGL(permute_kernel<int>, g, blk, 0, st, n, comps, (const int*)a, sa, (int*)b, sb, idx,
+                                   scatter); break;
+        case 8: hipLaunchKernelGGL(permute_kernel<double>, g, blk, 0, st, n, comps, (const double*)a, sa, (double*)b,
+                                   sb, idx, scatter); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ training
 constexpr int TG = 64;   // samples per chunk = lanes per workgroup
 constexpr int TP = LP_H + 1;

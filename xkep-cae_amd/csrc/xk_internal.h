@@ -94,11 +94,17 @@ struct Ctx {
     int64_t lp_gprev_n = 0, lp_gprev_cap = 0;
     int64_t n_watch = 0, n_prune = 0;
     bool lp_on = false;          // the fused assembly runs the pair kernel over the watch list
+    // while lp_on, the watched pairs' state lives in these compact arrays (pair
+    // order of the watch list, capacity n_watch) and PW points at them
+    std::vector<DBuf> wb;
+    PairSoA PW{};
     // pair-state snapshot (xk_pairs_snapshot / xk_pairs_restore)
     std::vector<DBuf> snap;
     int64_t snap_n = -1;
 
     int grow_pairs(int64_t need);
+    int lp_compact(hipStream_t st);                 // gather the watched pairs into wb / PW
+    int lp_sync(hipStream_t st, bool keep_on);      // scatter them back (and end the watch list)
     void bind_pairs();
     int build_pattern();
     int ensure_pattern();
@@ -119,6 +125,7 @@ struct Ctx {
         d_mu_elem.release();
         for (DBuf* b : {&lp_prm, &lp_feat, &lp_gprev, &lp_keep, &lp_pos, &lp_watch, &lp_prune, &lp_count}) b->release();
         for (auto& b : snap) b.release();
+        for (auto& b : wb) b.release();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         prof_ev.clear();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -168,6 +175,8 @@ hipError_t launch_lp_split(int64_t n, const int64_t* keep, const int64_t* pos, i
 hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g_on, int* count, hipStream_t st);
 hipError_t launch_lp_grad(int64_t n, const float* X, const float* y, const float* prm, float pos_w, float* partial,
                           float* grad, hipStream_t st);
+hipError_t launch_permute(int esz, int64_t n, int comps, const void* a, int64_t sa, void* b, int64_t sb,
+                          const int* idx, int scatter, hipStream_t st);
 hipError_t launch_lp_adam(float* prm, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
                           float eps, hipStream_t st);
 int pair_blocks(int64_t n_pairs, int bs);
