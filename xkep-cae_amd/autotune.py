@@ -71,17 +71,21 @@ def _stage_us(asm, calls: int):
 
 
 def tune_launch_shapes(asm, *, calls: int = 10) -> dict:
-    """Pair and gather workgroup sizes minimising their kernels' event time."""
+    """Pair and gather workgroup sizes minimising the event time of the pair,
+    gather and finish kernels (the finish kernel reduces one partial per pair
+    and per gather workgroup, so it depends on both)."""
     ctx = asm.ctx
     table = {"pair_block": {}, "gather_block": {}}
     for pb in PAIR_BLOCKS:
         ctx.set_tuning(pair_block=pb)
-        table["pair_block"][pb] = float(_stage_us(asm, calls)[0])
+        st = _stage_us(asm, calls)
+        table["pair_block"][pb] = float(st[0] + st[3])
     best_pb = min(table["pair_block"], key=table["pair_block"].get)
     ctx.set_tuning(pair_block=best_pb)
     for gb in GATHER_BLOCKS:
         ctx.set_tuning(gather_block=gb)
-        table["gather_block"][gb] = float(_stage_us(asm, calls)[2])
+        st = _stage_us(asm, calls)
+        table["gather_block"][gb] = float(st[2] + st[3])
     best_gb = min(table["gather_block"], key=table["gather_block"].get)
     ctx.set_tuning(gather_block=best_gb)
     asm.graph = None

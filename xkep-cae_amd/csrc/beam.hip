@@ -33,11 +33,19 @@
 
 #include "xk_common.h"
 
+#include <type_traits>
+
 namespace xk {
+
+// Half-angle of a rotation: angle a, sin(a/2), cos(a/2) (the tangent
+// operators of the FP32 tangent reuse them instead of trigonometry of their own).
+struct HalfAngle {
+    double a, s, c;
+};
 
 // Unit quaternion (w, v) of a rotation vector, Taylor below |th| < 1e-12
 // (_beam_cr.py:685-726), returned as the rotation matrix.
-__device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th) {
+__device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th, HalfAngle* ha = nullptr) {
     double ang = norm(th);
     double half = ang / 2.0;
     double sinc, ch;
@@ -49,6 +57,7 @@ __device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th) {
         sinc = sin(half) / ang;
         ch = cos(half);
     }
+    if (ha) *ha = HalfAngle{ang, sinc * ang, ch};
     double w = ch, x = sinc * th.x, y = sinc * th.y, z = sinc * th.z;
     double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
     double wx = w * x, wy = w * y, wz = w * z;
@@ -62,50 +71,97 @@ __device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th) {
 // Logarithmic map by Shepperd's method: branch = argmax(tr, R00, R11, R22)
 // (first on ties), w >= 0, normalise (floor 1e-15), 2 atan2(|v|, w)/|v| v,
 // 2 v below |v| < 1e-12 (_beam_cr.py:729-787).
-__device__ __forceinline__ V3<double> rotmat_to_rotvec(const M3<double>& R) {
+// FAST: the divisions by s and by |q| are one reciprocal and products (FP32
+// tangent form; the FP64 form divides as the reference does).
+template <bool FAST = false>
+__device__ __forceinline__ V3<double> rotmat_to_rotvec(const M3<double>& R, HalfAngle* ha = nullptr) {
     double tr = R.m[0] + R.m[4] + R.m[8];
     int best = 0;
     double bv = tr;
     if (R.m[0] > bv) { best = 1; bv = R.m[0]; }
     if (R.m[4] > bv) { best = 2; bv = R.m[4]; }
     if (R.m[8] > bv) { best = 3; }
-    double q0, q1, q2, q3, s;
+    // branch b: s = 2 sqrt(arg), the component b is s / 4, the others num_k / s
+    double arg, n0, n1, n2;
     if (best == 0) {
-        s = 2.0 * sqrt(tr + 1.0);
-        q0 = 0.25 * s;
-        q1 = (R.m[7] - R.m[5]) / s;
-        q2 = (R.m[2] - R.m[6]) / s;
-        q3 = (R.m[3] - R.m[1]) / s;
+        arg = tr + 1.0;
+        n0 = R.m[7] - R.m[5]; n1 = R.m[2] - R.m[6]; n2 = R.m[3] - R.m[1];
     } else if (best == 1) {
-        s = 2.0 * sqrt(((1.0 + R.m[0]) - R.m[4]) - R.m[8]);
-        q0 = (R.m[7] - R.m[5]) / s;
-        q1 = 0.25 * s;
-        q2 = (R.m[1] + R.m[3]) / s;
-        q3 = (R.m[2] + R.m[6]) / s;
+        arg = ((1.0 + R.m[0]) - R.m[4]) - R.m[8];
+        n0 = R.m[7] - R.m[5]; n1 = R.m[1] + R.m[3]; n2 = R.m[2] + R.m[6];
     } else if (best == 2) {
-        s = 2.0 * sqrt(((1.0 + R.m[4]) - R.m[0]) - R.m[8]);
-        q0 = (R.m[2] - R.m[6]) / s;
-        q1 = (R.m[1] + R.m[3]) / s;
-        q2 = 0.25 * s;
-        q3 = (R.m[5] + R.m[7]) / s;
+        arg = ((1.0 + R.m[4]) - R.m[0]) - R.m[8];
+        n0 = R.m[2] - R.m[6]; n1 = R.m[1] + R.m[3]; n2 = R.m[5] + R.m[7];
     } else {
-        s = 2.0 * sqrt(((1.0 + R.m[8]) - R.m[0]) - R.m[4]);
-        q0 = (R.m[3] - R.m[1]) / s;
-        q1 = (R.m[2] + R.m[6]) / s;
-        q2 = (R.m[5] + R.m[7]) / s;
-        q3 = 0.25 * s;
+        arg = ((1.0 + R.m[8]) - R.m[0]) - R.m[4];
+        n0 = R.m[3] - R.m[1]; n1 = R.m[2] + R.m[6]; n2 = R.m[5] + R.m[7];
     }
+    const double s = 2.0 * sqrt(arg);
+    double d0, d1, d2;
+    if (FAST) {
+        const double is = 1.0 / s;
+        d0 = n0 * is; d1 = n1 * is; d2 = n2 * is;
+    } else {
+        d0 = n0 / s; d1 = n1 / s; d2 = n2 / s;
+    }
+    const double qb = 0.25 * s;
+    double q0, q1, q2, q3;
+    if (best == 0) { q0 = qb; q1 = d0; q2 = d1; q3 = d2; }
+    else if (best == 1) { q0 = d0; q1 = qb; q2 = d1; q3 = d2; }
+    else if (best == 2) { q0 = d0; q1 = d1; q2 = qb; q3 = d2; }
+    else { q0 = d0; q1 = d1; q2 = d2; q3 = qb; }
     if (q0 < 0.0) { q0 = -q0; q1 = -q1; q2 = -q2; q3 = -q3; }
     double qn = sqrt(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
     qn = qn > 1e-15 ? qn : 1e-15;
-    q0 /= qn; q1 /= qn; q2 /= qn; q3 /= qn;
+    if (FAST) {
+        const double iq = 1.0 / qn;
+        q0 *= iq; q1 *= iq; q2 *= iq; q3 *= iq;
+    } else {
+        q0 /= qn; q1 /= qn; q2 /= qn; q3 /= qn;
+    }
     V3<double> v{q1, q2, q3};
     double sh = norm(v);
     double coef = (sh < 1e-12) ? 2.0 : 2.0 * atan2(sh, q0) / sh;
+    if (ha) *ha = HalfAngle{coef * sh, sh, q0};
     return coef * v;
 }
 
-// T_s(th) and its inverse (_beam_cr.py:856-908), in the tangent precision.
+// T_s(th) and its inverse (_beam_cr.py:856-908) in FP32 from the half-angle
+// sine and cosine: c1 = (1 - cos a)/a^2 = 2 (sin(a/2)/a)^2; c2 = (1 - sin a/a)/a^2
+// and, for the inverse, 1/a^2 - (1 + cos a)/(2 a sin a) = 1/a^2 - cos(a/2)/(2 a sin(a/2)),
+// by their Taylor series below a = 1/2 (truncation < 1e-12) where the closed
+// forms cancel in FP32; the reference's own constants below a = 1e-10.
+__device__ __forceinline__ M3<float> tangent_op_f32(V3<double> thd, const HalfAngle& h, bool inverse) {
+    const float a = (float)h.a, s = (float)h.s, c = (float)h.c;
+    const float a2 = a * a;
+    float c1, c2;
+    if (!inverse) {
+        if (h.a < 1e-10) { c1 = 0.5f; c2 = 1.0f / 6.0f; }
+        else {
+            const float sa = s / a;
+            c1 = 2.0f * sa * sa;
+            if (a < 0.5f)
+                c2 = 1.0f / 6.0f + a2 * (-1.0f / 120.0f + a2 * (1.0f / 5040.0f + a2 * (-1.0f / 362880.0f +
+                                                                                         a2 * (1.0f / 39916800.0f))));
+            else c2 = (1.0f - 2.0f * s * c / a) / a2;
+        }
+    } else {
+        c1 = -0.5f;
+        if (h.a < 1e-10) c2 = 1.0f / 12.0f;
+        else if (a < 0.5f)
+            c2 = 1.0f / 12.0f + a2 * (1.0f / 720.0f + a2 * (1.0f / 30240.0f + a2 * (1.0f / 1209600.0f +
+                                                                                     a2 * (1.0f / 47900160.0f))));
+        else c2 = 1.0f / a2 - c / (2.0f * a * s);
+    }
+    V3<float> th = cvt<float>(thd);
+    M3<float> S = skew(th);
+    M3<float> S2 = mul(S, S);
+    M3<float> r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.m[k] = ((k % 4 == 0) ? 1.0f : 0.0f) + c1 * S.m[k] + c2 * S2.m[k];
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ M3<T> tangent_op(V3<double> thd, bool inverse) {
     double a = norm(thd);
@@ -146,8 +202,15 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
 
     // reference frame R_0 (rows e_x, e_y, e_z)
     V3<double> d0 = P2 - P1;
+    constexpr bool FAST = std::is_same<TT, float>::value;   // reciprocal-multiply normalisations
     double L0 = norm(d0);
-    V3<double> ex0{d0.x / L0, d0.y / L0, d0.z / L0};
+    V3<double> ex0;
+    if (FAST) {
+        const double i0 = 1.0 / L0;
+        ex0 = V3<double>{d0.x * i0, d0.y * i0, d0.z * i0};
+    } else {
+        ex0 = V3<double>{d0.x / L0, d0.y / L0, d0.z / L0};
+    }
     V3<double> vr;
     if (bc.has_vref) {
         vr = V3<double>{bc.vref[0], bc.vref[1], bc.vref[2]};
@@ -159,7 +222,12 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     V3<double> ez = cross(ex0, vr);
     double nz = norm(ez);
     nz = nz > 1e-10 ? nz : 1e-10;
-    ez = V3<double>{ez.x / nz, ez.y / nz, ez.z / nz};
+    if (FAST) {
+        const double iz = 1.0 / nz;
+        ez = V3<double>{ez.x * iz, ez.y * iz, ez.z * iz};
+    } else {
+        ez = V3<double>{ez.x / nz, ez.y / nz, ez.z / nz};
+    }
     V3<double> ey = cross(ez, ex0);
     M3<double> R0;
     R0.m[0] = ex0.x; R0.m[1] = ex0.y; R0.m[2] = ex0.z;
@@ -169,7 +237,13 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     // deformed chord
     V3<double> dd = (P2 + t2) - (P1 + t1);
     double Ld = norm(dd);
-    V3<double> exd{dd.x / Ld, dd.y / Ld, dd.z / Ld};
+    V3<double> exd;
+    if (FAST) {
+        const double id = 1.0 / Ld;
+        exd = V3<double>{dd.x * id, dd.y * id, dd.z * id};
+    } else {
+        exd = V3<double>{dd.x / Ld, dd.y / Ld, dd.z / Ld};
+    }
     if (!(L0 >= 1e-15) || !(Ld >= 1e-15)) atomicOr(flags, FLAG_DEGENERATE);
 
     // minimal rotation e_x0 -> e_xd (_beam_cr.py:790-828)
@@ -205,31 +279,52 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     M3<double> Rcr = mulNT(R0, Rrod);  // R_0 R_rod^T
 
     // nodal rotations relative to the corotated frame
-    M3<double> Rd1 = mulNT(mul(Rcr, rotvec_to_rotmat(r1)), R0);
-    M3<double> Rd2 = mulNT(mul(Rcr, rotvec_to_rotmat(r2)), R0);
-    V3<double> th1 = rotmat_to_rotvec(Rd1);
-    V3<double> th2 = rotmat_to_rotvec(Rd2);
+    HalfAngle hr1, hr2, ht1, ht2;
+    M3<double> Rd1 = mulNT(mul(Rcr, rotvec_to_rotmat(r1, &hr1)), R0);
+    M3<double> Rd2 = mulNT(mul(Rcr, rotvec_to_rotmat(r2, &hr2)), R0);
+    V3<double> th1 = rotmat_to_rotvec<FAST>(Rd1, &ht1);
+    V3<double> th2 = rotmat_to_rotvec<FAST>(Rd2, &ht2);
     double dax = Ld - L0;
 
     // local Timoshenko stiffness coefficients (_beam_cr.py:911-998)
     const double L = L0;
-    double phy = 12.0 * bc.E * bc.Iy / (bc.ky * bc.G * bc.A * L * L);
-    double phz = 12.0 * bc.E * bc.Iz / (bc.kz * bc.G * bc.A * L * L);
-    if (bc.scf > 0.0) {
-        phy = phy * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iy))));
-        phz = phz * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iz))));
+    double ea, gj, k15, k55, k511, m15, m44, m410;
+    if (FAST) {
+        // one reciprocal of L, L^2 and of each shear-correction denominator
+        const double iL = 1.0 / L, iL2 = iL * iL;
+        double phy = (12.0 * bc.E * bc.Iy / (bc.ky * bc.G * bc.A)) * iL2;
+        double phz = (12.0 * bc.E * bc.Iz / (bc.kz * bc.G * bc.A)) * iL2;
+        if (bc.scf > 0.0) {
+            phy = phy * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iy))));
+            phz = phz * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iz))));
+        }
+        const double idy = 1.0 / (1.0 + phy), idz = 1.0 / (1.0 + phz);
+        ea = bc.E * bc.A * iL;
+        gj = bc.G * bc.J * iL;
+        k15 = 6.0 * (bc.E * bc.Iz * iL2) * idz;
+        k55 = (4.0 + phz) * (bc.E * bc.Iz * iL) * idz;
+        k511 = (2.0 - phz) * (bc.E * bc.Iz * iL) * idz;
+        m15 = 6.0 * (bc.E * bc.Iy * iL2) * idy;
+        m44 = (4.0 + phy) * (bc.E * bc.Iy * iL) * idy;
+        m410 = (2.0 - phy) * (bc.E * bc.Iy * iL) * idy;
+    } else {
+        double phy = 12.0 * bc.E * bc.Iy / (bc.ky * bc.G * bc.A * L * L);
+        double phz = 12.0 * bc.E * bc.Iz / (bc.kz * bc.G * bc.A * L * L);
+        if (bc.scf > 0.0) {
+            phy = phy * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iy))));
+            phz = phz * (1.0 / (1.0 + bc.scf * (L * L * bc.A / (12.0 * bc.Iz))));
+        }
+        const double dy = 1.0 + phy, dz = 1.0 + phz;
+        const double L2 = L * L;
+        ea = bc.E * bc.A / L;
+        gj = bc.G * bc.J / L;
+        k15 = 6.0 * (bc.E * bc.Iz / L2) / dz;          // Ke[1][5]
+        k55 = (4.0 + phz) * (bc.E * bc.Iz / L) / dz;   // Ke[5][5]
+        k511 = (2.0 - phz) * (bc.E * bc.Iz / L) / dz;  // Ke[5][11]
+        m15 = 6.0 * (bc.E * bc.Iy / L2) / dy;          // -Ke[2][4]
+        m44 = (4.0 + phy) * (bc.E * bc.Iy / L) / dy;   // Ke[4][4]
+        m410 = (2.0 - phy) * (bc.E * bc.Iy / L) / dy;  // Ke[4][10]
     }
-    const double dy = 1.0 + phy, dz = 1.0 + phz;
-    const double L2 = L * L, L3 = L2 * L;
-    const double ea = bc.E * bc.A / L;
-    const double gj = bc.G * bc.J / L;
-    const double k15 = 6.0 * (bc.E * bc.Iz / L2) / dz;          // Ke[1][5]
-    const double k55 = (4.0 + phz) * (bc.E * bc.Iz / L) / dz;   // Ke[5][5]
-    const double k511 = (2.0 - phz) * (bc.E * bc.Iz / L) / dz;  // Ke[5][11]
-    const double m15 = 6.0 * (bc.E * bc.Iy / L2) / dy;          // -Ke[2][4]
-    const double m44 = (4.0 + phy) * (bc.E * bc.Iy / L) / dy;   // Ke[4][4]
-    const double m410 = (2.0 - phy) * (bc.E * bc.Iy / L) / dy;  // Ke[4][10]
-    (void)L3;
 
     // corotated element force f_cr = Ke d_cr (only nonzero d_cr entries used)
     double fcr[12];
@@ -264,10 +359,18 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
 #pragma unroll
         for (int k = 0; k < 9; ++k) dpsi.m[k] = il * dpsi.m[k];
     }
-    M3<T> Ti1 = tangent_op<T>(th1, true);
-    M3<T> Ti2 = tangent_op<T>(th2, true);
-    M3<T> Ts1 = tangent_op<T>(r1, false);
-    M3<T> Ts2 = tangent_op<T>(r2, false);
+    M3<T> Ti1, Ti2, Ts1, Ts2;
+    if constexpr (std::is_same<T, float>::value) {
+        Ti1 = tangent_op_f32(th1, ht1, true);
+        Ti2 = tangent_op_f32(th2, ht2, true);
+        Ts1 = tangent_op_f32(r1, hr1, false);
+        Ts2 = tangent_op_f32(r2, hr2, false);
+    } else {
+        Ti1 = tangent_op<T>(th1, true);
+        Ti2 = tangent_op<T>(th2, true);
+        Ts1 = tangent_op<T>(r1, false);
+        Ts2 = tangent_op<T>(r2, false);
+    }
     M3<T> P1m = mul(Ti1, dpsi);            // B[3:6, 0:3]  (B[3:6, 6:9] = -P1)
     M3<T> Q1m = mul(mul(Ti1, Rc), Ts1);    // B[3:6, 3:6]
     M3<T> P2m = mul(Ti2, dpsi);            // B[9:12, 0:3] (B[9:12, 6:9] = -P2)

@@ -194,7 +194,15 @@ __global__ void __launch_bounds__(FB) finish_kernel(int n_part, const double* __
         s = (((s + p0) + p1) + p2) + p3;
     }
     for (; k < n_part; k += FB) s += partial[k];
-    for (k = threadIdx.x; k < n_act; k += FB) a += act_part[k];
+    k = threadIdx.x;
+    for (; k + 7 * FB < n_act; k += 8 * FB) {
+        int q[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] = act_part[k + j * FB];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a += q[j];
+    }
+    for (; k < n_act; k += FB) a += act_part[k];
     red[threadIdx.x] = s;
     ired[threadIdx.x] = a;
     __syncthreads();
