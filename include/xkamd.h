@@ -40,6 +40,13 @@ enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1 };
  * node in more than two elements). */
 enum { XK_OPT_STAGED = 256 };
 
+/* OR-ed into xk_ctx_create's precision: on the direct path, build the tangent
+ * pattern of each increment on the host instead of the device.  Both builds
+ * give the same pattern and the same contribution lists (so bitwise the same
+ * assembly); the option exists to check the device build against the host
+ * one.  No counterpart in the reference. */
+enum { XK_OPT_HOST_PATTERN = 512 };
+
 /* status flags returned (bitwise OR) by assembly calls */
 enum {
     XK_FLAG_NONFINITE = 1,     /* a non-finite residual entry was produced */

@@ -23,6 +23,7 @@ vp = C.c_void_p
 PAIR_GEOM, PAIR_HYST, PAIR_HUBER, PAIR_FRIC, PAIR_TAN_C, PAIR_TAN_F = 1, 2, 4, 8, 16, 32
 PREC = {"fp64": 0, "mixed": 1}
 OPT_STAGED = 256
+OPT_HOST_PATTERN = 512
 
 
 class XkTuning(C.Structure):

@@ -64,9 +64,10 @@ class DeviceContext:
     """Mesh, section, contact radii and pair list resident on the GPU."""
 
     def __init__(self, node_coords, connectivity, section: Section, *, radii=None, scf=None, v_ref=None,
-                 precision: str = "fp64", staged: bool = False):
+                 precision: str = "fp64", staged: bool = False, host_pattern: bool = False):
         """``staged`` forces the staged element path; otherwise meshes made of
-        element chains use the direct path (see ``direct``)."""
+        element chains use the direct path (see ``direct``), whose tangent
+        pattern is built on the device unless ``host_pattern``."""
         torch = require_gpu()
         self.torch = torch
         self.precision = precision
@@ -86,7 +87,8 @@ class DeviceContext:
         L.check(L.lib().xk_ctx_create(C.byref(h), self.n_nodes, self.n_elems, X.ctypes.data, conn.ctypes.data,
                                       sec.ctypes.data, float(scf) if scf else 0.0,
                                       vr.ctypes.data if vr is not None else None, self.radii_np.ctypes.data,
-                                      L.PREC[precision] | (L.OPT_STAGED if staged else 0)), "xk_ctx_create")
+                                      L.PREC[precision] | (L.OPT_STAGED if staged else 0)
+                                      | (L.OPT_HOST_PATTERN if host_pattern else 0)), "xk_ctx_create")
         self.h = h
         self.direct = bool(L.lib().xk_ctx_direct(h))
         self.dev = torch.device("cuda")
@@ -137,8 +139,18 @@ class DeviceContext:
         self._pattern_cache = (self._pattern_epoch, out)
         return out
 
+    def pattern_nnz(self) -> int:
+        """Scalar entries of the current tangent pattern (builds it if needed)."""
+        nnz = C.c_int64()
+        nb = C.c_int64()
+        ns = C.c_int64()
+        L.check(L.lib().xk_pattern_info(self.h, C.byref(nnz), C.byref(nb), C.byref(ns)), "xk_pattern_info")
+        return nnz.value
+
     def new_K(self):
-        return self.torch.empty(max(self.pattern()[2], 1), dtype=self.kdtype, device=self.dev)
+        """Device buffer for the tangent values of the current pattern (the host
+        copy of the pattern is made only when ``pattern()`` asks for it)."""
+        return self.torch.empty(max(self.pattern_nnz(), 1), dtype=self.kdtype, device=self.dev)
 
     def to_csr(self, K_vals) -> sp.csr_matrix:
         indptr, indices, nnz = self.pattern()

@@ -169,6 +169,13 @@ int Ctx::build_pattern() {
     // a watch list belongs to the pattern it was selected on; its pairs' state goes back first
     if (lp_sync(nullptr, false) != 0) return -1;
     XK_TRY(hipDeviceSynchronize());
+    if (direct && !host_build) {
+        if (build_pattern_gpu(nullptr) != 0) return -1;
+        XK_TRY(d_sd.ensure((size_t)std::max(n_sinc, 1) * SD_REC * sizeof(double)));
+        XK_TRY(d_sg.ensure((size_t)std::max(n_sinc, 1) * SG_REC * (prec == 0 ? 8 : 4)));
+        pattern_valid = true;
+        return 0;
+    }
     const int64_t np = n_pairs;
     std::vector<int8_t> st(np);
     std::vector<double> pn(np);
@@ -294,7 +301,18 @@ int Ctx::build_pattern() {
     XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * (prec == 0 ? 8 : 4)));
     n_sinc = ns;
     n_blocks = nblk;
+    host_pattern_valid = true;
     pattern_valid = true;
+    return 0;
+}
+
+int Ctx::host_pattern() {
+    if (host_pattern_valid) return 0;
+    gptr_h.resize(2 * (size_t)n_nodes + 1);
+    gcol_h.resize(n_blocks);
+    XK_TRY(hipMemcpy(gptr_h.data(), d_gptr.p, sizeof(int) * gptr_h.size(), hipMemcpyDeviceToHost));
+    if (n_blocks) XK_TRY(hipMemcpy(gcol_h.data(), d_gcol.p, sizeof(int) * (size_t)n_blocks, hipMemcpyDeviceToHost));
+    host_pattern_valid = true;
     return 0;
 }
 
@@ -365,7 +383,8 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
         return -1;
     }
     const bool force_staged = (precision & XK_OPT_STAGED) != 0;
-    precision &= ~XK_OPT_STAGED;
+    const bool host_pattern = (precision & XK_OPT_HOST_PATTERN) != 0;
+    precision &= ~(XK_OPT_STAGED | XK_OPT_HOST_PATTERN);
     if (precision != XK_PREC_FP64 && precision != XK_PREC_MIXED) {
         set_error("xk_ctx_create: unknown precision");
         return -1;
@@ -379,6 +398,7 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
     c->n_nodes = n_nodes;
     c->n_elems = n_elems;
     c->prec = precision;
+    c->host_build = host_pattern;
     c->X_h.assign(node_coords, node_coords + 3 * (size_t)n_nodes);
     c->conn_h.assign(connectivity, connectivity + 2 * (size_t)n_elems);
     c->rad_h.assign(n_elems, 0.0);
@@ -438,6 +458,13 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
         c->direct = chain && !force_staged;
         if (c->direct) {
             if (upload(c->d_eflag, ef) != hipSuccess) fail("element flags");
+            // node flags of the device pattern build
+            std::vector<int8_t> nf(n_nodes, 0);
+            for (int e = 0; e < n_elems; ++e) {
+                nf[c->conn_h[2 * e]] |= 3;
+                nf[c->conn_h[2 * e + 1]] |= 1;
+            }
+            if (!rc && upload(c->d_nflag, nf) != hipSuccess) fail("node flags");
             if (!rc && (c->d_fint.ensure(sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess ||
                         hipMemset(c->d_fint.p, 0, sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess))
                 fail("internal force buffer");
@@ -696,7 +723,7 @@ int xk_pattern_info(xk_ctx* c, int64_t* nnz, int64_t* n_blocks, int64_t* n_contr
 
 int xk_pattern_export(xk_ctx* c, int64_t* indptr, int32_t* indices) {
     if (!c) return -1;
-    if (c->ensure_pattern() != 0) return -1;
+    if (c->ensure_pattern() != 0 || c->host_pattern() != 0) return -1;
     const int ng = 2 * c->n_nodes;
     int64_t pos = 0;
     for (int gr = 0; gr < ng; ++gr) {

@@ -54,7 +54,13 @@ struct Ctx {
     // tangent pattern of the current increment
     bool pattern_valid = false;
     int n_sinc = 0, n_blocks = 0;
-    std::vector<int> gptr_h, gcol_h;
+    std::vector<int> gptr_h, gcol_h;   // host copy, valid when host_pattern_valid
+    bool host_pattern_valid = false;
+    bool host_build = false;   // direct path: build the pattern on the host (XK_OPT_HOST_PATTERN)
+    DBuf d_nflag;          // direct path, per node: bit0 has an element, bit1 an element joins n and n + 1
+    DBuf d_bbeam;          // direct path, per block: 1 if the beam part holds it
+    std::vector<DBuf> pat;  // scratch of the device pattern build
+    DBuf pat_tmp;           // radix-sort scratch
     DBuf d_gptr, d_gcol, d_brow, d_blk_bptr, d_blk_bsrc, d_blk_pptr, d_blk_psrc, d_node_pptr, d_node_psrc;
     DBuf d_sd, d_sg;
     // broadphase scratch and candidate bookkeeping
@@ -107,6 +113,8 @@ struct Ctx {
     int lp_sync(hipStream_t st, bool keep_on);      // scatter them back (and end the watch list)
     void bind_pairs();
     int build_pattern();
+    int build_pattern_gpu(hipStream_t st);   // direct path: the pattern built on the device
+    int host_pattern();                      // gptr_h / gcol_h of the current pattern
     int ensure_pattern();
     BlockLists block_lists() const;
     NodeLists node_lists() const;
@@ -119,6 +127,10 @@ struct Ctx {
         for (auto& b : bp) b.release();
         d_act.release();
         d_eflag.release();
+        d_nflag.release();
+        d_bbeam.release();
+        pat_tmp.release();
+        for (auto& b : pat) b.release();
         d_eslot.release();
         d_cdesc.release();
         d_fint.release();
