@@ -50,7 +50,8 @@ enum { XK_OPT_HOST_PATTERN = 512 };
 /* status flags returned (bitwise OR) by assembly calls */
 enum {
     XK_FLAG_NONFINITE = 1,     /* a non-finite residual entry was produced */
-    XK_FLAG_DEGENERATE = 2     /* an element of (near) zero length was met */
+    XK_FLAG_DEGENERATE = 2,    /* an element of (near) zero length was met */
+    XK_FLAG_SINGULAR = 4       /* the device linear solve met a zero pivot */
 };
 
 /* pair-kernel modes (bitwise OR) */
@@ -187,6 +188,33 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
                           double k_pen, double smoothing_delta, double k_t_ratio, double mu_eff,
                           double* R, void* K_vals, int64_t K_cap, double* res_norm2, int32_t* n_active,
                           void* stream);
+
+/* Linear solve of the attempt on the device, replacing LinearSolveProcess
+ * (contact/solver/_newton_steps.py:338-366, SciPy spsolve of
+ * (D K D + I - D) du = -D R, D = diag(free dofs)).  Direct element path only
+ * (element chains).  Call after xk_assemble_iteration with K_vals on the same
+ * ctx, with the same u, fixed_mask and dyn.  The tangent is taken as
+ * K = A_b + sum_p U_p G_p U_p^T: the beam part A_b = K_beam + (1 - alpha_m) c0 M
+ * (the element kernel re-run, factored along each chain) and the pairs'
+ * staged 3x3 tangent blocks G_p with U_p = c_p (x) I3 (Woodbury, a dense
+ * capacitance matrix per contact component, rocSOLVER LU); in the FP64 form
+ * this equals the assembled K up to rounding, in the mixed form K holds the
+ * same sum rounded to FP32.  The result is refined against that operator:
+ * with tol <= 0 exactly max_refine steps, stream-ordered; with tol > 0 until
+ * ||D R + A~ du|| <= tol ||D R|| or max_refine steps, reading the norm on the
+ * host after each step (the capacitance is ill-conditioned on strands, so
+ * 2-8 steps reach 1e-12).  du (device, ndof) is written; res_rel (device
+ * double, nullable) receives the final relative residual, n_refine (host,
+ * nullable) the steps taken.  A zero pivot sets XK_FLAG_SINGULAR in
+ * xk_status_flags.  The first call after a pattern change sets up the solve
+ * synchronously. */
+int xk_linear_solve(xk_ctx* ctx, const double* u, const uint8_t* fixed_mask, const xk_dyn* dyn,
+                    const double* R, double* du, int max_refine, double tol, double* res_rel,
+                    int32_t* n_refine, void* stream);
+/* Copy of one of the linear solve's internal device buffers (index `which`,
+ * layouts in csrc/solve.hip) for tests; *size_bytes receives its size.
+ * Synchronous.  No counterpart in the reference. */
+int xk_solve_buffer(xk_ctx* ctx, int which, void* host_dst, int64_t nbytes, int64_t* size_bytes);
 
 /* ---- Learned candidate prescreen.  No counterpart in the reference: an
  * 8-64-64-1 BF16 MLP (inference on the matrix cores) chooses which candidate

@@ -1,0 +1,161 @@
+"""Device linear solve of a Newton attempt against SciPy's SuperLU
+(``spsolve``, the reference's LinearSolveProcess, contact/solver/_newton_steps.py:338-366),
+through the C-ABI.
+
+The solve inverts A_w = A_b + U G U^T: the beam blocks with the mass term and
+the pairs' staged 3x3 tangent blocks, summed in FP64.  The test rebuilds A_w
+on the host from the solve's buffers and checks
+  * A_w against the assembled tangent K (masked): equal up to rounding in the
+    FP64 form (1e-14), up to FP32 rounding in the mixed form (1e-6, K sums
+    the same blocks in FP32);
+  * the device residual ||b - A_w du|| / ||b|| within 1e-10 (the solve refines
+    to 1e-12 in fewer than 16 steps; SuperLU reaches about 1e-12);
+  * du against SuperLU on A_w within 1e-7: two solutions with residuals near
+    1e-12 differ by up to cond(A) times that, and cond(A) reaches 1e5;
+  * FP64 form: the same two bars against the assembled K itself.
+In the mixed form the solution is not compared with SuperLU on the FP32 K:
+on penalty-tied strands a 1e-7 relative change of the contact entries moves
+the solution by tens of percent (measured), so the FP32 rounding of K does
+not determine it; the Newton trajectory tests check the mixed form.
+Cases: the 91-wire strand with the same-layer filter and without it (pairs
+within one wire, many more active pairs), three 37-wire instances in one
+context (three contact components), and a state without contact (chain solve
+alone)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-10
+SOL_TOL = 1e-7
+
+
+def masked(K, fixed):
+    n = K.shape[0]
+    free = np.ones(n)
+    free[fixed] = 0.0
+    D = sp.diags(free)
+    return (D @ K.astype(np.float64) @ D + sp.diags(1.0 - free)).tocsr(), D
+
+
+def operator_from_buffers(ctx, D):
+    """A_w = A~_b + U~ G U~^T from the solve's buffers (A~_b already masked)."""
+    nn, n = ctx.n_nodes, ctx.ndof
+    Ab = ctx.solve_buffer("Ab", np.float64)[:108 * nn].reshape(nn, 6, 18)
+    node = np.arange(nn)[:, None, None]
+    r = np.arange(6)[None, :, None]
+    k = np.arange(18)[None, None, :]
+    cnode = node - 1 + k // 6
+    rows = np.broadcast_to(6 * node + r, Ab.shape)
+    cols = np.broadcast_to(6 * cnode + k % 6, Ab.shape)
+    ok = (cnode >= 0) & (cnode < nn) & (Ab != 0.0)
+    A_b = sp.csr_matrix((Ab[ok], (rows[ok], cols[ok])), shape=(n, n))
+    ns = ctx.pattern_counts()[2]
+    if ns == 0:
+        return A_b
+    sn = ctx.solve_buffer("slot_nodes", np.int32)[:4 * ns].reshape(ns, 4)
+    sG = ctx.solve_buffer("slot_G", np.float64)[:9 * ns].reshape(ns, 3, 3)
+    sc = ctx.solve_buffer("slot_c", np.float64)[:4 * ns].reshape(ns, 4)
+    q = np.arange(ns)[:, None, None]
+    a = np.arange(3)[None, None, :]
+    ur = np.broadcast_to(6 * sn[:, :, None] + a, (ns, 4, 3))
+    uc = np.broadcast_to(3 * q + a, (ns, 4, 3))
+    uv = np.broadcast_to(sc[:, :, None], (ns, 4, 3))
+    U = D @ sp.csr_matrix((uv.ravel(), (ur.ravel(), uc.ravel())), shape=(n, 3 * ns))
+    return (A_b + U @ sp.block_diag(list(sG)) @ U.T).tocsr()
+
+
+def solve_case(ctx, asm, fixed, precision):
+    torch = ctx.torch
+    R, K, _, nact = asm.assemble()
+    du = torch.empty(ctx.ndof, dtype=torch.float64, device=ctx.dev)
+    rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+    steps = ctx.linear_solve(asm.u, asm.fixed_mask, R, du, asm._dyn, max_refine=16, tol=1e-12, res_rel=rel)
+    torch.cuda.synchronize()
+    assert ctx.status_flags() & 4 == 0, "zero pivot"
+    assert steps < 16
+    A_K, D = masked(asm.K_csr(), fixed)
+    A_w = operator_from_buffers(ctx, D)
+    b = -R.cpu().numpy()
+    b[fixed] = 0.0
+    x = du.cpu().numpy()
+    out = dict(steps=steps, nact=int(nact.item()), dev_res=float(rel.item()),
+               op_diff=sla.norm(A_K - A_w) / sla.norm(A_K),
+               res_w=np.linalg.norm(A_w @ x - b) / np.linalg.norm(b))
+    xw = sla.spsolve(A_w.tocsc(), b)
+    out["err_w"] = np.linalg.norm(x - xw) / np.linalg.norm(xw)
+    if precision == "fp64":
+        xk = sla.spsolve(A_K.tocsc(), b)
+        out["res_K"] = np.linalg.norm(A_K @ x - b) / np.linalg.norm(b)
+        out["err_K"] = np.linalg.norm(x - xk) / np.linalg.norm(xk)
+    print(" ".join(f"{k} {v:.2e}" if isinstance(v, float) else f"{k} {v}" for k, v in out.items()))
+    assert out["dev_res"] < RES_TOL and out["res_w"] < RES_TOL
+    assert out["err_w"] < SOL_TOL
+    if precision == "fp64":
+        assert out["op_diff"] < 1e-14
+        assert out["res_K"] < RES_TOL and out["err_K"] < SOL_TOL
+    else:
+        assert out["op_diff"] < 1e-6
+    return out
+
+
+def strand_assembly(X, conn, radii, case, st_like, precision, cfg=None):
+    C, A, T, E = pkg("context"), pkg("assembly"), pkg("time_integration"), pkg("elements")
+    s = C.circle_section(case.wire_radius, case.E, case.nu)
+    ctx = C.DeviceContext(X, conn, s, radii=radii, precision=precision)
+    Md = E.ULCRBeamAssembler(X, conn, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z).lumped_mass_diag(
+        case.rho)
+    dt = case.dt()
+    ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
+    ga.predict_device(ctx.torch.zeros(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev), dt)
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                           f_ext=st_like.f_ext, fixed_dofs=st_like.fixed_dofs, time=ga, dt=dt, config=cfg)
+    return ctx, asm
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("layers", [True, False], ids=["between_layers", "all_pairs"])
+def test_device_solve_matches_superlu_at_91_wires(gpu, precision, layers):
+    wl, Kc = pkg("workloads"), pkg("contact")
+    case = wl.C3
+    st = wl.make_state(case)
+    m = st.mesh
+    cfg = Kc.ContactConfig(mu=case.mu, exclude_same_layer=layers,
+                           elem_layer_map=wl.layer_map(st.elem_layer) if layers else None)
+    ctx, asm = strand_assembly(m.node_coords, m.connectivity, m.radii, case, st, precision, cfg)
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(st.u)
+    out = solve_case(ctx, asm, st.fixed_dofs, precision)
+    assert out["nact"] > 0
+
+
+def test_device_solve_on_batched_instances(gpu):
+    wl = pkg("workloads")
+    case = wl.C2
+    b = wl.concat_states([wl.make_state(case, seed=s) for s in (37, 38, 39)])
+    ctx, asm = strand_assembly(b.node_coords, b.connectivity, b.radii, case, b, "fp64")
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(b.u)
+    out = solve_case(ctx, asm, b.fixed_dofs, "fp64")
+    assert out["nact"] > 0
+
+
+def test_device_solve_without_contact(gpu):
+    """The 7-wire strand with its pair list emptied: the capacitance is empty
+    and the chain solve alone is exact (no refinement step needed)."""
+    wl = pkg("workloads")
+    case = wl.C1
+    st = wl.make_state(case)
+    m = st.mesh
+    ctx, asm = strand_assembly(m.node_coords, m.connectivity, m.radii, case, st, "fp64")
+    asm.begin_increment(np.zeros(ctx.ndof))
+    ctx.set_pairs(np.zeros(0, np.int32), np.zeros(0, np.int32))
+    asm.set_u(0.01 * st.u)
+    out = solve_case(ctx, asm, st.fixed_dofs, "fp64")
+    assert out["nact"] == 0 and ctx.n_pairs == 0
+    assert out["steps"] == 0 and out["res_K"] < 1e-13

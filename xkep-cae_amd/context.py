@@ -139,13 +139,18 @@ class DeviceContext:
         self._pattern_cache = (self._pattern_epoch, out)
         return out
 
-    def pattern_nnz(self) -> int:
-        """Scalar entries of the current tangent pattern (builds it if needed)."""
+    def pattern_counts(self):
+        """(scalar entries, 3x3 blocks, contributing pairs) of the current
+        tangent pattern (builds it if needed)."""
         nnz = C.c_int64()
         nb = C.c_int64()
         ns = C.c_int64()
         L.check(L.lib().xk_pattern_info(self.h, C.byref(nnz), C.byref(nb), C.byref(ns)), "xk_pattern_info")
-        return nnz.value
+        return nnz.value, nb.value, ns.value
+
+    def pattern_nnz(self) -> int:
+        """Scalar entries of the current tangent pattern (builds it if needed)."""
+        return self.pattern_counts()[0]
 
     def new_K(self):
         """Device buffer for the tangent values of the current pattern (the host
@@ -304,6 +309,36 @@ class DeviceContext:
             C.byref(dyn) if dyn is not None else None, float(k_pen), float(smoothing_delta), float(k_t_ratio),
             float(mu_eff), L.ptr(R), L.ptr(K_vals), self._cap(K_vals), L.ptr(res_norm2), L.ptr(n_active),
             self.stream), "xk_assemble_iteration")
+
+    def linear_solve(self, u, fixed_mask, R, du, dyn: "L.XkDyn | None" = None, *, max_refine: int = 16,
+                     tol: float = 1e-12, res_rel=None) -> int:
+        """du = -(D K D + I - D)^-1 D R on the device (LinearSolveProcess,
+        _newton_steps.py:338-366), after ``assemble_iteration`` with a tangent
+        at the same u, fixed_mask and dyn; refined until the relative residual
+        is <= tol (tol <= 0: exactly max_refine steps, no host read).
+        res_rel: optional device float64 scalar receiving the final relative
+        residual.  Returns the refinement steps taken."""
+        for a, nm in ((u, "u"), (fixed_mask, "fixed_mask"), (R, "R"), (du, "du")):
+            self._check_vec(a, self.ndof, nm)
+        steps = C.c_int32()
+        L.check(L.lib().xk_linear_solve(
+            self.h, L.ptr(u), L.ptr(fixed_mask), C.byref(dyn) if dyn is not None else None, L.ptr(R),
+            L.ptr(du), int(max_refine), float(tol), L.ptr(res_rel), C.byref(steps), self.stream),
+            "xk_linear_solve")
+        return steps.value
+
+    SOLVE_BUFFERS = ("Kb", "Ab", "Lf", "Sinv", "slist", "slot_nodes", "slot_G", "slot_c", "pos", "cslot", "cn",
+                     "cnode", "ch_m", "ch_cn0", "ch_yoff", "ch_coff", "Y", "C", "S", "ipiv", "info", "W", "b", "x",
+                     "r", "t")
+
+    def solve_buffer(self, name: str, dtype) -> np.ndarray:
+        """Host copy of one of the linear solve's device buffers (tests)."""
+        k = self.SOLVE_BUFFERS.index(name)
+        n = C.c_int64()
+        L.check(L.lib().xk_solve_buffer(self.h, k, None, 0, C.byref(n)), "xk_solve_buffer")
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype)
+        L.check(L.lib().xk_solve_buffer(self.h, k, out.ctypes.data, out.nbytes, None), "xk_solve_buffer")
+        return out
 
     def profile_begin(self, max_calls: int):
         L.check(L.lib().xk_profile_begin(self.h, int(max_calls)), "xk_profile_begin")

@@ -169,6 +169,8 @@ int Ctx::build_pattern() {
     // a watch list belongs to the pattern it was selected on; its pairs' state goes back first
     if (lp_sync(nullptr, false) != 0) return -1;
     XK_TRY(hipDeviceSynchronize());
+    solve_ready = false;
+    tangent_staged = false;
     if (direct && !host_build) {
         if (build_pattern_gpu(nullptr) != 0) return -1;
         XK_TRY(d_sd.ensure((size_t)std::max(n_sinc, 1) * SD_REC * sizeof(double)));
@@ -465,6 +467,17 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
                 nf[c->conn_h[2 * e + 1]] |= 1;
             }
             if (!rc && upload(c->d_nflag, nf) != hipSuccess) fail("node flags");
+            // element chains: a new chain starts where no element joins n - 1 and n
+            c->node_chain_h.assign(n_nodes, 0);
+            for (int n = 0; n < n_nodes; ++n) {
+                if (n == 0 || !(nf[n - 1] & 2)) c->cstart_h.push_back(n);
+                c->node_chain_h[n] = (int)c->cstart_h.size() - 1;
+            }
+            c->cstart_h.push_back(n_nodes);
+            c->n_chains = (int)c->cstart_h.size() - 1;
+            if (!rc && (upload(c->d_cstart, c->cstart_h) != hipSuccess ||
+                        upload(c->d_node_chain, c->node_chain_h) != hipSuccess))
+                fail("element chains");
             if (!rc && (c->d_fint.ensure(sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess ||
                         hipMemset(c->d_fint.p, 0, sizeof(double) * 6 * (size_t)n_nodes) != hipSuccess))
                 fail("internal force buffer");
@@ -712,6 +725,27 @@ int xk_pairs_gather(xk_ctx* c, double* f, void* K_vals, int64_t K_cap, void* str
 
 int xk_pattern_build(xk_ctx* c) { return c ? c->build_pattern() : -1; }
 
+int xk_solve_buffer(xk_ctx* c, int which, void* host, int64_t nbytes, int64_t* size) {
+    return c ? c->solve_buffer(which, host, nbytes, size) : -1;
+}
+
+int xk_linear_solve(xk_ctx* c, const double* u, const uint8_t* fixed_mask, const xk_dyn* dyn, const double* R,
+                    double* du, int max_refine, double tol, double* res_rel, int32_t* n_refine, void* stream) {
+    if (!c || !u || !fixed_mask || !R || !du || max_refine < 0) {
+        set_error("xk_linear_solve: invalid arguments");
+        return -1;
+    }
+    double mass_coef = 0.0;
+    const double* M = nullptr;
+    if (dyn && dyn->dt > 1e-30) {
+        const double c0 = 1.0 / (dyn->beta * (dyn->dt * dyn->dt));   // as the fused assembly forms it
+        mass_coef = (1.0 - dyn->alpha_m) * c0;
+        M = dyn->M_diag;
+    }
+    return c->linear_solve(u, fixed_mask, mass_coef, M, R, du, max_refine, tol, res_rel, n_refine,
+                           (hipStream_t)stream);
+}
+
 int xk_pattern_info(xk_ctx* c, int64_t* nnz, int64_t* n_blocks, int64_t* n_contrib) {
     if (!c) return -1;
     if (c->ensure_pattern() != 0) return -1;
@@ -766,6 +800,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.allow_deact = 0;
     pp.n_sinc = c->n_sinc;
     const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    c->tangent_staged = K_vals != nullptr;   // the pair kernel stages every contributing pair's tangent record
     // with the prescreen on, the pair kernel visits the watch list (every
     // contributing pair and the ones the network keeps), compacted
     const bool lp = c->lp_on;

@@ -1,0 +1,813 @@
+// Linear solve of one Newton attempt on the device (direct element path),
+// the role of LinearSolveProcess (contact/solver/_newton_steps.py:338-366):
+//
+//   (D K D + I - D) du = -D R            D = diag(free dofs)
+//
+// The tangent splits as K = A_b + sum_p U_p G_p U_p^T:
+//   A_b  = K_beam + (1 - alpha_m) c0 M, block tridiagonal along every element
+//          chain (6x6 node blocks; chains never couple through elements);
+//   U_p  = c_p (x) I3 on the translational dofs of the pair's four nodes and
+//   G_p    the pair's 3x3 tangent block: exactly the pair kernel's staged
+//          tangent record, which the gather scatters as c_k c_l G_p.
+// With the mask on both (A~_b, U~), the Woodbury identity gives
+//   A^-1 r = A~_b^-1 (r - U~ w),   (I + G U~^T A~_b^-1 U~) w = G U~^T A~_b^-1 r.
+// A~_b is factored by block LU along each chain (6x6 pivot blocks inverted by
+// Gauss-Jordan with partial pivoting); the capacitance matrix
+// I + G U~^T A~_b^-1 U~ (3 rows per contributing pair) is dense within a
+// connected component of the contact graph (pairs sharing a chain) and is
+// LU-factored per component by rocSOLVER.  Penalty springs that tie the soft,
+// mass-dominated modes of whole wires make the capacitance ill-conditioned
+// (condition numbers near 1e12 on strands): the result's residual is
+// U (S w - t), so it carries the capacitance solve's backward error times
+// ||S|| ||w|| (relative residuals 1e-4 .. 1e-2 on strands).  Iterative
+// refinement against the assembled K contracts it by that factor per step
+// and reaches the residual of a sparse direct solve in a few steps.
+
+#include "../../include/xkamd.h"
+#include "xk_common.h"
+#include "xk_structs.h"
+#include "xk_internal.h"
+
+#include <algorithm>
+#include <numeric>
+#include <rocsolver/rocsolver.h>
+
+namespace xk {
+
+enum SolveBuf {
+    SV_KB, SV_AB, SV_LF, SV_SINV, SV_SLIST, SV_SN, SV_SG, SV_SC, SV_POS, SV_CSLOT, SV_CN, SV_CNODE, SV_CHM,
+    SV_CHCN0, SV_CHYOFF, SV_CHCOFF, SV_Y, SV_C, SV_S, SV_IPIV, SV_INFO, SV_W, SV_B, SV_X, SV_R, SV_T, SV_RED,
+    SV_COUNT
+};
+
+// ------------------------------------------------------------------ kernels
+
+__device__ __forceinline__ bool sv_link(const int8_t* nflag, int n, int n_nodes) {
+    return n >= 0 && n < n_nodes && (nflag[n] & 2);
+}
+
+__device__ __forceinline__ int sv_find(const int* __restrict__ gptr, const int* __restrict__ gcol, int row, int col) {
+    int lo = gptr[row], hi = gptr[row + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (gcol[mid] < col) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Per node n, its 6 scalar rows over the columns of nodes n-1, n, n+1
+// (6 x 18, FP64), masked: a fixed row or column becomes the identity's.
+template <class KT>
+__global__ void sv_extract_kernel(int n_nodes, const int8_t* __restrict__ nflag, const int* __restrict__ gptr,
+                                  const int* __restrict__ gcol, const KT* __restrict__ Kb,
+                                  const uint8_t* __restrict__ fixed, double* __restrict__ Ab) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n_nodes) return;
+    const int n = t / 6, r = t - 6 * (t / 6);
+    const int gr = 2 * n + r / 3, a = r % 3;
+    const int r0 = gptr[gr], deg = gptr[gr + 1] - r0;
+    const KT* row = Kb + 9 * (size_t)r0 + 3 * (size_t)deg * a;
+    const bool has = (nflag[n] & 1) != 0;
+    double out[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) out[k] = 0.0;
+#pragma unroll
+    for (int d = -1; d <= 1; ++d) {
+        const int m = n + d;
+        const bool present = d == 0 ? has : (d < 0 ? sv_link(nflag, m, n_nodes) : sv_link(nflag, n, n_nodes));
+        if (!present) continue;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int blk = sv_find(gptr, gcol, gr, 2 * m + h);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) out[6 * (d + 1) + 3 * h + j] = (double)row[3 * (blk - r0) + j];
+        }
+    }
+    const int di = 6 * n + r;
+    const bool fi = fixed[di] != 0;
+    double* o = Ab + 108 * (size_t)n + 18 * r;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+        const int m = n - 1 + k / 6;
+        const int dj = 6 * m + k % 6;
+        double v = out[k];
+        if (m >= 0 && m < n_nodes && (fi || fixed[dj])) v = (di == dj) ? 1.0 : 0.0;
+        o[k] = v;
+    }
+}
+
+// Block LU along each chain: S_n0 = D_n0, L_n = F_{n-1} S_{n-1}^-1,
+// S_n = D_n - L_n E_{n-1}; stores L_n and S_n^-1.  One workgroup per chain.
+__global__ __launch_bounds__(64) void sv_factor_kernel(const int* __restrict__ cstart,
+                                                       const double* __restrict__ Ab, double* __restrict__ Lf,
+                                                       double* __restrict__ Sinv, int* __restrict__ flags) {
+    const int c = blockIdx.x;
+    const int n0 = cstart[c], n1 = cstart[c + 1];
+    __shared__ double M[6][13];
+    __shared__ double Pv[36];
+    __shared__ double T[36];
+    __shared__ int piv;
+    const int l = threadIdx.x, i = l / 6, j = l - 6 * (l / 6);
+    bool singular = false;
+    for (int n = n0; n < n1; ++n) {
+        const double* a = Ab + 108 * (size_t)n;
+        double s = 0.0;
+        if (l < 36) {
+            s = a[18 * i + 6 + j];
+            if (n > n0) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) t = fma(a[18 * i + k], Pv[6 * k + j], t);
+                T[l] = t;
+            }
+        }
+        __syncthreads();
+        if (n > n0 && l < 36) {
+            const double* e = Ab + 108 * (size_t)(n - 1);
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t = fma(T[6 * i + k], e[18 * k + 12 + j], t);
+            s -= t;
+            Lf[36 * (size_t)n + l] = T[l];
+        }
+        if (l < 36) {
+            M[i][j] = s;
+            M[i][6 + j] = i == j ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < 6; ++k) {
+            if (l == 0) {
+                int p = k;
+                double best = fabs(M[k][k]);
+                for (int r = k + 1; r < 6; ++r)
+                    if (fabs(M[r][k]) > best) {
+                        best = fabs(M[r][k]);
+                        p = r;
+                    }
+                piv = p;
+            }
+            __syncthreads();
+            const int p = piv;
+            if (p != k && l < 12) {
+                const double tmp = M[k][l];
+                M[k][l] = M[p][l];
+                M[p][l] = tmp;
+            }
+            __syncthreads();
+            const double d = M[k][k];
+            if (d == 0.0) singular = true;
+            __syncthreads();
+            if (l < 12) M[k][l] = M[k][l] / d;
+            __syncthreads();
+            double f0 = 0.0, f1 = 0.0;
+            const int t0 = l, t1 = l + 64;
+            if (t0 < 72) f0 = M[t0 / 12][k];
+            if (t1 < 72) f1 = M[t1 / 12][k];
+            __syncthreads();
+            if (t0 < 72 && t0 / 12 != k) M[t0 / 12][t0 % 12] -= f0 * M[k][t0 % 12];
+            if (t1 < 72 && t1 / 12 != k) M[t1 / 12][t1 % 12] -= f1 * M[k][t1 % 12];
+            __syncthreads();
+        }
+        if (l < 36) {
+            const double v = M[i][6 + j];
+            Sinv[36 * (size_t)n + l] = v;
+            Pv[l] = v;
+        }
+        __syncthreads();
+    }
+    if (singular && l == 0) atomicOr(flags, XK_FLAG_SINGULAR);
+}
+
+// slot -> pair
+__global__ void sv_slist_kernel(int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_pairs && sinc[i] >= 0) slist[sinc[i]] = (int)i;
+}
+
+__global__ void sv_slot_nodes_kernel(int ns, const int* __restrict__ slist, PairSoA P, int4* __restrict__ sn) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ns) return;
+    const int i = slist[q];
+    sn[q] = make_int4(P.na0[i], P.na1[i], P.nb0[i], P.nb1[i]);
+}
+
+// the pair's staged tangent record -> G (3x3) and c (4) in FP64
+template <class GT>
+__global__ void sv_slot_pack_kernel(int ns, const GT* __restrict__ sg, double* __restrict__ sG,
+                                    double* __restrict__ sc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ns) return;
+    const GT* r = sg + SG_REC * (size_t)q;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sG[9 * (size_t)q + k] = (double)r[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sc[4 * (size_t)q + k] = (double)r[SG_C + k];
+}
+
+// Compliance of each chain at its contact nodes' translational dofs:
+// C_w[(k, a), (l, b)] = (A~_b^-1)[(node k, a), (node l, b)].  One workgroup per
+// chain, one lane per right-hand side e_(l, b); forward sweep into Y, backward
+// sweep reading it.
+__global__ __launch_bounds__(256) void sv_compliance_kernel(
+    const int* __restrict__ cstart, const int* __restrict__ cn, const int* __restrict__ cnode,
+    const int* __restrict__ ch_cn0, const int* __restrict__ ch_m, const int64_t* __restrict__ ch_yoff,
+    const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed, const double* __restrict__ Ab,
+    const double* __restrict__ Lf, const double* __restrict__ Sinv, double* __restrict__ Y, double* __restrict__ C) {
+    const int c = blockIdx.x;
+    const int m = ch_m[c];
+    if (m == 0) return;
+    const int n0 = cstart[c], n1 = cstart[c + 1], cn0 = ch_cn0[c];
+    double* Yc = Y + ch_yoff[c];
+    double* Cc = C + ch_coff[c];
+    for (int rho = threadIdx.x; rho < m; rho += blockDim.x) {
+        const int s = cnode[cn0 + rho / 3], comp = rho % 3;
+        const bool live = fixed[6 * s + comp] == 0;
+        double y[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) y[a] = (live && a == comp) ? 1.0 : 0.0;
+        for (int n = s; n < n1; ++n) {
+            if (n > s) {
+                const double* L = Lf + 36 * (size_t)n;
+                double t[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) v = fma(L[6 * a + b], y[b], v);
+                    t[a] = -v;
+                }
+#pragma unroll
+                for (int a = 0; a < 6; ++a) y[a] = t[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 6; ++a) Yc[((size_t)(n - n0) * 6 + a) * m + rho] = y[a];
+        }
+        double x[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) x[a] = 0.0;
+        for (int n = n1 - 1; n >= n0; --n) {
+            double v[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) v[a] = n >= s ? Yc[((size_t)(n - n0) * 6 + a) * m + rho] : 0.0;
+            if (n < n1 - 1) {
+                const double* e = Ab + 108 * (size_t)n + 12;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) t = fma(e[18 * a + b], x[b], t);
+                    v[a] -= t;
+                }
+            }
+            const double* Si = Sinv + 36 * (size_t)n;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) t = fma(Si[6 * a + b], v[b], t);
+                x[a] = t;
+            }
+            const int k = cn[n];
+            if (k >= 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) Cc[(size_t)(3 * (k - cn0) + a) * m + rho] = x[a];
+            }
+        }
+    }
+}
+
+// Capacitance block (i, q) of one component, column-major with ld = 3 Pc:
+// I delta_iq + G_i sum_{k,l} c_ik c_ql M_k C[node_k(i), node_l(q)] M_l.
+__global__ void sv_capacitance_kernel(int Pc, const int* __restrict__ cslot, const int4* __restrict__ sn,
+                                      const double* __restrict__ sG, const double* __restrict__ sc,
+                                      const int* __restrict__ node_chain, const int* __restrict__ cn,
+                                      const int* __restrict__ ch_cn0, const int* __restrict__ ch_m,
+                                      const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed,
+                                      const double* __restrict__ C, double* __restrict__ S) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lq = blockIdx.y;
+    if (li >= Pc) return;
+    const int i = cslot[li], q = cslot[lq];
+    const int4 a4 = sn[i], b4 = sn[q];
+    const int na[4] = {a4.x, a4.y, a4.z, a4.w};
+    const int nb[4] = {b4.x, b4.y, b4.z, b4.w};
+    double B[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const int a = na[k], ch = node_chain[a];
+        const double ck = sc[4 * (size_t)i + k];
+        for (int l = 0; l < 4; ++l) {
+            const int b = nb[l];
+            if (node_chain[b] != ch) continue;
+            const int m = ch_m[ch], cn0 = ch_cn0[ch];
+            const double* Cb = C + ch_coff[ch] + (size_t)(3 * (cn[a] - cn0)) * m + 3 * (cn[b] - cn0);
+            const double w = ck * sc[4 * (size_t)q + l];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (fixed[6 * a + r]) continue;
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+                    if (!fixed[6 * b + s]) B[3 * r + s] = fma(w, Cb[(size_t)r * m + s], B[3 * r + s]);
+            }
+        }
+    }
+    const double* G = sG + 9 * (size_t)i;
+    const size_t ld = 3 * (size_t)Pc;
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double v = (li == lq && r == s) ? 1.0 : 0.0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) v = fma(G[3 * r + t], B[3 * t + s], v);
+            S[(3 * (size_t)lq + s) * ld + 3 * (size_t)li + r] = v;
+        }
+}
+
+__global__ void sv_info_kernel(int nc, const int64_t* __restrict__ info, int* __restrict__ flags) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc && info[c] != 0) atomicOr(flags, XK_FLAG_SINGULAR);
+}
+
+// x <- A~_b^-1 x along every chain (one lane per chain)
+__global__ void sv_chain_solve_kernel(int n_chains, const int* __restrict__ cstart, const double* __restrict__ Ab,
+                                      const double* __restrict__ Lf, const double* __restrict__ Sinv,
+                                      double* __restrict__ x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_chains) return;
+    const int n0 = cstart[c], n1 = cstart[c + 1];
+    double y[6];
+    for (int n = n0; n < n1; ++n) {
+        double v[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) v[a] = x[6 * (size_t)n + a];
+        if (n > n0) {
+            const double* L = Lf + 36 * (size_t)n;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) t = fma(L[6 * a + b], y[b], t);
+                v[a] -= t;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            y[a] = v[a];
+            x[6 * (size_t)n + a] = v[a];
+        }
+    }
+    double z[6];
+    for (int n = n1 - 1; n >= n0; --n) {
+        double v[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) v[a] = x[6 * (size_t)n + a];
+        if (n < n1 - 1) {
+            const double* e = Ab + 108 * (size_t)n + 12;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) t = fma(e[18 * a + b], z[b], t);
+                v[a] -= t;
+            }
+        }
+        const double* Si = Sinv + 36 * (size_t)n;
+        double w[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) t = fma(Si[6 * a + b], v[b], t);
+            w[a] = t;
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            z[a] = w[a];
+            x[6 * (size_t)n + a] = w[a];
+        }
+    }
+}
+
+// W[3 pos(q) + r] = (G_q U~_q^T y)_r
+__global__ void sv_project_kernel(int ns, const int* __restrict__ pos, const int4* __restrict__ sn,
+                                  const double* __restrict__ sG, const double* __restrict__ sc,
+                                  const uint8_t* __restrict__ fixed, const double* __restrict__ y,
+                                  double* __restrict__ W) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ns) return;
+    const int4 n4 = sn[q];
+    const int nd[4] = {n4.x, n4.y, n4.z, n4.w};
+    double u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double ck = sc[4 * (size_t)q + k];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const size_t d = 6 * (size_t)nd[k] + a;
+            if (!fixed[d]) u[a] = fma(ck, y[d], u[a]);
+        }
+    }
+    const double* G = sG + 9 * (size_t)q;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) W[3 * (size_t)pos[q] + r] = G[3 * r] * u[0] + G[3 * r + 1] * u[1] + G[3 * r + 2] * u[2];
+}
+
+// out = r - U~ W  (node lists hold the codes 4 slot + k of the pairs at each node)
+__global__ void sv_lift_kernel(int n_nodes, const int* __restrict__ pptr, const int* __restrict__ psrc,
+                               const int* __restrict__ pos, const double* __restrict__ sc,
+                               const uint8_t* __restrict__ fixed, const double* __restrict__ r,
+                               const double* __restrict__ W, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n_nodes) return;
+    const int n = t / 6, a = t - 6 * (t / 6);
+    double v = r[t];
+    if (a < 3 && !fixed[t]) {
+        for (int p = pptr[n]; p < pptr[n + 1]; ++p) {
+            const int code = psrc[p], q = code >> 2, k = code & 3;
+            v = fma(-sc[4 * (size_t)q + k], W[3 * (size_t)pos[q] + a], v);
+        }
+    }
+    out[t] = v;
+}
+
+// out = b - A~_b x from the chain blocks (one lane per scalar row); the
+// contact part U~ G U~^T x is subtracted by sv_project_kernel + sv_lift_kernel
+__global__ void sv_chain_residual_kernel(int n_nodes, const double* __restrict__ Ab, const double* __restrict__ b,
+                                         const double* __restrict__ x, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n_nodes) return;
+    const int n = t / 6, r = t - 6 * (t / 6);
+    const double* a = Ab + 108 * (size_t)n + 18 * r;
+    double v = 0.0;
+    const int k0 = n > 0 ? 0 : 6, k1 = n + 1 < n_nodes ? 18 : 12;
+    for (int k = k0; k < k1; ++k) v = fma(a[k], x[6 * (size_t)(n - 1) + k], v);
+    out[t] = b[t] - v;
+}
+
+__global__ void sv_rhs_kernel(int ndof, const double* __restrict__ R, const uint8_t* __restrict__ fixed,
+                              double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ndof) b[i] = fixed[i] ? 0.0 : -R[i];
+}
+
+__global__ void sv_axpy_kernel(int ndof, const double* __restrict__ dx, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ndof) x[i] += dx[i];
+}
+
+// rel = ||r|| / ||b|| (one workgroup; fixed order, so deterministic)
+__global__ __launch_bounds__(1024) void sv_relnorm_kernel(int ndof, const double* __restrict__ r,
+                                                          const double* __restrict__ b, double* __restrict__ rel) {
+    __shared__ double s0[1024], s1[1024];
+    double a = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < ndof; i += 1024) {
+        a = fma(r[i], r[i], a);
+        c = fma(b[i], b[i], c);
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *rel = s1[0] > 0.0 ? sqrt(s0[0] / s1[0]) : sqrt(s0[0]);
+}
+
+// ------------------------------------------------------------------ host
+
+static unsigned sv_grid(int64_t n, int bs = 256) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+#define SV_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            set_error(std::string("linear solve: ") + #expr + ": " + hipGetErrorString(_e)); \
+            return -1;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+#define SV_RB(expr)                                                                        \
+    do {                                                                                   \
+        rocblas_status _s = (expr);                                                        \
+        if (_s != rocblas_status_success) {                                                \
+            set_error(std::string("linear solve: ") + #expr + ": " + rocblas_status_to_string(_s)); \
+            return -1;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+template <class T>
+static hipError_t sv_upload(DBuf& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+int Ctx::solve_buffer(int which, void* host, int64_t nbytes, int64_t* size) {
+    if (which < 0 || which >= SV_COUNT || (int)sv.size() != SV_COUNT) {
+        set_error("xk_solve_buffer: no such buffer (run a solve first)");
+        return -1;
+    }
+    if (size) *size = (int64_t)sv[which].bytes;
+    if (host && nbytes > 0) {
+        SV_TRY(hipDeviceSynchronize());
+        SV_TRY(hipMemcpy(host, sv[which].p, std::min<size_t>((size_t)nbytes, sv[which].bytes), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+void Ctx::release_solver() {
+    for (auto& b : sv) b.release();
+    if (sv_handle) (void)rocblas_destroy_handle((rocblas_handle)sv_handle);
+    sv_handle = nullptr;
+    solve_ready = false;
+}
+
+// Per pattern: slots, contact nodes and their per-chain numbering, the
+// components of the contact graph and the buffer sizes.
+int Ctx::solve_setup() {
+    sv.resize(SV_COUNT);
+    const int ns = n_sinc;
+    SV_TRY(sv[SV_SLIST].ensure(sizeof(int) * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_SN].ensure(sizeof(int4) * (size_t)std::max(ns, 1)));
+    if (ns > 0) {
+        hipLaunchKernelGGL(sv_slist_kernel, dim3(sv_grid(n_pairs)), dim3(256), 0, 0, n_pairs, (const int*)P.sinc,
+                           (int*)sv[SV_SLIST].p);
+        SV_TRY(hipGetLastError());
+        hipLaunchKernelGGL(sv_slot_nodes_kernel, dim3(sv_grid(ns)), dim3(256), 0, 0, ns, (const int*)sv[SV_SLIST].p,
+                           P, (int4*)sv[SV_SN].p);
+        SV_TRY(hipGetLastError());
+    }
+    std::vector<int4> sn(ns);
+    if (ns) SV_TRY(hipMemcpy(sn.data(), sv[SV_SN].p, sizeof(int4) * ns, hipMemcpyDeviceToHost));
+    // contact nodes, numbered in node order (so contiguous per chain)
+    std::vector<int> cn(n_nodes, -1), cnode;
+    for (const int4& s : sn)
+        for (int v : {s.x, s.y, s.z, s.w}) cn[v] = 0;
+    for (int n = 0; n < n_nodes; ++n)
+        if (cn[n] == 0) {
+            cn[n] = (int)cnode.size();
+            cnode.push_back(n);
+        }
+    std::vector<int> ch_cn0(n_chains), ch_m(n_chains);
+    std::vector<int64_t> ch_yoff(n_chains), ch_coff(n_chains);
+    int64_t ytot = 0, ctot = 0;
+    {
+        int k = 0;
+        for (int c = 0; c < n_chains; ++c) {
+            const int n0 = cstart_h[c], n1 = cstart_h[c + 1];
+            ch_cn0[c] = k;
+            int cnt = 0;
+            while (k < (int)cnode.size() && cnode[k] < n1) {
+                ++k;
+                ++cnt;
+            }
+            const int m = 3 * cnt;
+            ch_m[c] = m;
+            ch_yoff[c] = ytot;
+            ch_coff[c] = ctot;
+            ytot += (int64_t)(n1 - n0) * 6 * m;
+            ctot += (int64_t)m * m;
+        }
+    }
+    // components: chains joined by pairs (union-find), slots ordered by component
+    std::vector<int> parent(n_chains);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (const int4& s : sn) {
+        const int a = find(node_chain_h[s.x]), b = find(node_chain_h[s.z]);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<int> comp_of_root(n_chains, -1), slot_comp(ns);
+    int ncomp = 0;
+    for (int q = 0; q < ns; ++q) {
+        const int r = find(node_chain_h[sn[q].x]);
+        if (comp_of_root[r] < 0) comp_of_root[r] = ncomp++;
+        slot_comp[q] = comp_of_root[r];
+    }
+    sv_comp_n.assign(ncomp, 0);
+    for (int q = 0; q < ns; ++q) sv_comp_n[slot_comp[q]]++;
+    sv_comp_pos.assign(ncomp + 1, 0);
+    for (int c = 0; c < ncomp; ++c) sv_comp_pos[c + 1] = sv_comp_pos[c] + sv_comp_n[c];
+    std::vector<int> pos(ns), cslot(ns);
+    {
+        std::vector<int64_t> fill(sv_comp_pos.begin(), sv_comp_pos.end() - 1);
+        for (int q = 0; q < ns; ++q) {
+            const int p = (int)fill[slot_comp[q]]++;
+            pos[q] = p;
+            cslot[p] = q;
+        }
+    }
+    sv_comp_soff.assign(ncomp + 1, 0);
+    for (int c = 0; c < ncomp; ++c) sv_comp_soff[c + 1] = sv_comp_soff[c] + 9 * (int64_t)sv_comp_n[c] * sv_comp_n[c];
+    const int64_t stot = sv_comp_soff[ncomp];
+    if ((double)stot * 8.0 > 64.0 * (1ull << 30)) {
+        set_error("linear solve: the contact capacitance matrices exceed 64 GiB");
+        return -1;
+    }
+    if (ns > 0 && *std::max_element(sv_comp_n.begin(), sv_comp_n.end()) > 65535) {
+        set_error("linear solve: a contact component holds more than 65535 contributing pairs");
+        return -1;
+    }
+    SV_TRY(sv_upload(sv[SV_CN], cn));
+    SV_TRY(sv_upload(sv[SV_CNODE], cnode));
+    SV_TRY(sv_upload(sv[SV_CHCN0], ch_cn0));
+    SV_TRY(sv_upload(sv[SV_CHM], ch_m));
+    SV_TRY(sv_upload(sv[SV_CHYOFF], ch_yoff));
+    SV_TRY(sv_upload(sv[SV_CHCOFF], ch_coff));
+    SV_TRY(sv_upload(sv[SV_POS], pos));
+    SV_TRY(sv_upload(sv[SV_CSLOT], cslot));
+    const size_t nd = 6 * (size_t)n_nodes;
+    SV_TRY(sv[SV_KB].ensure((size_t)std::max(n_blocks, 1) * 9 * (prec == 0 ? 8 : 4)));
+    SV_TRY(sv[SV_AB].ensure(sizeof(double) * 108 * (size_t)n_nodes));
+    SV_TRY(sv[SV_LF].ensure(sizeof(double) * 36 * (size_t)n_nodes));
+    SV_TRY(sv[SV_SINV].ensure(sizeof(double) * 36 * (size_t)n_nodes));
+    SV_TRY(sv[SV_SG].ensure(sizeof(double) * 9 * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_SC].ensure(sizeof(double) * 4 * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_Y].ensure(sizeof(double) * (size_t)std::max<int64_t>(ytot, 1)));
+    SV_TRY(sv[SV_C].ensure(sizeof(double) * (size_t)std::max<int64_t>(ctot, 1)));
+    SV_TRY(sv[SV_S].ensure(sizeof(double) * (size_t)std::max<int64_t>(stot, 1)));
+    SV_TRY(sv[SV_IPIV].ensure(sizeof(int64_t) * 3 * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_INFO].ensure(sizeof(int64_t) * (size_t)std::max(ncomp, 1)));
+    SV_TRY(sv[SV_W].ensure(sizeof(double) * 3 * (size_t)std::max(ns, 1)));
+    for (int b : {SV_B, SV_X, SV_R, SV_T}) SV_TRY(sv[b].ensure(sizeof(double) * nd));
+    SV_TRY(sv[SV_RED].ensure(64));
+    if (!sv_handle) {
+        rocblas_handle h = nullptr;
+        SV_RB(rocblas_create_handle(&h));
+        sv_handle = h;
+    }
+    solve_ready = true;
+    return 0;
+}
+
+// x <- A~^-1 r (Woodbury), r and x device vectors of 6 n_nodes; T scratch
+int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
+    const int ns = n_sinc;
+    const int nd = 6 * n_nodes;
+    double* T = (double*)sv[SV_T].p;
+    double* W = (double*)sv[SV_W].p;
+    const uint8_t* fixed = sv_fixed;
+    SV_TRY(hipMemcpyAsync(T, r, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 64)), dim3(64), 0, st, n_chains,
+                       (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
+                       (const double*)sv[SV_SINV].p, T);
+    SV_TRY(hipGetLastError());
+    if (ns == 0) {
+        SV_TRY(hipMemcpyAsync(x, T, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+        return 0;
+    }
+    hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, ns, (const int*)sv[SV_POS].p,
+                       (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, fixed,
+                       (const double*)T, W);
+    SV_TRY(hipGetLastError());
+    rocblas_handle h = (rocblas_handle)sv_handle;
+    for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+        const int64_t n = 3 * (int64_t)sv_comp_n[c];
+        SV_RB(rocsolver_dgetrs_64(h, rocblas_operation_none, n, 1, (double*)sv[SV_S].p + sv_comp_soff[c], n,
+                                  (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], W + 3 * sv_comp_pos[c], n));
+    }
+    hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes, (const int*)d_node_pptr.p,
+                       (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p, fixed, r,
+                       (const double*)W, x);
+    SV_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 64)), dim3(64), 0, st, n_chains,
+                       (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
+                       (const double*)sv[SV_SINV].p, x);
+    SV_TRY(hipGetLastError());
+    return 0;
+}
+
+int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, const double* M, const double* R,
+                      double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st) {
+    if (!direct) {
+        set_error("linear solve: the device solve needs element chains (the direct element path)");
+        return -1;
+    }
+    if (!pattern_valid || !tangent_staged) {
+        set_error("linear solve: assemble the iteration with K on the current pattern first");
+        return -1;
+    }
+    if (!solve_ready && solve_setup() != 0) return -1;
+    sv_fixed = fixed;
+    SV_RB(rocblas_set_stream((rocblas_handle)sv_handle, st));
+    const int ns = n_sinc;
+    const int nd = 6 * n_nodes;
+    // beam-only tangent with the mass term, then the chain factors
+    SV_TRY(launch_elements(u, false, sv[SV_KB].p, mass_coef, M, st));
+    if (prec == 0)
+        hipLaunchKernelGGL(sv_extract_kernel<double>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+                           (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
+                           (const double*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
+    else
+        hipLaunchKernelGGL(sv_extract_kernel<float>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+                           (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
+                           (const float*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
+    SV_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sv_factor_kernel, dim3(n_chains), dim3(64), 0, st, (const int*)d_cstart.p,
+                       (const double*)sv[SV_AB].p, (double*)sv[SV_LF].p, (double*)sv[SV_SINV].p, (int*)d_flags.p);
+    SV_TRY(hipGetLastError());
+    if (ns > 0) {
+        if (prec == 0)
+            hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
+                               (const double*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
+        else
+            hipLaunchKernelGGL(sv_slot_pack_kernel<float>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
+                               (const float*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
+        SV_TRY(hipGetLastError());
+        hipLaunchKernelGGL(sv_compliance_kernel, dim3(n_chains), dim3(256), 0, st, (const int*)d_cstart.p,
+                           (const int*)sv[SV_CN].p, (const int*)sv[SV_CNODE].p, (const int*)sv[SV_CHCN0].p,
+                           (const int*)sv[SV_CHM].p, (const int64_t*)sv[SV_CHYOFF].p,
+                           (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_AB].p,
+                           (const double*)sv[SV_LF].p, (const double*)sv[SV_SINV].p, (double*)sv[SV_Y].p,
+                           (double*)sv[SV_C].p);
+        SV_TRY(hipGetLastError());
+        rocblas_handle h = (rocblas_handle)sv_handle;
+        for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+            const int Pc = sv_comp_n[c];
+            double* S = (double*)sv[SV_S].p + sv_comp_soff[c];
+            hipLaunchKernelGGL(sv_capacitance_kernel, dim3(sv_grid(Pc, 64), Pc), dim3(64), 0, st, Pc,
+                               (const int*)sv[SV_CSLOT].p + sv_comp_pos[c], (const int4*)sv[SV_SN].p,
+                               (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, (const int*)d_node_chain.p,
+                               (const int*)sv[SV_CN].p, (const int*)sv[SV_CHCN0].p, (const int*)sv[SV_CHM].p,
+                               (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_C].p, S);
+            SV_TRY(hipGetLastError());
+            const int64_t n = 3 * (int64_t)Pc;
+            SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
+                                      (int64_t*)sv[SV_INFO].p + c));
+        }
+        hipLaunchKernelGGL(sv_info_kernel, dim3(sv_grid((int64_t)sv_comp_n.size())), dim3(256), 0, st,
+                           (int)sv_comp_n.size(), (const int64_t*)sv[SV_INFO].p, (int*)d_flags.p);
+        SV_TRY(hipGetLastError());
+    }
+    double* b = (double*)sv[SV_B].p;
+    double* r = (double*)sv[SV_R].p;
+    double* x = (double*)sv[SV_X].p;
+    hipLaunchKernelGGL(sv_rhs_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, R, fixed, b);
+    SV_TRY(hipGetLastError());
+    if (solve_apply(b, du, st) != 0) return -1;
+    // iterative refinement against the assembled K: a fixed number of steps
+    // (tol <= 0, stream-ordered) or until ||b - A du|| <= tol ||b|| (one
+    // host read of the norm per step)
+    // residual of the operator the solve inverts, A~_b + U~ G U~^T (in the
+    // FP64 form equal to the assembled K up to rounding; in the mixed form K
+    // accumulates its contact blocks in FP32 and differs by that rounding)
+    auto residual = [&]() -> int {
+        hipLaunchKernelGGL(sv_chain_residual_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+                           (const double*)sv[SV_AB].p, (const double*)b, (const double*)du, x);
+        SV_TRY(hipGetLastError());
+        if (ns > 0) {
+            hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, ns, (const int*)sv[SV_POS].p,
+                               (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p,
+                               fixed, (const double*)du, (double*)sv[SV_W].p);
+            SV_TRY(hipGetLastError());
+            hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes, (const int*)d_node_pptr.p,
+                               (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p,
+                               fixed, (const double*)x, (const double*)sv[SV_W].p, r);
+        } else {
+            SV_TRY(hipMemcpyAsync(r, x, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+        }
+        SV_TRY(hipGetLastError());
+        return 0;
+    };
+    double* red = (double*)sv[SV_RED].p;
+    int steps = 0;
+    bool have_r = false;
+    for (; steps < refine; ++steps) {
+        if (residual() != 0) return -1;
+        if (tol > 0.0) {
+            double rel = 0.0;
+            hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r,
+                               (const double*)b, red);
+            SV_TRY(hipGetLastError());
+            SV_TRY(hipMemcpyAsync(&rel, red, sizeof(double), hipMemcpyDeviceToHost, st));
+            SV_TRY(hipStreamSynchronize(st));
+            if (rel <= tol) {
+                have_r = true;
+                break;
+            }
+        }
+        if (solve_apply(r, x, st) != 0) return -1;
+        hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, (const double*)x, du);
+        SV_TRY(hipGetLastError());
+    }
+    if (n_refine) *n_refine = steps;
+    if (res_rel) {
+        if (!have_r && residual() != 0) return -1;
+        hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r, (const double*)b,
+                           res_rel);
+        SV_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace xk

@@ -87,6 +87,18 @@ struct Ctx {
     DBuf d_cdesc;   // int4 descriptors of the blocks with pair contributions
     DBuf d_fint;    // internal force written by the element kernel (6 n_nodes)
     int n_cdesc = 0;
+    // element chains (direct path): maximal runs of nodes joined n -> n + 1
+    int n_chains = 0;
+    std::vector<int> cstart_h, node_chain_h;
+    DBuf d_cstart, d_node_chain;
+    // device linear solve: per-pattern set-up, buffers, dense capacitance blocks
+    bool solve_ready = false;
+    bool tangent_staged = false;   // the last fused assembly staged the pairs' tangent records on this pattern
+    std::vector<DBuf> sv;
+    std::vector<int> sv_comp_n;                    // contributing pairs per component
+    std::vector<int64_t> sv_comp_pos, sv_comp_soff;  // first position / capacitance offset per component
+    void* sv_handle = nullptr;                     // rocblas handle
+    const uint8_t* sv_fixed = nullptr;
     // per-element friction coefficients (xk_ctx_set_friction)
     DBuf d_mu_elem;
     bool has_mu_elem = false;
@@ -113,6 +125,12 @@ struct Ctx {
     int lp_sync(hipStream_t st, bool keep_on);      // scatter them back (and end the watch list)
     void bind_pairs();
     int build_pattern();
+    int solve_setup();
+    int solve_apply(const double* r, double* x, hipStream_t st);
+    int linear_solve(const double* u, const uint8_t* fixed, double mass_coef, const double* M, const double* R,
+                     double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st);
+    void release_solver();
+    int solve_buffer(int which, void* host, int64_t nbytes, int64_t* size);
     int build_pattern_gpu(hipStream_t st);   // direct path: the pattern built on the device
     int host_pattern();                      // gptr_h / gcol_h of the current pattern
     int ensure_pattern();
@@ -128,6 +146,9 @@ struct Ctx {
         d_act.release();
         d_eflag.release();
         d_nflag.release();
+        d_cstart.release();
+        d_node_chain.release();
+        release_solver();
         d_bbeam.release();
         pat_tmp.release();
         for (auto& b : pat) b.release();

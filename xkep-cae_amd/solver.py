@@ -22,11 +22,15 @@ The per-attempt work (narrowphase, forces, tangent, effective terms, norm) is
 one call of the backend's ``assemble``; the linear solve is SciPy's SuperLU on
 the host, as in the reference.  The backend protocol is small so that the same
 driver runs on the device (``DeviceBackend``) or on any other assembly with
-the reference's semantics.
+the reference's semantics.  ``DeviceBackend(device_solve=True)`` instead runs
+the Newton attempts of an increment on the device: the assembly, the linear
+solve (``xk_linear_solve``), the line search and the updates stay in HBM and
+the host reads a few scalars per attempt.
 """
 
 from __future__ import annotations
 
+import math
 from collections import deque
 from dataclasses import dataclass, field
 
@@ -208,7 +212,10 @@ class NewtonOutcome:
 def newton_dynamic(backend, u: np.ndarray, u_ref: np.ndarray, f_ext: np.ndarray, fixed_dofs: np.ndarray,
                    dt: float, f_ext_ref_norm: float, dynamic_ref: bool, cfg: SolverConfig) -> NewtonOutcome:
     """One increment's Newton iterations (NewtonDynamicProcess.process,
-    _newton_dynamic.py:115-389).  ``u`` is updated in place."""
+    _newton_dynamic.py:115-389).  ``u`` is updated in place.  A backend that
+    runs the whole loop itself (``device_newton``) is handed the call."""
+    if getattr(backend, "device_newton", False):
+        return backend.newton(u, u_ref, f_ext, fixed_dofs, dt, f_ext_ref_norm, dynamic_ref, cfg)
     energy_ref = None
     incr_f_ref = 0.0
     consecutive_increase = 0
@@ -398,12 +405,16 @@ class DeviceBackend:
     the host for the SuperLU solve."""
 
     def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
-                 k_t_ratio: float = 0.5, fixed_dofs=None, config=None):
+                 k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False):
         from .assembly import StrandAssembly
         from .time_integration import GeneralizedAlphaProcess
 
         self.ctx = ctx
         self.torch = ctx.torch
+        self.device_newton = bool(device_solve)
+        self.solve_residuals = []   # device solve: relative residual of every linear solve
+        self.solve_tol = 1e-12      # device solve: target relative residual (SuperLU reaches about this)
+        self.max_refine = 16
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
         self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
                                   fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0)
@@ -440,6 +451,101 @@ class DeviceBackend:
 
     def correct(self, u, dt):
         self.ga.correct(u, None, dt)
+
+    def newton(self, u, u_ref, f_ext, fixed_dofs, dt, f_ext_ref_norm, dynamic_ref, cfg) -> NewtonOutcome:
+        """``newton_dynamic`` with the vectors in HBM and the linear solve on
+        the device (LinearSolveProcess -> xk_linear_solve, refined to a
+        relative residual of ``solve_tol``); the same control flow, thresholds
+        and order of updates.  The
+        host reads ||R||^2 and n_active per attempt, one norm per line-search
+        step and (||du||, ||u||, |du.R|, status) per update; ``u`` (host) is
+        updated in place at the end."""
+        torch, asm, ctx = self.torch, self.asm, self.ctx
+        if dt != self._dt:
+            asm.set_dt(dt)
+            self._dt = dt
+        asm.set_u(u)
+        asm.set_u_ref(u_ref)
+        asm.set_f_ext(f_ext)
+        ud = asm.u
+        mask = asm.fixed_mask.bool()
+        du = torch.empty_like(ud)
+        rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+        f_try = torch.empty_like(ud)
+        energy_ref = None
+        incr_f_ref = 0.0
+        consecutive_increase = 0
+        prev_ratio = float("inf")
+        out = NewtonOutcome(False, 0, 0, False, None)
+        for att in range(cfg.max_nr_attempts):
+            out.n_attempts += 1
+            R, K, res2, n_act = asm.assemble()
+            ctx.pairs_gather(f=self._f)
+            f_c = -self._f
+            out.f_c = f_c
+            # ConvergenceCheckProcess, force part (_newton_steps.py:194-213)
+            res = math.sqrt(float(res2.item()))
+            n_active = int(n_act.item())
+            f_ref = incr_f_ref if incr_f_ref > 1e-30 else f_ext_ref_norm
+            if dynamic_ref and att == 0 and res > 1e-30:
+                f_ref = res
+            if att == 0 and dynamic_ref:
+                incr_f_ref = f_ref
+            out.n_active = n_active
+            ratio = res / f_ref
+            out.res_history.append(ratio)
+            out.active_history.append(n_active)
+            if ratio < cfg.tol_force:
+                out.converged = True
+                break
+            # divergence (_newton_dynamic.py:258-285)
+            if att > 0 and ratio > prev_ratio * 1.01:
+                consecutive_increase += 1
+            else:
+                consecutive_increase = 0
+            prev_ratio = ratio
+            if (consecutive_increase >= cfg.divergence_window and not ratio < 0.01) or (att >= 10 and ratio > 100.0):
+                out.diverged = True
+                break
+            ctx.linear_solve(ud, asm.fixed_mask, R, du, asm._dyn, max_refine=self.max_refine,
+                             tol=self.solve_tol, res_rel=rel)
+            if ctx.status_flags() & 4:
+                break   # zero pivot: the solve failed (LinearSolveProcess returns success=False)
+            self.solve_residuals.append(rel)
+            rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+            scale = 1.0
+            if cfg.use_line_search:
+                # _ncp_line_search (contact/solver/_utils.py:31-61), f_c frozen
+                alpha = 1.0
+                thr = 3.0 * max(res, 1e-30)
+                for _ in range(cfg.line_search_max_steps):
+                    ctx.beam(ud + alpha * du, f_int=f_try)
+                    r_try = f_try + f_c - asm.f_ext
+                    r_try[mask] = 0.0
+                    if float(torch.linalg.vector_norm(r_try).item()) < thr:
+                        break
+                    alpha *= 0.5
+                scale *= alpha
+            if cfg.du_norm_cap > 0.0:
+                dn = float(torch.linalg.vector_norm(scale * du).item())
+                un = max(float(torch.linalg.vector_norm(ud).item()), 1.0)
+                if dn > cfg.du_norm_cap * un:
+                    scale *= cfg.du_norm_cap * un / dn
+            du.mul_(scale)
+            ud.add_(du)
+            # ConvergenceCheckProcess, displacement / energy part (_newton_steps.py:215-243)
+            s = torch.stack([torch.linalg.vector_norm(du), torch.linalg.vector_norm(ud), torch.dot(du, R)]).cpu()
+            du_norm, u_norm, energy = float(s[0]), float(s[1]), abs(float(s[2]))
+            if u_norm > 1e-30 and du_norm / u_norm < cfg.tol_disp:
+                out.converged = True
+                break
+            if energy_ref is None:
+                energy_ref = energy if energy > 1e-30 else 1.0
+            if energy_ref > 1e-30 and energy / energy_ref < 1e-10:
+                out.converged = True
+                break
+        u[:] = ud.cpu().numpy()
+        return out
 
     def checkpoint(self):
         self.ga.checkpoint()
