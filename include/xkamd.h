@@ -201,9 +201,9 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
  * this equals the assembled K up to rounding, in the mixed form K holds the
  * same sum rounded to FP32.  The result is refined against that operator:
  * with tol <= 0 exactly max_refine steps, stream-ordered; with tol > 0 until
- * ||D R + A~ du|| <= tol ||D R|| or max_refine steps, reading the norm on the
- * host after each step (the capacitance is ill-conditioned on strands, so
- * 2-8 steps reach 1e-12).  du (device, ndof) is written; res_rel (device
+ * ||D R + A~ du|| <= tol ||D R||, a step no longer halves that residual (its
+ * rounding floor, about 1e-12 on strands) or max_refine steps, reading the
+ * norm on the host after each step (2 steps reach 1e-11 on strands).  du (device, ndof) is written; res_rel (device
  * double, nullable) receives the final relative residual, n_refine (host,
  * nullable) the steps taken.  A zero pivot sets XK_FLAG_SINGULAR in
  * xk_status_flags.  The first call after a pattern change sets up the solve

@@ -9,7 +9,8 @@ on the host from the solve's buffers and checks
     FP64 form (1e-14), up to FP32 rounding in the mixed form (1e-6, K sums
     the same blocks in FP32);
   * the device residual ||b - A_w du|| / ||b|| within 1e-10 (the solve refines
-    to 1e-12 in fewer than 16 steps; SuperLU reaches about 1e-12);
+    to 1e-11 or its rounding floor near 1e-12 in a few steps; SuperLU reaches
+    about 1e-12);
   * du against SuperLU on A_w within 1e-7: two solutions with residuals near
     1e-12 differ by up to cond(A) times that, and cond(A) reaches 1e5;
   * FP64 form: the same two bars against the assembled K itself.
@@ -75,7 +76,7 @@ def solve_case(ctx, asm, fixed, precision):
     R, K, _, nact = asm.assemble()
     du = torch.empty(ctx.ndof, dtype=torch.float64, device=ctx.dev)
     rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
-    steps = ctx.linear_solve(asm.u, asm.fixed_mask, R, du, asm._dyn, max_refine=16, tol=1e-12, res_rel=rel)
+    steps = ctx.linear_solve(asm.u, asm.fixed_mask, R, du, asm._dyn, max_refine=16, tol=1e-11, res_rel=rel)
     torch.cuda.synchronize()
     assert ctx.status_flags() & 4 == 0, "zero pivot"
     assert steps < 16

@@ -311,11 +311,12 @@ class DeviceContext:
             self.stream), "xk_assemble_iteration")
 
     def linear_solve(self, u, fixed_mask, R, du, dyn: "L.XkDyn | None" = None, *, max_refine: int = 16,
-                     tol: float = 1e-12, res_rel=None) -> int:
+                     tol: float = 1e-11, res_rel=None) -> int:
         """du = -(D K D + I - D)^-1 D R on the device (LinearSolveProcess,
         _newton_steps.py:338-366), after ``assemble_iteration`` with a tangent
         at the same u, fixed_mask and dyn; refined until the relative residual
-        is <= tol (tol <= 0: exactly max_refine steps, no host read).
+        is <= tol or stops decreasing (tol <= 0: exactly max_refine steps, no
+        host read).
         res_rel: optional device float64 scalar receiving the final relative
         residual.  Returns the refinement steps taken."""
         for a, nm in ((u, "u"), (fixed_mask, "fixed_mask"), (R, "R"), (du, "du")):
@@ -329,7 +330,7 @@ class DeviceContext:
 
     SOLVE_BUFFERS = ("Kb", "Ab", "Lf", "Sinv", "slist", "slot_nodes", "slot_G", "slot_c", "pos", "cslot", "cn",
                      "cnode", "ch_m", "ch_cn0", "ch_yoff", "ch_coff", "Y", "C", "S", "ipiv", "info", "W", "b", "x",
-                     "r", "t")
+                     "r", "t", "red", "perm", "W2")
 
     def solve_buffer(self, name: str, dtype) -> np.ndarray:
         """Host copy of one of the linear solve's device buffers (tests)."""

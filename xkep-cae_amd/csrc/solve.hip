@@ -37,7 +37,7 @@ namespace xk {
 enum SolveBuf {
     SV_KB, SV_AB, SV_LF, SV_SINV, SV_SLIST, SV_SN, SV_SG, SV_SC, SV_POS, SV_CSLOT, SV_CN, SV_CNODE, SV_CHM,
     SV_CHCN0, SV_CHYOFF, SV_CHCOFF, SV_Y, SV_C, SV_S, SV_IPIV, SV_INFO, SV_W, SV_B, SV_X, SV_R, SV_T, SV_RED,
-    SV_COUNT
+    SV_PERM, SV_W2, SV_COUNT
 };
 
 // ------------------------------------------------------------------ kernels
@@ -331,64 +331,185 @@ __global__ void sv_info_kernel(int nc, const int64_t* __restrict__ info, int* __
     if (c < nc && info[c] != 0) atomicOr(flags, XK_FLAG_SINGULAR);
 }
 
-// x <- A~_b^-1 x along every chain (one lane per chain)
-__global__ void sv_chain_solve_kernel(int n_chains, const int* __restrict__ cstart, const double* __restrict__ Ab,
-                                      const double* __restrict__ Lf, const double* __restrict__ Sinv,
-                                      double* __restrict__ x) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_chains) return;
+// x <- A~_b^-1 x along every chain: 8 lanes per chain (lanes 0..5 hold the
+// node's six components), 8 chains per wavefront; rows of the 6x6 blocks are
+// combined across the chain's lanes by shuffles.
+__global__ __launch_bounds__(64) void sv_chain_solve_kernel(int n_chains, const int* __restrict__ cstart,
+                                                            const double* __restrict__ Ab,
+                                                            const double* __restrict__ Lf,
+                                                            const double* __restrict__ Sinv, double* __restrict__ x) {
+    const int lane = threadIdx.x, base = lane & ~7, a = lane & 7;
+    const int c = blockIdx.x * 8 + (lane >> 3);
+    if (c >= n_chains || a >= 6) return;
     const int n0 = cstart[c], n1 = cstart[c + 1];
-    double y[6];
+    // forward: y_n = x_n - L_n y_{n-1}; node n + 1's row and value are loaded
+    // before node n's shuffles so that the loads overlap the dependent chain
+    double y = 0.0;
+    double xv = x[6 * (size_t)n0 + a];
+    double L[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int n = n0; n < n1; ++n) {
-        double v[6];
+        double xn = 0.0, Ln[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (n + 1 < n1) {
+            xn = x[6 * (size_t)(n + 1) + a];
+            const double* p = Lf + 36 * (size_t)(n + 1) + 6 * a;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) v[a] = x[6 * (size_t)n + a];
+            for (int b = 0; b < 6; ++b) Ln[b] = p[b];
+        }
+        double v = xv;
         if (n > n0) {
-            const double* L = Lf + 36 * (size_t)n;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                double t = 0.0;
-#pragma unroll
-                for (int b = 0; b < 6; ++b) t = fma(L[6 * a + b], y[b], t);
-                v[a] -= t;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            y[a] = v[a];
-            x[6 * (size_t)n + a] = v[a];
-        }
-    }
-    double z[6];
-    for (int n = n1 - 1; n >= n0; --n) {
-        double v[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) v[a] = x[6 * (size_t)n + a];
-        if (n < n1 - 1) {
-            const double* e = Ab + 108 * (size_t)n + 12;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                double t = 0.0;
-#pragma unroll
-                for (int b = 0; b < 6; ++b) t = fma(e[18 * a + b], z[b], t);
-                v[a] -= t;
-            }
-        }
-        const double* Si = Sinv + 36 * (size_t)n;
-        double w[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
             double t = 0.0;
 #pragma unroll
-            for (int b = 0; b < 6; ++b) t = fma(Si[6 * a + b], v[b], t);
-            w[a] = t;
+            for (int b = 0; b < 6; ++b) t = fma(L[b], __shfl(y, base + b, 64), t);
+            v -= t;
         }
+        y = v;
+        x[6 * (size_t)n + a] = v;
+        xv = xn;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            z[a] = w[a];
-            x[6 * (size_t)n + a] = w[a];
+        for (int b = 0; b < 6; ++b) L[b] = Ln[b];
+    }
+    // backward: x_n = S_n^-1 (y_n - E_n x_{n+1}), prefetched the same way
+    double z = 0.0;
+    double yv = x[6 * (size_t)(n1 - 1) + a];
+    double E[6], Si[6];
+    {
+        const double* pe = Ab + 108 * (size_t)(n1 - 1) + 18 * a + 12;
+        const double* ps = Sinv + 36 * (size_t)(n1 - 1) + 6 * a;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            E[b] = pe[b];
+            Si[b] = ps[b];
         }
     }
+    for (int n = n1 - 1; n >= n0; --n) {
+        double yn = 0.0, En[6], Sn[6];
+        if (n - 1 >= n0) {
+            yn = x[6 * (size_t)(n - 1) + a];
+            const double* pe = Ab + 108 * (size_t)(n - 1) + 18 * a + 12;
+            const double* ps = Sinv + 36 * (size_t)(n - 1) + 6 * a;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                En[b] = pe[b];
+                Sn[b] = ps[b];
+            }
+        }
+        double v = yv;
+        if (n < n1 - 1) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) t = fma(E[b], __shfl(z, base + b, 64), t);
+            v -= t;
+        }
+        double w = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) w = fma(Si[b], __shfl(v, base + b, 64), w);
+        z = w;
+        x[6 * (size_t)n + a] = w;
+        yv = yn;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            E[b] = En[b];
+            Si[b] = Sn[b];
+        }
+    }
+}
+
+// ---- dense LU solves of the capacitance blocks (column-major, ld = n).
+// Substitution, right-looking, in row blocks of TB: step s updates the rows
+// below block s with block s's solved values (workgroup 0 takes block s + 1
+// and then solves its diagonal block with the block's rows held in
+// registers).  Backward stable, unlike solves through inverted diagonal blocks.
+constexpr int TB = 64;
+
+// acc - sum_{k < m} A[i + (k0 + k) ld] ys[k], all loads in flight for full blocks
+__device__ __forceinline__ double sv_block_dot(double acc, const double* __restrict__ A, size_t ld, int i, int k0,
+                                               int m, const double* ys) {
+    const double* a = A + i + (size_t)k0 * ld;
+    if (m == TB) {
+        double p[TB];
+#pragma unroll
+        for (int k = 0; k < TB; ++k) p[k] = a[(size_t)k * ld];
+#pragma unroll
+        for (int k = 0; k < TB; ++k) acc = fma(-p[k], ys[k], acc);
+    } else {
+        for (int k = 0; k < m; ++k) acc = fma(-a[(size_t)k * ld], ys[k], acc);
+    }
+    return acc;
+}
+
+// y[i] = t[perm[i]]
+__global__ void sv_permute_kernel(int n, const int* __restrict__ perm, const double* __restrict__ t,
+                                  double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = t[perm[i]];
+}
+
+// unit lower triangle: step s (-1: only the diagonal solve of block 0)
+__global__ __launch_bounds__(TB) void sv_lower_step_kernel(int n, const double* __restrict__ LU,
+                                                           double* __restrict__ y, int s) {
+    __shared__ double ys[TB];
+    const int t = threadIdx.x;
+    const size_t ld = (size_t)n;
+    const int nb = (n + TB - 1) / TB;
+    if (s >= 0) {
+        const int k0 = s * TB, k1 = min(n, k0 + TB);
+        if (k0 + t < k1) ys[t] = y[k0 + t];
+        __syncthreads();
+        const int i = (s + 1) * TB + blockIdx.x * TB + t;
+        if (i < n) y[i] = sv_block_dot(y[i], LU, ld, i, k0, k1 - k0, ys);
+    }
+    if (blockIdx.x != 0 || s + 1 >= nb) return;
+    // diagonal solve of block s + 1 (its rows were updated by this workgroup)
+    __syncthreads();
+    const int r0 = (s + 1) * TB, m = min(TB, n - r0);
+    double row[TB];
+#pragma unroll
+    for (int k = 0; k < TB; ++k) row[k] = (t < m && k < t) ? LU[(r0 + t) + (size_t)(r0 + k) * ld] : 0.0;
+    double v = t < m ? y[r0 + t] : 0.0;
+#pragma unroll
+    for (int k = 0; k < TB; ++k) {
+        if (k >= m) continue;   // uniform; keeps the loop unrolled (row[] stays in registers)
+        if (t == k) ys[k] = v;
+        __syncthreads();
+        if (t > k) v = fma(-row[k], ys[k], v);
+    }
+    if (t < m) y[r0 + t] = v;
+}
+
+// upper triangle with its diagonal: step s (nb: only the diagonal solve of the
+// last block); updates the rows above block s, workgroup 0 solves block s - 1
+__global__ __launch_bounds__(TB) void sv_upper_step_kernel(int n, const double* __restrict__ LU,
+                                                           double* __restrict__ y, int s) {
+    __shared__ double ys[TB];
+    const int t = threadIdx.x;
+    const size_t ld = (size_t)n;
+    const int nb = (n + TB - 1) / TB;
+    if (s < nb) {
+        const int k0 = s * TB, k1 = min(n, k0 + TB);
+        if (k0 + t < k1) ys[t] = y[k0 + t];
+        __syncthreads();
+        // rows above block s, from the row block just above it downwards
+        const int i = (s - 1) * TB - blockIdx.x * TB + t;
+        if (i >= 0 && i < k0) y[i] = sv_block_dot(y[i], LU, ld, i, k0, k1 - k0, ys);
+    }
+    if (blockIdx.x != 0 || s - 1 < 0) return;
+    __syncthreads();
+    const int r0 = (s - 1) * TB, m = min(TB, n - r0);
+    double row[TB];
+#pragma unroll
+    for (int k = 0; k < TB; ++k) row[k] = (t < m && k >= t && k < m) ? LU[(r0 + t) + (size_t)(r0 + k) * ld] : 0.0;
+    double v = t < m ? y[r0 + t] : 0.0;
+#pragma unroll
+    for (int k = TB - 1; k >= 0; --k) {
+        if (k >= m) continue;
+        if (t == k) {
+            v = v / row[k];
+            ys[k] = v;
+        }
+        __syncthreads();
+        if (t < k) v = fma(-row[k], ys[k], v);
+    }
+    if (t < m) y[r0 + t] = v;
 }
 
 // W[3 pos(q) + r] = (G_q U~_q^T y)_r
@@ -639,6 +760,8 @@ int Ctx::solve_setup() {
     SV_TRY(sv[SV_IPIV].ensure(sizeof(int64_t) * 3 * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_INFO].ensure(sizeof(int64_t) * (size_t)std::max(ncomp, 1)));
     SV_TRY(sv[SV_W].ensure(sizeof(double) * 3 * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_W2].ensure(sizeof(double) * 3 * (size_t)std::max(ns, 1)));
+    SV_TRY(sv[SV_PERM].ensure(sizeof(int) * 3 * (size_t)std::max(ns, 1)));
     for (int b : {SV_B, SV_X, SV_R, SV_T}) SV_TRY(sv[b].ensure(sizeof(double) * nd));
     SV_TRY(sv[SV_RED].ensure(64));
     if (!sv_handle) {
@@ -658,7 +781,7 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
     double* W = (double*)sv[SV_W].p;
     const uint8_t* fixed = sv_fixed;
     SV_TRY(hipMemcpyAsync(T, r, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 64)), dim3(64), 0, st, n_chains,
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
                        (const double*)sv[SV_SINV].p, T);
     SV_TRY(hipGetLastError());
@@ -670,17 +793,29 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
                        (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, fixed,
                        (const double*)T, W);
     SV_TRY(hipGetLastError());
-    rocblas_handle h = (rocblas_handle)sv_handle;
+    double* W2 = (double*)sv[SV_W2].p;
     for (size_t c = 0; c < sv_comp_n.size(); ++c) {
-        const int64_t n = 3 * (int64_t)sv_comp_n[c];
-        SV_RB(rocsolver_dgetrs_64(h, rocblas_operation_none, n, 1, (double*)sv[SV_S].p + sv_comp_soff[c], n,
-                                  (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], W + 3 * sv_comp_pos[c], n));
+        const int n = 3 * sv_comp_n[c];
+        const int nb = (n + TB - 1) / TB;
+        const double* LU = (const double*)sv[SV_S].p + sv_comp_soff[c];
+        double* y = W2 + 3 * sv_comp_pos[c];
+        hipLaunchKernelGGL(sv_permute_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n,
+                           (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)(W + 3 * sv_comp_pos[c]), y);
+        SV_TRY(hipGetLastError());
+        for (int s = -1; s <= nb - 2; ++s) {
+            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(s < 0 ? 1 : nb - s - 1), dim3(TB), 0, st, n, LU, y, s);
+            SV_TRY(hipGetLastError());
+        }
+        for (int s = nb; s >= 1; --s) {
+            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(s == nb ? 1 : s), dim3(TB), 0, st, n, LU, y, s);
+            SV_TRY(hipGetLastError());
+        }
     }
     hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes, (const int*)d_node_pptr.p,
                        (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p, fixed, r,
-                       (const double*)W, x);
+                       (const double*)W2, x);
     SV_TRY(hipGetLastError());
-    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 64)), dim3(64), 0, st, n_chains,
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
                        (const double*)sv[SV_SINV].p, x);
     SV_TRY(hipGetLastError());
@@ -748,6 +883,26 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
         hipLaunchKernelGGL(sv_info_kernel, dim3(sv_grid((int64_t)sv_comp_n.size())), dim3(256), 0, st,
                            (int)sv_comp_n.size(), (const int64_t*)sv[SV_INFO].p, (int*)d_flags.p);
         SV_TRY(hipGetLastError());
+        // the row interchanges of every component as one permutation (host)
+        const size_t np3 = 3 * (size_t)ns;
+        piv_h.resize(np3);
+        perm_h.resize(np3);
+        SV_TRY(hipMemcpyAsync(piv_h.data(), sv[SV_IPIV].p, sizeof(int64_t) * np3, hipMemcpyDeviceToHost, st));
+        SV_TRY(hipStreamSynchronize(st));
+        for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+            const int64_t o = 3 * sv_comp_pos[c], n = 3 * (int64_t)sv_comp_n[c];
+            int* p = perm_h.data() + o;
+            for (int64_t i = 0; i < n; ++i) p[i] = (int)i;
+            for (int64_t k = 0; k < n; ++k) {
+                const int64_t q = piv_h[o + k] - 1;
+                if (q < 0 || q >= n) {
+                    set_error("linear solve: pivot index out of range");
+                    return -1;
+                }
+                std::swap(p[k], p[q]);
+            }
+        }
+        SV_TRY(hipMemcpyAsync(sv[SV_PERM].p, perm_h.data(), sizeof(int) * np3, hipMemcpyHostToDevice, st));
     }
     double* b = (double*)sv[SV_B].p;
     double* r = (double*)sv[SV_R].p;
@@ -782,6 +937,7 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     double* red = (double*)sv[SV_RED].p;
     int steps = 0;
     bool have_r = false;
+    double prev = HUGE_VAL;
     for (; steps < refine; ++steps) {
         if (residual() != 0) return -1;
         if (tol > 0.0) {
@@ -791,10 +947,12 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
             SV_TRY(hipGetLastError());
             SV_TRY(hipMemcpyAsync(&rel, red, sizeof(double), hipMemcpyDeviceToHost, st));
             SV_TRY(hipStreamSynchronize(st));
-            if (rel <= tol) {
+            // done, or at the rounding floor (a step no longer halves the residual)
+            if (rel <= tol || rel > 0.5 * prev) {
                 have_r = true;
                 break;
             }
+            prev = rel;
         }
         if (solve_apply(r, x, st) != 0) return -1;
         hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, (const double*)x, du);

@@ -99,6 +99,8 @@ struct Ctx {
     std::vector<int64_t> sv_comp_pos, sv_comp_soff;  // first position / capacitance offset per component
     void* sv_handle = nullptr;                     // rocblas handle
     const uint8_t* sv_fixed = nullptr;
+    std::vector<int64_t> piv_h;   // host copies of the capacitance pivots and their permutation
+    std::vector<int> perm_h;
     // per-element friction coefficients (xk_ctx_set_friction)
     DBuf d_mu_elem;
     bool has_mu_elem = false;

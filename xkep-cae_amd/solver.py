@@ -413,7 +413,7 @@ class DeviceBackend:
         self.torch = ctx.torch
         self.device_newton = bool(device_solve)
         self.solve_residuals = []   # device solve: relative residual of every linear solve
-        self.solve_tol = 1e-12      # device solve: target relative residual (SuperLU reaches about this)
+        self.solve_tol = 1e-11      # device solve: target relative residual (SuperLU reaches about 1e-12)
         self.max_refine = 16
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
         self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
