@@ -219,10 +219,61 @@ def run(label, case, states, prec, torch, steps, warmup, tune=False, batch=None,
     return out
 
 
+def newton_attempt(case, torch, wl, reps=5):
+    """One Newton attempt at 91 wires: the fused assembly plus the device
+    linear solve (xk_linear_solve, refined to 1e-11) against the same assembly
+    plus SciPy SuperLU on the host (the reference's LinearSolveProcess), FP64
+    and FP32-mixed tangents."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    st = wl.make_state(case)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        ctx, asm, _ = setup(case, [st], prec, torch)
+        du = torch.empty(ctx.ndof, dtype=torch.float64, device=ctx.dev)
+        ta, ts, steps = [], [], []
+        for rep in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            R, K, _, _ = asm.assemble()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            steps.append(ctx.linear_solve(asm.u, asm.fixed_mask, R, du, asm._dyn, max_refine=16, tol=1e-11))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep:   # the first call also sets up the solve for the pattern
+                ta.append(t1 - t0)
+                ts.append(t2 - t1)
+        rec = {"assembly_ms": round(1e3 * float(np.median(ta)), 3), "device_solve_ms": round(1e3 * float(np.median(ts)), 2),
+               "refinement_steps": int(np.median(steps)), "contributing_pairs": int(ctx.pattern_counts()[2])}
+        if prec == "fp64":
+            Kc = asm.K_csr()
+            n = Kc.shape[0]
+            free = np.ones(n)
+            free[st.fixed_dofs] = 0.0
+            D = sp.diags(free)
+            A = (D @ Kc @ D + sp.diags(1.0 - free)).tocsc()
+            b = -R.cpu().numpy()
+            b[st.fixed_dofs] = 0.0
+            t0 = time.perf_counter()
+            sla.spsolve(A, b)
+            rec["host_superlu_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            rec["solve_speedup_vs_superlu"] = round(rec["host_superlu_ms"] / rec["device_solve_ms"], 1)
+        log(f"[bench] Newton attempt 91 wires {prec}: {rec}")
+        out[prec] = rec
+        del ctx, asm
+        torch.cuda.empty_cache()
+    out["workload"] = ("91-wire strand, one Newton attempt: fused assembly + device linear solve "
+                       "(chain block LU + Woodbury over the pairs' tangent blocks) vs SciPy SuperLU on the host")
+    return out
+
+
 def side_measurements(batched, torch, wl, args):
     """Beside the headline: the learned prescreen on the batched 91-wire
     workload, the uneven 64-instance batch, and the 1000-wire strand (FP32-mixed,
-    HIP-graph replay, per-step times)."""
+    HIP-graph replay, per-step times), and one 91-wire Newton attempt with the
+    device linear solve against host SuperLU."""
     out = {}
     steps = max(20, args.steps // 4)
     P = importlib.import_module("xkep-cae_amd.prescreen")
@@ -255,6 +306,9 @@ def side_measurements(batched, torch, wl, args):
     out["wires_1000"] = {"n_elem": r5["n_elem"], "n_pair": r5["n_pair"], "us_per_step": round(r5["ms"] * 1e3, 2),
                          "value": round(r5["value"], 1), "stages_us": np.round(r5["stages"] * 1e3, 2).tolist(),
                          "increment_setup_ms": round(r5["t_inc"] * 1e3, 1)}
+    del r5
+    torch.cuda.empty_cache()
+    out["newton_attempt_91wire"] = newton_attempt(wl.C3, torch, wl)
     return out
 
 
