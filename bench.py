@@ -322,6 +322,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes and grid cell")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the side measurements (uneven 64-instance batch, 1000 wires, learned prescreen)")
+    ap.add_argument("--newton-only", action="store_true",
+                    help="only the 91-wire Newton attempt: device linear solve vs host SuperLU")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -346,6 +348,9 @@ def main():
 
     wl = importlib.import_module("xkep-cae_amd.workloads")
     case = wl.C3
+    if args.newton_only:
+        print(json.dumps({"newton_attempt_91wire": newton_attempt(case, torch, wl)}), flush=True)
+        return
     st = wl.make_state(case)
     plain = run("fp64 x1", case, [st], "fp64", torch, args.steps, args.warmup)
     single = run("mixed x1", case, [st], "mixed", torch, args.steps, args.warmup)
@@ -354,6 +359,11 @@ def main():
     states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
     batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup,
                   tune=not args.no_autotune)
+    # the FP64 plain form on the same batch (same tuned shapes through the cache)
+    plain_b = run(f"fp64 x{args.batch}", case, states, "fp64", torch, args.steps, args.warmup,
+                  tune=not args.no_autotune)
+    del plain_b["asm"], plain_b["ctx"]
+    torch.cuda.empty_cache()
 
     extra = {} if args.no_extra else side_measurements(batched, torch, wl, args)
     nnz_beam = beam_nnz(batched["ctx"].conn_np)
@@ -406,6 +416,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_fp64_gpu": round(value / plain["value"], 3),
+        "fp64_batched": {"value": round(plain_b["value"], 1), "us_per_step": round(plain_b["ms"] * 1e3, 2),
+                         "stages_us": np.round(plain_b["stages"] * 1e3, 2).tolist()},
+        "speedup_vs_fp64_gpu_same_batch": round(value / plain_b["value"], 3),
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
