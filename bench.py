@@ -269,6 +269,35 @@ def newton_attempt(case, torch, wl, reps=5):
     return out
 
 
+def newton_loop(torch, wl, force=0.1):
+    """A whole dynamic solve of the 91-wire strand geometry (radial squeeze of
+    the outer wires, workloads.squeeze_case, Coulomb mu 0.3, FP64): Newton
+    attempts with host SuperLU, on the device, and on the device with the
+    attempts after each increment's first replayed from HIP graphs.  The
+    three trajectories (attempts and active counts per call) must agree."""
+    TP = importlib.import_module("xkep-cae_amd.train_prescreen")
+    sq = wl.squeeze_case(force, n_wires=91, n_elems=150, length=80.0, radius=0.4, lay_angle_deg=12.0, mu=0.3)
+    out, traj = {}, {}
+    for mode in ("device_graph", "device", "host_superlu"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, b = TP.run_case(sq, precision="fp64", device_solve=mode != "host_superlu",
+                             use_graph=mode == "device_graph")
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        att = sum(c.n_attempts for c in res.calls)
+        traj[mode] = [(c.n_attempts, c.n_active) for c in res.calls]
+        out[mode] = {"seconds": round(t, 3), "attempts": att, "ms_per_attempt": round(1e3 * t / max(att, 1), 2)}
+        if mode == "device_graph":
+            out[mode]["graphs_captured"] = b.graph_attempts_captured
+        log(f"[bench] Newton loop 91 wires {mode}: {out[mode]}")
+    out["trajectories_equal"] = traj["device_graph"] == traj["device"] == traj["host_superlu"]
+    out["calls"] = traj["device"]
+    out["workload"] = (f"91-wire strand (150 el/wire, r 0.4, L 80), dynamic radial squeeze {force} N per outer wire, "
+                       "mu 0.3, 4 increments x up to 30 attempts (the reference's line search halves to 1/32)")
+    return out
+
+
 def side_measurements(batched, torch, wl, args):
     """Beside the headline: the learned prescreen on the batched 91-wire
     workload, the uneven 64-instance batch, and the 1000-wire strand (FP32-mixed,
@@ -309,6 +338,7 @@ def side_measurements(batched, torch, wl, args):
     del r5
     torch.cuda.empty_cache()
     out["newton_attempt_91wire"] = newton_attempt(wl.C3, torch, wl)
+    out["newton_loop_91wire"] = newton_loop(torch, wl)
     return out
 
 
@@ -349,7 +379,8 @@ def main():
     wl = importlib.import_module("xkep-cae_amd.workloads")
     case = wl.C3
     if args.newton_only:
-        print(json.dumps({"newton_attempt_91wire": newton_attempt(case, torch, wl)}), flush=True)
+        print(json.dumps({"newton_attempt_91wire": newton_attempt(case, torch, wl),
+                          "newton_loop_91wire": newton_loop(torch, wl)}), flush=True)
         return
     st = wl.make_state(case)
     plain = run("fp64 x1", case, [st], "fp64", torch, args.steps, args.warmup)

@@ -211,6 +211,26 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
 int xk_linear_solve(xk_ctx* ctx, const double* u, const uint8_t* fixed_mask, const xk_dyn* dyn,
                     const double* R, double* du, int max_refine, double tol, double* res_rel,
                     int32_t* n_refine, void* stream);
+/* Line search, update and convergence norms of the attempt on the device,
+ * replacing LineSearchUpdateProcess (contact/solver/_newton_steps.py:396-434)
+ * with _ncp_line_search (contact/solver/_utils.py:31-61) and the
+ * displacement / energy norms of ConvergenceCheckProcess (:215-243).
+ * alpha = 1, 1/2, ... (line_search_steps candidates, 0 = no search): the
+ * first alpha whose frozen-contact static residual
+ * f_int(u + alpha du) - f_contact - f_ext (fixed dofs zeroed) is finite and
+ * below diverge_factor * sqrt(*res_norm2) is taken, else 2^-line_search_steps;
+ * with du_norm_cap > 0, alpha du is scaled to at most du_norm_cap * max(||u||, 1);
+ * then du = alpha du, u += du.  f_contact is xk_pairs_gather's pair force at
+ * the attempt (the reference's -f_c).  status (device, >= 24 doubles):
+ * [0] alpha, [1] ||du||, [2] ||u||, [3] |du . R|, [4] index of the accepted
+ * candidate (line_search_steps if none), [8 + k] candidate k's residual norm.
+ * Direct element path only; stream-ordered, no host synchronisation, no
+ * memset (capturable in a HIP graph). */
+int xk_newton_update(xk_ctx* ctx, double* u, double* du, const double* R, const double* f_contact,
+                     const double* f_ext, const uint8_t* fixed_mask, const double* res_norm2,
+                     int line_search_steps, double diverge_factor, double du_norm_cap, double* status,
+                     void* stream);
+
 /* Copy of one of the linear solve's internal device buffers (index `which`,
  * layouts in csrc/solve.hip) for tests; *size_bytes receives its size.
  * Synchronous.  No counterpart in the reference. */

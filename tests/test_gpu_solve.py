@@ -160,3 +160,44 @@ def test_device_solve_without_contact(gpu):
     out = solve_case(ctx, asm, st.fixed_dofs, "fp64")
     assert out["nact"] == 0 and ctx.n_pairs == 0
     assert out["steps"] == 0 and out["res_K"] < 1e-13
+
+
+def test_captured_solve_and_update_replays_match_eager(gpu):
+    """The device solve (three fixed refinement steps) + line search + update,
+    captured in a HIP graph, replayed three times from the same state with
+    eager work in between: every replay is bitwise the eager run (u, du and
+    the status words)."""
+    wl, Kc = pkg("workloads"), pkg("contact")
+    case = wl.C3
+    st = wl.make_state(case)
+    m = st.mesh
+    cfg = Kc.ContactConfig(mu=case.mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(st.elem_layer))
+    ctx, asm = strand_assembly(m.node_coords, m.connectivity, m.radii, case, st, "fp64", cfg)
+    torch = ctx.torch
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(st.u)
+    R, K, res2, _ = asm.assemble()
+    fc = torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev)
+    ctx.pairs_gather(f=fc)
+    u0 = asm.u.clone()
+    du = torch.empty_like(u0)
+    status = torch.zeros(24, dtype=torch.float64, device=ctx.dev)
+    rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+
+    def step():
+        ctx.linear_solve(asm.u, asm.fixed_mask, R, du, asm._dyn, max_refine=3, tol=0.0, res_rel=rel)
+        ctx.newton_update(asm.u, du, R, fc, asm.f_ext, asm.fixed_mask, res2, status, line_search_steps=5)
+
+    step()
+    torch.cuda.synchronize()
+    ue, due, ste = asm.u.clone(), du.clone(), status.clone()
+    assert float(rel.item()) < RES_TOL and float(ste[0]) > 0.0
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    for rep in range(3):
+        asm.u.copy_(u0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(asm.u, ue) and torch.equal(du, due) and torch.equal(status, ste), rep
+        _ = float((asm.u * 2.0).sum().item())

@@ -105,12 +105,14 @@ def test_newton_trajectory_matches_reference_on_oracle_assembly():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("device_solve", [False, True], ids=["host_superlu", "device_solve"])
+@pytest.mark.parametrize("mode", ["host_superlu", "device_solve", "device_graph"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_newton_trajectory_matches_reference_on_device(gpu, precision, device_solve):
-    """The device assembly under the driver, with SuperLU on the host or with
-    the Newton attempts on the device (device linear solve, line search and
-    updates in HBM)."""
+def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode):
+    """The device assembly under the driver: with SuperLU on the host, with the
+    Newton attempts on the device (device linear solve, line search and
+    updates in HBM), and with the attempts after each increment's first as
+    replays of two captured HIP graphs."""
+    device_solve = mode != "host_superlu"
     g, prm, cfg = case()
     S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
     r = float(g["radii"][0])
@@ -120,12 +122,16 @@ def test_newton_trajectory_matches_reference_on_device(gpu, precision, device_so
                             elem_layer_map=wl.layer_map(g["layer"]))
     b = S.DeviceBackend(ctx, M_diag=g["M_diag"], rho_inf=prm["rho_inf"], k_pen=prm["k_pen"],
                         smoothing_delta=prm["smoothing_delta"], mu=prm["mu"], k_t_ratio=prm["k_t_ratio"],
-                        fixed_dofs=g["fixed"], config=kcfg, device_solve=device_solve)
+                        fixed_dofs=g["fixed"], config=kcfg, device_solve=device_solve,
+                        use_graph=mode == "device_graph")
     res = S.ContactFrictionSolver(b, fixed_dofs=g["fixed"], f_ext_total=g["f_ext"], t_total=prm["t_total"],
                                   config=cfg).solve()
     check_trajectory(g, res, precision)
     if device_solve:
         rel = [float(r.item()) for r in b.solve_residuals]
         assert len(rel) > 0 and max(rel) < 1e-10, max(rel)
+    if mode == "device_graph":
+        assert b.graph_error is None, b.graph_error
+        assert b.graph_attempts_captured > 0
     tol = 1e-8 if precision == "fp64" else 1e-4
     assert np.linalg.norm(res.u - g["u_final"]) <= tol * np.linalg.norm(g["u_final"])

@@ -328,6 +328,22 @@ class DeviceContext:
             "xk_linear_solve")
         return steps.value
 
+    def newton_update(self, u, du, R, f_contact, f_ext, fixed_mask, res_norm2, status, *,
+                      line_search_steps: int = 5, diverge_factor: float = 3.0, du_norm_cap: float = 0.0):
+        """Line search + update + norms of a Newton attempt on the device
+        (LineSearchUpdateProcess with _ncp_line_search, _newton_steps.py:396-434,
+        _utils.py:31-61); ``status`` (device float64, >= 24) receives alpha,
+        ||du||, ||u||, |du.R| and the accepted candidate.  No host sync."""
+        for a, nm in ((u, "u"), (du, "du"), (R, "R"), (f_contact, "f_contact"), (f_ext, "f_ext"),
+                      (fixed_mask, "fixed_mask")):
+            self._check_vec(a, self.ndof, nm)
+        if status.numel() < 24:
+            raise ValueError("status needs 24 entries")
+        L.check(L.lib().xk_newton_update(
+            self.h, L.ptr(u), L.ptr(du), L.ptr(R), L.ptr(f_contact), L.ptr(f_ext), L.ptr(fixed_mask),
+            L.ptr(res_norm2), int(line_search_steps), float(diverge_factor), float(du_norm_cap), L.ptr(status),
+            self.stream), "xk_newton_update")
+
     SOLVE_BUFFERS = ("Kb", "Ab", "Lf", "Sinv", "slist", "slot_nodes", "slot_G", "slot_c", "pos", "cslot", "cn",
                      "cnode", "ch_m", "ch_cn0", "ch_yoff", "ch_coff", "Y", "C", "S", "ipiv", "info", "W", "b", "x",
                      "r", "t", "red", "perm", "W2")

@@ -725,6 +725,17 @@ int xk_pairs_gather(xk_ctx* c, double* f, void* K_vals, int64_t K_cap, void* str
 
 int xk_pattern_build(xk_ctx* c) { return c ? c->build_pattern() : -1; }
 
+int xk_newton_update(xk_ctx* c, double* u, double* du, const double* R, const double* f_contact,
+                     const double* f_ext, const uint8_t* fixed_mask, const double* res_norm2, int line_search_steps,
+                     double diverge_factor, double du_norm_cap, double* status, void* stream) {
+    if (!c || !u || !du || !R || !f_contact || !f_ext || !fixed_mask || !res_norm2 || !status) {
+        set_error("xk_newton_update: null argument");
+        return -1;
+    }
+    return c->newton_update(u, du, R, f_contact, f_ext, fixed_mask, res_norm2, line_search_steps, diverge_factor,
+                            du_norm_cap, status, (hipStream_t)stream);
+}
+
 int xk_solve_buffer(xk_ctx* c, int which, void* host, int64_t nbytes, int64_t* size) {
     return c ? c->solve_buffer(which, host, nbytes, size) : -1;
 }

@@ -437,6 +437,38 @@ __device__ __forceinline__ double sv_block_dot(double acc, const double* __restr
     return acc;
 }
 
+// The row interchanges of one capacitance block (LAPACK pivots) as a
+// permutation, perm[i] = original row that ends in row i: one workgroup, the
+// sequential interchanges in LDS (blocks up to PERM_MAX rows), so the solve
+// needs no host round trip (and can be captured in a graph).
+constexpr int PERM_MAX = 16384;
+
+__global__ __launch_bounds__(256) void sv_perm_kernel(int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
+                                                      int* __restrict__ flags) {
+    __shared__ int p[PERM_MAX];
+    __shared__ int q[PERM_MAX];
+    bool bad = false;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        p[i] = i;
+        const int64_t v = ipiv[i] - 1;
+        bad |= v < i || v >= n;   // partial pivoting picks a row at or below the diagonal
+        q[i] = (int)v;
+    }
+    __syncthreads();
+    if (bad) atomicOr(flags, XK_FLAG_SINGULAR);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < n; ++k) {
+            const int r = q[k];
+            if (r < k || r >= n) continue;
+            const int a = p[k];
+            p[k] = p[r];
+            p[r] = a;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = p[i];
+}
+
 // y[i] = t[perm[i]]
 __global__ void sv_permute_kernel(int n, const int* __restrict__ perm, const double* __restrict__ t,
                                   double* __restrict__ y) {
@@ -883,8 +915,20 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
         hipLaunchKernelGGL(sv_info_kernel, dim3(sv_grid((int64_t)sv_comp_n.size())), dim3(256), 0, st,
                            (int)sv_comp_n.size(), (const int64_t*)sv[SV_INFO].p, (int*)d_flags.p);
         SV_TRY(hipGetLastError());
-        // the row interchanges of every component as one permutation (host)
+        // the row interchanges of every component as one permutation: on the
+        // device when every block fits the permutation kernel, else on the host
+        bool on_device = true;
+        for (int nc : sv_comp_n) on_device &= 3 * nc <= PERM_MAX;
+        if (on_device) {
+            for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+                hipLaunchKernelGGL(sv_perm_kernel, dim3(1), dim3(256), 0, st, 3 * sv_comp_n[c],
+                                   (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
+                                   (int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (int*)d_flags.p);
+                SV_TRY(hipGetLastError());
+            }
+        }
         const size_t np3 = 3 * (size_t)ns;
+        if (!on_device) {
         piv_h.resize(np3);
         perm_h.resize(np3);
         SV_TRY(hipMemcpyAsync(piv_h.data(), sv[SV_IPIV].p, sizeof(int64_t) * np3, hipMemcpyDeviceToHost, st));
@@ -903,6 +947,7 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
             }
         }
         SV_TRY(hipMemcpyAsync(sv[SV_PERM].p, perm_h.data(), sizeof(int) * np3, hipMemcpyHostToDevice, st));
+        }
     }
     double* b = (double*)sv[SV_B].p;
     double* r = (double*)sv[SV_R].p;

@@ -132,6 +132,10 @@ struct Ctx {
     int linear_solve(const double* u, const uint8_t* fixed, double mass_coef, const double* M, const double* R,
                      double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st);
     void release_solver();
+    DBuf nt_try;   // trial displacement of the device line search
+    int newton_update(double* u, double* du, const double* R, const double* f_contact, const double* f_ext,
+                      const uint8_t* fixed, const double* res_norm2, int max_steps, double diverge_factor,
+                      double cap, double* status, hipStream_t st);
     int solve_buffer(int which, void* host, int64_t nbytes, int64_t* size);
     int build_pattern_gpu(hipStream_t st);   // direct path: the pattern built on the device
     int host_pattern();                      // gptr_h / gcol_h of the current pattern
@@ -150,6 +154,7 @@ struct Ctx {
         d_nflag.release();
         d_cstart.release();
         d_node_chain.release();
+        nt_try.release();
         release_solver();
         d_bbeam.release();
         pat_tmp.release();

@@ -405,7 +405,8 @@ class DeviceBackend:
     the host for the SuperLU solve."""
 
     def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
-                 k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False):
+                 k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False,
+                 use_graph: bool = False):
         from .assembly import StrandAssembly
         from .time_integration import GeneralizedAlphaProcess
 
@@ -415,6 +416,10 @@ class DeviceBackend:
         self.solve_residuals = []   # device solve: relative residual of every linear solve
         self.solve_tol = 1e-11      # device solve: target relative residual (SuperLU reaches about 1e-12)
         self.max_refine = 16
+        self.use_graph = bool(device_solve and use_graph)   # attempts after the first as HIP-graph replays
+        self.graph_refine = 3       # fixed refinement steps inside the captured solve
+        self.graph_attempts_captured = 0
+        self.graph_error = None
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
         self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
                                   fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0)
@@ -453,13 +458,16 @@ class DeviceBackend:
         self.ga.correct(u, None, dt)
 
     def newton(self, u, u_ref, f_ext, fixed_dofs, dt, f_ext_ref_norm, dynamic_ref, cfg) -> NewtonOutcome:
-        """``newton_dynamic`` with the vectors in HBM and the linear solve on
-        the device (LinearSolveProcess -> xk_linear_solve, refined to a
-        relative residual of ``solve_tol``); the same control flow, thresholds
-        and order of updates.  The
-        host reads ||R||^2 and n_active per attempt, one norm per line-search
-        step and (||du||, ||u||, |du.R|, status) per update; ``u`` (host) is
-        updated in place at the end."""
+        """``newton_dynamic`` with the vectors in HBM: the linear solve
+        (LinearSolveProcess -> xk_linear_solve) and the line search + update
+        (LineSearchUpdateProcess -> xk_newton_update) on the device, with the
+        same control flow, thresholds and order of updates.  The host reads
+        (||R||^2, n_active) after the assembly and (alpha, ||du||, ||u||,
+        |du.R|) after the update of every attempt.  With ``use_graph`` the
+        attempts after the first replay two captured HIP graphs: G1 =
+        assembly + pair force, G2 = linear solve (a fixed number of
+        refinement steps) + line search + update.  ``u`` (host) is updated in
+        place at the end."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
         if dt != self._dt:
             asm.set_dt(dt)
@@ -468,24 +476,33 @@ class DeviceBackend:
         asm.set_u_ref(u_ref)
         asm.set_f_ext(f_ext)
         ud = asm.u
-        mask = asm.fixed_mask.bool()
         du = torch.empty_like(ud)
-        rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
-        f_try = torch.empty_like(ud)
+        status = torch.zeros(24, dtype=torch.float64, device=ctx.dev)
+        ls_steps = cfg.line_search_max_steps if cfg.use_line_search else 0
+        graphs = None
+        graph_ok = self.use_graph and asm.prescreen is None
         energy_ref = None
         incr_f_ref = 0.0
         consecutive_increase = 0
         prev_ratio = float("inf")
         out = NewtonOutcome(False, 0, 0, False, None)
+
+        def solve_and_update(rel, refine, tol):
+            ctx.linear_solve(ud, asm.fixed_mask, asm.R, du, asm._dyn, max_refine=refine, tol=tol, res_rel=rel)
+            ctx.newton_update(ud, du, asm.R, self._f, asm.f_ext, asm.fixed_mask, asm.res2, status,
+                              line_search_steps=ls_steps, diverge_factor=3.0, du_norm_cap=cfg.du_norm_cap)
+
         for att in range(cfg.max_nr_attempts):
             out.n_attempts += 1
-            R, K, res2, n_act = asm.assemble()
-            ctx.pairs_gather(f=self._f)
-            f_c = -self._f
-            out.f_c = f_c
+            if graphs is None:
+                asm.assemble()
+                ctx.pairs_gather(f=self._f)
+            else:
+                graphs[0].replay()
+            out.f_c = self._f
             # ConvergenceCheckProcess, force part (_newton_steps.py:194-213)
-            res = math.sqrt(float(res2.item()))
-            n_active = int(n_act.item())
+            res = math.sqrt(float(asm.res2.item()))
+            n_active = int(asm.n_active.item())
             f_ref = incr_f_ref if incr_f_ref > 1e-30 else f_ext_ref_norm
             if dynamic_ref and att == 0 and res > 1e-30:
                 f_ref = res
@@ -507,35 +524,18 @@ class DeviceBackend:
             if (consecutive_increase >= cfg.divergence_window and not ratio < 0.01) or (att >= 10 and ratio > 100.0):
                 out.diverged = True
                 break
-            ctx.linear_solve(ud, asm.fixed_mask, R, du, asm._dyn, max_refine=self.max_refine,
-                             tol=self.solve_tol, res_rel=rel)
+            rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+            if graphs is None:
+                solve_and_update(rel, self.max_refine, self.solve_tol)
+            else:
+                rel = graphs[2]
+                graphs[1].replay()
+            self.solve_residuals.append(rel.clone() if graphs is not None else rel)
             if ctx.status_flags() & 4:
                 break   # zero pivot: the solve failed (LinearSolveProcess returns success=False)
-            self.solve_residuals.append(rel)
-            rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
-            scale = 1.0
-            if cfg.use_line_search:
-                # _ncp_line_search (contact/solver/_utils.py:31-61), f_c frozen
-                alpha = 1.0
-                thr = 3.0 * max(res, 1e-30)
-                for _ in range(cfg.line_search_max_steps):
-                    ctx.beam(ud + alpha * du, f_int=f_try)
-                    r_try = f_try + f_c - asm.f_ext
-                    r_try[mask] = 0.0
-                    if float(torch.linalg.vector_norm(r_try).item()) < thr:
-                        break
-                    alpha *= 0.5
-                scale *= alpha
-            if cfg.du_norm_cap > 0.0:
-                dn = float(torch.linalg.vector_norm(scale * du).item())
-                un = max(float(torch.linalg.vector_norm(ud).item()), 1.0)
-                if dn > cfg.du_norm_cap * un:
-                    scale *= cfg.du_norm_cap * un / dn
-            du.mul_(scale)
-            ud.add_(du)
+            st = status[:4].cpu().numpy()
+            du_norm, u_norm, energy = float(st[1]), float(st[2]), float(st[3])
             # ConvergenceCheckProcess, displacement / energy part (_newton_steps.py:215-243)
-            s = torch.stack([torch.linalg.vector_norm(du), torch.linalg.vector_norm(ud), torch.dot(du, R)]).cpu()
-            du_norm, u_norm, energy = float(s[0]), float(s[1]), abs(float(s[2]))
             if u_norm > 1e-30 and du_norm / u_norm < cfg.tol_disp:
                 out.converged = True
                 break
@@ -544,8 +544,35 @@ class DeviceBackend:
             if energy_ref > 1e-30 and energy / energy_ref < 1e-10:
                 out.converged = True
                 break
+            if graph_ok and graphs is None:
+                graphs = self._capture_attempt(solve_and_update)
+                if graphs is None:
+                    graph_ok = False
+                else:
+                    self.graph_attempts_captured += 1
         u[:] = ud.cpu().numpy()
         return out
+
+    def _capture_attempt(self, solve_and_update):
+        """G1 (assembly + pair force) and G2 (solve with graph_refine fixed
+        refinement steps + line search + update), recorded without being run
+        (the eager first attempt has set the solve up).  None if capture is
+        not possible; the reason is kept in ``graph_error``."""
+        torch, asm, ctx = self.torch, self.asm, self.ctx
+        rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+        try:
+            g1 = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g1):
+                asm.assemble()
+                ctx.pairs_gather(f=self._f)
+            g2 = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g2):
+                solve_and_update(rel, self.graph_refine, 0.0)
+        except Exception as e:  # noqa: BLE001 - any capture failure falls back to eager attempts
+            self.graph_error = repr(e)
+            torch.cuda.synchronize()
+            return None
+        return g1, g2, rel
 
     def checkpoint(self):
         self.ga.checkpoint()

@@ -31,8 +31,11 @@ W = importlib.import_module(__package__ + ".workloads")
 TRAIN_RUNS = [(f, nw) for f in (60.0, 80.0, 100.0, 120.0, 140.0) for nw in (7, 19)]
 
 
-def run_case(case, *, precision: str = "fp64", net=None, log: bool = False):
-    """Solve one squeeze case on the device; returns (SolverResult, backend)."""
+def run_case(case, *, precision: str = "fp64", net=None, log: bool = False, device_solve: bool = False,
+             use_graph: bool = False):
+    """Solve one squeeze case on the device; returns (SolverResult, backend).
+    device_solve / use_graph: the Newton attempts on the device (linear solve,
+    line search, update), after each increment's first as HIP-graph replays."""
     r = float(case["radii"][0])
     sec = C.circle_section(r, W.E_STEEL, W.NU_STEEL)
     ctx = C.DeviceContext(case["X"], case["conn"], sec, radii=case["radii"], precision=precision)
@@ -41,7 +44,8 @@ def run_case(case, *, precision: str = "fp64", net=None, log: bool = False):
     kcfg = K.ContactConfig(mu=case["mu"], k_t_ratio=case["ktr"], exclude_same_layer=True,
                            elem_layer_map=W.layer_map(case["layer"]))
     b = S.DeviceBackend(ctx, M_diag=Md, rho_inf=case["rho_inf"], k_pen=case["k_pen"], smoothing_delta=case["sd"],
-                        mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg)
+                        mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg,
+                        device_solve=device_solve, use_graph=use_graph)
     if net is not None:
         b.use_prescreen(net, elem_layer=case["layer"])
     if log:
