@@ -518,13 +518,40 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
     const int e_lane = blockIdx.x * DIRECT_OWN + l;
     const bool live = e_lane < n_elems;
     const int e = live ? e_lane : n_elems - 1;
-    double fe[12];
-    M3<TT> Xb[4], Yb[4], Zb[4];
-    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, K != nullptr, flags);
     const int fl = eflag[e];
     const bool prv = live && (fl & 1), nxt = (fl & 2) != 0;
     const bool own = live && l < DIRECT_OWN;
     const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
+    // Row offsets, block slots and mass terms of the rows this lane writes,
+    // loaded before the element math so that their latency hides behind it
+    // (loaded at the write-out they cost one dependent round trip per row).
+    // Index 0/1 of gq/bq/mq: node 2's rows, then the chain-start rows of node 1.
+    int gq[2][3] = {}, bq[2][2] = {};
+    double mq[2][6] = {};
+    const bool first = own && !prv;
+    if (K) {   // uniform
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gq[0][k] = gptr[2 * n2 + k];
+        bq[0][0] = eslot[16 * (size_t)e + 8];
+        bq[0][1] = eslot[16 * (size_t)e + 12];
+        if (mass_coef != 0.0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) mq[0][k] = M[6 * (size_t)n2 + k];
+        }
+        if (first) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) gq[1][k] = gptr[2 * n1 + k];
+            bq[1][0] = eslot[16 * (size_t)e];
+            bq[1][1] = eslot[16 * (size_t)e + 4];
+            if (mass_coef != 0.0) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) mq[1][k] = M[6 * (size_t)n1 + k];
+            }
+        }
+    }
+    double fe[12];
+    M3<TT> Xb[4], Yb[4], Zb[4];
+    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, K != nullptr, flags);
     // node-1 force part for the previous lane
     __shared__ double fsh[BEAM_BLOCK * 6];
 #pragma unroll
@@ -546,13 +573,12 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
     for (int p = 0; p < 2; ++p) {
         // own node-2 rows: columns [node 1 | node 2 | next element's node 2]
         {
-            const int gr = 2 * n2 + p;
-            const int r0 = gptr[gr], deg = gptr[gr + 1] - r0;
-            const int b0 = eslot[16 * (size_t)e + 4 * (2 + p)];
+            const int r0 = gq[0][p], deg = gq[0][p + 1] - r0;
+            const int b0 = bq[0][p];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 TT* row = img + (l * 3 + i) * RUN;
-                const TT m = mass_coef != 0.0 ? TT(mass_coef * M[3 * (size_t)gr + i]) : TT(0);
+                const TT m = mass_coef != 0.0 ? TT(mass_coef * mq[0][3 * p + i]) : TT(0);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -579,33 +605,50 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
             }
         }
         __syncthreads();
-        // entry t = l + 64 it of the owned image: row rr = t / RUN, column k = t % RUN
+        // entry t = l + 64 it of the image: row rr = t / RUN, column k = t % RUN.
+        // The image holds BEAM_BLOCK * 3 rows, so every t < BEAM_BLOCK * 3 * RUN
+        // reads inside it; rows of the last lane (not owned) carry offset -1.
+        // Each group of U entries reads its offsets and values first and stores
+        // after, so the LDS reads of the group overlap instead of each store
+        // waiting on its own chain of reads.
         {
-            constexpr int TOTAL = DIRECT_OWN * 3 * RUN;
+            constexpr int NIT = 3 * RUN;   // BEAM_BLOCK * 3 * RUN / BEAM_BLOCK
+            constexpr int U = 6;
+            static_assert(NIT % U == 0, "store loop unroll");
             constexpr int STEP_R = BEAM_BLOCK / RUN, STEP_K = BEAM_BLOCK % RUN;
-            int rr = l / RUN, k = l % RUN;
-#pragma unroll 6
-            for (int t = l; t < TOTAL; t += BEAM_BLOCK) {
-                const int2 d = dst[rr];
-                if (d.x >= 0 && k < d.y) K[(size_t)d.x + k] = img[t];
-                rr += STEP_R;
-                k += STEP_K;
-                if (k >= RUN) { k -= RUN; ++rr; }
+            int rr = l / RUN, k = l % RUN, t = l;
+#pragma unroll 1
+            for (int it0 = 0; it0 < NIT; it0 += U) {
+                int2 dd[U];
+                TT vv[U];
+                int kk[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    dd[j] = dst[rr];
+                    vv[j] = img[t];
+                    kk[j] = k;
+                    t += BEAM_BLOCK;
+                    rr += STEP_R;
+                    k += STEP_K;
+                    if (k >= RUN) { k -= RUN; ++rr; }
+                }
+#pragma unroll
+                for (int j = 0; j < U; ++j)
+                    if ((dd[j].x >= 0) & (kk[j] < dd[j].y)) K[(size_t)dd[j].x + kk[j]] = vv[j];
             }
         }
         __syncthreads();
     }
     // first node of a chain: its rows hold this element's node-1 blocks only
-    if (own && !prv) {
+    if (first) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const int gr = 2 * n1 + p;
-            const int r0 = gptr[gr], deg = gptr[gr + 1] - r0;
-            const int b0 = eslot[16 * (size_t)e + 4 * p];
+            const int r0 = gq[1][p], deg = gq[1][p + 1] - r0;
+            const int b0 = bq[1][p];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const size_t d = 9 * (size_t)r0 + (size_t)i * 3 * deg + 3 * (size_t)(b0 - r0);
-                const TT m = mass_coef != 0.0 ? TT(mass_coef * M[3 * (size_t)gr + i]) : TT(0);
+                const TT m = mass_coef != 0.0 ? TT(mass_coef * mq[1][3 * p + i]) : TT(0);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
