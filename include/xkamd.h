@@ -51,7 +51,9 @@ enum { XK_OPT_HOST_PATTERN = 512 };
 enum {
     XK_FLAG_NONFINITE = 1,     /* a non-finite residual entry was produced */
     XK_FLAG_DEGENERATE = 2,    /* an element of (near) zero length was met */
-    XK_FLAG_SINGULAR = 4       /* the device linear solve met a zero pivot */
+    XK_FLAG_SINGULAR = 4,      /* the device linear solve met a zero pivot */
+    XK_FLAG_COAT_UNPLACED = 8  /* a pair with a coating contribution held no slot of the pattern
+                                  (its terms are missing: probe, rebuild, assemble again) */
 };
 
 /* pair-kernel modes (bitwise OR) */
@@ -284,6 +286,43 @@ int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, 
  * reference, whose solve has one coefficient: it lets independent solves with
  * their own coefficients (mu_eff = 1) share one context and one launch. */
 int xk_ctx_set_friction(xk_ctx* ctx, const double* mu_elem);
+
+/* Kelvin-Voigt coating pair terms.  Replaces _create_coating_strategy and
+ * KelvinVoigtCoatingProcess (xkep_cae/contact/coating/strategy.py:80-400) with
+ * the coating branches of UpdateGeometryProcess and the active-set update
+ * (xkep_cae/contact/_manager_process.py:351-360, :486-494): on when
+ * stiffness > 0 (the config's coating_stiffness / coating_damping /
+ * coating_mu / coating_k_t_ratio, xkep_cae/contact/_contact_pair.py:164-167).
+ * core_radii: per element, host (NULL: the contact radii, which is what the
+ * reference's solver passes, contact/solver/process.py:229); pairs created
+ * later and the existing ones take them.  Every pair-kernel call then uses
+ * the core gap, writes the compression, and the fused assembly adds the
+ * coating force, coating friction and their stiffness.  Synchronous. */
+int xk_ctx_set_coating(xk_ctx* ctx, double stiffness, double damping, double mu, double k_t_ratio,
+                       const double* core_radii);
+/* The dashpot's time step of the following assemblies,
+ * max(load_frac - load_frac_prev, 1e-15) (contact/solver/_newton_steps.py:112, :297). */
+int xk_coating_set_dt(xk_ctx* ctx, double coat_dt);
+/* Compression of every pair at x = X + u, nothing else written: pairs with a
+ * compression > 0 are marked for the next pattern build; n_unplaced = how
+ * many of them hold no slot of the current pattern (then the pattern is
+ * rebuilt by the next assembly call).  A frozen-INACTIVE pair can start
+ * pressing its coating within an increment, and the reference applies the
+ * coating force to it regardless of status; call this before an assembly
+ * whenever the coating is on.  No counterpart in the reference (whose
+ * assembly has no fixed pattern).  Synchronous. */
+int xk_coating_probe(xk_ctx* ctx, const double* u, int64_t* n_unplaced, void* stream);
+/* End of a converged step: coating_compression_prev <- coating_compression
+ * (contact/solver/process.py:539-547).  Stream-ordered. */
+int xk_coating_commit(xk_ctx* ctx, void* stream);
+/* Coating state of the pair list (_contact_pair.py:57-62, core radii :79-80),
+ * host arrays of n_pairs (z_t: n_pairs x 2), each nullable.  Synchronous. */
+int xk_pairs_get_coating(xk_ctx* ctx, double* core_a, double* core_b, double* compression,
+                         double* compression_prev, double* z_t, int8_t* stick, double* q_trial_norm,
+                         double* dissipation);
+int xk_pairs_set_coating(xk_ctx* ctx, const double* compression, const double* compression_prev,
+                         const double* z_t, const int8_t* stick, const double* q_trial_norm,
+                         const double* dissipation);
 
 /* Launch shapes and the broadphase grid (no counterpart in the reference;
  * set by the autotuner): grid cell = cell_factor x mean box extent when a call

@@ -5,6 +5,7 @@ Test infrastructure only (see ``oracle/__init__.py``).
 Restates
   - ``ContactForceAssemblyProcess``  contact/solver/_newton_steps.py:79-135
   - ``TangentAssemblyProcess``       contact/solver/_newton_steps.py:289-313
+  - the Kelvin-Voigt coating terms of both (``coating.py``)
   - ``GeneralizedAlphaProcess``      time_integration/strategy.py:96-232
   - the dynamic residual/stiffness steps of one Newton attempt
     (contact/solver/_newton_dynamic.py:164-192, :294-315) and the active count
@@ -18,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 import scipy.sparse as sp
 
-from . import beam, contact
+from . import beam, coating, contact
 
 
 class GeneralizedAlpha:
@@ -115,12 +116,18 @@ class IterationResult:
 
 def assemble_iteration(prob: Problem, pairs: contact.Pairs, u, u_ref, f_ext, fixed_dofs,
                        ga: GeneralizedAlpha | None, dt: float, *, tangent: bool = True,
-                       mu_eff: float | None = None) -> IterationResult:
+                       mu_eff: float | None = None, coat_dt: float = 1e-15) -> IterationResult:
     """One Newton attempt's assembly: narrowphase with frozen active set, Huber
     force (writes p_n), Coulomb friction (writes z_t/status), residual
     R = f_int - (f_c + f_fric) - f_ext, R[fixed] = 0, Generalized-alpha
     correct + effective residual, R[fixed] = 0, ||R||_2, n_active = #(p_n > 0),
-    and (optionally) K_T = K_beam + K_c - K_fric + (1 - alpha_m) c0 M."""
+    and (optionally) K_T = K_beam + K_c - K_fric + (1 - alpha_m) c0 M.
+
+    With the coating on (cfg.coating_stiffness > 0) the Kelvin-Voigt terms
+    enter after friction, f_c = ((f_huber + f_fric) + f_coat) + f_coat_fric
+    (_newton_steps.py:110-121) and K_T = ((K_beam + K_c) + K_coat) + K_coat_fric
+    - K_fric (:296-311); coat_dt = max(load_frac - load_frac_prev, 1e-15) is
+    the dashpot's time step (:112, :297)."""
     ndof = prob.ndof
     mu_eff = prob.mu if mu_eff is None else mu_eff
     xd = deformed_coords(prob.X, u)
@@ -128,7 +135,14 @@ def assemble_iteration(prob: Problem, pairs: contact.Pairs, u, u_ref, f_ext, fix
     f_c, _, p = contact.huber_evaluate(p, prob.k_pen, prob.smoothing_delta, ndof)
     fr = contact.friction_evaluate(p, u, u_ref, prob.k_pen, prob.k_t_ratio, mu_eff, ndof)
     p = fr.pairs
-    fc_neg = -(f_c + fr.f_fric)
+    f_c = f_c + fr.f_fric
+    coat = prob.cfg.coating_stiffness > 0.0
+    if coat:
+        f_c = f_c + coating.forces(p, prob.cfg, coat_dt, ndof)
+        if prob.cfg.coating_mu > 0.0:
+            f_cf, p = coating.friction_forces(p, prob.cfg, u, u_ref, ndof)
+            f_c = f_c + f_cf
+    fc_neg = -f_c
     _, f_int = beam.assemble_cr_beam3d(prob.X, prob.conn, u, prob.sec, stiffness=False)
     R = f_int + fc_neg - f_ext
     R[fixed_dofs] = 0.0
@@ -144,6 +158,12 @@ def assemble_iteration(prob: Problem, pairs: contact.Pairs, u, u_ref, f_ext, fix
         K_beam, _ = beam.assemble_cr_beam3d(prob.X, prob.conn, u, prob.sec, internal_force=False)
         K_c = contact.huber_tangent(p, prob.k_pen, prob.smoothing_delta, ndof)
         K = K_beam + K_c
+        if coat:
+            m, b = coating.stiffness_blocks(p, prob.cfg, coat_dt)
+            K = K + contact.blocks_to_csr(p, m, b, ndof)
+            if prob.cfg.coating_mu > 0.0:
+                m, b = coating.friction_stiffness_blocks(p, prob.cfg)
+                K = K + contact.blocks_to_csr(p, m, b, ndof)
         K_fric = None
         if fr.mask.any():
             K_fric = contact.friction_tangent(p, fr, ndof)

@@ -46,6 +46,9 @@ class ContactConfig:
     exclude_same_layer: bool = False
     elem_layer_map: dict | None = None
     coating_stiffness: float = 0.0
+    coating_damping: float = 0.0
+    coating_mu: float = 0.0
+    coating_k_t_ratio: float = 0.5
 
 
 def _z(n, *shape, dtype=float):
@@ -78,6 +81,13 @@ class Pairs:
     status: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
     stick: np.ndarray = field(default_factory=lambda: np.zeros(0, bool))
     dissipation: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # Kelvin-Voigt coating state (_contact_pair.py:57-62)
+    coating_compression: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    coating_compression_prev: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    coating_z_t: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
+    coating_stick: np.ndarray = field(default_factory=lambda: np.zeros(0, bool))
+    coating_q_trial_norm: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    coating_dissipation: np.ndarray = field(default_factory=lambda: np.zeros(0))
 
     def __len__(self) -> int:
         return len(self.elem_a)
@@ -100,6 +110,8 @@ class Pairs:
             s=_z(n), t=_z(n), gap=_z(n), normal=_z(n, 3), tangent1=_z(n, 3), tangent2=_z(n, 3),
             k_pen=_z(n), k_t=_z(n), p_n=_z(n), z_t=_z(n, 2), q_trial_norm=_z(n),
             status=np.zeros(n, np.int64), stick=np.ones(n, bool), dissipation=_z(n),
+            coating_compression=_z(n), coating_compression_prev=_z(n), coating_z_t=_z(n, 2),
+            coating_stick=np.ones(n, bool), coating_q_trial_norm=_z(n), coating_dissipation=_z(n),
         )
         return p
 
@@ -376,7 +388,12 @@ def build_contact_frame(normals, prev_t1, prev_n, has_prev, has_prev_n):
 def update_geometry(pairs: Pairs, coords, cfg: ContactConfig, *,
                     allow_deactivation: bool = True, freeze_active_set: bool = False) -> Pairs:
     """``UpdateGeometryProcess.process`` (_manager_process.py:327-403) with
-    ``_update_active_set_state`` (:475-494) when not frozen."""
+    ``_update_active_set_state`` (:475-494) when not frozen.
+
+    With the coating on (coating_stiffness > 0) the gap is the core gap
+    dist - (core_a + core_b), the coating compression is
+    max(0, (r_a - core_a) + (r_b - core_b) - gap_core) (:351-360), and a pair
+    with compression > 0 is activated and never deactivated."""
     p = pairs.copy()
     if len(p) == 0:
         return p
@@ -384,17 +401,22 @@ def update_geometry(pairs: Pairs, coords, cfg: ContactConfig, *,
     s, t, dist, nrm, _ = closest_point_segments(
         coords[p.nodes_a[:, 0]], coords[p.nodes_a[:, 1]],
         coords[p.nodes_b[:, 0]], coords[p.nodes_b[:, 1]])
-    if cfg.coating_stiffness > 0.0:
-        raise NotImplementedError("coating terms are outside this path")
-    gap = dist - (p.radius_a + p.radius_b)
+    coat = cfg.coating_stiffness > 0.0
+    if coat:
+        gap = dist - (p.core_radius_a + p.core_radius_b)
+        coat_total = (p.radius_a - p.core_radius_a) + (p.radius_b - p.core_radius_b)
+        p.coating_compression = np.maximum(0.0, coat_total - gap)
+    else:
+        gap = dist - (p.radius_a + p.radius_b)
     has_prev = np.sqrt(np.einsum("ij,ij->i", p.tangent1, p.tangent1)) > 1e-10
     has_prev_n = np.sqrt(np.einsum("ij,ij->i", p.normal, p.normal)) > 1e-10
     n, t1, t2 = build_contact_frame(nrm, p.tangent1, p.normal, has_prev, has_prev_n)
     p.s, p.t, p.gap, p.normal, p.tangent1, p.tangent2 = s, t, gap, n, t1, t2
     if not freeze_active_set:
         inact = p.status == INACTIVE
-        on = inact & (gap <= cfg.g_on)
-        off = (~inact) & allow_deactivation & (gap >= cfg.g_off)
+        coat_active = (p.coating_compression > 0.0) if coat else np.zeros(len(p), bool)
+        on = inact & ((gap <= cfg.g_on) | coat_active)
+        off = (~inact) & allow_deactivation & (gap >= cfg.g_off) & ~coat_active
         p.status = np.where(on, ACTIVE, np.where(off, INACTIVE, p.status))
     return p
 

@@ -8,6 +8,7 @@ import numpy as np
 
 from oracle import assembly as oasm
 from oracle import beam as obeam
+from oracle import coating as ocoat
 from oracle import contact as ocon
 
 
@@ -20,6 +21,14 @@ class OracleBackend:
         self.ga = oasm.GeneralizedAlpha(M_diag, rho_inf)
         self.pairs = ocon.Pairs()
         self.fixed = None
+        self.coating = cfg.coating_stiffness > 0.0
+        self.coat_dt = 1e-15
+
+    def set_coat_dt(self, coat_dt):
+        self.coat_dt = coat_dt
+
+    def commit_coating(self):
+        self.pairs = ocoat.commit(self.pairs)
 
     def _x(self, u):
         return self.X + u.reshape(-1, 6)[:, :3]
@@ -40,7 +49,8 @@ class OracleBackend:
         return self.ga.predict(u, dt)
 
     def assemble(self, u, u_ref, f_ext, dt):
-        r = oasm.assemble_iteration(self.prob, self.pairs, u, u_ref, f_ext, self.fixed, self.ga, dt)
+        r = oasm.assemble_iteration(self.prob, self.pairs, u, u_ref, f_ext, self.fixed, self.ga, dt,
+                                    coat_dt=self.coat_dt)
         self.pairs = r.pairs
         return r.R, r.K, r.n_active, r.f_c
 

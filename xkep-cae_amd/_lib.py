@@ -24,6 +24,7 @@ PAIR_GEOM, PAIR_HYST, PAIR_HUBER, PAIR_FRIC, PAIR_TAN_C, PAIR_TAN_F = 1, 2, 4, 8
 PREC = {"fp64": 0, "mixed": 1}
 OPT_STAGED = 256
 OPT_HOST_PATTERN = 512
+FLAG_COAT_UNPLACED = 8
 
 
 class XkTuning(C.Structure):
@@ -73,6 +74,12 @@ SIGNATURES = {
     "xk_lp_select": (C.c_int, [vp, c_int64_p, vp]),
     "xk_lp_judge": (C.c_int, [vp, vp, vp, C.c_double, c_int64_p, vp]),
     "xk_lp_off": (C.c_int, [vp]),
+    "xk_ctx_set_coating": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp]),
+    "xk_coating_set_dt": (C.c_int, [vp, C.c_double]),
+    "xk_coating_probe": (C.c_int, [vp, vp, c_int64_p, vp]),
+    "xk_coating_commit": (C.c_int, [vp, vp]),
+    "xk_pairs_get_coating": (C.c_int, [vp] + [vp] * 8),
+    "xk_pairs_set_coating": (C.c_int, [vp] + [vp] * 6),
     "xk_pairs_snapshot": (C.c_int, [vp]),
     "xk_pairs_restore": (C.c_int, [vp]),
     "xk_mlp_n_params": (C.c_int, []),

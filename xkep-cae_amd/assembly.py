@@ -13,6 +13,12 @@ synchronisation, capturable in a HIP graph.
 ``begin_increment`` runs the once-per-increment part (contact/solver/process.py:406-429):
 deformed coordinates, DetectCandidates on the GPU, UpdateGeometry with
 hysteresis, and the tangent pattern of the increment.
+
+With ``config.coating_stiffness > 0`` the Kelvin-Voigt coating terms
+(contact/coating/strategy.py:80-384) join the pair kernel; each assembly then
+first probes the coating compression at the current u, and a pair that
+starts pressing its coating within the increment without a slot in the
+tangent pattern makes the assembly rebuild the pattern before it runs.
 """
 
 from __future__ import annotations
@@ -28,7 +34,7 @@ from .time_integration import GeneralizedAlphaProcess
 class StrandAssembly:
     def __init__(self, ctx: DeviceContext, *, k_pen: float, smoothing_delta: float, k_t_ratio: float = 0.5,
                  mu: float = 0.3, f_ext=None, fixed_dofs=None, config: ContactConfig | None = None,
-                 time: GeneralizedAlphaProcess | None = None, dt: float = 0.0):
+                 time: GeneralizedAlphaProcess | None = None, dt: float = 0.0, core_radii=None):
         torch = ctx.torch
         self.ctx = ctx
         self.torch = torch
@@ -38,6 +44,11 @@ class StrandAssembly:
         self.mu = float(mu)
         self.cfg = config or ContactConfig(mu=mu, k_t_ratio=k_t_ratio)
         self._layer = layer_array(self.cfg, ctx.n_elems)
+        self.coating = self.cfg.coating_stiffness > 0.0
+        self.n_coat_rebuilds = 0
+        if self.coating:
+            ctx.set_coating(self.cfg.coating_stiffness, self.cfg.coating_damping, self.cfg.coating_mu,
+                            self.cfg.coating_k_t_ratio, core_radii)
         n = ctx.ndof
         dev = ctx.dev
         self.f_ext = ctx.to_dev(np.zeros(n) if f_ext is None else f_ext)
@@ -87,6 +98,16 @@ class StrandAssembly:
 
     def set_f_ext(self, f_ext):
         self.f_ext.copy_(self.ctx.to_dev(f_ext))
+
+    def set_coat_dt(self, coat_dt: float):
+        """Time step of the coating dashpot: max(load_frac - load_frac_prev, 1e-15)
+        (contact/solver/_newton_steps.py:112)."""
+        self.ctx.coating_set_dt(coat_dt)
+
+    def commit_coating(self):
+        """End of a converged step: compression_prev <- compression."""
+        if self.coating:
+            self.ctx.coating_commit()
 
     def set_dt(self, dt: float):
         """Time step of the inertia terms (a captured graph is dropped)."""
@@ -146,15 +167,22 @@ class StrandAssembly:
         snap = self.ctx.get_pairs()
         if len(snap["elem_a"]) == 0:
             return 0
+        coat = self.ctx.get_coating() if self.coating else None
         self.ctx.pairs_run(L.PAIR_GEOM, coords=self.deformed_coords())
-        gap = self.ctx.get_pairs()["gap"]
+        gap = self.ctx.get_pairs()["gap"]   # the core gap with the coating on (_initial_penetration.py:134-137)
         self.ctx.set_pairs(snap["elem_a"], snap["elem_b"], state=snap)
+        if coat is not None:
+            self.ctx.set_coating_state(coat)
         return int(np.sum(gap < 0.0))
 
     def assemble(self, u=None, *, tangent: bool = True):
         """Launch one fused assembly; returns device tensors (R, K, ||R||^2, n_active)."""
         if u is not None:
             self.set_u(u)
+        if self.coating and self.ctx.coating_probe(self.u) > 0:
+            self.K = None   # a pair pressing its coating has no slot yet: new pattern
+            self.graph = None
+            self.n_coat_rebuilds += 1
         if self.K is None:
             self.K = self.ctx.new_K()
         self.ctx.assemble_iteration(self.u, self.u_ref, self.f_ext, self.fixed_mask, self.R,
@@ -166,6 +194,8 @@ class StrandAssembly:
     # -------------------------------------------------------------- HIP graph
     def capture(self, *, tangent: bool = True):
         """Capture one assembly (reading ``self.u``) into a HIP graph."""
+        if self.coating:
+            raise RuntimeError("an assembly with the coating on probes the pattern on the host: not capturable")
         torch = self.torch
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())

@@ -51,6 +51,9 @@ class ContactConfig:
     midpoint_prescreening: bool = True
     prescreening_margin: float = 0.0
     coating_stiffness: float = 0.0
+    coating_damping: float = 0.0
+    coating_mu: float = 0.0
+    coating_k_t_ratio: float = 0.5
     smoothing_delta: float = 0.0
 
 

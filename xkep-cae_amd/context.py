@@ -208,6 +208,65 @@ class DeviceContext:
         out["tangent2"] = np.cross(out["normal"], out["tangent1"])
         return out
 
+    # ------------------------------------------------------------ coating
+    def set_coating(self, stiffness: float, damping: float = 0.0, mu: float = 0.0, k_t_ratio: float = 0.5,
+                    core_radii=None):
+        """Kelvin-Voigt coating terms on (stiffness > 0) or off; core radii per
+        element (None: the contact radii)."""
+        cr = None
+        if core_radii is not None:
+            cr = np.ascontiguousarray(np.broadcast_to(np.asarray(core_radii, float), (self.n_elems,)))
+        L.check(L.lib().xk_ctx_set_coating(self.h, float(stiffness), float(damping), float(mu), float(k_t_ratio),
+                                           None if cr is None else cr.ctypes.data), "xk_ctx_set_coating")
+        self.invalidate_pattern()
+
+    def coating_set_dt(self, coat_dt: float):
+        L.check(L.lib().xk_coating_set_dt(self.h, float(coat_dt)), "xk_coating_set_dt")
+
+    def coating_probe(self, u) -> int:
+        """#(pairs pressing their coating at X + u without a slot of the pattern); > 0
+        means the next assembly rebuilds the pattern first."""
+        n = C.c_int64()
+        L.check(L.lib().xk_coating_probe(self.h, L.ptr(u), C.byref(n), self.stream), "xk_coating_probe")
+        if n.value > 0:
+            self.invalidate_pattern()   # the host copy of the pattern is stale from here
+        return n.value
+
+    def coating_commit(self):
+        L.check(L.lib().xk_coating_commit(self.h, self.stream), "xk_coating_commit")
+
+    def get_coating(self) -> dict:
+        n = self.n_pairs
+        out = {k: np.empty(n) for k in ("core_radius_a", "core_radius_b", "coating_compression",
+                                         "coating_compression_prev", "coating_q_trial_norm",
+                                         "coating_dissipation")}
+        out["coating_z_t"] = np.empty((n, 2))
+        out["coating_stick"] = np.empty(n, np.int8)
+        if n:
+            d = lambda k: out[k].ctypes.data
+            L.check(L.lib().xk_pairs_get_coating(
+                self.h, d("core_radius_a"), d("core_radius_b"), d("coating_compression"),
+                d("coating_compression_prev"), d("coating_z_t"), d("coating_stick"), d("coating_q_trial_norm"),
+                d("coating_dissipation")), "xk_pairs_get_coating")
+        return out
+
+    def set_coating_state(self, state: dict):
+        keep = []
+
+        def p(name, dtype):
+            a = state.get(name)
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data
+
+        L.check(L.lib().xk_pairs_set_coating(
+            self.h, p("coating_compression", np.float64), p("coating_compression_prev", np.float64),
+            p("coating_z_t", np.float64), p("coating_stick", np.int8), p("coating_q_trial_norm", np.float64),
+            p("coating_dissipation", np.float64)), "xk_pairs_set_coating")
+        self.invalidate_pattern()
+
     # ------------------------------------------------------------ kernels
     def _check_vec(self, a, n, name):
         if a is not None and a.numel() != n:

@@ -369,8 +369,8 @@ __global__ void newflag_kernel(int64_t nc, const int* __restrict__ I, const int*
 
 __global__ void append_kernel(int64_t nc, const int64_t* __restrict__ isnew, const int64_t* __restrict__ pos,
                               const int* __restrict__ I, const int* __restrict__ J, const int* __restrict__ conn,
-                              const double* __restrict__ rad, PairSoA P, int* __restrict__ ea, int* __restrict__ eb,
-                              int64_t base, int64_t* __restrict__ newkeys) {
+                              const double* __restrict__ rad, const double* __restrict__ crad, PairSoA P,
+                              int* __restrict__ ea, int* __restrict__ eb, int64_t base, int64_t* __restrict__ newkeys) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nc || !isnew[q]) return;
     const int64_t p = base + pos[q];
@@ -388,6 +388,13 @@ __global__ void append_kernel(int64_t nc, const int64_t* __restrict__ isnew, con
     P.status[p] = ST_INACTIVE;
     P.stick[p] = 1;
     P.sinc[p] = -1;
+    // coating: core radii of the elements, fresh state (_contact_pair.py:57-62)
+    P.cra[p] = crad[i]; P.crb[p] = crad[j];
+    P.cc[p] = 0.0; P.ccp[p] = 0.0;
+    P.cz[p] = 0.0; P.cz[cap + p] = 0.0;
+    P.cstick[p] = 1;
+    P.cqtn[p] = 0.0; P.cdiss[p] = 0.0;
+    P.cneed[p] = 0;
     newkeys[pos[q]] = pkey(i, j);
 }
 
@@ -666,7 +673,8 @@ int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double 
         if (n_new > 0) {
             if (c->grow_pairs(np + n_new) != 0) return -1;
             hipLaunchKernelGGL(append_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, isnew, pos, I2, J2,
-                               (const int*)c->d_conn.p, (const double*)c->d_rad.p, c->P, c->ea, c->eb, np, newkeys);
+                               (const int*)c->d_conn.p, (const double*)c->d_rad.p, (const double*)c->d_crad.p, c->P,
+                               c->ea, c->eb, np, newkeys);
             BP_TRY(hipGetLastError());
             // merged sorted keys
             DBuf merged;

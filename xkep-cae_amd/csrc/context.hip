@@ -68,12 +68,22 @@ static void bucket(int n_keys, const std::vector<int>& keys, const std::vector<i
 }
 
 // bytes per pair of each pair-state array, and its component count
-static const size_t PB_SZ[PB_COUNT] = {4, 4, 4, 4, 8, 8, 8, 8, 8, 24, 24, 8, 8, 8, 8, 8, 16, 1, 1, 4, 4, 4};
-static int pb_comps(int k) { return (k == PB_N || k == PB_T1) ? 3 : (k == PB_Z ? 2 : 1); }
+static const size_t PB_SZ[PB_COUNT] = {4, 4, 4, 4, 8, 8, 8, 8, 8, 24, 24, 8, 8, 8, 8, 8, 16, 1, 1, 4, 4, 4,
+                                       8, 8, 8, 8, 16, 1, 8, 8, 1};
+static int pb_comps(int k) { return (k == PB_N || k == PB_T1) ? 3 : ((k == PB_Z || k == PB_CZ) ? 2 : 1); }
 // arrays the pair kernel writes (the others are the pair's identity)
 static bool pb_written(int k) {
     return !(k == PB_NA0 || k == PB_NA1 || k == PB_NB0 || k == PB_NB1 || k == PB_RA || k == PB_RB ||
-             k == PB_SINC || k == PB_EA || k == PB_EB);
+             k == PB_SINC || k == PB_EA || k == PB_EB || k == PB_CRA || k == PB_CRB);
+}
+
+static void bind_coating(PairSoA& P, std::vector<DBuf>& b) {
+    P.cra = (double*)b[PB_CRA].p; P.crb = (double*)b[PB_CRB].p;
+    P.cc = (double*)b[PB_CC].p; P.ccp = (double*)b[PB_CCP].p;
+    P.cz = (double*)b[PB_CZ].p;
+    P.cstick = (int8_t*)b[PB_CSTICK].p;
+    P.cqtn = (double*)b[PB_CQTN].p; P.cdiss = (double*)b[PB_CDISS].p;
+    P.cneed = (int8_t*)b[PB_CNEED].p;
 }
 
 static PairSoA soa_of(std::vector<DBuf>& b, int64_t cap) {
@@ -87,6 +97,7 @@ static PairSoA soa_of(std::vector<DBuf>& b, int64_t cap) {
     P.qtn = (double*)b[PB_QTN].p; P.diss = (double*)b[PB_DISS].p; P.z = (double*)b[PB_Z].p;
     P.status = (int8_t*)b[PB_STATUS].p; P.stick = (int8_t*)b[PB_STICK].p;
     P.sinc = (int*)b[PB_SINC].p;
+    bind_coating(P, b);
     P.cap = cap;
     return P;
 }
@@ -160,6 +171,7 @@ void Ctx::bind_pairs() {
     P.qtn = (double*)pb[PB_QTN].p; P.diss = (double*)pb[PB_DISS].p; P.z = (double*)pb[PB_Z].p;
     P.status = (int8_t*)pb[PB_STATUS].p; P.stick = (int8_t*)pb[PB_STICK].p;
     P.sinc = (int*)pb[PB_SINC].p;
+    bind_coating(P, pb);
     P.cap = cap;
     ea = (int*)pb[PB_EA].p; eb = (int*)pb[PB_EB].p;
 }
@@ -179,23 +191,29 @@ int Ctx::build_pattern() {
         return 0;
     }
     const int64_t np = n_pairs;
-    std::vector<int8_t> st(np);
-    std::vector<double> pn(np);
+    std::vector<int8_t> st(np), cneed(coat_on ? np : 0);
+    std::vector<double> pn(np), cc(coat_on ? np : 0), ccp(coat_on ? np : 0);
     std::vector<int> nd[4];
     if (np) {
         XK_TRY(hipMemcpy(st.data(), P.status, np, hipMemcpyDeviceToHost));
         XK_TRY(hipMemcpy(pn.data(), P.p_n, np * 8, hipMemcpyDeviceToHost));
+        if (coat_on) {
+            XK_TRY(hipMemcpy(cc.data(), P.cc, np * 8, hipMemcpyDeviceToHost));
+            XK_TRY(hipMemcpy(ccp.data(), P.ccp, np * 8, hipMemcpyDeviceToHost));
+            XK_TRY(hipMemcpy(cneed.data(), P.cneed, np, hipMemcpyDeviceToHost));
+        }
         int* src[4] = {P.na0, P.na1, P.nb0, P.nb1};
         for (int k = 0; k < 4; ++k) {
             nd[k].resize(np);
             XK_TRY(hipMemcpy(nd[k].data(), src[k], np * 4, hipMemcpyDeviceToHost));
         }
     }
-    // contributing pairs of the increment: status != INACTIVE or p_n > 0
+    // contributing pairs of the increment: status != INACTIVE or p_n > 0; with the
+    // coating on also a compression (current, previous step or probed) > 0
     std::vector<int> sinc_of(np, -1);
     std::vector<int> sinc_list;
     for (int64_t i = 0; i < np; ++i)
-        if (st[i] != ST_INACTIVE || pn[i] > 0.0) {
+        if (st[i] != ST_INACTIVE || pn[i] > 0.0 || (coat_on && (cc[i] > 0.0 || ccp[i] > 0.0 || cneed[i]))) {
             sinc_of[i] = (int)sinc_list.size();
             sinc_list.push_back((int)i);
         }
@@ -373,6 +391,35 @@ using namespace xk;
 
 struct xk_ctx : public Ctx {};
 
+namespace xk {
+
+// core radii of the pair list from the per-element core radii
+__global__ void coat_core_kernel(int64_t np, const int* __restrict__ ea, const int* __restrict__ eb,
+                                 const double* __restrict__ crad, double* __restrict__ cra, double* __restrict__ crb) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) {
+        cra[p] = crad[ea[p]];
+        crb[p] = crad[eb[p]];
+    }
+}
+
+// end of a converged step: compression_prev <- compression
+__global__ void coat_commit_kernel(int64_t np, const double* __restrict__ cc, double* __restrict__ ccp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) ccp[p] = cc[p];
+}
+
+// #(pairs the probe marked that hold no slot of the pattern): one atomic per wave
+__global__ void coat_unplaced_kernel(int64_t np, const int8_t* __restrict__ cneed, const int* __restrict__ sinc,
+                                     int* __restrict__ count) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool hit = p < np && cneed[p] != 0 && sinc[p] < 0;
+    const unsigned long long m = __ballot(hit);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
+}
+
+}  // namespace xk
+
 extern "C" {
 
 const char* xk_last_error(void) { return g_last_error.c_str(); }
@@ -419,6 +466,9 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
     if (upload(c->d_X, c->X_h) != hipSuccess) fail("X");
     if (!rc && upload(c->d_conn, c->conn_h) != hipSuccess) fail("conn");
     if (!rc && upload(c->d_rad, c->rad_h) != hipSuccess) fail("radius");
+    // core radii default to the contact radii (no coating layer)
+    c->crad_h = c->rad_h;
+    if (!rc && upload(c->d_crad, c->crad_h) != hipSuccess) fail("core radius");
     // static element -> node lists (element order)
     if (!rc) {
         std::vector<int> k, cd, ptr, src;
@@ -571,11 +621,12 @@ int xk_pairs_set(xk_ctx* c, int64_t n, const int32_t* ea, const int32_t* eb, con
     }
     const int64_t cap = c->cap;
     std::vector<int> na0(n), na1(n), nb0(n), nb1(n);
-    std::vector<double> ra(n), rb(n);
+    std::vector<double> ra(n), rb(n), cra(n), crb(n);
     for (int64_t i = 0; i < n; ++i) {
         na0[i] = c->conn_h[2 * ea[i]]; na1[i] = c->conn_h[2 * ea[i] + 1];
         nb0[i] = c->conn_h[2 * eb[i]]; nb1[i] = c->conn_h[2 * eb[i] + 1];
         ra[i] = c->rad_h[ea[i]]; rb[i] = c->rad_h[eb[i]];
+        cra[i] = c->crad_h[ea[i]]; crb[i] = c->crad_h[eb[i]];
     }
     auto put = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
     auto put_or_zero = [&](void* dst, const void* src, size_t bytes) {
@@ -614,6 +665,17 @@ int xk_pairs_set(xk_ctx* c, int64_t n, const int32_t* ea, const int32_t* eb, con
     if (stick) XK_TRY(put(c->P.stick, stick, n));
     else XK_TRY(hipMemset(c->P.stick, 1, n));
     XK_TRY(hipMemset(c->P.sinc, 0xff, n * 4));  // -1: not contributing until the pattern is built
+    // coating: core radii of the elements, state of a new pair (xk_pairs_set_coating sets it)
+    XK_TRY(put(c->P.cra, cra.data(), n * 8));
+    XK_TRY(put(c->P.crb, crb.data(), n * 8));
+    XK_TRY(hipMemset(c->P.cc, 0, n * 8));
+    XK_TRY(hipMemset(c->P.ccp, 0, n * 8));
+    XK_TRY(hipMemset(c->P.cz, 0, n * 8));
+    XK_TRY(hipMemset(c->P.cz + cap, 0, n * 8));
+    XK_TRY(hipMemset(c->P.cstick, 1, n));
+    XK_TRY(hipMemset(c->P.cqtn, 0, n * 8));
+    XK_TRY(hipMemset(c->P.cdiss, 0, n * 8));
+    XK_TRY(hipMemset(c->P.cneed, 0, n));
     c->pattern_valid = false;
     return 0;
 }
@@ -667,6 +729,10 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     if (!c) return -1;
     if (c->lp_sync((hipStream_t)stream, false) != 0) return -1;
     const int phys = PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F;
+    if (modes & PAIR_PROBE) {
+        set_error("xk_pairs_run: the compression probe is xk_coating_probe");
+        return -1;
+    }
     if ((modes & PAIR_HYST) && (modes & phys)) {
         set_error("xk_pairs_run: the hysteresis update cannot be combined with force/tangent modes");
         return -1;
@@ -695,6 +761,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     pp.g_off = g_off;
     pp.allow_deact = allow_deactivation;
     pp.n_sinc = c->n_sinc;
+    c->coat_params(pp);
     XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_sd.p,
                         c->d_sg.p, nullptr, (hipStream_t)stream, nullptr, c->pair_block));
     if (modes & PAIR_HYST) c->pattern_valid = false;
@@ -810,6 +877,11 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.g_off = 0.0;
     pp.allow_deact = 0;
     pp.n_sinc = c->n_sinc;
+    c->coat_params(pp);
+    if (c->coat_on && c->lp_on) {
+        set_error("xk_assemble_iteration: the learned prescreen is not available with the coating on");
+        return -1;
+    }
     const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
     c->tangent_staged = K_vals != nullptr;   // the pair kernel stages every contributing pair's tangent record
     // with the prescreen on, the pair kernel visits the watch list (every
@@ -947,6 +1019,10 @@ int xk_lp_features(xk_ctx* c, const double* coords_ref, const double* u, const i
 }
 
 int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
+    if (c && c->coat_on) {
+        set_error("xk_lp_select: the learned prescreen is not available with the coating on");
+        return -1;
+    }
     if (!c) return -1;
     if (!c->lp_ready) {
         set_error("xk_lp_select: no parameters (xk_lp_set_params)");
@@ -1117,6 +1193,138 @@ int xk_ctx_get_tuning(const xk_ctx* c, xk_tuning* t) {
     t->cell_factor = c->cell_factor;
     t->pair_block = c->pair_block;
     t->gather_block = c->gather_block;
+    return 0;
+}
+
+int xk_ctx_set_coating(xk_ctx* c, double stiffness, double damping, double mu, double k_t_ratio,
+                       const double* core_radii) {
+    if (!c) return -1;
+    if (!(damping >= 0.0) || !(mu >= 0.0) || !(k_t_ratio >= 0.0)) {
+        set_error("xk_ctx_set_coating: damping, mu and k_t_ratio must be >= 0");
+        return -1;
+    }
+    if (core_radii) {
+        for (int e = 0; e < c->n_elems; ++e)
+            if (!(core_radii[e] >= 0.0)) {
+                set_error("xk_ctx_set_coating: core radii must be >= 0");
+                return -1;
+            }
+        c->crad_h.assign(core_radii, core_radii + c->n_elems);
+    } else {
+        c->crad_h = c->rad_h;
+    }
+    XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, false) != 0) return -1;
+    XK_TRY(upload(c->d_crad, c->crad_h));
+    if (c->n_pairs > 0) {
+        hipLaunchKernelGGL(coat_core_kernel, dim3((unsigned)((c->n_pairs + 255) / 256)), dim3(256), 0, nullptr,
+                           c->n_pairs, (const int*)c->ea, (const int*)c->eb, (const double*)c->d_crad.p, c->P.cra,
+                           c->P.crb);
+        XK_TRY(hipGetLastError());
+        XK_TRY(hipDeviceSynchronize());
+    }
+    // _create_coating_strategy: Kelvin-Voigt when the stiffness is > 0, none otherwise
+    c->coat_on = stiffness > 0.0;
+    c->coat_k = stiffness;
+    c->coat_c = damping;
+    c->coat_mu = mu;
+    c->coat_ktr = k_t_ratio;
+    c->pattern_valid = false;
+    return 0;
+}
+
+int xk_coating_set_dt(xk_ctx* c, double coat_dt) {
+    if (!c) return -1;
+    c->coat_dt = coat_dt;
+    return 0;
+}
+
+int xk_coating_probe(xk_ctx* c, const double* u, int64_t* n_unplaced, void* stream) {
+    if (!c || !n_unplaced) return -1;
+    *n_unplaced = 0;
+    if (!c->coat_on || c->n_pairs == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (c->lp_sync(st, false) != 0) return -1;
+    PairParams pp;
+    c->coat_params(pp);
+    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, nullptr, PAIR_GEOM | PAIR_PROBE,
+                        pp, (double*)c->d_sd.p, c->d_sg.p, nullptr, st, nullptr, c->pair_block));
+    XK_TRY(c->d_coat_cnt.ensure(sizeof(int)));
+    XK_TRY(hipMemsetAsync(c->d_coat_cnt.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(coat_unplaced_kernel, dim3((unsigned)((c->n_pairs + 255) / 256)), dim3(256), 0, st,
+                       c->n_pairs, (const int8_t*)c->P.cneed, (const int*)c->P.sinc, (int*)c->d_coat_cnt.p);
+    XK_TRY(hipGetLastError());
+    int cnt = 0;
+    XK_TRY(hipMemcpyAsync(&cnt, c->d_coat_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    XK_TRY(hipStreamSynchronize(st));
+    *n_unplaced = cnt;
+    if (cnt > 0) c->pattern_valid = false;   // the next pattern build gives the marked pairs a slot
+    return 0;
+}
+
+int xk_coating_commit(xk_ctx* c, void* stream) {
+    if (!c) return -1;
+    if (c->n_pairs == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (c->lp_sync(st, false) != 0) return -1;
+    hipLaunchKernelGGL(coat_commit_kernel, dim3((unsigned)((c->n_pairs + 255) / 256)), dim3(256), 0, st,
+                       c->n_pairs, (const double*)c->P.cc, c->P.ccp);
+    XK_TRY(hipGetLastError());
+    return 0;
+}
+
+int xk_pairs_get_coating(xk_ctx* c, double* core_a, double* core_b, double* compression,
+                         double* compression_prev, double* z_t, int8_t* stick, double* q_trial_norm,
+                         double* dissipation) {
+    if (!c) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, false) != 0) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    const int64_t n = c->n_pairs, cap = c->cap;
+    if (n == 0) return 0;
+    auto get = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        return dst ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+    };
+    XK_TRY(get(core_a, c->P.cra, n * 8));
+    XK_TRY(get(core_b, c->P.crb, n * 8));
+    XK_TRY(get(compression, c->P.cc, n * 8));
+    XK_TRY(get(compression_prev, c->P.ccp, n * 8));
+    XK_TRY(get(stick, c->P.cstick, n));
+    XK_TRY(get(q_trial_norm, c->P.cqtn, n * 8));
+    XK_TRY(get(dissipation, c->P.cdiss, n * 8));
+    if (z_t) {
+        std::vector<double> col(n);
+        for (int k = 0; k < 2; ++k) {
+            XK_TRY(hipMemcpy(col.data(), c->P.cz + k * cap, n * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) z_t[2 * i + k] = col[i];
+        }
+    }
+    return 0;
+}
+
+int xk_pairs_set_coating(xk_ctx* c, const double* compression, const double* compression_prev, const double* z_t,
+                         const int8_t* stick, const double* q_trial_norm, const double* dissipation) {
+    if (!c) return -1;
+    XK_TRY(hipDeviceSynchronize());
+    if (c->lp_sync(nullptr, false) != 0) return -1;
+    const int64_t n = c->n_pairs, cap = c->cap;
+    if (n == 0) return 0;
+    auto put = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        return src ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    XK_TRY(put(c->P.cc, compression, n * 8));
+    XK_TRY(put(c->P.ccp, compression_prev, n * 8));
+    XK_TRY(put(c->P.cstick, stick, n));
+    XK_TRY(put(c->P.cqtn, q_trial_norm, n * 8));
+    XK_TRY(put(c->P.cdiss, dissipation, n * 8));
+    if (z_t) {
+        std::vector<double> col(n);
+        for (int k = 0; k < 2; ++k) {
+            for (int64_t i = 0; i < n; ++i) col[i] = z_t[2 * i + k];
+            XK_TRY(hipMemcpy(c->P.cz + k * cap, col.data(), n * 8, hipMemcpyHostToDevice));
+        }
+    }
+    c->pattern_valid = false;
     return 0;
 }
 

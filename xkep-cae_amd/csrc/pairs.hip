@@ -15,10 +15,20 @@
 //   tangents      Huber material + geometric, friction material
 //                 sum_ab D_ab t_a t_b^T and geometric M / dist
 //                 (contact_force/strategy.py:172-254, friction/_assembly.py:108-257)
+//   coating       with the Kelvin-Voigt coating on (pp.coat_on): core gap and
+//                 compression cc = max(0, coat_total - gap_core), activation by
+//                 cc > 0 (_manager_process.py:351-360, :486-494), the force
+//                 k cc + c (cc - cc_prev) / dt along -n, the coating Coulomb
+//                 return mapping (k_t = k k_t_ratio, p_n = k cc), and the
+//                 stiffness k_eff n n^T + T D T^T (coating/strategy.py:99-370)
+//   probe         (PAIR_PROBE, alone) the compression at x only: cneed[i] =
+//                 cc > 0; nothing else is written
 //
 // Every 3x3 node block (k, l) of a pair's contribution to K is c_k c_l G with
 // one 3x3 G per pair and c = [(1-s), s, -(1-t), -t]; every node force is
-// c_k F with one 3-vector F.  The kernel therefore stages per contributing
+// c_k F with one 3-vector F.  The coating terms have this form too: their
+// weights [-(1-s), -s, 1-t, t] are -c, so they enter F with a minus sign and
+// G as they are (w_k w_l = c_k c_l).  The kernel therefore stages per contributing
 // pair only a force record c[4], F[3] (FP64) and a tangent record G[9], c[4]
 // (FP64 or FP32), at the pair's contributing slot of the increment; the
 // gather kernel expands them.  Pair blocks are mapped XCD-contiguously, so
@@ -54,7 +64,9 @@ __device__ __forceinline__ V3<double> min_axis(V3<double> v) {
     return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
 }
 
-template <class GT, int BS>
+// COAT: the Kelvin-Voigt coating terms are compiled in (pp.coat_on); the
+// plain instantiation carries none of their registers.
+template <class GT, int BS, bool COAT>
 __global__ void __launch_bounds__(BS)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
@@ -72,6 +84,8 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
         V3<double> n, t1;
         const int a0 = P.na0[i], a1 = P.na1[i], b0 = P.nb0[i], b1 = P.nb1[i];
         const double ra = P.ra[i], rb = P.rb[i];
+        constexpr bool coat = COAT;
+        double cc = 0.0;   // coating compression
         if (modes & PAIR_GEOM) {
             V3<double> xA0 = node_x(Xr, u, a0), xA1 = node_x(Xr, u, a1);
             V3<double> xB0 = node_x(Xr, u, b0), xB1 = node_x(Xr, u, b1);
@@ -98,7 +112,18 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             double sd_ = dist > 1e-30 ? dist : 1e-30;
             V3<double> nrm{diff.x / sd_, diff.y / sd_, diff.z / sd_};
             if (dist < 1e-30) nrm = V3<double>{0.0, 0.0, 1.0};
-            gap = dist - (ra + rb);
+            if (coat) {
+                const double cra = P.cra[i], crb = P.crb[i];
+                gap = dist - (cra + crb);
+                const double x = ((ra - cra) + (rb - crb)) - gap;
+                cc = x > 0.0 ? x : 0.0;
+            } else {
+                gap = dist - (ra + rb);
+            }
+            if (modes & PAIR_PROBE) {
+                P.cneed[i] = cc > 0.0 ? 1 : 0;
+                goto done_pair;
+            }
 
             // frame (n, t1); t2 = n x t1 is never stored
             double nn_ = sqrt(dot(nrm, nrm));
@@ -148,11 +173,13 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             P.s[i] = s; P.t[i] = t; P.gap[i] = gap;
             P.n[i] = n.x; P.n[cap + i] = n.y; P.n[2 * cap + i] = n.z;
             P.t1[i] = t1.x; P.t1[cap + i] = t1.y; P.t1[2 * cap + i] = t1.z;
+            if (coat) P.cc[i] = cc;
             if (modes & PAIR_HYST) {
+                const bool coat_active = coat && cc > 0.0;
                 int8_t ns = st;
                 if (st == ST_INACTIVE) {
-                    if (gap <= pp.g_on) ns = ST_ACTIVE;
-                } else if (pp.allow_deact && gap >= pp.g_off) {
+                    if (gap <= pp.g_on || coat_active) ns = ST_ACTIVE;
+                } else if (pp.allow_deact && gap >= pp.g_off && !coat_active) {
                     ns = ST_INACTIVE;
                 }
                 if (ns != st) { P.status[i] = ns; st = ns; }
@@ -161,9 +188,21 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             s = P.s[i]; t = P.t[i]; gap = P.gap[i];
             n = V3<double>{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
             t1 = V3<double>{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
+            if (coat) cc = P.cc[i];
         }
 
         const int si = P.sinc[i];
+        // coating: history reset of a separated pair (every pair, strategy.py:220-233);
+        // a pair with a coating contribution but no slot of the pattern is reported
+        const bool coat_force = coat && (modes & PAIR_HUBER);
+        if (coat_force && pp.coat_mu > 0.0 && cc <= 0.0) {
+            P.cz[i] = 0.0; P.cz[cap + i] = 0.0;
+            P.cstick[i] = 1;
+            P.cqtn[i] = 0.0;
+            P.cdiss[i] = 0.0;
+        }
+        if (coat && si < 0 && (modes & (PAIR_HUBER | PAIR_TAN_C)) && (cc > 0.0 || P.ccp[i] > 0.0))
+            atomicOr(pp.flags, FLAG_COAT_UNPLACED);
         if (si >= 0 && (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F))) {
             const V3<double> t2 = cross(n, t1);
             double p_n = P.p_n[i];
@@ -210,6 +249,37 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                     double f1 = fabs(q1) < 1e-30 ? 0.0 : q1;
                     double f2 = fabs(q2) < 1e-30 ? 0.0 : q2;
                     F = F + (f1 * t1 + f2 * t2);
+                }
+            }
+            if (coat_force) {
+                // Kelvin-Voigt normal force (strategy.py:99-144): node forces w_k f_n n = c_k (-f_n n)
+                const double ccp = P.ccp[i];
+                if (!(cc <= 0.0 && ccp <= 0.0)) {
+                    double f_n = pp.coat_k * cc;
+                    if (pp.coat_c > 0.0 && pp.coat_dt > 0.0) f_n = f_n + pp.coat_c * ((cc - ccp) / pp.coat_dt);
+                    if (!(fabs(f_n) < 1e-30)) F = F - f_n * n;
+                }
+                // coating friction (strategy.py:194-286): return mapping of the coating history
+                if (pp.coat_mu > 0.0 && cc > 0.0) {
+                    const double pnc = pp.coat_k * cc, kt = pp.coat_kt;
+                    V3<double> duA = (1.0 - s) * node_du(u, u_ref, a0) + s * node_du(u, u_ref, a1);
+                    V3<double> duB = (1.0 - t) * node_du(u, u_ref, b0) + t * node_du(u, u_ref, b1);
+                    V3<double> dur = duB - duA;
+                    double du1 = dot(dur, t1), du2 = dot(dur, t2);
+                    double q1 = P.cz[i] + kt * du1;
+                    double q2 = P.cz[cap + i] + kt * du2;
+                    double qtn = sqrt(q1 * q1 + q2 * q2);
+                    double fy = pp.coat_mu * pnc;
+                    bool stick = qtn <= fy;
+                    if (!stick) {
+                        q1 = (fy * q1) / qtn;
+                        q2 = (fy * q2) / qtn;
+                    }
+                    P.cz[i] = q1; P.cz[cap + i] = q2;
+                    P.cstick[i] = stick ? 1 : 0;
+                    P.cqtn[i] = qtn;
+                    P.cdiss[i] = q1 * du1 + q2 * du2;
+                    F = F - (q1 * t1 + q2 * t2);
                 }
             }
             if (modes & (PAIR_HUBER | PAIR_FRIC)) {
@@ -297,6 +367,41 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                         for (int k = 0; k < 9; ++k) G[k] += sgn * H[k];
                     }
                 }
+                if (coat && (modes & PAIR_TAN_C)) {
+                    // coating stiffness k_eff n n^T (strategy.py:146-192), pairs with cc > 1e-15
+                    if (cc > 1e-15) {
+                        double k_eff = pp.coat_k;
+                        if (pp.coat_c > 0.0 && pp.coat_dt > 0.0) k_eff += pp.coat_c / pp.coat_dt;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) G[3 * a + b] += k_eff * (nv[a] * nv[b]);
+                    }
+                    // coating friction stiffness T D T^T (strategy.py:288-370), pairs with cc > 0
+                    if (pp.coat_mu > 0.0 && cc > 0.0) {
+                        const double kt = pp.coat_kt, pnc = pp.coat_k * cc;
+                        const double q1 = P.cz[i], q2 = P.cz[cap + i];
+                        const double zn = sqrt(q1 * q1 + q2 * q2), qtn = P.cqtn[i];
+                        double D00, D01, D10, D11;
+                        if (P.cstick[i] || zn < 1e-30 || qtn < 1e-30) {
+                            D00 = kt; D01 = 0.0; D10 = 0.0; D11 = kt;
+                        } else {
+                            const double h1 = q1 / zn, h2 = q2 / zn;
+                            const double rk = ((pp.coat_mu * pnc) / qtn) * kt;
+                            D00 = rk * (1.0 - h1 * h1); D01 = rk * (0.0 - h1 * h2);
+                            D10 = rk * (0.0 - h2 * h1); D11 = rk * (1.0 - h2 * h2);
+                        }
+                        const double T1[3] = {t1.x, t1.y, t1.z}, T2[3] = {t2.x, t2.y, t2.z};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            // row a of T D = [T1[a] D00 + T2[a] D10, T1[a] D01 + T2[a] D11]
+                            const double r0 = T1[a] * D00 + T2[a] * D10;
+                            const double r1 = T1[a] * D01 + T2[a] * D11;
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) G[3 * a + b] += r0 * T1[b] + r1 * T2[b];
+                        }
+                    }
+                }
                 double* rec = sd + SD_REC * (size_t)si;
                 rec[0] = 1.0 - s;
                 rec[1] = s;
@@ -313,6 +418,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             active_here = p_n > 0.0 ? 1 : 0;
         }
     }
+done_pair:
     if (act_part) {
         // #(p_n > 0) of this workgroup: wave ballots, then one store per workgroup
         __shared__ int wc[BS / 64];
@@ -336,8 +442,12 @@ static void launch_pairs_bs(int64_t n_pairs, const PairSoA& P, const double* Xr,
                             int modes, const PairParams& pp, double* sd, GT* sg, int* act_part, hipStream_t st,
                             const int* idx) {
     dim3 blk(BS), grd((unsigned)xcd_padded(pair_blocks(n_pairs, BS)));
-    hipLaunchKernelGGL((pair_kernel<GT, BS>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part,
-                       idx);
+    if (pp.coat_on)
+        hipLaunchKernelGGL((pair_kernel<GT, BS, true>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg,
+                           act_part, idx);
+    else
+        hipLaunchKernelGGL((pair_kernel<GT, BS, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg,
+                           act_part, idx);
 }
 
 template <class GT>

@@ -1,7 +1,8 @@
 // Tangent pattern of an increment, built on the GPU (direct element path).
 //
 // The pattern is the union of the beam blocks and the blocks of the pairs that
-// can contribute within the increment (status != INACTIVE or p_n > 0), as a
+// can contribute within the increment (status != INACTIVE or p_n > 0, with the
+// coating on also a compression > 0), as a
 // 3x3-block CSR over DOF groups (2n translations of node n, 2n+1 rotations).
 // On element chains with consecutive node numbers the beam blocks of the row
 // of node n are implied: column groups of n-1 (if an element joins n-1 and n),
@@ -40,10 +41,16 @@ __device__ __forceinline__ int pair_node(const PairSoA& P, int64_t i, int k) {
     return k == 0 ? P.na0[i] : (k == 1 ? P.na1[i] : (k == 2 ? P.nb0[i] : P.nb1[i]));
 }
 
+// contributing: status != INACTIVE or p_n > 0; with the coating on (cc != nullptr)
+// also a compression (current, previous step or probed) > 0
 __global__ void pat_contrib_kernel(int64_t n, const int8_t* __restrict__ status, const double* __restrict__ p_n,
-                                   int64_t* __restrict__ keep) {
+                                   const double* __restrict__ cc, const double* __restrict__ ccp,
+                                   const int8_t* __restrict__ cneed, int64_t* __restrict__ keep) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keep[i] = (status[i] != ST_INACTIVE || p_n[i] > 0.0) ? 1 : 0;
+    if (i >= n) return;
+    bool k = status[i] != ST_INACTIVE || p_n[i] > 0.0;
+    if (cc) k = k || cc[i] > 0.0 || ccp[i] > 0.0 || cneed[i] != 0;
+    keep[i] = k ? 1 : 0;
 }
 
 __global__ void pat_sinc_kernel(int64_t n, const int64_t* __restrict__ keep, const int64_t* __restrict__ pos,
@@ -246,7 +253,9 @@ int Ctx::build_pattern_gpu(hipStream_t st) {
     int64_t* pos = (int64_t*)S[1].p;
     int* slist = (int*)S[2].p;
     if (np > 0) {
-        hipLaunchKernelGGL(pat_contrib_kernel, dim3(grid(np)), dim3(256), 0, st, np, P.status, P.p_n, keep);
+        hipLaunchKernelGGL(pat_contrib_kernel, dim3(grid(np)), dim3(256), 0, st, np, P.status, P.p_n,
+                           coat_on ? (const double*)P.cc : nullptr, (const double*)P.ccp, (const int8_t*)P.cneed,
+                           keep);
         PT_TRY(hipGetLastError());
         if (scan_exclusive_i64(this, 0, keep, pos, np, &ns, st) != 0) return -1;
         hipLaunchKernelGGL(pat_sinc_kernel, dim3(grid(np)), dim3(256), 0, st, np, keep, pos, P.sinc, slist);

@@ -113,6 +113,7 @@ XK_HD V3<T> cvt(V3<S> a) { return {T(a.x), T(a.y), T(a.z)}; }
 // Device status word bits (mirror of XK_FLAG_* in include/xkamd.h).
 constexpr int FLAG_NONFINITE = 1;
 constexpr int FLAG_DEGENERATE = 2;
+constexpr int FLAG_COAT_UNPLACED = 8;
 
 // Pair-kernel modes (mirror of XK_PAIR_*).
 constexpr int PAIR_GEOM = 1;
@@ -121,6 +122,7 @@ constexpr int PAIR_HUBER = 4;
 constexpr int PAIR_FRIC = 8;
 constexpr int PAIR_TAN_C = 16;
 constexpr int PAIR_TAN_F = 32;
+constexpr int PAIR_PROBE = 64;
 
 // ContactStatus codes (xkep_cae/contact/_types.py:8-11).
 constexpr int8_t ST_INACTIVE = 0;

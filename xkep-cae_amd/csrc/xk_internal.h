@@ -33,6 +33,7 @@ namespace xk {
 enum PairBuf {
     PB_NA0, PB_NA1, PB_NB0, PB_NB1, PB_RA, PB_RB, PB_S, PB_T, PB_GAP, PB_N, PB_T1,
     PB_PN, PB_KPEN, PB_KT, PB_QTN, PB_DISS, PB_Z, PB_STATUS, PB_STICK, PB_SINC, PB_EA, PB_EB,
+    PB_CRA, PB_CRB, PB_CC, PB_CCP, PB_CZ, PB_CSTICK, PB_CQTN, PB_CDISS, PB_CNEED,
     PB_COUNT
 };
 
@@ -104,6 +105,22 @@ struct Ctx {
     // per-element friction coefficients (xk_ctx_set_friction)
     DBuf d_mu_elem;
     bool has_mu_elem = false;
+    // Kelvin-Voigt coating (xk_ctx_set_coating): per-element core radii, parameters,
+    // the dashpot time step of the next assemblies (xk_coating_set_dt)
+    std::vector<double> crad_h;
+    DBuf d_crad;
+    DBuf d_coat_cnt;   // probe: #(pairs with compression > 0 and no slot)
+    bool coat_on = false;
+    double coat_k = 0.0, coat_c = 0.0, coat_mu = 0.0, coat_ktr = 0.5, coat_dt = 1e-15;
+    void coat_params(PairParams& pp) const {
+        pp.coat_on = coat_on ? 1 : 0;
+        pp.coat_k = coat_k;
+        pp.coat_c = coat_c;
+        pp.coat_mu = coat_mu;
+        pp.coat_kt = coat_k * coat_ktr;
+        pp.coat_dt = coat_dt;
+        pp.flags = (int*)d_flags.p;
+    }
     // launch shapes (xk_ctx_set_tuning)
     int pair_block = 256, gather_block = 256;
     // learned prescreen: parameters, features, watch / prune lists of the increment
@@ -150,6 +167,8 @@ struct Ctx {
         for (auto& b : pb) b.release();
         for (auto& b : bp) b.release();
         d_act.release();
+        d_crad.release();
+        d_coat_cnt.release();
         d_eflag.release();
         d_nflag.release();
         d_cstart.release();

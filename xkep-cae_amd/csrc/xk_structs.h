@@ -17,17 +17,31 @@ struct PairSoA {
     double* z;
     int8_t* status; int8_t* stick;
     int* sinc;                                // contributing-pair index of the increment, or -1
+    // Kelvin-Voigt coating: core radii, compression (current, previous step),
+    // coating friction history z (component-major [2][cap]), stick, |q_trial|,
+    // dissipation; cneed: compression > 0 seen by the probe (pattern input)
+    double* cra; double* crb;
+    double* cc; double* ccp;
+    double* cz;
+    int8_t* cstick;
+    double* cqtn; double* cdiss;
+    int8_t* cneed;
     int64_t cap;
 };
 
 struct PairParams {
-    double k_pen, delta_h, k_t_ratio, mu_eff, g_on, g_off;
-    int allow_deact;
-    int n_sinc;
+    double k_pen = 0.0, delta_h = 0.0, k_t_ratio = 0.0, mu_eff = 0.0, g_on = 0.0, g_off = 0.0;
+    int allow_deact = 0;
+    int n_sinc = 0;
     // per-element friction coefficient (nullable): pair i uses mu_elem[ea[i]]
     // scaled by mu_eff instead of mu_eff (batches of independent solves)
-    const double* mu_elem;
-    const int* ea;
+    const double* mu_elem = nullptr;
+    const int* ea = nullptr;
+    // Kelvin-Voigt coating (coat_on = 0: off): stiffness, damping, friction
+    // coefficient, tangential stiffness k * k_t_ratio, dashpot time step
+    int coat_on = 0;
+    double coat_k = 0.0, coat_c = 0.0, coat_mu = 0.0, coat_kt = 0.0, coat_dt = 0.0;
+    int* flags = nullptr;   // status flags (FLAG_COAT_UNPLACED)
 };
 
 struct NodeLists {

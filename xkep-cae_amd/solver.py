@@ -318,6 +318,8 @@ class ContactFrictionSolver:
       assemble(u, u_ref, f_ext, dt) -> (R_eff, K_eff, n_active, f_c)
       internal_force(u) -> f_int
       correct(u, dt); checkpoint(); restore_checkpoint()
+    and, when its ``coating`` attribute is true, set_coat_dt(coat_dt) and
+    commit_coating().
     """
 
     def __init__(self, backend, *, fixed_dofs, f_ext_total, t_total: float, f_ext_base=None,
@@ -339,10 +341,12 @@ class ContactFrictionSolver:
         if dynamic_ref:
             f_ref_norm = 1.0
         b = self.b
-        # set-up candidates and the initial-penetration check (process.py:258-324)
+        # set-up candidates and the initial-penetration check (process.py:258-324;
+        # with the coating on a core penetration is allowed, :320)
+        coat = bool(getattr(b, "coating", False))
         b.detect(u)
         n_pen = b.initial_penetrations(u)
-        if n_pen > 0:
+        if n_pen > 0 and not coat:
             raise ValueError(f"initial penetration detected in {n_pen} pairs")
         ckpt = (u.copy(), u_ref.copy())
         b.checkpoint()
@@ -358,6 +362,8 @@ class ContactFrictionSolver:
             incr_count += 1
             n_incr += 1
             f_ext = self.f_base + load_frac * self.f_total
+            if coat:
+                b.set_coat_dt(max(load_frac - load_frac_prev, 1e-15))   # _newton_steps.py:112, :297
             u = b.predict(u, dt)
             b.detect(u)
             b.update_geometry(u)
@@ -386,6 +392,8 @@ class ContactFrictionSolver:
                     n_incr -= 1
                     continue
                 return SolverResult(u, False, n_incr, total, stepping.n_cutbacks, calls, n_rechecks, watches)
+            if coat:
+                b.commit_coating()   # coating_compression_prev <- coating_compression (process.py:539-547)
             b.correct(u, dt)
             stepping.success(load_frac, load_frac_prev, step.n_attempts, step.n_active, prev_n_active)
             prev_n_active = step.n_active
@@ -406,7 +414,7 @@ class DeviceBackend:
 
     def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
                  k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False,
-                 use_graph: bool = False):
+                 use_graph: bool = False, core_radii=None):
         from .assembly import StrandAssembly
         from .time_integration import GeneralizedAlphaProcess
 
@@ -422,7 +430,8 @@ class DeviceBackend:
         self.graph_error = None
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
         self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
-                                  fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0)
+                                  fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0,
+                                  core_radii=core_radii)
         self._f = self.torch.zeros(ctx.ndof, dtype=self.torch.float64, device=ctx.dev)
         self._dt = None
 
@@ -434,6 +443,16 @@ class DeviceBackend:
 
     def initial_penetrations(self, u) -> int:
         return self.asm.count_penetrations(u)
+
+    @property
+    def coating(self) -> bool:
+        return self.asm.coating
+
+    def set_coat_dt(self, coat_dt: float):
+        self.asm.set_coat_dt(coat_dt)
+
+    def commit_coating(self):
+        self.asm.commit_coating()
 
     def predict(self, u, dt):
         return self.ga.predict(u, dt)
@@ -480,7 +499,8 @@ class DeviceBackend:
         status = torch.zeros(24, dtype=torch.float64, device=ctx.dev)
         ls_steps = cfg.line_search_max_steps if cfg.use_line_search else 0
         graphs = None
-        graph_ok = self.use_graph and asm.prescreen is None
+        # with the coating on every assembly probes the pattern first (host sync): eager attempts
+        graph_ok = self.use_graph and asm.prescreen is None and not asm.coating
         energy_ref = None
         incr_f_ref = 0.0
         consecutive_increase = 0
