@@ -20,6 +20,8 @@ beam      CR Timoshenko element f_int / analytical tangent / lumped mass
 contact   broadphase, candidate filters, narrowphase, frame transport,
           hysteresis, Huber normal force, Coulomb return mapping, friction
           tangents (material + geometric)
+coating   Kelvin-Voigt coating force (dashpot), stiffness, coating Coulomb
+          friction and its stiffness
 assembly  per-iteration composition R = f_int - (f_c + f_fric) - f_ext,
           K_T = K_beam + K_c - K_fric, Generalized-alpha effective terms
 newton    restated dynamic Newton driver (host spsolve) for Newton-count checks
@@ -44,6 +46,10 @@ PINNING = {
     "contact.huber_*": "closed-form Huber values/derivative, FD of f_c at fixed frame, golden vectors",
     "contact.friction_*": "return-map closed forms (stick/slip, |q|=mu p_n), D_t eigenvalues, "
     "golden vectors (sign quirks of the material block)",
+    "coating.*": "the known answers of the reference's coating tests (linearity, action-reaction, "
+    "force signs, dashpot rate, k_eff = k + c/dt, symmetry, PSD, Coulomb limit, history reset), node "
+    "forces and return mapping in closed form, K_coat = FD of the coating force along a normal approach",
+    "contact.update_geometry (coating)": "core gap, compression and activation in closed form",
     "assembly.*": "golden vectors of R and K_T from the reference on a 7-wire state",
     "newton.*": "Newton-iteration counts and active-set trajectory from the reference",
 }
