@@ -11,9 +11,10 @@ friction-loop state quirks (stale p_n, status rewrite, z_t overwrite) and the
 composed R / K_T of whole Newton attempts across two increments, and the
 trajectory of the reference's own dynamic Newton solver (attempt counts,
 cutbacks, active counts, residual histories) on a small strand, which only
-running it can pin.
+running it can pin, and the strand generator's layouts (angle offsets, lay
+radii, numbering) over its branches.
 
-Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [element|assembly|newton ...]
+Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [element|assembly|newton|mesh ...]
 """
 
 from __future__ import annotations
@@ -313,7 +314,41 @@ def newton_fixture():
     np.savez_compressed(os.path.join(HERE, "newton_7wire.npz"), **out)
 
 
-FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture}
+# strand generator configurations: (n_strands, wire_diameter, pitch, length, n_elems, keyword arguments)
+MESH_CASES = [
+    (7, 2.0, 40.0, 40.0, 40, {}),
+    (3, 1.0, 30.0, 30.0, 30, {"lay_direction": -1}),
+    (19, 1.0, 25.0, 0.0, 40, {"n_pitches": 1.5, "gap": 0.05}),
+    (37, 1.0, 60.0, 60.0, 64, {"strand_diameter": 8.2}),
+    (10, 1.0, 30.0, 30.0, 32, {"strand_diameter": 5.6, "coating_thickness": 0.05}),
+    (13, 0.8, 30.0, 30.0, 32, {"coating_thickness": 0.1, "gap": 0.02}),
+    (3, 1.0, 30.0, 30.0, 30, {"strand_diameter": 2.5}),
+]
+
+
+def mesh_fixture():
+    """The reference's strand generator on configurations that cover its
+    branches (three wires, partial layers, explicit outer diameter with the
+    surplus spread over the layers, coating thickness, pitches, lay direction)."""
+    import warnings
+
+    from xkep_cae.mesh._twisted_wire import _make_twisted_wire_mesh
+
+    out = {}
+    for k, (n, d, pitch, length, ne, kw) in enumerate(MESH_CASES):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            m = _make_twisted_wire_mesh(n, d, pitch, length, ne, **kw)
+        out[f"c{k}_X"] = m.node_coords
+        out[f"c{k}_conn"] = m.connectivity.astype(np.int64)
+        out[f"c{k}_info"] = np.array([[i.strand_id, i.layer, i.angle_offset, i.lay_radius, i.wire_radius,
+                                       i.lay_direction] for i in m.strand_infos])
+        out[f"c{k}_length"] = np.array(m.length)
+    np.savez_compressed(os.path.join(HERE, "mesh_layouts.npz"), **out)
+
+
+FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture,
+            "mesh": mesh_fixture}
 
 if __name__ == "__main__":
     for name in sys.argv[1:] or list(FIXTURES):
