@@ -79,11 +79,14 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
     if (w < n_pairs) {
         const int64_t i = idx ? (int64_t)idx[w] : w;
         const int64_t cap = P.cap;
-        int8_t st = P.status[i];
+        // The per-pair state streams through once per assembly (about 200 bytes a
+        // pair); loaded and stored non-temporally it does not evict the node
+        // vectors and records that the gather kernel reads next.
+        int8_t st = __builtin_nontemporal_load(&P.status[i]);
         double s, t, gap;
         V3<double> n, t1;
-        const int a0 = P.na0[i], a1 = P.na1[i], b0 = P.nb0[i], b1 = P.nb1[i];
-        const double ra = P.ra[i], rb = P.rb[i];
+        const int a0 = __builtin_nontemporal_load(&P.na0[i]), a1 = __builtin_nontemporal_load(&P.na1[i]), b0 = __builtin_nontemporal_load(&P.nb0[i]), b1 = __builtin_nontemporal_load(&P.nb1[i]);
+        const double ra = __builtin_nontemporal_load(&P.ra[i]), rb = __builtin_nontemporal_load(&P.rb[i]);
         constexpr bool coat = COAT;
         double cc = 0.0;   // coating compression
         if (modes & PAIR_GEOM) {
@@ -129,8 +132,8 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             double nn_ = sqrt(dot(nrm, nrm));
             nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
             n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
-            V3<double> pt1{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
-            V3<double> pn{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
+            V3<double> pt1{__builtin_nontemporal_load(&P.t1[i]), __builtin_nontemporal_load(&P.t1[cap + i]), __builtin_nontemporal_load(&P.t1[2 * cap + i])};
+            V3<double> pn{__builtin_nontemporal_load(&P.n[i]), __builtin_nontemporal_load(&P.n[cap + i]), __builtin_nontemporal_load(&P.n[2 * cap + i])};
             bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
             bool has_n = sqrt(dot(pn, pn)) > 1e-10;
             bool done = false;
@@ -170,9 +173,15 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 rn = rn > 1e-30 ? rn : 1e-30;
                 t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
             }
-            P.s[i] = s; P.t[i] = t; P.gap[i] = gap;
-            P.n[i] = n.x; P.n[cap + i] = n.y; P.n[2 * cap + i] = n.z;
-            P.t1[i] = t1.x; P.t1[cap + i] = t1.y; P.t1[2 * cap + i] = t1.z;
+            __builtin_nontemporal_store(s, &P.s[i]);
+            __builtin_nontemporal_store(t, &P.t[i]);
+            __builtin_nontemporal_store(gap, &P.gap[i]);
+            __builtin_nontemporal_store(n.x, &P.n[i]);
+            __builtin_nontemporal_store(n.y, &P.n[cap + i]);
+            __builtin_nontemporal_store(n.z, &P.n[2 * cap + i]);
+            __builtin_nontemporal_store(t1.x, &P.t1[i]);
+            __builtin_nontemporal_store(t1.y, &P.t1[cap + i]);
+            __builtin_nontemporal_store(t1.z, &P.t1[2 * cap + i]);
             if (coat) P.cc[i] = cc;
             if (modes & PAIR_HYST) {
                 const bool coat_active = coat && cc > 0.0;
@@ -191,7 +200,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             if (coat) cc = P.cc[i];
         }
 
-        const int si = P.sinc[i];
+        const int si = __builtin_nontemporal_load(&P.sinc[i]);
         // coating: history reset of a separated pair (every pair, strategy.py:220-233);
         // a pair with a coating contribution but no slot of the pattern is reported
         const bool coat_force = coat && (modes & PAIR_HUBER);
