@@ -44,13 +44,13 @@ def coating_params(case):
         coat_dt
 
 
-def device_run(case, st, prec, core, kw, coat_dt, steps):
+def device_run(case, st, prec, core, kw, coat_dt, steps, **ctx_opts):
     """Device: increments at the displacement list `steps`; the last one's
     assembly (R, K csr, n_active, pairs, coating state, rebuilds)."""
     C, A, Kc = pkg("context"), pkg("assembly"), pkg("contact")
     m = st.mesh
     s = C.circle_section(case.wire_radius, case.E, case.nu)
-    ctx = C.DeviceContext(m.node_coords, m.connectivity, s, radii=m.radii, precision=prec)
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, s, radii=m.radii, precision=prec, **ctx_opts)
     cfg = Kc.ContactConfig(mu=case.mu, **kw)
     asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
                            f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, config=cfg, core_radii=core)
@@ -98,10 +98,10 @@ def order(d):
     return np.lexsort((key[:, 1], key[:, 0])), key
 
 
-def compare(case, st, prec, core_frac, steps):
+def compare(case, st, prec, core_frac, steps, **ctx_opts):
     kw, coat_dt = coating_params(case)
     core = None if core_frac is None else core_frac * st.mesh.radii
-    R, K, nact, pairs, coat, rebuilds = device_run(case, st, prec, core, kw, coat_dt, steps)
+    R, K, nact, pairs, coat, rebuilds = device_run(case, st, prec, core, kw, coat_dt, steps, **ctx_opts)
     ref = oracle_run(case, st, core, kw, coat_dt, steps)
     cc = ref.pairs.coating_compression
     print(f"pairs {len(cc)} compressed {int((cc > 0).sum())} n_active {ref.n_active} rebuilds {rebuilds}")
@@ -144,6 +144,18 @@ def test_coating_assembly_matches_oracle_7wire(gpu, precision, core_frac):
     st = wl.make_state(wl.C1)
     half = 0.5 * st.u
     compare(wl.C1, st, precision, core_frac, [(half, half), (st.u, st.u)])
+
+
+@pytest.mark.parametrize("opts", [{"staged": True}, {"host_pattern": True}], ids=["staged_path", "host_pattern"])
+def test_coating_on_the_other_assembly_paths(gpu, opts):
+    """The staged element path (any connectivity) and the host-built pattern
+    carry the coating terms as well (two increments, coating layer, FP64)."""
+    wl = pkg("workloads")
+    st = wl.make_state(wl.C1)
+    half = 0.5 * st.u
+    compare(wl.C1, st, "fp64", 0.9, [(half, half), (st.u, st.u)], **opts)
+    zero = np.zeros_like(st.u)
+    assert compare(wl.C1, st, "fp64", None, [(zero, st.u)], **opts) >= 1   # probe + rebuild on this path
 
 
 def test_coating_assembly_matches_oracle_37wire(gpu):
