@@ -33,11 +33,20 @@
 
 namespace xk {
 
+// Sum over the 64 lanes of a wave by a butterfly: every lane gets the same
+// value, formed in the same order each call.
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // node_mode: -1 no residual work, 0 = f_int only, 1 = pair force only, 2 = full residual
 template <class KT, class ST, class GT, int GB>
 __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __restrict__ Ke,
                                                      const GT* __restrict__ sg, KT* __restrict__ K) {
-    __shared__ double red[GB];
+    __shared__ double red[GB / 64];
     double ss = 0.0;
     const bool node_block = (int)blockIdx.x < g.node_blocks;
     if (node_block) {
@@ -164,15 +173,17 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
             K[base0 + 2] = KT(v2);
         }
     }
-    // residual partial of this workgroup (fixed-shape tree: deterministic)
+    // residual partial of this workgroup: butterfly within each wave, then the
+    // waves' sums in wave order (fixed shape: deterministic)
     if (g.partial && node_block) {
-        red[threadIdx.x] = ss;
+        ss = wave_sum(ss);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
         __syncthreads();
-        for (int w = GB / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = red[0];
+            for (int w = 1; w < GB / 64; ++w) t += red[w];
+            g.partial[blockIdx.x] = t;
         }
-        if (threadIdx.x == 0) g.partial[blockIdx.x] = red[0];
     }
 }
 
@@ -185,39 +196,52 @@ constexpr int FB = 1024;  // finish kernel lanes
 __global__ void __launch_bounds__(FB) finish_kernel(int n_part, const double* __restrict__ partial,
                                                     int n_act, const int* __restrict__ act_part,
                                                     double* __restrict__ res_norm2, int* __restrict__ n_active) {
-    __shared__ double red[FB];
-    __shared__ int ired[FB];
-    // each lane sums its strided entries in index order, four loads in flight
+    __shared__ double red[FB / 64];
+    __shared__ int ired[FB / 64];
+    // each lane sums its strided entries in index order; one chunk issues all its
+    // loads (8 partials and 16 counts per lane) before the first add
+    constexpr int CP = 8, CA = 16;
     double s = 0.0;
     int a = 0;
-    int k = threadIdx.x;
-    for (; k + 3 * FB < n_part; k += 4 * FB) {
-        const double p0 = partial[k], p1 = partial[k + FB], p2 = partial[k + 2 * FB], p3 = partial[k + 3 * FB];
-        s = (((s + p0) + p1) + p2) + p3;
-    }
-    for (; k < n_part; k += FB) s += partial[k];
-    k = threadIdx.x;
-    for (; k + 7 * FB < n_act; k += 8 * FB) {
-        int q[8];
+    for (int k0 = threadIdx.x; k0 < n_part || k0 < n_act; k0 += CA * FB) {
+        double p[CP];
+        int q[CA];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) q[j] = act_part[k + j * FB];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a += q[j];
-    }
-    for (; k < n_act; k += FB) a += act_part[k];
-    red[threadIdx.x] = s;
-    ired[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = FB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[threadIdx.x] += red[threadIdx.x + w];
-            ired[threadIdx.x] += ired[threadIdx.x + w];
+        for (int j = 0; j < CP; ++j) {
+            const int k = k0 + j * FB;
+            p[j] = k < n_part ? partial[k] : 0.0;
         }
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            const int k = k0 + j * FB;
+            q[j] = k < n_act ? act_part[k] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < CP; ++j) s += p[j];
+#pragma unroll
+        for (int j = CP; j < CA; ++j) {
+            const int k = k0 + j * FB;
+            if (k < n_part) s += partial[k];
+        }
+#pragma unroll
+        for (int j = 0; j < CA; ++j) a += q[j];
     }
+    s = wave_sum(s);
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = s;
+        ired[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        if (res_norm2) *res_norm2 = red[0];
-        if (n_active) *n_active = ired[0];
+        double t = red[0];
+        int c = ired[0];
+        for (int w = 1; w < FB / 64; ++w) {
+            t += red[w];
+            c += ired[w];
+        }
+        if (res_norm2) *res_norm2 = t;
+        if (n_active) *n_active = c;
     }
 }
 

@@ -89,6 +89,15 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
         const double ra = __builtin_nontemporal_load(&P.ra[i]), rb = __builtin_nontemporal_load(&P.rb[i]);
         constexpr bool coat = COAT;
         double cc = 0.0;   // coating compression
+        // The previous frame and the record slot are loaded with the ids, before
+        // the dependent node gathers, so the three loads overlap.
+        const int si = __builtin_nontemporal_load(&P.sinc[i]);
+        const bool geom = (modes & PAIR_GEOM) && !(modes & PAIR_PROBE);
+        V3<double> pt1{0.0, 0.0, 0.0}, pn{0.0, 0.0, 0.0};
+        if (geom) {
+            pt1 = V3<double>{__builtin_nontemporal_load(&P.t1[i]), __builtin_nontemporal_load(&P.t1[cap + i]), __builtin_nontemporal_load(&P.t1[2 * cap + i])};
+            pn = V3<double>{__builtin_nontemporal_load(&P.n[i]), __builtin_nontemporal_load(&P.n[cap + i]), __builtin_nontemporal_load(&P.n[2 * cap + i])};
+        }
         if (modes & PAIR_GEOM) {
             V3<double> xA0 = node_x(Xr, u, a0), xA1 = node_x(Xr, u, a1);
             V3<double> xB0 = node_x(Xr, u, b0), xB1 = node_x(Xr, u, b1);
@@ -132,8 +141,6 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             double nn_ = sqrt(dot(nrm, nrm));
             nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
             n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
-            V3<double> pt1{__builtin_nontemporal_load(&P.t1[i]), __builtin_nontemporal_load(&P.t1[cap + i]), __builtin_nontemporal_load(&P.t1[2 * cap + i])};
-            V3<double> pn{__builtin_nontemporal_load(&P.n[i]), __builtin_nontemporal_load(&P.n[cap + i]), __builtin_nontemporal_load(&P.n[2 * cap + i])};
             bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
             bool has_n = sqrt(dot(pn, pn)) > 1e-10;
             bool done = false;
@@ -200,7 +207,6 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             if (coat) cc = P.cc[i];
         }
 
-        const int si = __builtin_nontemporal_load(&P.sinc[i]);
         // coating: history reset of a separated pair (every pair, strategy.py:220-233);
         // a pair with a coating contribution but no slot of the pattern is reported
         const bool coat_force = coat && (modes & PAIR_HUBER);
