@@ -63,7 +63,13 @@ enum {
     XK_PAIR_HUBER = 4,         /* HuberContactForceProcess.evaluate */
     XK_PAIR_FRICTION = 8,      /* CoulombReturnMappingProcess.evaluate */
     XK_PAIR_TANGENT_C = 16,    /* stage Huber tangent blocks */
-    XK_PAIR_TANGENT_F = 32     /* stage friction tangent blocks (material + geometric) */
+    XK_PAIR_TANGENT_F = 32,    /* stage friction tangent blocks (material + geometric) */
+    /* Kelvin-Voigt coating terms (the coating on: xk_ctx_set_coating), as the
+     * methods of KelvinVoigtCoatingProcess (contact/coating/strategy.py:80-384) */
+    XK_PAIR_COAT_FORCE = 128,      /* .forces: k cc + c (cc - cc_prev) / dt along -n (:97-144) */
+    XK_PAIR_COAT_FRICTION = 256,   /* .friction_forces: coating return mapping, history update (:206-297) */
+    XK_PAIR_COAT_TANGENT = 512,    /* .stiffness: stage k_eff (w n)(w n)^T blocks (:146-204) */
+    XK_PAIR_COAT_TANGENT_F = 1024  /* .friction_stiffness: stage (w w) T D T^T blocks (:299-379) */
 };
 
 /* Mesh, material and section.  Replaces ULCRBeamAssembler.__init__
@@ -141,7 +147,10 @@ int xk_prescreen_radii(xk_ctx* ctx, const double* coords, double margin, double 
  * deformed coordinates; u_ref (device) is the friction reference.  Covers
  * UpdateGeometryProcess (_manager_process.py:327-403, hysteresis :475-494),
  * HuberContactForceProcess.evaluate/tangent (contact_force/strategy.py:126-254),
- * CoulombReturnMappingProcess.evaluate/tangent (friction/strategy.py:111-164). */
+ * CoulombReturnMappingProcess.evaluate/tangent (friction/strategy.py:111-164)
+ * and the KelvinVoigtCoatingProcess methods (coating/strategy.py:97-379).
+ * With the coating on, every geometry update invalidates the tangent pattern
+ * (a new compression gives a pair a coating contribution). */
 int xk_pairs_run(xk_ctx* ctx, int modes, const double* coords_ref, const double* u,
                  const double* u_ref, double k_pen, double smoothing_delta, double k_t_ratio,
                  double mu_eff, double g_on, double g_off, int allow_deactivation,

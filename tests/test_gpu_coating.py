@@ -165,6 +165,69 @@ def test_coating_assembly_matches_oracle_37wire(gpu):
     compare(wl.C2, st, "fp64", 0.9, [(half, half), (st.u, st.u)])
 
 
+@pytest.mark.parametrize("core_frac", [None, 0.9], ids=["core_eq_radius", "coating_layer"])
+def test_coating_strategy_api_matches_oracle(gpu, core_frac):
+    """The drop-in Processes and the coating Strategy, one call each:
+    DetectCandidates with core radii, UpdateGeometry with the coating branch,
+    then KelvinVoigtCoatingProcess.forces / stiffness / friction_forces /
+    friction_stiffness against the oracle's; a second step after the
+    compression is committed (dashpot rate and coating history carried)."""
+    wl = pkg("workloads")
+    C, K = pkg("context"), pkg("contact")
+    case = wl.C1
+    st = wl.make_state(case)
+    m = st.mesh
+    ndof = 6 * len(m.node_coords)
+    kw, coat_dt = coating_params(case)
+    core = None if core_frac is None else core_frac * m.radii
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, C.circle_section(case.wire_radius, case.E, case.nu),
+                          radii=m.radii)
+    cfg = K.ContactConfig(mu=case.mu, **kw)
+    mgr = K.ContactManager(ctx, cfg)
+    ocfg = ocon.ContactConfig(mu=case.mu, **kw)
+    det, geo = K.DetectCandidatesProcess(), K.UpdateGeometryProcess()
+    det.process(K.DetectCandidatesInput(mgr, m.node_coords, m.connectivity, m.radii, core_radii=core))
+    op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, ocfg, core_radii=core)
+    x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
+    det.process(K.DetectCandidatesInput(mgr, x, m.connectivity, m.radii, core_radii=core))
+    op, _ = ocon.detect_candidates(op, x, m.connectivity, m.radii, ocfg, core_radii=core)
+    strat = K._create_coating_strategy(coating_stiffness=cfg.coating_stiffness)
+    assert isinstance(strat, K.KelvinVoigtCoatingProcess)
+    assert isinstance(K._create_coating_strategy(coating_stiffness=0.0), K.NoCoatingProcess)
+    u_ref = np.zeros(ndof)
+    for step, (u, dt) in enumerate(((st.u, coat_dt), (1.05 * st.u, 0.5 * coat_dt))):
+        x = m.node_coords + u.reshape(-1, 6)[:, :3]
+        geo.process(K.UpdateGeometryInput(mgr, x))
+        op = ocon.update_geometry(op, x, ocfg)
+        o, key = order(ctx.get_pairs())
+        okey = np.stack([np.minimum(op.elem_a, op.elem_b), np.maximum(op.elem_a, op.elem_b)], 1)
+        oo = np.lexsort((okey[:, 1], okey[:, 0]))
+        assert np.array_equal(key[o], okey[oo])
+        assert np.array_equal(ctx.get_pairs()["status"][o], op.status[oo])
+        coat = ctx.get_coating()
+        cc = op.coating_compression
+        assert (cc > 0).sum() > 0
+        assert np.abs(coat["coating_compression"][o] - cc[oo]).max() <= FP64_TOL * cc.max()
+        f = strat.forces(mgr.pairs, x, cfg, dt)
+        assert rel(f, ocoat.forces(op, ocfg, dt, ndof)) < FP64_TOL
+        mk, bk = ocoat.stiffness_blocks(op, ocfg, dt)
+        assert srel(strat.stiffness(mgr.pairs, x, cfg, ndof, dt), ocon.blocks_to_csr(op, mk, bk, ndof)) < FP64_TOL
+        ff = strat.friction_forces(mgr.pairs, x, cfg, u, u_ref)
+        off, op = ocoat.friction_forces(op, ocfg, u, u_ref, ndof)
+        assert rel(ff, off) < FP64_TOL
+        mf, bf = ocoat.friction_stiffness_blocks(op, ocfg)
+        assert srel(strat.friction_stiffness(mgr.pairs, x, cfg, ndof), ocon.blocks_to_csr(op, mf, bf, ndof)) < FP64_TOL
+        coat = ctx.get_coating()
+        zr = op.coating_z_t[oo]
+        assert np.abs(coat["coating_z_t"][o] - zr).max() <= FP64_TOL * np.abs(zr).max()
+        assert np.array_equal(coat["coating_stick"][o].astype(bool), op.coating_stick[oo])
+        print(f"step {step}: pairs {len(cc)} compressed {int((cc > 0).sum())} slipping "
+              f"{int((~op.coating_stick & (cc > 0)).sum())}")
+        ctx.coating_commit()
+        op = ocoat.commit(op)
+        u_ref = u
+
+
 def test_pair_pressing_within_the_increment_gets_a_slot(gpu):
     """Active set frozen at the rest configuration (no pair in contact, no
     compression), assembly at the perturbed u where pairs overlap: the

@@ -21,6 +21,7 @@ c_int8_p = C.POINTER(C.c_int8)
 vp = C.c_void_p
 
 PAIR_GEOM, PAIR_HYST, PAIR_HUBER, PAIR_FRIC, PAIR_TAN_C, PAIR_TAN_F = 1, 2, 4, 8, 16, 32
+PAIR_COAT_F, PAIR_COAT_FRIC, PAIR_COAT_K, PAIR_COAT_KF = 128, 256, 512, 1024
 PREC = {"fp64": 0, "mixed": 1}
 OPT_STAGED = 256
 OPT_HOST_PATTERN = 512

@@ -10,6 +10,8 @@ Mirrors the reference's interfaces:
     behind ``ContactForceStrategy`` (core/strategies/protocols.py:16-45)
   - ``CoulombReturnMappingProcess`` (contact/friction/strategy.py:50-168)
     behind ``FrictionStrategy`` (protocols.py:48-78)
+  - ``NoCoatingProcess`` / ``KelvinVoigtCoatingProcess`` and
+    ``_create_coating_strategy`` (contact/coating/strategy.py:19-408)
 
 The pair state lives on the GPU (structure of arrays in the device context).
 ``manager.pairs`` materialises read-only views on demand; the strategies never
@@ -220,6 +222,10 @@ class DetectCandidatesProcess:
             r = np.full(ctx.n_elems, float(inp.radii)) if np.isscalar(inp.radii) else np.asarray(inp.radii, float)
             if not np.array_equal(r, ctx.radii_np):
                 raise ValueError("radii differ from the device context's")
+        if cfg.coating_stiffness > 0.0 or ctx.coat_on or inp.core_radii is not None:
+            # core radii of the pairs created here (None: the contact radii, :184-189)
+            ctx.ensure_coating(cfg.coating_stiffness, cfg.coating_damping, cfg.coating_mu, cfg.coating_k_t_ratio,
+                               inp.core_radii)
         coords = ctx.to_dev(inp.node_coords)
         layer = layer_array(cfg, ctx.n_elems)
         ctx.detect_candidates(coords, margin=inp.margin, cell_size=inp.cell_size, elem_layer=layer,
@@ -250,9 +256,10 @@ class UpdateGeometryProcess:
     def process(self, inp: UpdateGeometryInput) -> UpdateGeometryOutput:
         m = inp.manager
         cfg = m.config
-        if cfg.coating_stiffness > 0.0:
-            raise NotImplementedError("coating terms are outside this path")
         ctx = m.ctx
+        if cfg.coating_stiffness > 0.0 or ctx.coat_on:
+            # with the coating on: core gap, compression, activation by compression (:351-360, :486-494)
+            ctx.ensure_coating(cfg.coating_stiffness, cfg.coating_damping, cfg.coating_mu, cfg.coating_k_t_ratio)
         coords = ctx.to_dev(inp.node_coords)
         modes = L.PAIR_GEOM | (0 if inp.freeze_active_set else L.PAIR_HYST)
         ctx.pairs_run(modes, coords=coords, g_on=cfg.g_on, g_off=cfg.g_off,
@@ -417,3 +424,101 @@ class CoulombReturnMappingProcess:
         K = ctx.new_K()
         ctx.pairs_gather(K_vals=K)
         return ctx.to_csr(K)
+
+
+def _coating_params(config):
+    return (config.coating_stiffness, config.coating_damping, config.coating_mu, config.coating_k_t_ratio)
+
+
+class NoCoatingProcess:
+    """``NoCoatingProcess`` (contact/coating/strategy.py:19-77): every term is zero."""
+
+    def forces(self, pairs, node_coords, config, dt: float) -> np.ndarray:
+        return np.zeros(len(node_coords) * getattr(config, "ndof_per_node", 6))
+
+    def stiffness(self, pairs, node_coords, config, ndof_total: int, dt: float) -> sp.csr_matrix:
+        return sp.csr_matrix((ndof_total, ndof_total))
+
+    def friction_forces(self, pairs, node_coords, config, u_cur, u_ref) -> np.ndarray:
+        return np.zeros_like(u_cur)
+
+    def friction_stiffness(self, pairs, node_coords, config, ndof_total: int) -> sp.csr_matrix:
+        return sp.csr_matrix((ndof_total, ndof_total))
+
+    def process(self, input_data) -> None:
+        return None
+
+
+class KelvinVoigtCoatingProcess:
+    """``KelvinVoigtCoatingProcess`` (contact/coating/strategy.py:80-383): the
+    coating terms of the pairs' current state (s, t, normal, frame and
+    compression as the last UpdateGeometry left them), each method one pair
+    kernel launch with its coating mode and one gather.  ``pairs`` is the
+    device manager's pair list (or the manager); ``config`` carries the
+    coating parameters; ``node_coords`` only sizes the result, as in the
+    reference."""
+
+    def _ctx(self, pairs, config, ndof=None):
+        m = CoulombReturnMappingProcess._manager_of(pairs)
+        ctx = m.ctx
+        if config.coating_stiffness <= 0.0:
+            raise ValueError("KelvinVoigtCoatingProcess needs coating_stiffness > 0 (_create_coating_strategy)")
+        if ndof is not None and int(ndof) != ctx.ndof:
+            raise ValueError("ndof_total differs from the device context's")
+        ctx.ensure_coating(*_coating_params(config))
+        return ctx
+
+    def _force(self, ctx, modes, u=None, u_ref=None) -> np.ndarray:
+        f = ctx.torch.empty(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev)
+        if ctx.n_pairs == 0:
+            return np.zeros(ctx.ndof)
+        ctx.pairs_run(modes, u=None if u is None else ctx.to_dev(u), u_ref=None if u_ref is None else ctx.to_dev(u_ref))
+        ctx.pairs_gather(f=f)
+        return f.cpu().numpy()
+
+    def _tangent(self, ctx, modes) -> sp.csr_matrix:
+        if ctx.n_pairs == 0:
+            return sp.csr_matrix((ctx.ndof, ctx.ndof))
+        ctx.pairs_run(modes)
+        K = ctx.new_K()
+        ctx.pairs_gather(K_vals=K)
+        return ctx.to_csr(K)
+
+    def forces(self, pairs, node_coords, config, dt: float) -> np.ndarray:
+        """Kelvin-Voigt normal force k cc + c (cc - cc_prev) / dt of every pair
+        with cc > 0 or cc_prev > 0, -f_n n on A and +f_n n on B (:97-144)."""
+        ctx = self._ctx(pairs, config)
+        ctx.coating_set_dt(dt)
+        return self._force(ctx, L.PAIR_COAT_F)
+
+    def stiffness(self, pairs, node_coords, config, ndof_total: int, dt: float) -> sp.csr_matrix:
+        """k_eff g_n g_n^T with k_eff = k + c / dt over the pairs with cc > 1e-15 (:146-204)."""
+        ctx = self._ctx(pairs, config, ndof_total)
+        ctx.coating_set_dt(dt)
+        return self._tangent(ctx, L.PAIR_COAT_K)
+
+    def friction_forces(self, pairs, node_coords, config, u_cur, u_ref) -> np.ndarray:
+        """Coulomb return mapping of the coating history (k_t = k k_t_ratio,
+        p_n = k cc) and its force; the history of separated pairs is reset (:206-297)."""
+        if config.coating_mu <= 0.0:
+            return np.zeros_like(u_cur)
+        ctx = self._ctx(pairs, config)
+        return self._force(ctx, L.PAIR_COAT_FRIC, u=u_cur, u_ref=u_ref)
+
+    def friction_stiffness(self, pairs, node_coords, config, ndof_total: int) -> sp.csr_matrix:
+        """(w_k w_l) T D T^T over the pairs with cc > 0, from the history the
+        last friction_forces left (:299-379)."""
+        if config.coating_mu <= 0.0:
+            return sp.csr_matrix((ndof_total, ndof_total))
+        ctx = self._ctx(pairs, config, ndof_total)
+        return self._tangent(ctx, L.PAIR_COAT_KF)
+
+    def process(self, input_data) -> None:
+        return None
+
+
+def _create_coating_strategy(*, coating_stiffness: float = 0.0):
+    """Kelvin-Voigt when the stiffness is > 0, none otherwise (strategy.py:386-408)."""
+    if coating_stiffness <= 0.0:
+        return NoCoatingProcess()
+    return KelvinVoigtCoatingProcess()

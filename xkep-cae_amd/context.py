@@ -95,6 +95,8 @@ class DeviceContext:
         self.X = torch.from_numpy(X).to(self.dev)
         self.kdtype = torch.float64 if precision == "fp64" else torch.float32
         self._pattern_epoch = 0
+        self.coat_on = False
+        self.coat_key = None
         self._pattern_cache = None
 
     # ------------------------------------------------------------ plumbing
@@ -218,7 +220,27 @@ class DeviceContext:
             cr = np.ascontiguousarray(np.broadcast_to(np.asarray(core_radii, float), (self.n_elems,)))
         L.check(L.lib().xk_ctx_set_coating(self.h, float(stiffness), float(damping), float(mu), float(k_t_ratio),
                                            None if cr is None else cr.ctypes.data), "xk_ctx_set_coating")
+        self.coat_on = stiffness > 0.0
+        self.coat_key = (float(stiffness), float(damping), float(mu), float(k_t_ratio),
+                         None if cr is None else cr.tobytes())
+        self._core_radii = cr
         self.invalidate_pattern()
+
+    KEEP = object()
+
+    def ensure_coating(self, stiffness: float, damping: float = 0.0, mu: float = 0.0, k_t_ratio: float = 0.5,
+                       core_radii=KEEP):
+        """``set_coating`` unless the context holds these parameters already;
+        ``core_radii=KEEP`` keeps the core radii set before (the contact radii
+        if none were)."""
+        if core_radii is self.KEEP:
+            cr = getattr(self, "_core_radii", None)
+        else:
+            cr = None if core_radii is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(core_radii, float), (self.n_elems,)))
+        key = (float(stiffness), float(damping), float(mu), float(k_t_ratio), None if cr is None else cr.tobytes())
+        if key != self.coat_key:
+            self.set_coating(stiffness, damping, mu, k_t_ratio, cr)
 
     def coating_set_dt(self, coat_dt: float):
         L.check(L.lib().xk_coating_set_dt(self.h, float(coat_dt)), "xk_coating_set_dt")
@@ -292,7 +314,7 @@ class DeviceContext:
         L.check(L.lib().xk_pairs_run(self.h, modes, L.ptr(coords), L.ptr(u), L.ptr(u_ref), float(k_pen),
                                      float(smoothing_delta), float(k_t_ratio), float(mu_eff), float(g_on),
                                      float(g_off), int(bool(allow_deactivation)), self.stream), "xk_pairs_run")
-        if modes & L.PAIR_HYST:
+        if (modes & L.PAIR_HYST) or (self.coat_on and modes & L.PAIR_GEOM):
             self.invalidate_pattern()
 
     def pairs_gather(self, f=None, K_vals=None):

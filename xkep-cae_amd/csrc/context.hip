@@ -728,7 +728,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
                  int allow_deactivation, void* stream) {
     if (!c) return -1;
     if (c->lp_sync((hipStream_t)stream, false) != 0) return -1;
-    const int phys = PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F;
+    const int phys = PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F | PAIR_COAT;
     if (modes & PAIR_PROBE) {
         set_error("xk_pairs_run: the compression probe is xk_coating_probe");
         return -1;
@@ -745,8 +745,12 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
         set_error("xk_pairs_run: geometry needs coordinates");
         return -1;
     }
-    if ((modes & PAIR_FRIC) && !u) {
+    if ((modes & (PAIR_FRIC | PAIR_COAT_FRIC)) && !u) {
         set_error("xk_pairs_run: friction needs u");
+        return -1;
+    }
+    if ((modes & PAIR_COAT) && !c->coat_on) {
+        set_error("xk_pairs_run: the coating modes need the coating on (xk_ctx_set_coating)");
         return -1;
     }
     if ((modes & phys) && c->ensure_pattern() != 0) return -1;
@@ -764,7 +768,9 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     c->coat_params(pp);
     XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_sd.p,
                         c->d_sg.p, nullptr, (hipStream_t)stream, nullptr, c->pair_block));
-    if (modes & PAIR_HYST) c->pattern_valid = false;
+    // a new compression can give an INACTIVE pair a coating contribution: the
+    // pattern is rebuilt after every geometry update with the coating on
+    if ((modes & PAIR_HYST) || (c->coat_on && (modes & PAIR_GEOM))) c->pattern_valid = false;
     c->last_modes = modes;
     return 0;
 }
@@ -882,7 +888,8 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         set_error("xk_assemble_iteration: the learned prescreen is not available with the coating on");
         return -1;
     }
-    const int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    int modes = PAIR_GEOM | PAIR_HUBER | PAIR_FRIC | (K_vals ? (PAIR_TAN_C | PAIR_TAN_F) : 0);
+    if (c->coat_on) modes |= PAIR_COAT_F | PAIR_COAT_FRIC | (K_vals ? (PAIR_COAT_K | PAIR_COAT_KF) : 0);
     c->tangent_staged = K_vals != nullptr;   // the pair kernel stages every contributing pair's tangent record
     // with the prescreen on, the pair kernel visits the watch list (every
     // contributing pair and the ones the network keeps), compacted

@@ -17,10 +17,11 @@
 //                 (contact_force/strategy.py:172-254, friction/_assembly.py:108-257)
 //   coating       with the Kelvin-Voigt coating on (pp.coat_on): core gap and
 //                 compression cc = max(0, coat_total - gap_core), activation by
-//                 cc > 0 (_manager_process.py:351-360, :486-494), the force
-//                 k cc + c (cc - cc_prev) / dt along -n, the coating Coulomb
-//                 return mapping (k_t = k k_t_ratio, p_n = k cc), and the
-//                 stiffness k_eff n n^T + T D T^T (coating/strategy.py:99-370)
+//                 cc > 0 (_manager_process.py:351-360, :486-494); by mode bit
+//                 the force k cc + c (cc - cc_prev) / dt along -n (COAT_F), the
+//                 coating Coulomb return mapping (k_t = k k_t_ratio, p_n = k cc;
+//                 COAT_FRIC), the stiffness k_eff n n^T (COAT_K) and T D T^T
+//                 (COAT_KF) (coating/strategy.py:99-370)
 //   probe         (PAIR_PROBE, alone) the compression at x only: cneed[i] =
 //                 cc > 0; nothing else is written
 //
@@ -209,16 +210,17 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
 
         // coating: history reset of a separated pair (every pair, strategy.py:220-233);
         // a pair with a coating contribution but no slot of the pattern is reported
-        const bool coat_force = coat && (modes & PAIR_HUBER);
-        if (coat_force && pp.coat_mu > 0.0 && cc <= 0.0) {
+        const bool coat_f = coat && (modes & PAIR_COAT_F);
+        const bool coat_fr = coat && (modes & PAIR_COAT_FRIC);
+        if (coat_fr && pp.coat_mu > 0.0 && cc <= 0.0) {
             P.cz[i] = 0.0; P.cz[cap + i] = 0.0;
             P.cstick[i] = 1;
             P.cqtn[i] = 0.0;
             P.cdiss[i] = 0.0;
         }
-        if (coat && si < 0 && (modes & (PAIR_HUBER | PAIR_TAN_C)) && (cc > 0.0 || P.ccp[i] > 0.0))
+        if (coat && si < 0 && (modes & PAIR_COAT) && (cc > 0.0 || P.ccp[i] > 0.0))
             atomicOr(pp.flags, FLAG_COAT_UNPLACED);
-        if (si >= 0 && (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F))) {
+        if (si >= 0 && (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_TAN_C | PAIR_TAN_F | PAIR_COAT))) {
             const V3<double> t2 = cross(n, t1);
             double p_n = P.p_n[i];
             V3<double> F{0.0, 0.0, 0.0};
@@ -266,7 +268,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                     F = F + (f1 * t1 + f2 * t2);
                 }
             }
-            if (coat_force) {
+            if (coat_f) {
                 // Kelvin-Voigt normal force (strategy.py:99-144): node forces w_k f_n n = c_k (-f_n n)
                 const double ccp = P.ccp[i];
                 if (!(cc <= 0.0 && ccp <= 0.0)) {
@@ -274,6 +276,8 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                     if (pp.coat_c > 0.0 && pp.coat_dt > 0.0) f_n = f_n + pp.coat_c * ((cc - ccp) / pp.coat_dt);
                     if (!(fabs(f_n) < 1e-30)) F = F - f_n * n;
                 }
+            }
+            if (coat_fr) {
                 // coating friction (strategy.py:194-286): return mapping of the coating history
                 if (pp.coat_mu > 0.0 && cc > 0.0) {
                     const double pnc = pp.coat_k * cc, kt = pp.coat_kt;
@@ -297,7 +301,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                     F = F - (q1 * t1 + q2 * t2);
                 }
             }
-            if (modes & (PAIR_HUBER | PAIR_FRIC)) {
+            if (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_COAT_F | PAIR_COAT_FRIC)) {
                 double* rec = sd + SD_REC * (size_t)si;
                 rec[0] = 1.0 - s;
                 rec[1] = s;
@@ -307,7 +311,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 rec[5] = F.y;
                 rec[6] = F.z;
             }
-            if (modes & (PAIR_TAN_C | PAIR_TAN_F)) {
+            if (modes & (PAIR_TAN_C | PAIR_TAN_F | PAIR_COAT_K | PAIR_COAT_KF)) {
                 double G[9];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) G[k] = 0.0;
@@ -382,9 +386,9 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                         for (int k = 0; k < 9; ++k) G[k] += sgn * H[k];
                     }
                 }
-                if (coat && (modes & PAIR_TAN_C)) {
+                if (coat) {
                     // coating stiffness k_eff n n^T (strategy.py:146-192), pairs with cc > 1e-15
-                    if (cc > 1e-15) {
+                    if ((modes & PAIR_COAT_K) && cc > 1e-15) {
                         double k_eff = pp.coat_k;
                         if (pp.coat_c > 0.0 && pp.coat_dt > 0.0) k_eff += pp.coat_c / pp.coat_dt;
 #pragma unroll
@@ -393,7 +397,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                             for (int b = 0; b < 3; ++b) G[3 * a + b] += k_eff * (nv[a] * nv[b]);
                     }
                     // coating friction stiffness T D T^T (strategy.py:288-370), pairs with cc > 0
-                    if (pp.coat_mu > 0.0 && cc > 0.0) {
+                    if ((modes & PAIR_COAT_KF) && pp.coat_mu > 0.0 && cc > 0.0) {
                         const double kt = pp.coat_kt, pnc = pp.coat_k * cc;
                         const double q1 = P.cz[i], q2 = P.cz[cap + i];
                         const double zn = sqrt(q1 * q1 + q2 * q2), qtn = P.cqtn[i];

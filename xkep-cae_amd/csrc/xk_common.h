@@ -123,6 +123,11 @@ constexpr int PAIR_FRIC = 8;
 constexpr int PAIR_TAN_C = 16;
 constexpr int PAIR_TAN_F = 32;
 constexpr int PAIR_PROBE = 64;
+constexpr int PAIR_COAT_F = 128;      // coating normal force (Kelvin-Voigt)
+constexpr int PAIR_COAT_FRIC = 256;   // coating Coulomb return mapping and force
+constexpr int PAIR_COAT_K = 512;      // coating normal stiffness
+constexpr int PAIR_COAT_KF = 1024;    // coating friction stiffness
+constexpr int PAIR_COAT = PAIR_COAT_F | PAIR_COAT_FRIC | PAIR_COAT_K | PAIR_COAT_KF;
 
 // ContactStatus codes (xkep_cae/contact/_types.py:8-11).
 constexpr int8_t ST_INACTIVE = 0;
