@@ -38,9 +38,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(case, states, precision, torch, batch=None, mu_elem=None):
+def setup(case, states, precision, torch, batch=None, mu_elem=None, coating=None):
     """Context + assembly for one instance, a concatenated batch of instances
-    of one case, or a prepared uneven batch (per-element friction)."""
+    of one case, or a prepared uneven batch (per-element friction); ``coating``
+    (config fields, core radius fraction, coat_dt) switches the Kelvin-Voigt
+    coating terms on."""
     wl = importlib.import_module("xkep-cae_amd.workloads")
     C = importlib.import_module("xkep-cae_amd.context")
     A = importlib.import_module("xkep-cae_amd.assembly")
@@ -61,7 +63,8 @@ def setup(case, states, precision, torch, batch=None, mu_elem=None):
         X, conn, radii, u, f_ext, fixed = b.node_coords, b.connectivity, b.radii, b.u, b.f_ext, b.fixed_dofs
         layer = b.elem_layer
     mu = case.mu if mu_elem is None else 1.0
-    cfg = K.ContactConfig(mu=mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer))
+    coat_kw, core, coat_dt = ({}, None, 0.0) if coating is None else coating
+    cfg = K.ContactConfig(mu=mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer), **coat_kw)
     sec = C.circle_section(case.wire_radius, case.E, case.nu)
     ctx = C.DeviceContext(X, conn, sec, radii=radii, precision=precision)
     if mu_elem is not None:
@@ -72,7 +75,10 @@ def setup(case, states, precision, torch, batch=None, mu_elem=None):
     ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
     ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
     asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=mu,
-                           f_ext=f_ext, fixed_dofs=fixed, config=cfg, time=ga, dt=dt)
+                           f_ext=f_ext, fixed_dofs=fixed, config=cfg, time=ga, dt=dt,
+                           core_radii=None if core is None else core * radii)
+    if coating is not None:
+        asm.set_coat_dt(coat_dt)
     asm.bench_layer = layer
     asm.begin_increment(np.zeros(ctx.ndof))   # set-up broadphase on the reference configuration
     torch.cuda.synchronize()
@@ -219,6 +225,41 @@ def run(label, case, states, prec, torch, steps, warmup, tune=False, batch=None,
     return out
 
 
+def coated_assembly(case, states, torch, calls=50):
+    """The Kelvin-Voigt coating terms on (stiffness 0.1 k_pen, dashpot
+    0.5 k_coat coat_dt, coating mu 0.2, core radii 0.9 of the contact radii)
+    on the headline batch, FP32-mixed.  A coated assembly probes the
+    compression first (a launch and a host read) and is not graph-captured,
+    so both forms are timed eagerly, each call synchronised."""
+    out = {}
+    k = 0.1 * case.k_pen
+    coat_dt = 0.05
+    coat = (dict(coating_stiffness=k, coating_damping=0.5 * k * coat_dt, coating_mu=0.2, coating_k_t_ratio=0.5),
+            0.9, coat_dt)
+    for label, c in (("plain", None), ("coated", coat)):
+        ctx, asm, _ = setup(case, states, "mixed", torch, coating=c)
+        asm.assemble()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(calls):
+            t0 = time.perf_counter()
+            asm.assemble()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        rec = {"us_per_assembly_eager": round(1e6 * float(np.median(t)), 1)}
+        if c is not None:
+            cc = ctx.get_coating()["coating_compression"]
+            rec.update(compressed_pairs=int((cc > 0).sum()), pattern_rebuilds=int(asm.n_coat_rebuilds))
+        out[label] = rec
+        log(f"[bench] coated assembly x{len(states)} {label}: {rec}")
+        del ctx, asm
+        torch.cuda.empty_cache()
+    out["coating_overhead"] = round(out["coated"]["us_per_assembly_eager"] / out["plain"]["us_per_assembly_eager"], 3)
+    out["workload"] = (f"{len(states)} x 91-wire batch, FP32-mixed, Kelvin-Voigt coating (k 0.1 k_pen, "
+                       "dashpot, coating mu 0.2, core 0.9 r) against the same batch without it, eager calls")
+    return out
+
+
 def newton_attempt(case, torch, wl, reps=5):
     """One Newton attempt at 91 wires: the fused assembly plus the device
     linear solve (xk_linear_solve, refined to 1e-11) against the same assembly
@@ -337,6 +378,7 @@ def side_measurements(batched, torch, wl, args):
                          "increment_setup_ms": round(r5["t_inc"] * 1e3, 1)}
     del r5
     torch.cuda.empty_cache()
+    out["coated_assembly"] = coated_assembly(wl.C3, batched["states"], torch)
     out["newton_attempt_91wire"] = newton_attempt(wl.C3, torch, wl)
     out["newton_loop_91wire"] = newton_loop(torch, wl)
     return out
@@ -390,6 +432,7 @@ def main():
     states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
     batched = run(f"mixed x{args.batch}", case, states, "mixed", torch, args.steps, args.warmup,
                   tune=not args.no_autotune)
+    batched["states"] = states
     # the FP64 plain form on the same batch (same tuned shapes through the cache)
     plain_b = run(f"fp64 x{args.batch}", case, states, "fp64", torch, args.steps, args.warmup,
                   tune=not args.no_autotune)
