@@ -193,3 +193,21 @@ def test_geometry_branch_and_activation():
     assert p2.coating_compression[0] == 0.0 and p2.status[0] == ocon.INACTIVE      # released and gap >= g_off
     p3 = ocon.update_geometry(p, X, ocon.ContactConfig(g_on=0.0, g_off=1e-6))     # coating off: plain gap
     np.testing.assert_allclose(p3.gap, 0.55 - 0.6, rtol=1e-13)
+
+
+def test_no_coating_strategy_returns_zeros():
+    """_create_coating_strategy picks NoCoatingProcess at zero stiffness; its
+    terms are zeros of the reference's shapes (coating/strategy.py:19-77)."""
+    from conftest import pkg
+    K = pkg("contact")
+    nc = K._create_coating_strategy(coating_stiffness=0.0)
+    assert isinstance(nc, K.NoCoatingProcess)
+    assert isinstance(K._create_coating_strategy(coating_stiffness=1.0), K.KelvinVoigtCoatingProcess)
+    cfg = K.ContactConfig()
+    x = np.zeros((5, 3))
+    assert np.array_equal(nc.forces([], x, cfg, 0.1), np.zeros(30))
+    assert nc.stiffness([], x, cfg, 30, 0.1).shape == (30, 30) and nc.stiffness([], x, cfg, 30, 0.1).nnz == 0
+    u = np.ones(30)
+    assert np.array_equal(nc.friction_forces([], x, cfg, u, u), np.zeros(30))
+    assert nc.friction_stiffness([], x, cfg, 30).nnz == 0
+    assert nc.process(None) is None

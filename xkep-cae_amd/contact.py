@@ -469,9 +469,9 @@ class KelvinVoigtCoatingProcess:
         return ctx
 
     def _force(self, ctx, modes, u=None, u_ref=None) -> np.ndarray:
-        f = ctx.torch.empty(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev)
         if ctx.n_pairs == 0:
             return np.zeros(ctx.ndof)
+        f = ctx.torch.empty(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev)
         ctx.pairs_run(modes, u=None if u is None else ctx.to_dev(u), u_ref=None if u_ref is None else ctx.to_dev(u_ref))
         ctx.pairs_gather(f=f)
         return f.cpu().numpy()
