@@ -202,6 +202,74 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     assert srel(beam.assemble_tangent(st.u), Kr) < FP64_TOL
 
 
+def _rot(v, axis, ang):
+    """Rodrigues rotation of v about the unit axis."""
+    axis = axis / np.linalg.norm(axis)
+    return v * np.cos(ang) + np.cross(axis, v) * np.sin(ang) + axis * (axis @ v) * (1 - np.cos(ang))
+
+
+def test_narrowphase_and_frame_branches_match_oracle(gpu):
+    """Hand-built segment pairs through every branch of the narrowphase
+    (_compute.py:55-151): crossing, parallel overlapping and disjoint, s
+    clamped, t clamped then s recomputed, endpoint to endpoint, collinear,
+    zero-length A and B, intersecting (zero distance: fallback normal),
+    near-parallel.  Each geometry meets five previous frames (_compute.py:154-261):
+    none, a rotated previous normal (transport), the same normal (no
+    rotation), a previous tangent without a normal, and a previous tangent
+    along the new normal (the axis fallback).  Statuses alternate, so the
+    hysteresis switches both ways.  FP64 s, t, gap, normal, tangent1 and
+    status against the oracle's update_geometry."""
+    C = pkg("context")
+    L = pkg("_lib")
+    geoms = [
+        ((-1, 0, 0), (1, 0, 0), (0, -1, 0.3), (0, 1, 0.3)),          # crossing
+        ((0, 0, 0), (1, 0, 0), (0.2, 0.3, 0), (1.2, 0.3, 0)),         # parallel, overlapping
+        ((0, 0, 0), (1, 0, 0), (2, 0.3, 0), (3, 0.3, 0)),             # parallel, disjoint
+        ((0, 0, 0), (1, 0, 0), (-0.5, -1, 0.2), (-0.4, 1, 0.25)),     # s clamped
+        ((0, -1, 0), (0.1, 1, 0), (-2, 0.2, 0.3), (-1, 0.25, 0.3)),   # t clamped, s recomputed
+        ((0, 0, 0), (1, 0, 0), (1.5, 0.5, 0.1), (2.5, 1.5, 0.1)),     # endpoint to endpoint
+        ((0, 0, 0), (1, 0, 0), (0.5, 0, 0), (1.5, 0, 0)),             # collinear (parallel branch: s = 0)
+        ((0, 0, 0), (1, 0, 0), (0.3, 0.2, 0.1), (0.3, 0.2, 0.1)),     # zero-length B
+        ((0.5, 0.5, 0.5), (0.5, 0.5, 0.5), (0, 0, 0), (1, 0, 0)),     # zero-length A
+        ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0)),               # intersecting
+        ((0, 0, 0), (1, 0, 0), (0, 0.3, 0), (1, 0.3, 1e-4)),          # near-parallel
+    ]
+    X = np.array([pt for g in geoms for pt in g], float)
+    conn = np.arange(len(X)).reshape(-1, 2)
+    r = np.full(len(conn), 0.1)
+    ocfg = ocon.ContactConfig()
+    ea0 = 2 * np.arange(len(geoms))
+    eb0 = ea0 + 1
+    # the new normal of each geometry (fresh pairs), to build the previous frames from
+    n_new = ocon.update_geometry(ocon.Pairs.fresh(ea0, eb0, conn, r, r), X, ocfg).normal
+    ea, eb, tp, npv = [], [], [], []
+    for k in range(len(geoms)):
+        n = n_new[k]
+        perp = np.cross(n, [0.3, 0.5, 0.8])
+        perp /= np.linalg.norm(perp)
+        n_rot = _rot(n, perp + 0.5 * np.cross(n, perp), 0.35)
+        t_rot = np.cross(n_rot, [0.6, -0.2, 0.7])
+        t_rot /= np.linalg.norm(t_rot)
+        for t1, nn in ((np.zeros(3), np.zeros(3)), (t_rot, n_rot), (perp, n), (perp, np.zeros(3)), (n, n)):
+            ea.append(ea0[k]); eb.append(eb0[k]); tp.append(t1); npv.append(nn)
+    ea, eb, tp, npv = np.array(ea), np.array(eb), np.array(tp), np.array(npv)
+    status = (np.arange(len(ea)) % 2).astype(np.int8)
+    op = ocon.Pairs.fresh(ea, eb, conn, r, r)
+    op.tangent1, op.normal, op.status = tp.copy(), npv.copy(), status.astype(np.int64)
+    op = ocon.update_geometry(op, X, ocfg)
+    ctx = C.DeviceContext(X, conn, section(0.1), radii=r)
+    ctx.set_pairs(ea, eb, state={"tangent1": tp, "normal": npv, "status": status})
+    ctx.pairs_run(L.PAIR_GEOM | L.PAIR_HYST, g_on=ocfg.g_on, g_off=ocfg.g_off)
+    d = ctx.get_pairs()
+    for name in ("s", "t", "gap", "normal", "tangent1"):
+        np.testing.assert_allclose(d[name], getattr(op, name), rtol=0, atol=1e-12, err_msg=name)
+    assert np.array_equal(d["status"], op.status)
+    # the branches were met: zero-distance fallback normal, clamps at both ends, both statuses
+    assert np.all(d["normal"][9 * 5:10 * 5] == [0.0, 0.0, 1.0])
+    assert {0.0, 1.0} <= set(np.round(d["s"], 12)) and {0.0, 1.0} <= set(np.round(d["t"], 12))
+    assert set(d["status"].tolist()) == {0, 1}
+
+
 def test_non_chain_connectivity_takes_the_staged_path(gpu):
     """Reversed and reordered elements: the library detects that the mesh is
     not made of element chains and assembles through the staged path."""
