@@ -151,9 +151,24 @@ def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision, 
 def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     """Drop-in Processes / Strategies (DetectCandidates, UpdateGeometry, Huber,
     Coulomb evaluate + tangent) against the oracle, one call each."""
+    strategy_roundtrip("C2", staged)
+
+
+@pytest.mark.parametrize("sd,mu", [(0.0, 0.3), (0.0, 0.0), (None, 0.0)],
+                         ids=["plain_penalty", "plain_penalty_frictionless", "huber_frictionless"])
+def test_strategy_api_plain_penalty_and_frictionless(gpu, sd, mu):
+    """The same calls with smoothing_delta = 0 (p_n = max(0, k_pen (-gap)),
+    contact_force/strategy.py:126-170) and with mu = 0 (no friction force,
+    no friction tangent), on the 7-wire strand."""
+    strategy_roundtrip("C1", False, smoothing_delta=sd, mu=mu)
+
+
+def strategy_roundtrip(case_name, staged, smoothing_delta=None, mu=None):
     wl = pkg("workloads")
     C, K = pkg("context"), pkg("contact")
-    case = wl.C2
+    case = getattr(wl, case_name)
+    sd = case.smoothing_delta if smoothing_delta is None else smoothing_delta
+    mu = case.mu if mu is None else mu
     st = wl.make_state(case)
     m = st.mesh
     ctx = C.DeviceContext(m.node_coords, m.connectivity, section(m.wire_radius), radii=m.radii, staged=staged)
@@ -161,9 +176,9 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     s = section(m.wire_radius)
     beam = E.ULCRBeamAssembler(m.node_coords, m.connectivity, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y,
                                s.kappa_z, context=ctx)
-    cfg = K.ContactConfig(mu=case.mu)
+    cfg = K.ContactConfig(mu=mu)
     mgr = K.ContactManager(ctx, cfg)
-    ocfg = ocon.ContactConfig(mu=case.mu)
+    ocfg = ocon.ContactConfig(mu=mu)
     op, _ = ocon.detect_candidates(ocon.Pairs(), m.node_coords, m.connectivity, m.radii, ocfg)
     K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, m.node_coords, m.connectivity, m.radii))
     x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
@@ -176,23 +191,28 @@ def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     assert np.array_equal(d["elem_a"], op.elem_a) and np.array_equal(d["elem_b"], op.elem_b)
     assert np.array_equal(d["status"], op.status)
     ndof = ctx.ndof
-    hub = K.HuberContactForceProcess(ndof, smoothing_delta=case.smoothing_delta)
+    hub = K.HuberContactForceProcess(ndof, smoothing_delta=sd)
     fc, res = hub.evaluate(st.u, mgr, case.k_pen)
-    ofc, ores, op = ocon.huber_evaluate(op, case.k_pen, case.smoothing_delta, ndof)
+    ofc, ores, op = ocon.huber_evaluate(op, case.k_pen, sd, ndof)
     assert rel(fc, ofc) < FP64_TOL and rel(res, ores) < FP64_TOL
     fr = K.CoulombReturnMappingProcess(ndof, k_pen=case.k_pen, k_t_ratio=0.5)
-    ff, fres = fr.evaluate(st.u, mgr.pairs, case.mu, u_ref=st.u_ref)
-    ofr = ocon.friction_evaluate(op, st.u, st.u_ref, case.k_pen, 0.5, case.mu, ndof)
+    ff, fres = fr.evaluate(st.u, mgr.pairs, mu, u_ref=st.u_ref)
+    ofr = ocon.friction_evaluate(op, st.u, st.u_ref, case.k_pen, 0.5, mu, ndof)
     assert rel(ff, ofr.f_fric) < FP64_TOL
     # residual entries max(0, |q| - mu p_n) are rounding-level for slipping pairs:
     # compare on the scale of the friction bound mu p_n
     assert fres.shape == ofr.residual.shape
-    assert np.abs(fres - ofr.residual).max() <= 1e-12 * case.mu * ofr.pairs.p_n.max()
+    if fres.size:
+        assert np.abs(fres - ofr.residual).max() <= 1e-12 * mu * ofr.pairs.p_n.max()
     assert len(fr.friction_tangents) == int(ofr.mask.sum())
     Kc = hub.tangent(st.u, mgr, case.k_pen)
-    assert srel(Kc, ocon.huber_tangent(ofr.pairs, case.k_pen, case.smoothing_delta, ndof)) < FP64_TOL
-    Kf = fr.tangent(st.u, mgr.pairs, case.mu)
-    assert srel(Kf, ocon.friction_tangent(ofr.pairs, ofr, ndof)) < FP64_TOL
+    assert srel(Kc, ocon.huber_tangent(ofr.pairs, case.k_pen, sd, ndof)) < FP64_TOL
+    Kf = fr.tangent(st.u, mgr.pairs, mu)
+    oKf = ocon.friction_tangent(ofr.pairs, ofr, ndof)
+    if mu > 0.0:
+        assert srel(Kf, oKf) < FP64_TOL
+    else:
+        assert abs(Kf).sum() == 0.0 and abs(oKf).sum() == 0.0 and not np.any(ff)
     d = ctx.get_pairs()
     assert np.array_equal(d["status"], ofr.pairs.status)
     # beam callbacks sharing the context (pattern now holds contact blocks too)
