@@ -288,6 +288,17 @@ def test_narrowphase_and_frame_branches_match_oracle(gpu):
     assert np.all(d["normal"][9 * 5:10 * 5] == [0.0, 0.0, 1.0])
     assert {0.0, 1.0} <= set(np.round(d["s"], 12)) and {0.0, 1.0} <= set(np.round(d["t"], 12))
     assert set(d["status"].tolist()) == {0, 1}
+    # UpdateGeometryInput(allow_deactivation=False) and (freeze_active_set=True)
+    for kw, modes in (({"allow_deactivation": False}, L.PAIR_GEOM | L.PAIR_HYST), ({"freeze_active_set": True}, L.PAIR_GEOM)):
+        ref = ocon.Pairs.fresh(ea, eb, conn, r, r)
+        ref.tangent1, ref.normal, ref.status = tp.copy(), npv.copy(), status.astype(np.int64)
+        ref = ocon.update_geometry(ref, X, ocfg, **kw)
+        ctx.set_pairs(ea, eb, state={"tangent1": tp, "normal": npv, "status": status})
+        ctx.pairs_run(modes, g_on=ocfg.g_on, g_off=ocfg.g_off, allow_deactivation=kw.get("allow_deactivation", True))
+        d = ctx.get_pairs()
+        assert np.array_equal(d["status"], ref.status), kw
+        np.testing.assert_allclose(d["tangent1"], ref.tangent1, rtol=0, atol=1e-12)
+    assert np.array_equal(d["status"], status)      # frozen: statuses as set
 
 
 def test_non_chain_connectivity_takes_the_staged_path(gpu):
