@@ -262,13 +262,13 @@ int xk_lp_set_params(xk_ctx* ctx, const float* params_host, int n_params, float 
 int xk_lp_features(xk_ctx* ctx, const double* coords_ref, const double* u, const int32_t* elem_layer,
                    float layer_scale, float* features_out, void* stream);
 /* Inference + watch / prune lists of the current pattern; switches the fused
- * assembly to the watch list until the next xk_lp_judge / xk_lp_off, pattern
+ * assembly to the watch list until the next xk_lp_verify / xk_lp_off, pattern
  * rebuild or pair-list change.  Synchronous. */
 int xk_lp_select(xk_ctx* ctx, int64_t* n_watch, void* stream);
-/* The judge: exact narrowphase (geometry, active set frozen) over the pruned
+/* Verification pass: exact narrowphase (geometry, active set frozen) over the pruned
  * pairs at x = coords_ref + u, n_miss = #(pruned pairs with gap <= g_on);
  * switches the watch list off.  Synchronous. */
-int xk_lp_judge(xk_ctx* ctx, const double* coords_ref, const double* u, double g_on, int64_t* n_miss,
+int xk_lp_verify(xk_ctx* ctx, const double* coords_ref, const double* u, double g_on, int64_t* n_miss,
                 void* stream);
 int xk_lp_off(xk_ctx* ctx);
 /* Copy of the whole pair state on the device, and its restore (the pair list

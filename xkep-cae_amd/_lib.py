@@ -73,7 +73,7 @@ SIGNATURES = {
     "xk_lp_set_params": (C.c_int, [vp, vp, C.c_int, C.c_float]),
     "xk_lp_features": (C.c_int, [vp, vp, vp, vp, C.c_float, vp, vp]),
     "xk_lp_select": (C.c_int, [vp, c_int64_p, vp]),
-    "xk_lp_judge": (C.c_int, [vp, vp, vp, C.c_double, c_int64_p, vp]),
+    "xk_lp_verify": (C.c_int, [vp, vp, vp, C.c_double, c_int64_p, vp]),
     "xk_lp_off": (C.c_int, [vp]),
     "xk_ctx_set_coating": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp]),
     "xk_coating_set_dt": (C.c_int, [vp, C.c_double]),

@@ -147,12 +147,12 @@ class StrandAssembly:
         return self
 
     def end_call(self) -> int:
-        """After a Newton call: the judge (exact narrowphase over the pairs the
+        """After a Newton call: the verification pass (exact narrowphase over the pairs the
         prescreen left out, at the last assembled u) -> #misses; logs the
         training rows when logging."""
         miss = 0
         if self.n_watch is not None:
-            miss = self.ctx.lp_judge(self.u, self.cfg.g_on)
+            miss = self.ctx.lp_verify(self.u, self.cfg.g_on)
             self.n_watch = None
         if self.lp_log is not None and self._lp_feat is not None:
             self.lp_log.append((self._lp_feat, self.ctx.get_pairs()["gap"].copy()))

@@ -493,10 +493,10 @@ class DeviceContext:
         L.check(L.lib().xk_lp_select(self.h, C.byref(n), self.stream), "xk_lp_select")
         return n.value
 
-    def lp_judge(self, u, g_on: float = 0.0) -> int:
+    def lp_verify(self, u, g_on: float = 0.0) -> int:
         n = C.c_int64()
-        L.check(L.lib().xk_lp_judge(self.h, L.ptr(self.X), L.ptr(self.to_dev(u)), float(g_on), C.byref(n),
-                                    self.stream), "xk_lp_judge")
+        L.check(L.lib().xk_lp_verify(self.h, L.ptr(self.X), L.ptr(self.to_dev(u)), float(g_on), C.byref(n),
+                                    self.stream), "xk_lp_verify")
         return n.value
 
     def lp_off(self):

@@ -1063,7 +1063,7 @@ int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
     return 0;
 }
 
-int xk_lp_judge(xk_ctx* c, const double* coords_ref, const double* u, double g_on, int64_t* n_miss, void* stream) {
+int xk_lp_verify(xk_ctx* c, const double* coords_ref, const double* u, double g_on, int64_t* n_miss, void* stream) {
     if (!c || !coords_ref) return -1;
     hipStream_t st = (hipStream_t)stream;
     int64_t miss = 0;

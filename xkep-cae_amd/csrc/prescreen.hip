@@ -8,7 +8,7 @@
 // Inference runs on the matrix cores (v_mfma_f32_32x32x8_bf16: BF16 operands,
 // FP32 accumulation), one 32-pair tile per wave.  Pairs whose logit reaches
 // the threshold, and every contributing pair, form the watch list; the rest
-// are checked once per Newton call by the exact narrowphase (the judge).
+// are checked once per Newton call by the exact narrowphase (the verification pass).
 //
 // Training runs on the GPU too: a per-sample forward / backward on the vector
 // ALUs with BF16-rounded weights and inputs, per-workgroup gradient partials

@@ -4,7 +4,7 @@ No counterpart in the reference.  An 8-64-64-1 ReLU network scores every
 candidate pair at the start of an increment from features of the
 increment-start geometry (``DeviceContext.lp_features``); the fused assembly
 then runs its pair kernel only over the pairs the network keeps plus every
-contributing pair, and the judge (``DeviceContext.lp_judge``, the exact
+contributing pair, and the verification pass (``DeviceContext.lp_verify``, the exact
 narrowphase over the rest at the end of the Newton call) counts the pruned
 pairs that ended within contact range.  A miss makes the driver re-run the
 call on the full path, so every pair that can enter contact keeps the full
@@ -12,7 +12,7 @@ path's state.
 
 Training data are logged from the method's own runs: features at increment
 start, label 1 for a non-contributing pair whose exact gap at the end of the
-Newton call is within contact range (the event the judge counts).  The
+Newton call is within contact range (the event the verification pass counts).  The
 forward pass, the gradient of the weighted logistic loss and Adam all run in
 the library's kernels (``xk_mlp_forward``, ``xk_mlp_grad``, ``xk_mlp_adam``);
 the threshold is then set below the lowest positive logit of the training
