@@ -226,6 +226,12 @@ def test_coating_strategy_api_matches_oracle(gpu, core_frac):
         ctx.coating_commit()
         op = ocoat.commit(op)
         u_ref = u
+    # coating mu = 0: no coating friction (strategy.py:217-219, :309-311), the normal terms unchanged
+    cfg0 = K.ContactConfig(mu=case.mu, **dict(kw, coating_mu=0.0))
+    assert not np.any(strat.friction_forces(mgr.pairs, x, cfg0, u, u_ref))
+    assert strat.friction_stiffness(mgr.pairs, x, cfg0, ndof).nnz == 0
+    ocfg0 = ocon.ContactConfig(mu=case.mu, **dict(kw, coating_mu=0.0))
+    assert rel(strat.forces(mgr.pairs, x, cfg0, dt), ocoat.forces(op, ocfg0, dt, ndof)) < FP64_TOL
 
 
 def test_pair_pressing_within_the_increment_gets_a_slot(gpu):
