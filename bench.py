@@ -396,6 +396,8 @@ def main():
                     help="skip the side measurements (uneven 64-instance batch, 1000 wires, learned prescreen)")
     ap.add_argument("--newton-only", action="store_true",
                     help="only the 91-wire Newton attempt: device linear solve vs host SuperLU")
+    ap.add_argument("--coated-only", action="store_true",
+                    help="only the coated-assembly comparison on the batch (Kelvin-Voigt coating on and off)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -425,6 +427,10 @@ def main():
                           "newton_loop_91wire": newton_loop(torch, wl)}), flush=True)
         return
     st = wl.make_state(case)
+    if args.coated_only:
+        states = [st] + [wl.make_state(case, seed=case.seed + 1000 * b) for b in range(1, args.batch)]
+        print(json.dumps({"coated_assembly": coated_assembly(case, states, torch)}), flush=True)
+        return
     plain = run("fp64 x1", case, [st], "fp64", torch, args.steps, args.warmup)
     single = run("mixed x1", case, [st], "mixed", torch, args.steps, args.warmup)
     del plain["asm"], plain["ctx"], single["asm"], single["ctx"]
