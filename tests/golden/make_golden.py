@@ -14,7 +14,8 @@ cutbacks, active counts, residual histories) on a small strand, which only
 running it can pin, and the strand generator's layouts (angle offsets, lay
 radii, numbering) over its branches.
 
-Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [element|assembly|newton|mesh ...]
+Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+        [element|assembly|newton|newton_friction|mesh ...]
 """
 
 from __future__ import annotations
@@ -216,14 +217,15 @@ def assembly_fixture():
     np.savez_compressed(os.path.join(HERE, "assembly_7wire.npz"), **out)
 
 
-def newton_case():
-    """Inputs of the Newton-trajectory fixture: 7-wire strand (r = 1, L = 20, 20
+def newton_case(mu=0.0, force=100.0):
+    """Inputs of the Newton-trajectory fixtures: 7-wire strand (r = 1, L = 20, 20
     elements per wire, lay angle 10 deg), core and outer layer in contact only
     (same-layer exclusion by layer index), first node of every wire clamped,
     radial inward force at the mid node of every outer wire, lumped mass,
-    Generalized-alpha rho_inf = 0.8, frictionless, penalty 0.5 c0 m_ii (the
-    dynamic estimate of contact/penalty/strategy.py:216-262 at the nominal
-    step), increments capped at a quarter of the total time."""
+    Generalized-alpha rho_inf = 0.8, friction coefficient mu (0 = frictionless),
+    penalty 0.5 c0 m_ii (the dynamic estimate of
+    contact/penalty/strategy.py:216-262 at the nominal step), increments capped
+    at a quarter of the total time."""
     nw, r, L, m, ang = 7, 1.0, 20.0, 20, 10.0
     mesh = mesh_mod.make_strand(nw, r, L, m, lay_angle_deg=ang)
     X, conn = mesh.node_coords, mesh.connectivity.astype(np.int64)
@@ -235,8 +237,8 @@ def newton_case():
     mid = first[1:] + nn // 2
     nrm = X[mid, :2] / np.linalg.norm(X[mid, :2], axis=1)[:, None]
     f_ext = np.zeros(ndof)
-    f_ext[mid * 6 + 0] = -100.0 * nrm[:, 0]
-    f_ext[mid * 6 + 1] = -100.0 * nrm[:, 1]
+    f_ext[mid * 6 + 0] = -force * nrm[:, 0]
+    f_ext[mid * 6 + 1] = -force * nrm[:, 1]
     # t_total = 4 T1/40 of one wire (first bending mode, numerical_tests/three_point_bend_jig.py:841-858)
     Iv = s["Iy"]
     f1 = (np.pi / (2.0 * L**2)) * np.sqrt(E * Iv / (RHO * s["A"]))
@@ -248,11 +250,11 @@ def newton_case():
     dt_nom = t_total * dt_max_fraction
     k_pen = 0.5 * (1.0 / (beta * dt_nom**2)) * RHO * s["A"] * (L / m) / 2.0
     return dict(mesh=mesh, X=X, conn=conn, s=s, fixed=fixed, f_ext=f_ext, t_total=t_total, rho_inf=rho_inf,
-                dt_max_fraction=dt_max_fraction, k_pen=k_pen, mu=0.0, smoothing_delta=5000.0 / r, tol_force=1e-6,
+                dt_max_fraction=dt_max_fraction, k_pen=k_pen, mu=mu, smoothing_delta=5000.0 / r, tol_force=1e-6,
                 tol_disp=1e-8, max_nr_attempts=30, radius=r, layer=mesh.elem_layer())
 
 
-def newton_fixture():
+def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz"):
     """The reference's own dynamic solver (ContactFrictionProcess) on newton_case():
     every Newton call of every increment attempt (cutbacks included) with its
     time window, attempt count, active count, convergence flags and residual
@@ -262,7 +264,7 @@ def newton_fixture():
     from xkep_cae.core.data import (AssembleCallbacks, BoundaryData, ContactFrictionInputData,
                                     ContactSetupData, MeshData)
 
-    c = newton_case()
+    c = newton_case(mu, force)
     s = c["s"]
     X, conn = c["X"], c["conn"]
     asm = ULCRBeamAssembler(X, conn, s["E"], s["G"], s["A"], s["Iy"], s["Iz"], s["J"], s["kappa_y"], s["kappa_z"])
@@ -311,7 +313,7 @@ def newton_fixture():
                active_history=np.concatenate([np.asarray(cl[8], np.int64) for cl in calls]) if n else np.zeros(0),
                converged=np.array(res.converged), n_increments=np.array(res.n_increments),
                n_cutbacks=np.array(res.n_cutbacks), u_final=res.u)
-    np.savez_compressed(os.path.join(HERE, "newton_7wire.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, name), **out)
 
 
 # strand generator configurations: (n_strands, wire_diameter, pitch, length, n_elems, keyword arguments)
@@ -347,8 +349,21 @@ def mesh_fixture():
     np.savez_compressed(os.path.join(HERE, "mesh_layouts.npz"), **out)
 
 
+def newton_friction_fixtures():
+    """The same solver and strand with Coulomb friction at mu = 0.3 (the
+    friction coefficient of the strand workloads), under two squeeze loads.
+    Once the friction loop engages, the reference's Newton does not converge
+    on this strand (its friction material block enters K_T with the opposite
+    sign of the consistent derivative), so these trajectories pin the
+    frictional attempts, the divergence detection, the cutbacks and the final
+    failure: load 60 gives 10 calls (4 cutbacks, friction engaged in 5 of
+    them), load 100 gives 5 calls (one divergence abort)."""
+    newton_fixture(0.3, 60.0, "newton_7wire_mu03_f60.npz")
+    newton_fixture(0.3, 100.0, "newton_7wire_mu03_f100.npz")
+
+
 FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture,
-            "mesh": mesh_fixture}
+            "newton_friction": newton_friction_fixtures, "mesh": mesh_fixture}
 
 if __name__ == "__main__":
     for name in sys.argv[1:] or list(FIXTURES):

@@ -1,8 +1,12 @@
 """The Newton driver (xkep-cae_amd/solver.py, a restatement of the reference's
 ContactFrictionProcess / NewtonDynamicProcess loop) against the trajectory of
-the reference's own solver (tests/golden/newton_7wire.npz, written by running
+the reference's own solver (tests/golden/newton_7wire*.npz, written by running
 it): every Newton call of every increment attempt, cutbacks included, must
-have the same time window, attempt count, convergence and active count.
+have the same time window, attempt count, convergence, divergence flag and
+active count.  Three trajectories: frictionless (15 calls, converges), and
+Coulomb friction mu = 0.3 under loads 60 (10 calls) and 100 (5 calls, a
+divergence abort), where the reference's own Newton fails once friction
+engages and the driver must fail the same way.
 
 On the CPU the driver runs over the oracle's FP64 assembly; on the GPU over
 the device assembly (FP64 and FP32-mixed tangents), with the host SuperLU
@@ -20,8 +24,11 @@ from oracle import contact as ocon
 HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3}
 
 
-def case():
-    g = golden("newton_7wire.npz")
+FIXTURES = ["newton_7wire.npz", "newton_7wire_mu03_f60.npz", "newton_7wire_mu03_f100.npz"]
+
+
+def case(name="newton_7wire.npz"):
+    g = golden(name)
     p = g["params"]
     k_pen, mu, t_total, rho_inf, dtmax, sd, tolf, told, maxa, ktr = p
     S = pkg("solver")
@@ -84,10 +91,23 @@ def test_adaptive_stepping_schedule():
     assert st.queue[0] == 1.0
 
 
-def test_newton_trajectory_matches_reference_on_oracle_assembly():
+def test_friction_fixtures_engage_friction():
+    """The frictional fixtures are the frictionless case with mu = 0.3 and other
+    loads, and their trajectories pass through calls with active contact."""
+    g0 = golden("newton_7wire.npz")
+    for name, load in (("newton_7wire_mu03_f60.npz", 60.0), ("newton_7wire_mu03_f100.npz", 100.0)):
+        g = golden(name)
+        assert g["params"][1] == 0.3
+        assert np.array_equal(np.delete(g["params"], 1), np.delete(g0["params"], 1))
+        np.testing.assert_allclose(g["f_ext"] * 100.0 / load, g0["f_ext"], rtol=1e-14, atol=1e-13)
+        assert np.any(g["call_active"] > 0) and not bool(g["converged"])
+
+
+@pytest.mark.parametrize("fixture", FIXTURES)
+def test_newton_trajectory_matches_reference_on_oracle_assembly(fixture):
     from _oracle_backend import OracleBackend
 
-    g, prm, cfg = case()
+    g, prm, cfg = case(fixture)
     S = pkg("solver")
     r = float(g["radii"][0])
     sec = obeam.circle_section(r, 200e3, 0.3)
@@ -105,15 +125,16 @@ def test_newton_trajectory_matches_reference_on_oracle_assembly():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fixture", FIXTURES)
 @pytest.mark.parametrize("mode", ["host_superlu", "device_solve", "device_graph"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode):
+def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fixture):
     """The device assembly under the driver: with SuperLU on the host, with the
     Newton attempts on the device (device linear solve, line search and
     updates in HBM), and with the attempts after each increment's first as
     replays of two captured HIP graphs."""
     device_solve = mode != "host_superlu"
-    g, prm, cfg = case()
+    g, prm, cfg = case(fixture)
     S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
     r = float(g["radii"][0])
     sec = C.circle_section(r, 200e3, 0.3)
