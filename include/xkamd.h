@@ -289,6 +289,19 @@ int xk_mlp_grad(const float* X, const float* y, int64_t n, const float* params, 
                 float* grad_and_loss, float* work, int64_t work_len, void* stream);
 int xk_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr, float beta1,
                 float beta2, float eps, void* stream);
+/* xk_mlp_grad without the workspace argument and without any allocation: the
+ * per-workgroup gradient partials go to a fixed device array of the library
+ * (256 x 4802 floats, 4.9 MB, held by every GPU the library runs on until the
+ * process ends).  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP-graph capture, training loops that forbid
+ * hipMalloc).  Any n >= 1 fits (the gradient kernel's grid stops at 256
+ * workgroups); a layout that needed more would be rejected with an error.
+ * Calls no hipMalloc / hipFree; the first call on a device resolves the
+ * array's address there (make it outside graph capture).  Calls that overlap
+ * on one GPU (different streams, or threads) share the array and race;
+ * calls ordered on one stream do not.  Results are bitwise xk_mlp_grad's. */
+int xk_mlp_grad_noalloc(const float* X, const float* y, int64_t n, const float* params, float pos_weight,
+                        float* grad_and_loss, void* stream);
 
 /* Per-element friction coefficients (host, n_elems, >= 0; NULL = off): a pair
  * uses mu_eff x mu_elem[elem_a] instead of mu_eff.  No counterpart in the

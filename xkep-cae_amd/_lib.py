@@ -88,6 +88,7 @@ SIGNATURES = {
     "xk_mlp_forward": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
     "xk_mlp_grad": (C.c_int, [vp, vp, C.c_int64, vp, C.c_float, vp, vp, C.c_int64, vp]),
     "xk_mlp_adam": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
+    "xk_mlp_grad_noalloc": (C.c_int, [vp, vp, C.c_int64, vp, C.c_float, vp, vp]),
 }
 
 

@@ -15,9 +15,13 @@
 // summed in a fixed order (deterministic), and Adam.  Parameter layout (FP32,
 // 4801 values): W1 [8][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3.
 
+#include <mutex>
+#include <string>
+
 #include "xk_common.h"
 #include "xk_structs.h"
 #include "xk_internal.h"
+#include "../../include/xkamd.h"
 
 namespace xk {
 
@@ -430,4 +434,71 @@ hipError_t launch_lp_adam(float* prm, const float* g, float* m, float* v, int t,
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ allocation-free gradient
+// The per-workgroup gradient partials of xk_mlp_grad_noalloc live in this
+// fixed array instead of a caller workspace: one row of LP_NPARAM + 1 floats
+// (gradient, then loss) per workgroup of lp_grad_kernel, whose grid never
+// exceeds 256 workgroups (lp_grad_blocks), so 256 x 4802 floats = 4.9 MB.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code never passes `scratch` itself to a kernel: its host-side name is
+// not a device address.  The device address is looked up once per device.
+constexpr int64_t LP_SCRATCH_ROWS = 256;
+constexpr int64_t LP_SCRATCH_FLOATS = LP_SCRATCH_ROWS * (LP_NPARAM + 1);
+static __device__ __align__(256) float scratch[LP_SCRATCH_FLOATS];
+
+constexpr int LP_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[LP_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (resolved on the first
+// call on that device, with that device current).
+static int lp_scratch_ptr(float** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error(std::string("xk_mlp_grad_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+        return -1;
+    }
+    if (dev < 0 || dev >= LP_MAX_DEVICES) {
+        set_error("xk_mlp_grad_noalloc: device ordinal out of range");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("xk_mlp_grad_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return -1;
+        }
+        g_scratch_dev[dev] = (float*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
 }  // namespace xk
+
+extern "C" int xk_mlp_grad_noalloc(const float* X, const float* y, int64_t n, const float* params,
+                                   float pos_weight, float* grad_and_loss, void* stream) {
+    if (n <= 0 || !X || !y || !params || !grad_and_loss) {
+        xk::set_error("xk_mlp_grad_noalloc: invalid arguments");
+        return -1;
+    }
+    const int64_t need = (int64_t)xk::lp_grad_blocks(n) * (xk::LP_NPARAM + 1);
+    if (need > xk::LP_SCRATCH_FLOATS) {
+        xk::set_error("xk_mlp_grad_noalloc: " + std::to_string(n) + " rows need " + std::to_string(need) +
+                      " workspace floats, more than the fixed " + std::to_string(xk::LP_SCRATCH_FLOATS));
+        return -1;
+    }
+    float* work = nullptr;
+    if (xk::lp_scratch_ptr(&work) != 0) return -1;
+    // every partial row the reduction reads is written by this call's gradient
+    // kernel, so nothing needs clearing between calls
+    hipError_t e = xk::launch_lp_grad(n, X, y, params, pos_weight, work, grad_and_loss, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        xk::set_error(std::string("xk_mlp_grad_noalloc: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
