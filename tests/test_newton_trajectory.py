@@ -20,8 +20,12 @@ from oracle import beam as obeam
 from oracle import contact as ocon
 
 # residual-history tolerance: the per-attempt relative residuals of the
-# reference, of the oracle and of the device differ by summation order only
+# reference, of the oracle and of the device differ by summation order and by
+# the linear solve's rounding only; a residual left after a near-exact step
+# (1e-5 of the call's first one) carries that rounding at about 1e-11 of the
+# first residual, hence the absolute floor relative to the call's first entry
 HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3}
+HIST_FLOOR = {"oracle": 1e-9, "fp64": 1e-9, "mixed": 1e-6}
 
 
 FIXTURES = ["newton_7wire.npz", "newton_7wire_mu03_f60.npz", "newton_7wire_mu03_f100.npz"]
@@ -47,7 +51,7 @@ def check_trajectory(g, res, kind):
         assert c.n_attempts == int(g["call_attempts"][k]), k
         assert c.n_active == int(g["call_active"][k]), k
         assert c.diverged == bool(g["call_diverged"][k])
-        np.testing.assert_allclose(c.res_history, hist[k], rtol=HIST_RTOL[kind], atol=0)
+        np.testing.assert_allclose(c.res_history, hist[k], rtol=HIST_RTOL[kind], atol=HIST_FLOOR[kind] * hist[k][0])
     assert res.converged == bool(g["converged"])
     assert res.n_increments == int(g["n_increments"])
     assert res.n_cutbacks == int(g["n_cutbacks"])
