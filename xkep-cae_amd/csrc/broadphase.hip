@@ -7,14 +7,16 @@
 // _broadphase_aabb returns (xkep_cae/contact/_broadphase.py:13-96), here in
 // sorted (i, j) order.
 //
-// Method: a dense uniform grid.  Every box is binned into the cells its
-// floor(lo * inv) .. floor(hi * inv) range covers (counting sort by cell).
-// Lane i then visits the members j > i of its own cells and keeps a pair
-// only in the cell holding floor(max(lo_i, lo_j) * inv), the one cell both
-// boxes are guaranteed to share when they overlap -- so every pair is found
-// exactly once and the cell size only changes the work, never the set.  Each
-// lane writes its partners to a segment placed by an exclusive scan of the
-// counts and sorts it, which makes the list canonical.
+// Method: a dense uniform grid.  One reduction pass gives the boxes' bounds,
+// extents and a count of non-finite boxes (a diverged displacement is refused
+// before any cell loop runs); the host sizes the grid from them.  Every box is
+// binned into the cells its floor(lo * inv) .. floor(hi * inv) range covers
+// (counting sort by cell).  One wavefront per cell then tests the pairs of its
+// members and keeps a pair only in the cell holding floor(max(lo_i, lo_j) *
+// inv), the one cell both boxes are guaranteed to share when they overlap --
+// so every pair is found exactly once and the cell size only changes the
+// work, never the set.  The pairs' keys are radix-sorted, which makes the list
+// canonical.
 //
 // Filters of DetectCandidatesProcess (contact/_manager_process.py:200-217):
 // shared node, optional same-layer exclusion, and the midpoint prescreen
@@ -30,6 +32,8 @@
 #include <cmath>
 #include <string>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "xk_common.h"
 #include "xk_structs.h"
@@ -81,7 +85,7 @@ __global__ void scan_add_kernel(int64_t n, int64_t* __restrict__ out, const int6
 static int scan_rec(Ctx* c, int slot, const int64_t* in, int64_t* out, int64_t n, hipStream_t st) {
     if (n <= 0) return 0;
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (slot >= 16) {
+    if (slot >= 6) {
         set_error("scan: input too large");
         return -1;
     }
@@ -128,55 +132,72 @@ struct Box {
 
 __global__ void aabb_kernel(int n, const double* __restrict__ x, const int* __restrict__ conn,
                             const double* __restrict__ rad, double margin, double* __restrict__ lo,
-                            double* __restrict__ hi, double* __restrict__ ext) {
+                            double* __restrict__ hi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int a = conn[2 * i], b = conn[2 * i + 1];
     const double e = rad[i] + margin;
-    double mx = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        double p = x[3 * a + k], q = x[3 * b + k];
-        double l = fmin(p, q) - e, h = fmax(p, q) + e;
-        lo[3 * i + k] = l;
-        hi[3 * i + k] = h;
-        mx = k == 0 ? (h - l) : fmax(mx, h - l);
+        const double p = x[3 * a + k], q = x[3 * b + k];
+        // NaN propagates as through np.minimum / np.maximum (fmin and fmax would drop it)
+        const bool nan = isnan(p) || isnan(q);
+        lo[3 * i + k] = nan ? p + q : fmin(p, q) - e;
+        hi[3 * i + k] = nan ? p + q : fmax(p, q) + e;
     }
-    ext[i] = mx;
 }
 
-// per-block partial sums of ext (fixed-shape tree: deterministic)
-__global__ void __launch_bounds__(256) sum_kernel(int n, const double* __restrict__ v, double* __restrict__ part) {
-    __shared__ double sh[256];
-    double s = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += v[i];
-    sh[threadIdx.x] = s;
+// Per-workgroup reduction of the boxes (fixed-shape tree, no atomics, so the
+// sums are deterministic): [0..2] min lo, [3..5] max hi, [6..8] max extent per
+// axis, [9] sum over boxes of the largest extent, [10] boxes with a
+// non-finite bound.  The host folds the partials, refuses non-finite boxes
+// before any cell loop runs, and sizes the grid from the bounds.
+constexpr int BRED = 11;
+__global__ void __launch_bounds__(256)
+box_reduce_kernel(int n, const double* __restrict__ lo, const double* __restrict__ hi, double* __restrict__ part) {
+    __shared__ double sh[BRED][256];
+    double v[BRED];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { v[k] = HUGE_VAL; v[3 + k] = -HUGE_VAL; v[6 + k] = 0.0; }
+    v[9] = 0.0;
+    v[10] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double mx = 0.0;
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double l = lo[3 * i + k], h = hi[3 * i + k];
+            bad |= !(isfinite(l) && isfinite(h));
+            v[k] = fmin(v[k], l);
+            v[3 + k] = fmax(v[3 + k], h);
+            const double e = h - l;
+            v[6 + k] = fmax(v[6 + k], e);
+            mx = k == 0 ? e : fmax(mx, e);
+        }
+        v[9] += mx;
+        v[10] += bad ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < BRED; ++q) sh[q][threadIdx.x] = v[q];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            const int t = threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sh[k][t] = fmin(sh[k][t], sh[k][t + w]);
+                sh[3 + k][t] = fmax(sh[3 + k][t], sh[3 + k][t + w]);
+                sh[6 + k][t] = fmax(sh[6 + k][t], sh[6 + k][t + w]);
+            }
+            sh[9][t] += sh[9][t + w];
+            sh[10][t] += sh[10][t + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (threadIdx.x < BRED) part[(size_t)blockIdx.x * BRED + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __device__ __forceinline__ int64_t cell_of(double v, double inv) { return (int64_t)floor(v * inv); }
-
-// bounds of cell indices over all boxes; also flags too-wide boxes
-__global__ void cell_bounds_kernel(int n, const double* __restrict__ lo, const double* __restrict__ hi, double inv,
-                                   long long* __restrict__ bmin, long long* __restrict__ bmax,
-                                   long long* __restrict__ maxcount) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    long long cnt = 1;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        long long l = cell_of(lo[3 * i + k], inv), h = cell_of(hi[3 * i + k], inv);
-        atomicMin(&bmin[k], l);
-        atomicMax(&bmax[k], h);
-        cnt *= (h - l + 1);
-    }
-    atomicMax(maxcount, cnt);
-}
 
 struct Grid {
     double inv;
@@ -225,71 +246,111 @@ struct Filt {
     const int* layer;     // same-layer filter when non-null
 };
 
-// pass 0: count partners; pass 1: write them and sort the segment
-__global__ void pairs_kernel(int n, Box B, Grid g, const int64_t* __restrict__ start, const int* __restrict__ members,
-                             Filt f, int pass, int64_t* __restrict__ count, const int64_t* __restrict__ off,
-                             int* __restrict__ outI, int* __restrict__ outJ) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double li[3], hi_[3];
-    int64_t l[3], h[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        li[k] = B.lo[3 * i + k];
-        hi_[k] = B.hi[3 * i + k];
-        l[k] = cell_of(li[k], g.inv);
-        h[k] = cell_of(hi_[k], g.inv);
-    }
-    int ci0 = 0, ci1 = 0, lay = -1;
-    if (f.conn) { ci0 = f.conn[2 * i]; ci1 = f.conn[2 * i + 1]; }
-    if (f.layer) lay = f.layer[i];
-    int64_t k_out = 0;
-    const int64_t base = pass ? off[i] : 0;
-    for (int64_t cx = l[0]; cx <= h[0]; ++cx)
-        for (int64_t cy = l[1]; cy <= h[1]; ++cy)
-            for (int64_t cz = l[2]; cz <= h[2]; ++cz) {
-                const int64_t c = cell_lin(g, cx, cy, cz);
-                for (int64_t q = start[c]; q < start[c + 1]; ++q) {
-                    const int j = members[q];
-                    if (j <= i) continue;
-                    const double* lj = B.lo + 3 * j;
-                    const double* hj = B.hi + 3 * j;
-                    if (!(li[0] <= hj[0] && li[1] <= hj[1] && li[2] <= hj[2] && lj[0] <= hi_[0] &&
-                          lj[1] <= hi_[1] && lj[2] <= hi_[2]))
-                        continue;
-                    // canonical cell: the cell of max(lo_i, lo_j)
-                    if (cell_of(fmax(li[0], lj[0]), g.inv) != cx || cell_of(fmax(li[1], lj[1]), g.inv) != cy ||
-                        cell_of(fmax(li[2], lj[2]), g.inv) != cz)
-                        continue;
-                    if (f.conn) {
-                        const int cj0 = f.conn[2 * j], cj1 = f.conn[2 * j + 1];
-                        if (ci0 == cj0 || ci0 == cj1 || ci1 == cj0 || ci1 == cj1) continue;
-                    }
-                    if (f.layer) {
-                        const int lj_ = f.layer[j];
-                        if (lay == lj_ && lay >= 0) continue;
-                    }
-                    if (pass) {
-                        outI[base + k_out] = i;
-                        outJ[base + k_out] = j;
-                    }
-                    ++k_out;
-                }
-            }
-    if (!pass) {
-        count[i] = k_out;
+// One wavefront per grid cell: the cell's members are staged in LDS (up to
+// CELL_LDS of them; larger cells read the boxes from global memory) and the
+// wave's lanes stride over the m x m member slots, testing each unordered pair
+// once: inclusive overlap, then "this is the cell of max(lo_i, lo_j)", the one
+// cell two overlapping boxes must share, then the filters.  Pass 0 counts the
+// cell's pairs; pass 1 writes their keys (i << key_bits | j, i < j) at the
+// cell's offset, compacted by ballot.  The member order inside a cell depends
+// on the fill's atomics, so the keys are sorted afterwards.
+constexpr int CELL_LDS = 128;
+
+__global__ void __launch_bounds__(64)
+cell_pairs_kernel(int64_t ncell, Box B, Grid g, const int64_t* __restrict__ start, const int* __restrict__ members,
+                  Filt f, int pass, int key_bits, int64_t* __restrict__ count, const int64_t* __restrict__ off,
+                  uint64_t* __restrict__ keys) {
+    __shared__ double slo[3][CELL_LDS], shi[3][CELL_LDS];
+    __shared__ int sid[CELL_LDS], sc0[CELL_LDS], sc1[CELL_LDS], slay[CELL_LDS];
+    const int64_t c = blockIdx.x;
+    if (c >= ncell) return;
+    const int lane = threadIdx.x;
+    const int64_t b0 = start[c];
+    const int64_t m = start[c + 1] - b0;
+    if (m < 2) {
+        if (!pass && lane == 0) count[c] = 0;
         return;
     }
-    // insertion sort of this lane's partners by j
-    for (int64_t a = 1; a < k_out; ++a) {
-        int v = outJ[base + a];
-        int64_t b = a - 1;
-        while (b >= 0 && outJ[base + b] > v) {
-            outJ[base + b + 1] = outJ[base + b];
-            --b;
+    const int64_t cz = g.o[2] + c % g.d[2];
+    const int64_t cyx = c / g.d[2];
+    const int64_t cy = g.o[1] + cyx % g.d[1];
+    const int64_t cx = g.o[0] + cyx / g.d[1];
+    const bool staged = m <= CELL_LDS;
+    if (staged) {
+        for (int q = lane; q < m; q += 64) {
+            const int i = members[b0 + q];
+            sid[q] = i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { slo[k][q] = B.lo[3 * i + k]; shi[k][q] = B.hi[3 * i + k]; }
+            sc0[q] = f.conn ? f.conn[2 * i] : 0;
+            sc1[q] = f.conn ? f.conn[2 * i + 1] : 0;
+            slay[q] = f.layer ? f.layer[i] : -1;
         }
-        outJ[base + b + 1] = v;
     }
+    __syncthreads();
+    const int64_t base = pass ? off[c] : 0;
+    const int64_t mm = m * m;
+    int64_t running = 0;
+    for (int64_t q0 = 0; q0 < mm; q0 += 64) {
+        const int64_t q = q0 + lane;
+        bool keep = false;
+        int i = 0, j = 0;
+        if (q < mm) {
+            const int64_t a = q / m, b = q - a * m;
+            if (a < b) {
+                double li[3], hi_[3], lj[3], hj[3];
+                int ci0, ci1, cj0, cj1, lai, laj;
+                if (staged) {
+                    i = sid[a];
+                    j = sid[b];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        li[k] = slo[k][a]; hi_[k] = shi[k][a];
+                        lj[k] = slo[k][b]; hj[k] = shi[k][b];
+                    }
+                    ci0 = sc0[a]; ci1 = sc1[a]; cj0 = sc0[b]; cj1 = sc1[b];
+                    lai = slay[a]; laj = slay[b];
+                } else {
+                    i = members[b0 + a];
+                    j = members[b0 + b];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        li[k] = B.lo[3 * i + k]; hi_[k] = B.hi[3 * i + k];
+                        lj[k] = B.lo[3 * j + k]; hj[k] = B.hi[3 * j + k];
+                    }
+                    ci0 = f.conn ? f.conn[2 * i] : 0;
+                    ci1 = f.conn ? f.conn[2 * i + 1] : 0;
+                    cj0 = f.conn ? f.conn[2 * j] : 0;
+                    cj1 = f.conn ? f.conn[2 * j + 1] : 0;
+                    lai = f.layer ? f.layer[i] : -1;
+                    laj = f.layer ? f.layer[j] : -1;
+                }
+                keep = li[0] <= hj[0] && li[1] <= hj[1] && li[2] <= hj[2] && lj[0] <= hi_[0] && lj[1] <= hi_[1] &&
+                       lj[2] <= hi_[2];
+                keep = keep && cell_of(fmax(li[0], lj[0]), g.inv) == cx && cell_of(fmax(li[1], lj[1]), g.inv) == cy &&
+                       cell_of(fmax(li[2], lj[2]), g.inv) == cz;
+                if (keep && f.conn) keep = !(ci0 == cj0 || ci0 == cj1 || ci1 == cj0 || ci1 == cj1);
+                if (keep && f.layer) keep = !(lai == laj && lai >= 0);
+            }
+        }
+        const uint64_t mask = __ballot(keep);
+        if (pass && keep) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const uint64_t lo_id = (uint64_t)(i < j ? i : j), hi_id = (uint64_t)(i < j ? j : i);
+            keys[base + running + rank] = (lo_id << key_bits) | hi_id;
+        }
+        running += __popcll(mask);
+    }
+    if (!pass && lane == 0) count[c] = running;
+}
+
+__global__ void split_keys_kernel(int64_t n, const uint64_t* __restrict__ keys, int key_bits, int* __restrict__ I,
+                                  int* __restrict__ J) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint64_t k = keys[q];
+    I[q] = (int)(k >> key_bits);
+    J[q] = (int)(k & ((1ull << key_bits) - 1ull));
 }
 
 // ------------------------------------------------------- prescreen and merge
@@ -423,11 +484,12 @@ __global__ void keys_of_pairs_kernel(int64_t np, const int* __restrict__ ea, con
 
 static unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-// scratch slots: 0..5 scan recursion; 6 boxes; 7 ext/partials; 8 grid counts/start;
-// 9 members; 10 per-element counts/offsets; 11 raw I/J; 12 filtered I/J; 13 flags/positions;
-// 14 sorted existing keys; 15 new keys / merge output
+// scratch slots: 0..5 scan recursion; 6 boxes; 7 box-reduction partials; 8 grid counts/start;
+// 9 members; 10 per-cell pair counts/offsets; 11 raw I/J; 12 filtered I/J; 13 flags/positions;
+// 14 sorted existing keys; 15 new keys / merge output; 16 pair keys (emitted, sorted);
+// 17 radix-sort scratch
 enum { S_BOX = 6, S_EXT = 7, S_GRID = 8, S_MEM = 9, S_CNT = 10, S_RAW = 11, S_FILT = 12, S_FLAG = 13,
-       S_KEYS = 14, S_NEWK = 15 };
+       S_KEYS = 14, S_NEWK = 15, S_PKEY = 16, S_SORT = 17 };
 
 #define BP_TRY(expr)                                                                 \
     do {                                                                             \
@@ -438,8 +500,18 @@ enum { S_BOX = 6, S_EXT = 7, S_GRID = 8, S_MEM = 9, S_CNT = 10, S_RAW = 11, S_FI
         }                                                                            \
     } while (0)
 
+static hipError_t sort_keys_u64(Ctx* c, const uint64_t* in, uint64_t* out, int64_t n, int end_bit,
+                                hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    size_t tmp = 0;
+    hipError_t e = rocprim::radix_sort_keys(nullptr, tmp, in, out, (size_t)n, 0, end_bit, st);
+    if (e != hipSuccess) return e;
+    if ((e = c->bp[S_SORT].ensure(std::max<size_t>(tmp, 1))) != hipSuccess) return e;
+    return rocprim::radix_sort_keys(c->bp[S_SORT].p, tmp, in, out, (size_t)n, 0, end_bit, st);
+}
+
 // raw (filter=0) or shared-node/layer filtered (filter=1) broadphase of all
-// elements; result I/J (n_out) left in slot S_RAW.
+// elements; result I/J (n_out, sorted by (i, j)) left in slot S_RAW.
 static int run_broadphase(Ctx* c, const double* coords, double margin, double cell_size, int filter,
                           const int* layer, int64_t* n_out, hipStream_t st) {
     const int n = c->n_elems;
@@ -449,47 +521,58 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
     double* lo = (double*)c->bp[S_BOX].p;
     double* hi = lo + 3 * (size_t)n;
     const unsigned nbs = std::min<unsigned>(nblk(n), 1024);
-    BP_TRY(c->bp[S_EXT].ensure(sizeof(double) * ((size_t)n + nbs + 8) + 64));
-    double* ext = (double*)c->bp[S_EXT].p;
-    double* part = ext + n;
+    BP_TRY(c->bp[S_EXT].ensure(sizeof(double) * BRED * (size_t)nbs));
+    double* part = (double*)c->bp[S_EXT].p;
     hipLaunchKernelGGL(aabb_kernel, dim3(nblk(n)), dim3(256), 0, st, n, coords, (const int*)c->d_conn.p,
-                       (const double*)c->d_rad.p, margin, lo, hi, ext);
+                       (const double*)c->d_rad.p, margin, lo, hi);
     BP_TRY(hipGetLastError());
-    if (!(cell_size > 0.0)) {
-        // tuned default: 1.5 x mean of the boxes' largest extent
-        hipLaunchKernelGGL(sum_kernel, dim3(nbs), dim3(256), 0, st, n, ext, part);
-        std::vector<double> ph(nbs);
-        BP_TRY(hipMemcpyAsync(ph.data(), part, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
-        BP_TRY(hipStreamSynchronize(st));
-        double s = 0.0;
-        for (double v : ph) s += v;
-        cell_size = std::max(s / n * c->cell_factor, 1e-30);
-    }
-    Grid g{};
-    long long* bnd = (long long*)(part + nbs + 1);
-    for (int attempt = 0; attempt < 40; ++attempt) {
-        g.inv = 1.0 / cell_size;
-        long long init[7] = {LLONG_MAX, LLONG_MAX, LLONG_MAX, LLONG_MIN, LLONG_MIN, LLONG_MIN, 0};
-        BP_TRY(hipMemcpyAsync(bnd, init, sizeof(init), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(cell_bounds_kernel, dim3(nblk(n)), dim3(256), 0, st, n, lo, hi, g.inv, bnd, bnd + 3,
-                           bnd + 6);
-        long long got[7];
-        BP_TRY(hipMemcpyAsync(got, bnd, sizeof(got), hipMemcpyDeviceToHost, st));
-        BP_TRY(hipStreamSynchronize(st));
-        double cells = 1.0;
+    hipLaunchKernelGGL(box_reduce_kernel, dim3(nbs), dim3(256), 0, st, n, lo, hi, part);
+    BP_TRY(hipGetLastError());
+    std::vector<double> ph((size_t)nbs * BRED);
+    BP_TRY(hipMemcpyAsync(ph.data(), part, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, st));
+    BP_TRY(hipStreamSynchronize(st));
+    double bmin[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bmax[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    double emax[3] = {0.0, 0.0, 0.0}, esum = 0.0, nbad = 0.0;
+    for (unsigned b = 0; b < nbs; ++b) {
+        const double* v = ph.data() + (size_t)b * BRED;
         for (int k = 0; k < 3; ++k) {
-            g.o[k] = got[k];
-            g.d[k] = got[3 + k] - got[k] + 1;
-            cells *= (double)g.d[k];
+            bmin[k] = std::min(bmin[k], v[k]);
+            bmax[k] = std::max(bmax[k], v[3 + k]);
+            emax[k] = std::max(emax[k], v[6 + k]);
         }
-        // enlarge the cell (the set does not depend on it) until the grid and
-        // the per-box cell ranges stay bounded
-        if (cells <= (double)(1LL << 26) && got[6] <= 4096 && cells * 1.0 <= 64.0 * n + (1 << 20)) break;
-        cell_size *= 2.0;
-        if (attempt == 39) {
+        esum += v[9];
+        nbad += v[10];
+    }
+    if (nbad > 0.0 || !std::isfinite(esum)) {
+        set_error("broadphase: non-finite coordinates in " + std::to_string((long long)nbad) +
+                  " element boxes (the displacement has diverged)");
+        return -1;
+    }
+    if (!(cell_size > 0.0)) cell_size = std::max(esum / n * c->cell_factor, 1e-30);   // tuned default
+    // enlarge the cell (the set does not depend on it) until the grid and the
+    // per-box cell ranges stay bounded; all cell indices are formed in double
+    // and checked to be exact integers before they become grid extents
+    Grid g{};
+    double cells = 0.0;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt == 60) {
             set_error("broadphase: could not size the grid");
             return -1;
         }
+        g.inv = 1.0 / cell_size;
+        cells = 1.0;
+        double per_box = 1.0;
+        bool exact = true;
+        for (int k = 0; k < 3; ++k) {
+            const double a = std::floor(bmin[k] * g.inv), b = std::floor(bmax[k] * g.inv);
+            exact &= std::fabs(a) < 9.0e15 && std::fabs(b) < 9.0e15;
+            g.o[k] = exact ? (long long)a : 0;
+            g.d[k] = exact ? (long long)(b - a) + 1 : 0;
+            cells *= b - a + 1.0;
+            per_box *= std::floor(emax[k] * g.inv) + 2.0;
+        }
+        if (exact && cells <= (double)(1LL << 26) && per_box <= 4096.0 && cells <= 64.0 * n + (1 << 20)) break;
+        cell_size *= 2.0;
     }
     const int64_t ncell = g.d[0] * g.d[1] * g.d[2];
     BP_TRY(c->bp[S_GRID].ensure(sizeof(int64_t) * (size_t)(2 * ncell + 2) + sizeof(int) * (size_t)(ncell + 1)));
@@ -510,20 +593,33 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
     hipLaunchKernelGGL(grid_fill_kernel, dim3(nblk(n)), dim3(256), 0, st, n, B, g, gstart, gcnt, members);
     BP_TRY(hipGetLastError());
 
-    BP_TRY(c->bp[S_CNT].ensure(sizeof(int64_t) * 2 * (size_t)(n + 1)));
+    // per-cell pair counts, their offsets, the keys, sorted
+    BP_TRY(c->bp[S_CNT].ensure(sizeof(int64_t) * 2 * (size_t)(ncell + 1)));
     int64_t* cnt = (int64_t*)c->bp[S_CNT].p;
-    int64_t* off = cnt + n + 1;
+    int64_t* off = cnt + ncell + 1;
     Filt f{filter ? (const int*)c->d_conn.p : nullptr, filter ? layer : nullptr};
-    hipLaunchKernelGGL(pairs_kernel, dim3(nblk(n)), dim3(256), 0, st, n, B, g, gstart, members, f, 0, cnt, off,
-                       (int*)nullptr, (int*)nullptr);
+    int key_bits = 1;
+    while ((1LL << key_bits) < (int64_t)n) ++key_bits;
+    hipLaunchKernelGGL(cell_pairs_kernel, dim3((unsigned)ncell), dim3(64), 0, st, ncell, B, g, gstart, members, f, 0,
+                       key_bits, cnt, off, (uint64_t*)nullptr);
     BP_TRY(hipGetLastError());
     int64_t npairs = 0;
-    if (scan_exclusive_i64(c, 0, cnt, off, n, &npairs, st) != 0) return -1;
+    if (scan_exclusive_i64(c, 0, cnt, off, ncell, &npairs, st) != 0) return -1;
+    BP_TRY(c->bp[S_PKEY].ensure(sizeof(uint64_t) * 2 * (size_t)std::max<int64_t>(npairs, 1)));
+    uint64_t* keys = (uint64_t*)c->bp[S_PKEY].p;
+    uint64_t* sorted = keys + std::max<int64_t>(npairs, 1);
     BP_TRY(c->bp[S_RAW].ensure(sizeof(int) * 2 * (size_t)std::max<int64_t>(npairs, 1)));
     int* I = (int*)c->bp[S_RAW].p;
     int* J = I + std::max<int64_t>(npairs, 1);
-    hipLaunchKernelGGL(pairs_kernel, dim3(nblk(n)), dim3(256), 0, st, n, B, g, gstart, members, f, 1, cnt, off, I, J);
-    BP_TRY(hipGetLastError());
+    if (npairs > 0) {
+        hipLaunchKernelGGL(cell_pairs_kernel, dim3((unsigned)ncell), dim3(64), 0, st, ncell, B, g, gstart, members, f,
+                           1, key_bits, cnt, off, keys);
+        BP_TRY(hipGetLastError());
+        BP_TRY(sort_keys_u64(c, keys, sorted, npairs, 2 * key_bits, st));
+        hipLaunchKernelGGL(split_keys_kernel, dim3(nblk(npairs)), dim3(256), 0, st, npairs, (const uint64_t*)sorted,
+                           key_bits, I, J);
+        BP_TRY(hipGetLastError());
+    }
     c->raw_n = npairs;
     c->raw_cap = std::max<int64_t>(npairs, 1);
     c->last_cell = cell_size;

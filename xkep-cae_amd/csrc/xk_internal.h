@@ -65,7 +65,7 @@ struct Ctx {
     DBuf d_gptr, d_gcol, d_brow, d_blk_bptr, d_blk_bsrc, d_blk_pptr, d_blk_psrc, d_node_pptr, d_node_psrc;
     DBuf d_sd, d_sg;
     // broadphase scratch and candidate bookkeeping
-    DBuf bp[16];
+    DBuf bp[18];
     int64_t last_cand = 0;
     int64_t raw_n = 0, raw_cap = 1;
     double last_cell = 0.0;

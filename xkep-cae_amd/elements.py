@@ -84,13 +84,25 @@ class ULCRBeamAssembler:
         return K
 
     # ---------------------------------------------------------------- drop-in API
+    def _check_elements(self) -> None:
+        """Raise as the reference's CR element routine does on a zero-length
+        element (ValueError, elements/_beam_cr.py:54-55): the element kernel
+        flags XK_FLAG_DEGENERATE in the device status word (read and cleared
+        here, so this synchronises)."""
+        if self.ctx.status_flags() & 2:
+            raise ValueError("zero-length beam element (initial or deformed length below 1e-15)")
+
     def assemble_tangent(self, u_incr) -> sp.csr_matrix:
         """``ULCRBeamAssembler.assemble_tangent`` (_beam_assembler.py:73-93)."""
-        return self.ctx.to_csr(self.assemble_tangent_device(u_incr))
+        K = self.ctx.to_csr(self.assemble_tangent_device(u_incr))
+        self._check_elements()
+        return K
 
     def assemble_internal_force(self, u_incr) -> np.ndarray:
         """``ULCRBeamAssembler.assemble_internal_force`` (_beam_assembler.py:95-115)."""
-        return self.assemble_internal_force_device(u_incr).cpu().numpy()
+        f = self.assemble_internal_force_device(u_incr).cpu().numpy()
+        self._check_elements()
+        return f
 
     def update_reference(self, u_incr) -> None:
         """Updated-Lagrangian reference update (_beam_assembler.py:117-124); the

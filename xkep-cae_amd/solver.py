@@ -37,6 +37,9 @@ from dataclasses import dataclass, field
 import numpy as np
 import scipy.sparse as sp
 
+# device status word bits (XK_FLAG_* of include/xkamd.h)
+FLAG_NONFINITE, FLAG_DEGENERATE, FLAG_SINGULAR = 1, 2, 4
+
 
 @dataclass(frozen=True)
 class SolverConfig:
@@ -248,6 +251,8 @@ def newton_dynamic(backend, u: np.ndarray, u_ref: np.ndarray, f_ext: np.ndarray,
         if (consecutive_increase >= cfg.divergence_window and not ratio < 0.01) or (att >= 10 and ratio > 100.0):
             out.diverged = True
             break
+        if not math.isfinite(res):
+            break   # a non-finite system has no solution: LinearSolveProcess success=False (_newton_steps.py:360-366)
         du = linear_solve(K, R, fixed_dofs)
         if du is None:
             break
@@ -434,6 +439,19 @@ class DeviceBackend:
                                   core_radii=core_radii)
         self._f = self.torch.zeros(ctx.ndof, dtype=self.torch.float64, device=ctx.dev)
         self._dt = None
+        self.flags = 0              # every XK_FLAG_* bit the device status word has reported
+        self.solve_fail_tol = 1e-6  # a solve whose refined residual stays above this counts as failed
+        self.eager_resolves = 0     # graph attempts whose fixed refinement missed solve_tol, re-solved eagerly
+
+    def _read_flags(self) -> int:
+        """Read (and clear) the device status word, keep its bits, and raise as
+        the reference does on a degenerate element (the scalar CR routine's
+        ValueError, elements/_beam_cr.py:54-55)."""
+        f = self.ctx.status_flags()
+        self.flags |= f
+        if f & FLAG_DEGENERATE:
+            raise ValueError("zero-length beam element met by the element kernel (XK_FLAG_DEGENERATE)")
+        return f
 
     def detect(self, u):
         self.asm.begin_increment(u, geometry=False)
@@ -467,10 +485,12 @@ class DeviceBackend:
         asm.set_f_ext(f_ext)
         R, K, _, n_act = asm.assemble()
         self.ctx.pairs_gather(f=self._f)
+        self._read_flags()
         return (R.cpu().numpy(), asm.K_csr(), int(n_act.item()), -self._f.cpu().numpy())
 
     def internal_force(self, u):
         self.ctx.beam(self.ctx.to_dev(u), f_int=self._f)
+        self._read_flags()
         return self._f.cpu().numpy()
 
     def correct(self, u, dt):
@@ -507,8 +527,10 @@ class DeviceBackend:
         prev_ratio = float("inf")
         out = NewtonOutcome(False, 0, 0, False, None)
 
-        def solve_and_update(rel, refine, tol):
+        def solve(rel, refine, tol):
             ctx.linear_solve(ud, asm.fixed_mask, asm.R, du, asm._dyn, max_refine=refine, tol=tol, res_rel=rel)
+
+        def update():
             ctx.newton_update(ud, du, asm.R, self._f, asm.f_ext, asm.fixed_mask, asm.res2, status,
                               line_search_steps=ls_steps, diverge_factor=3.0, du_norm_cap=cfg.du_norm_cap)
 
@@ -523,6 +545,7 @@ class DeviceBackend:
             # ConvergenceCheckProcess, force part (_newton_steps.py:194-213)
             res = math.sqrt(float(asm.res2.item()))
             n_active = int(asm.n_active.item())
+            flags = self._read_flags()
             f_ref = incr_f_ref if incr_f_ref > 1e-30 else f_ext_ref_norm
             if dynamic_ref and att == 0 and res > 1e-30:
                 f_ref = res
@@ -544,15 +567,27 @@ class DeviceBackend:
             if (consecutive_increase >= cfg.divergence_window and not ratio < 0.01) or (att >= 10 and ratio > 100.0):
                 out.diverged = True
                 break
+            if flags & FLAG_NONFINITE or not math.isfinite(res):
+                break   # a non-finite system has no solution: LinearSolveProcess success=False
             rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
             if graphs is None:
-                solve_and_update(rel, self.max_refine, self.solve_tol)
+                solve(rel, self.max_refine, self.solve_tol)
             else:
-                rel = graphs[2]
+                rel = graphs[3]
                 graphs[1].replay()
-            self.solve_residuals.append(rel.clone() if graphs is not None else rel)
-            if ctx.status_flags() & 4:
-                break   # zero pivot: the solve failed (LinearSolveProcess returns success=False)
+                if float(rel.item()) > self.solve_tol:
+                    # the captured solve's fixed refinement fell short: solve this attempt eagerly
+                    self.eager_resolves += 1
+                    rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
+                    solve(rel, self.max_refine, self.solve_tol)
+            rel_v = float(rel.item())
+            self.solve_residuals.append(rel.clone())
+            if self._read_flags() & FLAG_SINGULAR or not rel_v <= self.solve_fail_tol:
+                break   # zero pivot or no accurate solution: the solve failed (LinearSolveProcess success=False)
+            if graphs is None:
+                update()
+            else:
+                graphs[2].replay()
             st = status[:4].cpu().numpy()
             du_norm, u_norm, energy = float(st[1]), float(st[2]), float(st[3])
             # ConvergenceCheckProcess, displacement / energy part (_newton_steps.py:215-243)
@@ -565,7 +600,7 @@ class DeviceBackend:
                 out.converged = True
                 break
             if graph_ok and graphs is None:
-                graphs = self._capture_attempt(solve_and_update)
+                graphs = self._capture_attempt(solve, update)
                 if graphs is None:
                     graph_ok = False
                 else:
@@ -573,11 +608,13 @@ class DeviceBackend:
         u[:] = ud.cpu().numpy()
         return out
 
-    def _capture_attempt(self, solve_and_update):
-        """G1 (assembly + pair force) and G2 (solve with graph_refine fixed
-        refinement steps + line search + update), recorded without being run
-        (the eager first attempt has set the solve up).  None if capture is
-        not possible; the reason is kept in ``graph_error``."""
+    def _capture_attempt(self, solve, update):
+        """G1 (assembly + pair force), G2 (solve with graph_refine fixed
+        refinement steps, writing its relative residual) and G3 (line search +
+        update), recorded without being run (the eager first attempt has set
+        the solve up).  The host reads the residual between G2 and G3 and
+        solves eagerly when the fixed refinement missed ``solve_tol``.  None if
+        capture is not possible; the reason is kept in ``graph_error``."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
         rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
         try:
@@ -587,12 +624,15 @@ class DeviceBackend:
                 ctx.pairs_gather(f=self._f)
             g2 = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g2):
-                solve_and_update(rel, self.graph_refine, 0.0)
+                solve(rel, self.graph_refine, 0.0)
+            g3 = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g3):
+                update()
         except Exception as e:  # noqa: BLE001 - any capture failure falls back to eager attempts
             self.graph_error = repr(e)
             torch.cuda.synchronize()
             return None
-        return g1, g2, rel
+        return g1, g2, g3, rel
 
     def checkpoint(self):
         self.ga.checkpoint()
