@@ -133,6 +133,32 @@ int xk_detect_candidates(xk_ctx* ctx, const double* coords, double margin, doubl
  * 2*cap) gets sorted (i<j) pairs.  Synchronous. */
 int xk_broadphase(xk_ctx* ctx, const double* coords, double margin, double cell_size,
                   int32_t* pairs_out, int64_t cap, int64_t* n_out, void* stream);
+/* The Strategy twin of candidate detection, _detect_candidates
+ * (xkep_cae/contact/geometry/strategy.py:135-199, behind
+ * ContactGeometryStrategy.detect, core/strategies/protocols.py:115-122):
+ * replaces the whole pair list with the broadphase pairs of coords (device,
+ * n_nodes x 3), filtered by shared nodes only when shared_node_filter is
+ * non-zero (the Strategy's exclude_same_layer), no layer filter, no midpoint
+ * prescreen, every pair with a fresh state, in sorted (i<j) order.  The
+ * initial narrowphase is the caller's xk_pairs_run (GEOM | HYSTERESIS with
+ * g_on = g_off = 0, _batch_update_geometry with config None, :205-305).
+ * n_pairs receives the new list's length.  Synchronous. */
+int xk_strategy_detect(xk_ctx* ctx, const double* coords, double margin, double cell_size,
+                       int shared_node_filter, int64_t* n_pairs, void* stream);
+/* Constraint Jacobian G = d g_n / d u of the pair list, replacing
+ * _build_constraint_jacobian_ptp (xkep_cae/contact/geometry/strategy.py:83-132,
+ * ContactGeometryStrategy.build_constraint_jacobian, protocols.py:139-147):
+ * one row per pair with status != INACTIVE in pair order, entries c_k n_d at
+ * DOF node_k * ndof_per_node + d of [a0, a1, b0, b1] with
+ * c = [1 - s, s, -(1 - t), -t], |c_k n_d| <= 1e-30 not stored, entries on one
+ * DOF summed, columns sorted (SciPy's CSR of the reference's COO).  Outputs
+ * are device arrays: active_idx (n_rows, int32: the pair index of each row),
+ * indptr (n_rows + 1, int64), indices (nnz, int32), data (nnz, float64); pass
+ * all four NULL to query n_rows and nnz only.  ndof_total only checks that
+ * the mesh's DOFs fit.  Synchronous. */
+int xk_pairs_constraint_jacobian(xk_ctx* ctx, int ndof_per_node, int64_t ndof_total, int64_t* n_rows,
+                                 int64_t* nnz, int32_t* active_idx, int64_t* indptr, int32_t* indices,
+                                 double* data, int64_t cap_rows, int64_t cap_nnz, void* stream);
 /* Candidate list of the last xk_detect_candidates (device int32 2*n). */
 int xk_last_candidates(xk_ctx* ctx, int32_t* pairs_out_host, int64_t cap);
 /* Sum of (r_a + r_b) over the shared-node-filtered candidates of the last

@@ -8,6 +8,9 @@ Restates, vectorised over pairs:
   - ``_closest_point_segments_batch``   contact/geometry/_compute.py:55-151
   - ``_build_contact_frame_batch``      contact/geometry/_compute.py:154-261
   - ``UpdateGeometryProcess``           contact/_manager_process.py:327-403, :475-494
+  - the ContactGeometryStrategy twin    contact/geometry/strategy.py:47-305
+    (``_detect_candidates``, ``_batch_update_geometry``,
+    ``_build_constraint_jacobian_ptp``)
   - ``HuberContactForceProcess``        contact/contact_force/strategy.py:104-254
   - ``_friction_return_mapping_loop``   contact/friction/_assembly.py:260-342
     with ``_return_mapping_core`` / ``_tangent_2x2_core`` (law_friction.py:89-153)
@@ -419,6 +422,66 @@ def update_geometry(pairs: Pairs, coords, cfg: ContactConfig, *,
         off = (~inact) & allow_deactivation & (gap >= cfg.g_off) & ~coat_active
         p.status = np.where(on, ACTIVE, np.where(off, INACTIVE, p.status))
     return p
+
+
+# --------------------------------------------------------------------------
+# ContactGeometryStrategy twin
+# --------------------------------------------------------------------------
+
+
+def strategy_update_geometry(pairs: Pairs, coords, config: ContactConfig | None = None) -> Pairs:
+    """``_batch_update_geometry`` (geometry/strategy.py:205-305), the update of
+    ``PointToPointProcess.update_geometry``: narrowphase and frame as
+    ``UpdateGeometryProcess``, then ``_update_active_set_hysteresis`` (:47-80)
+    with g_on / g_off / coating_stiffness from ``config`` (0, 0, 0 without
+    one), deactivation always allowed, never frozen."""
+    cfg = ContactConfig(g_on=0.0, g_off=0.0) if config is None else config
+    return update_geometry(pairs, coords, cfg, allow_deactivation=True, freeze_active_set=False)
+
+
+def strategy_detect(coords, conn, radii, *, exclude_same_layer: bool = True, margin: float = 0.0,
+                    cell_size: float | None = None) -> Pairs:
+    """``_detect_candidates`` (geometry/strategy.py:135-199): broadphase, the
+    shared-node filter when ``exclude_same_layer`` (no layer map, no midpoint
+    prescreen), every pair new with a fresh state and core radii 0.0 (the
+    ``_ContactPairOutput`` defaults, _contact_pair.py:79-80), then the initial
+    narrowphase without a config."""
+    coords = np.asarray(coords, float)
+    conn = np.asarray(conn, np.int64)
+    ne = len(conn)
+    if ne < 2:
+        return Pairs()
+    r_arr = np.full(ne, float(radii)) if np.isscalar(radii) else np.asarray(radii, float)
+    raw = broadphase_aabb(coords[conn[:, 0]], coords[conn[:, 1]], r_arr, margin=margin, cell_size=cell_size)
+    if exclude_same_layer and len(raw):
+        ca, cb = conn[raw[:, 0]], conn[raw[:, 1]]
+        share = ((ca[:, 0] == cb[:, 0]) | (ca[:, 0] == cb[:, 1]) |
+                 (ca[:, 1] == cb[:, 0]) | (ca[:, 1] == cb[:, 1]))
+        raw = raw[~share]
+    if len(raw) == 0:
+        return Pairs()
+    p = Pairs.fresh(raw[:, 0], raw[:, 1], conn, r_arr, np.zeros(ne))
+    return strategy_update_geometry(p, coords, None)
+
+
+def constraint_jacobian_ptp(pairs: Pairs, ndof_total: int, ndof_per_node: int = 6):
+    """``_build_constraint_jacobian_ptp`` (geometry/strategy.py:83-132):
+    G = d g_n / d u, one row per pair with status != INACTIVE (pair order),
+    entries c_k n_d at DOF node_k * ndof_per_node + d of [a0, a1, b0, b1],
+    |c_k n_d| <= 1e-30 dropped, COO -> CSR (duplicates summed).
+    Returns (G, active_indices)."""
+    act = np.nonzero(pairs.status != INACTIVE)[0]
+    if len(act) == 0:
+        return sp.csr_matrix((0, ndof_total)), []
+    c = shape_coeffs(pairs)[act]                                   # (m, 4)
+    nodes = np.stack([pairs.nodes_a[act, 0], pairs.nodes_a[act, 1],
+                      pairs.nodes_b[act, 0], pairs.nodes_b[act, 1]], axis=1)
+    vals = c[:, :, None] * pairs.normal[act][:, None, :]           # (m, 4, 3)
+    cols = nodes[:, :, None] * ndof_per_node + np.arange(3)[None, None, :]
+    rows = np.broadcast_to(np.arange(len(act))[:, None, None], vals.shape)
+    keep = np.abs(vals) > 1e-30
+    G = sp.coo_matrix((vals[keep], (rows[keep], cols[keep])), shape=(len(act), ndof_total)).tocsr()
+    return G, act.tolist()
 
 
 # --------------------------------------------------------------------------

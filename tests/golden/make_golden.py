@@ -15,7 +15,7 @@ running it can pin, and the strand generator's layouts (angle offsets, lay
 radii, numbering) over its branches.
 
 Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
-        [element|assembly|newton|newton_friction|mesh ...]
+        [element|assembly|newton|newton_friction|newton_91|strategy|mesh ...]
 """
 
 from __future__ import annotations
@@ -217,7 +217,7 @@ def assembly_fixture():
     np.savez_compressed(os.path.join(HERE, "assembly_7wire.npz"), **out)
 
 
-def newton_case(mu=0.0, force=100.0):
+def newton_case(mu=0.0, force=100.0, nw=7, r=1.0, L=20.0, m=20, ang=10.0):
     """Inputs of the Newton-trajectory fixtures: 7-wire strand (r = 1, L = 20, 20
     elements per wire, lay angle 10 deg), core and outer layer in contact only
     (same-layer exclusion by layer index), first node of every wire clamped,
@@ -226,7 +226,6 @@ def newton_case(mu=0.0, force=100.0):
     penalty 0.5 c0 m_ii (the dynamic estimate of
     contact/penalty/strategy.py:216-262 at the nominal step), increments capped
     at a quarter of the total time."""
-    nw, r, L, m, ang = 7, 1.0, 20.0, 20, 10.0
     mesh = mesh_mod.make_strand(nw, r, L, m, lay_angle_deg=ang)
     X, conn = mesh.node_coords, mesh.connectivity.astype(np.int64)
     s = section(r)
@@ -251,10 +250,10 @@ def newton_case(mu=0.0, force=100.0):
     k_pen = 0.5 * (1.0 / (beta * dt_nom**2)) * RHO * s["A"] * (L / m) / 2.0
     return dict(mesh=mesh, X=X, conn=conn, s=s, fixed=fixed, f_ext=f_ext, t_total=t_total, rho_inf=rho_inf,
                 dt_max_fraction=dt_max_fraction, k_pen=k_pen, mu=mu, smoothing_delta=5000.0 / r, tol_force=1e-6,
-                tol_disp=1e-8, max_nr_attempts=30, radius=r, layer=mesh.elem_layer())
+                tol_disp=1e-8, max_nr_attempts=30, radius=r, layer=mesh.elem_layer(), nw=nw)
 
 
-def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz"):
+def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz", **strand):
     """The reference's own dynamic solver (ContactFrictionProcess) on newton_case():
     every Newton call of every increment attempt (cutbacks included) with its
     time window, attempt count, active count, convergence flags and residual
@@ -264,7 +263,7 @@ def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz"):
     from xkep_cae.core.data import (AssembleCallbacks, BoundaryData, ContactFrictionInputData,
                                     ContactSetupData, MeshData)
 
-    c = newton_case(mu, force)
+    c = newton_case(mu, force, **strand)
     s = c["s"]
     X, conn = c["X"], c["conn"]
     asm = ULCRBeamAssembler(X, conn, s["E"], s["G"], s["A"], s["Iy"], s["Iz"], s["J"], s["kappa_y"], s["kappa_z"])
@@ -276,7 +275,7 @@ def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz"):
                               k_t_ratio=0.5)
     setup = ContactSetupData(manager=_ContactManagerInput(config=cfg), k_pen=c["k_pen"], mu=c["mu"])
     inp = ContactFrictionInputData(
-        mesh=MeshData(node_coords=X, connectivity=conn, radii=np.full(len(conn), c["radius"]), n_strands=7,
+        mesh=MeshData(node_coords=X, connectivity=conn, radii=np.full(len(conn), c["radius"]), n_strands=c["nw"],
                       layer_ids=lay),
         boundary=BoundaryData(fixed_dofs=c["fixed"], f_ext_total=c["f_ext"]),
         contact=setup,
@@ -314,6 +313,64 @@ def newton_fixture(mu=0.0, force=100.0, name="newton_7wire.npz"):
                converged=np.array(res.converged), n_increments=np.array(res.n_increments),
                n_cutbacks=np.array(res.n_cutbacks), u_final=res.u)
     np.savez_compressed(os.path.join(HERE, name), **out)
+
+
+def strategy_fixture():
+    """The reference's ContactGeometryStrategy classes (contact/geometry/strategy.py)
+    on a perturbed 7-wire strand: ``detect`` (shared-node filter on and off),
+    ``update_geometry`` at a second configuration with a config (g_on 0,
+    g_off 1e-6) and without one, a coating config, and
+    ``build_constraint_jacobian`` of each result; the three classes of the
+    factory give the same pairs.  Pairs are stored sorted by (min, max) element
+    key; each Jacobian row is stored with the key of its pair."""
+    from xkep_cae.contact.geometry.strategy import _create_contact_geometry_strategy
+
+    m = mesh_mod.make_strand(7, 1.0, 20.0, 20, lay_angle_deg=10.0)
+    X, conn = m.node_coords, m.connectivity.astype(np.int64)
+    rng = np.random.default_rng(31)
+    x0 = X + rng.normal(0.0, 0.02, X.shape)
+    x1 = x0 + rng.normal(0.0, 0.01, X.shape)
+    ndof = 6 * len(X)
+    out = {"X": X, "conn": conn, "radii": m.radii, "x0": x0, "x1": x1}
+
+    def jac(prefix, strat, pairs):
+        G, act = strat.build_constraint_jacobian(pairs, ndof, 6)
+        G = G.tocsr()
+        keys = np.array([(min(pairs[i].elem_a, pairs[i].elem_b), max(pairs[i].elem_a, pairs[i].elem_b))
+                         for i in act], np.int64).reshape(-1, 2)
+        out[f"{prefix}_G_keys"] = keys
+        out.update(csr_arrays(f"{prefix}_G", G))
+
+    for mode in ("point_to_point", "line_to_line", "mortar"):
+        strat = _create_contact_geometry_strategy(mode=mode)
+        pairs = strat.detect(x0, conn, m.radii)
+        if mode != "point_to_point":
+            ref = out["det_key"]
+            key = pair_arrays("tmp", pairs)["tmp_key"]
+            assert np.array_equal(key, ref), mode
+            continue
+        out.update(pair_arrays("det", pairs))
+        jac("det", strat, pairs)
+        out["det_gap_api"] = np.array([strat.compute_gap(p, x0) for p in pairs])
+        out["det_gap_api_key"] = np.array([(min(p.elem_a, p.elem_b), max(p.elem_a, p.elem_b)) for p in pairs])
+        cfg = _ContactConfigInput(g_on=0.0, g_off=1e-6)
+        strat.update_geometry(pairs, x1, config=cfg)
+        out.update(pair_arrays("upd", pairs))
+        jac("upd", strat, pairs)
+        strat.update_geometry(pairs, x0)
+        out.update(pair_arrays("upd2", pairs))
+        jac("upd2", strat, pairs)
+        ccfg = _ContactConfigInput(g_on=0.0, g_off=1e-6, coating_stiffness=1e3)
+        strat.update_geometry(pairs, x1, config=ccfg)
+        out.update(pair_arrays("coat", pairs))
+        out["coat_compression"] = np.array([pairs[i].state.coating_compression for i in
+                                            np.lexsort(([max(p.elem_a, p.elem_b) for p in pairs],
+                                                        [min(p.elem_a, p.elem_b) for p in pairs]))])
+    strat = _create_contact_geometry_strategy(mode="point_to_point", exclude_same_layer=False)
+    pairs = strat.detect(x0, conn, m.radii)
+    out.update(pair_arrays("all", pairs))
+    jac("all", strat, pairs)
+    np.savez_compressed(os.path.join(HERE, "strategy_7wire.npz"), **out)
 
 
 # strand generator configurations: (n_strands, wire_diameter, pitch, length, n_elems, keyword arguments)
@@ -362,8 +419,18 @@ def newton_friction_fixtures():
     newton_fixture(0.3, 100.0, "newton_7wire_mu03_f100.npz")
 
 
+def newton_91wire_fixture():
+    """The reference's dynamic solver on the 91-wire strand geometry of the
+    benchmark (r = 0.4, L = 80, 150 elements per wire, lay angle 12 deg),
+    squeezed by 0.1 per outer wire, Coulomb mu = 0.3: 4 Newton calls, 112
+    attempts, friction engaged on 432 pairs in the last two.  Slow (the
+    reference's per-pair Python loops over about 6e4 candidates)."""
+    newton_fixture(0.3, 0.1, "newton_91wire_mu03.npz", nw=91, r=0.4, L=80.0, m=150, ang=12.0)
+
+
 FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture,
-            "newton_friction": newton_friction_fixtures, "mesh": mesh_fixture}
+            "newton_friction": newton_friction_fixtures, "newton_91": newton_91wire_fixture,
+            "strategy": strategy_fixture, "mesh": mesh_fixture}
 
 if __name__ == "__main__":
     for name in sys.argv[1:] or list(FIXTURES):

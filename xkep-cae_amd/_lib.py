@@ -51,6 +51,9 @@ SIGNATURES = {
     "xk_detect_candidates": (C.c_int, [vp, vp, C.c_double, C.c_double, vp, C.c_double, c_int64_p, c_int64_p, vp]),
     "xk_broadphase": (C.c_int, [vp, vp, C.c_double, C.c_double, vp, C.c_int64, c_int64_p, vp]),
     "xk_last_candidates": (C.c_int, [vp, vp, C.c_int64]),
+    "xk_strategy_detect": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_int, c_int64_p, vp]),
+    "xk_pairs_constraint_jacobian": (C.c_int, [vp, C.c_int, C.c_int64, c_int64_p, c_int64_p, vp, vp, vp, vp,
+                                               C.c_int64, C.c_int64, vp]),
     "xk_prescreen_radii": (C.c_int, [vp, vp, C.c_double, C.c_double, vp, vp, C.c_int64, c_int64_p, vp]),
     "xk_pairs_run": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_int, vp]),

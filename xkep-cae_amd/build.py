@@ -1,10 +1,11 @@
 """Build the gfx950 assembly library ``libxkamd.so`` in-tree with hipcc.
 
 Every HIP source is compiled for gfx950 only.  Packed-FP32 instruction
-selection is disabled for all device code.  The pair and broadphase sources
-are compiled with FP contraction off: their branch decisions (parallel test,
-clamps, gap thresholds, AABB bounds, midpoint cutoff) must be formed with the
-same roundings as the FP64 NumPy formulation they replace.
+selection is disabled for all device code.  The pair, broadphase and
+geometry sources are compiled with FP contraction off: their branch decisions
+(parallel test, clamps, gap thresholds, AABB bounds, midpoint cutoff) and the
+constraint Jacobian's products and sums must be formed with the same roundings
+as the FP64 NumPy formulation they replace.
 """
 
 from __future__ import annotations
@@ -25,7 +26,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-NO_CONTRACT = {"pairs.hip", "broadphase.hip"}
+NO_CONTRACT = {"pairs.hip", "broadphase.hip", "geometry.hip"}
 
 
 def hipcc() -> str:

@@ -100,6 +100,7 @@ class ContactState:
     status: ContactStatus = ContactStatus.INACTIVE
     stick: bool = True
     dissipation: float = 0.0
+    coating_compression: float = 0.0
 
 
 @dataclass(frozen=True)
@@ -135,6 +136,7 @@ class DevicePairList:
         if self._snap is None:
             ctx = self.manager.ctx
             d = ctx.get_pairs()
+            cc = ctx.get_coating()["coating_compression"] if ctx.n_pairs else np.zeros(0)
             conn = ctx.conn_np
             rad = ctx.radii_np
             out = []
@@ -144,7 +146,8 @@ class DevicePairList:
                                   tangent1=d["tangent1"][i], tangent2=d["tangent2"][i], k_pen=d["k_pen"][i],
                                   k_t=d["k_t"][i], p_n=d["p_n"][i], z_t=d["z_t"][i],
                                   q_trial_norm=d["q_trial_norm"][i], status=ContactStatus(int(d["status"][i])),
-                                  stick=bool(d["stick"][i]), dissipation=d["dissipation"][i])
+                                  stick=bool(d["stick"][i]), dissipation=d["dissipation"][i],
+                                  coating_compression=float(cc[i]))
                 out.append(ContactPair(ea, eb, conn[ea].astype(np.int64), conn[eb].astype(np.int64), st,
                                        float(rad[ea]), float(rad[eb])))
             self._snap = out

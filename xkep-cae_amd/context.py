@@ -374,6 +374,40 @@ class DeviceContext:
         self._n_cand = nc.value
         return nc.value
 
+    def strategy_detect(self, coords, *, shared_node_filter: bool = True, margin: float = 0.0,
+                        cell_size=None) -> int:
+        """Replace the pair list by the broadphase pairs of ``coords`` with a
+        fresh state (``_detect_candidates`` before its narrowphase); returns
+        the number of pairs."""
+        n = C.c_int64()
+        L.check(L.lib().xk_strategy_detect(self.h, L.ptr(coords), float(margin), float(cell_size or 0.0),
+                                           int(bool(shared_node_filter)), C.byref(n), self.stream),
+                "xk_strategy_detect")
+        self.invalidate_pattern()
+        return n.value
+
+    def constraint_jacobian(self, ndof_total: int, ndof_per_node: int = 6):
+        """(G, active_indices): the constraint Jacobian of the pair list as a
+        SciPy CSR matrix (rows = pairs with status != INACTIVE, in pair order)."""
+        lib = L.lib()
+        rows, nnz = C.c_int64(), C.c_int64()
+        args = (self.h, int(ndof_per_node), int(ndof_total), C.byref(rows), C.byref(nnz))
+        L.check(lib.xk_pairs_constraint_jacobian(*args, None, None, None, None, 0, 0, self.stream),
+                "xk_pairs_constraint_jacobian")
+        if rows.value == 0:
+            return sp.csr_matrix((0, int(ndof_total))), []
+        t = self.torch
+        act = t.empty(rows.value, dtype=t.int32, device=self.dev)
+        indptr = t.empty(rows.value + 1, dtype=t.int64, device=self.dev)
+        idx = t.empty(nnz.value, dtype=t.int32, device=self.dev)
+        data = t.empty(nnz.value, dtype=t.float64, device=self.dev)
+        L.check(lib.xk_pairs_constraint_jacobian(*args, L.ptr(act), L.ptr(indptr), L.ptr(idx), L.ptr(data),
+                                                 act.numel(), idx.numel(), self.stream),
+                "xk_pairs_constraint_jacobian")
+        G = sp.csr_matrix((data.cpu().numpy(), idx.cpu().numpy(), indptr.cpu().numpy()),
+                          shape=(rows.value, int(ndof_total)))
+        return G, act.cpu().numpy().astype(np.int64).tolist()
+
     def last_candidates(self) -> np.ndarray:
         """Filtered candidate pairs (i<j) of the last ``detect_candidates``, sorted."""
         n = getattr(self, "_n_cand", 0)

@@ -248,9 +248,9 @@ struct Filt {
 
 // One wavefront per grid cell: the cell's members are staged in LDS (up to
 // CELL_LDS of them; larger cells read the boxes from global memory) and the
-// wave's lanes stride over the m x m member slots, testing each unordered pair
-// once: inclusive overlap, then "this is the cell of max(lo_i, lo_j)", the one
-// cell two overlapping boxes must share, then the filters.  Pass 0 counts the
+// wave's lanes stride over the cell's member pairs, testing each unordered
+// pair once: inclusive overlap, then "this is the cell of max(lo_i, lo_j)",
+// the one cell two overlapping boxes must share, then the filters.  Pass 0 counts the
 // cell's pairs; pass 1 writes their keys (i << key_bits | j, i < j) at the
 // cell's offset, compacted by ballot.  The member order inside a cell depends
 // on the fill's atomics, so the keys are sorted afterwards.
@@ -289,50 +289,20 @@ cell_pairs_kernel(int64_t ncell, Box B, Grid g, const int64_t* __restrict__ star
     }
     __syncthreads();
     const int64_t base = pass ? off[c] : 0;
-    const int64_t mm = m * m;
     int64_t running = 0;
-    for (int64_t q0 = 0; q0 < mm; q0 += 64) {
-        const int64_t q = q0 + lane;
-        bool keep = false;
-        int i = 0, j = 0;
-        if (q < mm) {
-            const int64_t a = q / m, b = q - a * m;
-            if (a < b) {
-                double li[3], hi_[3], lj[3], hj[3];
-                int ci0, ci1, cj0, cj1, lai, laj;
-                if (staged) {
-                    i = sid[a];
-                    j = sid[b];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        li[k] = slo[k][a]; hi_[k] = shi[k][a];
-                        lj[k] = slo[k][b]; hj[k] = shi[k][b];
-                    }
-                    ci0 = sc0[a]; ci1 = sc1[a]; cj0 = sc0[b]; cj1 = sc1[b];
-                    lai = slay[a]; laj = slay[b];
-                } else {
-                    i = members[b0 + a];
-                    j = members[b0 + b];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        li[k] = B.lo[3 * i + k]; hi_[k] = B.hi[3 * i + k];
-                        lj[k] = B.lo[3 * j + k]; hj[k] = B.hi[3 * j + k];
-                    }
-                    ci0 = f.conn ? f.conn[2 * i] : 0;
-                    ci1 = f.conn ? f.conn[2 * i + 1] : 0;
-                    cj0 = f.conn ? f.conn[2 * j] : 0;
-                    cj1 = f.conn ? f.conn[2 * j + 1] : 0;
-                    lai = f.layer ? f.layer[i] : -1;
-                    laj = f.layer ? f.layer[j] : -1;
-                }
-                keep = li[0] <= hj[0] && li[1] <= hj[1] && li[2] <= hj[2] && lj[0] <= hi_[0] && lj[1] <= hi_[1] &&
-                       lj[2] <= hi_[2];
-                keep = keep && cell_of(fmax(li[0], lj[0]), g.inv) == cx && cell_of(fmax(li[1], lj[1]), g.inv) == cy &&
-                       cell_of(fmax(li[2], lj[2]), g.inv) == cz;
-                if (keep && f.conn) keep = !(ci0 == cj0 || ci0 == cj1 || ci1 == cj0 || ci1 == cj1);
-                if (keep && f.layer) keep = !(lai == laj && lai >= 0);
-            }
-        }
+    // one pair test: inclusive overlap, canonical cell, filters
+    auto test = [&](int i, int j, const double* li, const double* hi_, const double* lj, const double* hj, int ci0,
+                    int ci1, int cj0, int cj1, int lai, int laj) -> bool {
+        (void)i; (void)j;
+        bool keep = li[0] <= hj[0] && li[1] <= hj[1] && li[2] <= hj[2] && lj[0] <= hi_[0] && lj[1] <= hi_[1] &&
+                    lj[2] <= hi_[2];
+        keep = keep && cell_of(fmax(li[0], lj[0]), g.inv) == cx && cell_of(fmax(li[1], lj[1]), g.inv) == cy &&
+               cell_of(fmax(li[2], lj[2]), g.inv) == cz;
+        if (keep && f.conn) keep = !(ci0 == cj0 || ci0 == cj1 || ci1 == cj0 || ci1 == cj1);
+        if (keep && f.layer) keep = !(lai == laj && lai >= 0);
+        return keep;
+    };
+    auto emit = [&](bool keep, int i, int j) {
         const uint64_t mask = __ballot(keep);
         if (pass && keep) {
             const int rank = __popcll(mask & ((1ull << lane) - 1ull));
@@ -340,6 +310,61 @@ cell_pairs_kernel(int64_t ncell, Box B, Grid g, const int64_t* __restrict__ star
             keys[base + running + rank] = (lo_id << key_bits) | hi_id;
         }
         running += __popcll(mask);
+    };
+    if (staged) {
+        // the m (m - 1) / 2 member pairs a < b in row-major order, lanes over
+        // the pair index p; row a starts at S(a) = a (2m - a - 1) / 2
+        const int M = (int)m;
+        const int P = M * (M - 1) / 2;
+        const float t = (float)(2 * M - 1);
+        for (int p0 = 0; p0 < P; p0 += 64) {
+            const int p = p0 + lane;
+            bool keep = false;
+            int i = 0, j = 0;
+            if (p < P) {
+                int a = (int)((t - sqrtf(t * t - 8.0f * (float)p)) * 0.5f);
+                a = a < 0 ? 0 : (a > M - 2 ? M - 2 : a);
+                while (a > 0 && a * (2 * M - a - 1) / 2 > p) --a;
+                while (a < M - 2 && (a + 1) * (2 * M - a - 2) / 2 <= p) ++a;
+                const int b = p - a * (2 * M - a - 1) / 2 + a + 1;
+                i = sid[a];
+                j = sid[b];
+                double li[3], hi_[3], lj[3], hj[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    li[k] = slo[k][a]; hi_[k] = shi[k][a];
+                    lj[k] = slo[k][b]; hj[k] = shi[k][b];
+                }
+                keep = test(i, j, li, hi_, lj, hj, sc0[a], sc1[a], sc0[b], sc1[b], slay[a], slay[b]);
+            }
+            emit(keep, i, j);
+        }
+    } else {
+        // a cell with more members than the LDS stage holds: every ordered
+        // slot (a, b) of the m x m square, boxes from global memory
+        const int64_t mm = m * m;
+        for (int64_t q0 = 0; q0 < mm; q0 += 64) {
+            const int64_t q = q0 + lane;
+            bool keep = false;
+            int i = 0, j = 0;
+            if (q < mm) {
+                const int64_t a = q / m, b = q - a * m;
+                if (a < b) {
+                    i = members[b0 + a];
+                    j = members[b0 + b];
+                    double li[3], hi_[3], lj[3], hj[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        li[k] = B.lo[3 * i + k]; hi_[k] = B.hi[3 * i + k];
+                        lj[k] = B.lo[3 * j + k]; hj[k] = B.hi[3 * j + k];
+                    }
+                    keep = test(i, j, li, hi_, lj, hj, f.conn ? f.conn[2 * i] : 0, f.conn ? f.conn[2 * i + 1] : 0,
+                                f.conn ? f.conn[2 * j] : 0, f.conn ? f.conn[2 * j + 1] : 0,
+                                f.layer ? f.layer[i] : -1, f.layer ? f.layer[j] : -1);
+                }
+            }
+            emit(keep, i, j);
+        }
     }
     if (!pass && lane == 0) count[c] = running;
 }
@@ -510,8 +535,9 @@ static hipError_t sort_keys_u64(Ctx* c, const uint64_t* in, uint64_t* out, int64
     return rocprim::radix_sort_keys(c->bp[S_SORT].p, tmp, in, out, (size_t)n, 0, end_bit, st);
 }
 
-// raw (filter=0) or shared-node/layer filtered (filter=1) broadphase of all
-// elements; result I/J (n_out, sorted by (i, j)) left in slot S_RAW.
+// raw (filter=0), shared-node/layer filtered (filter=1) or shared-node
+// filtered (filter=2) broadphase of all elements; result I/J (n_out, sorted by
+// (i, j)) left in slot S_RAW.
 static int run_broadphase(Ctx* c, const double* coords, double margin, double cell_size, int filter,
                           const int* layer, int64_t* n_out, hipStream_t st) {
     const int n = c->n_elems;
@@ -597,7 +623,7 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
     BP_TRY(c->bp[S_CNT].ensure(sizeof(int64_t) * 2 * (size_t)(ncell + 1)));
     int64_t* cnt = (int64_t*)c->bp[S_CNT].p;
     int64_t* off = cnt + ncell + 1;
-    Filt f{filter ? (const int*)c->d_conn.p : nullptr, filter ? layer : nullptr};
+    Filt f{filter ? (const int*)c->d_conn.p : nullptr, filter == 1 ? layer : nullptr};
     int key_bits = 1;
     while ((1LL << key_bits) < (int64_t)n) ++key_bits;
     hipLaunchKernelGGL(cell_pairs_kernel, dim3((unsigned)ncell), dim3(64), 0, st, ncell, B, g, gstart, members, f, 0,
@@ -624,6 +650,71 @@ static int run_broadphase(Ctx* c, const double* coords, double margin, double ce
     c->raw_cap = std::max<int64_t>(npairs, 1);
     c->last_cell = cell_size;
     *n_out = npairs;
+    return 0;
+}
+
+// The pair-list update of DetectCandidatesProcess (contact/_manager_process.py:219-250)
+// for the sorted candidate list I2/J2 (n2): existing pairs outside it become
+// INACTIVE (state kept), candidates not yet in the list are appended with a
+// fresh state.  Synchronous.
+static int merge_candidates(Ctx* c, const int* I2, const int* J2, int64_t n2, hipStream_t st) {
+
+    // existing pairs outside the candidates -> INACTIVE
+    const int64_t np = c->n_pairs;
+    if (np > 0) {
+        hipLaunchKernelGGL(deactivate_kernel, dim3(nblk(np)), dim3(256), 0, st, np, c->ea, c->eb, n2, I2, J2,
+                           c->P.status);
+        BP_TRY(hipGetLastError());
+    }
+    // sorted keys of the existing pairs
+    if (!c->keys_valid) {
+        BP_TRY(c->bp[S_KEYS].ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(np, 1)));
+        if (np > 0) {
+            std::vector<int64_t> hk(np);
+            std::vector<int> a(np), b(np);
+            BP_TRY(hipMemcpy(a.data(), c->ea, 4 * np, hipMemcpyDeviceToHost));
+            BP_TRY(hipMemcpy(b.data(), c->eb, 4 * np, hipMemcpyDeviceToHost));
+            for (int64_t p = 0; p < np; ++p) {
+                int lo_ = std::min(a[p], b[p]), hi_ = std::max(a[p], b[p]);
+                hk[p] = ((int64_t)lo_ << 32) | (int64_t)(uint32_t)hi_;
+            }
+            std::sort(hk.begin(), hk.end());
+            BP_TRY(hipMemcpy(c->bp[S_KEYS].p, hk.data(), 8 * np, hipMemcpyHostToDevice));
+        }
+        c->keys_valid = true;
+    }
+    int64_t n_new = 0;
+    if (n2 > 0) {
+        BP_TRY(c->bp[S_NEWK].ensure(sizeof(int64_t) * (3 * (size_t)n2 + (size_t)np + 2)));
+        int64_t* isnew = (int64_t*)c->bp[S_NEWK].p;
+        int64_t* pos = isnew + n2;
+        int64_t* newkeys = pos + n2;
+        hipLaunchKernelGGL(newflag_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, I2, J2, np,
+                           (const int64_t*)c->bp[S_KEYS].p, isnew);
+        BP_TRY(hipGetLastError());
+        if (scan_exclusive_i64(c, 0, isnew, pos, n2, &n_new, st) != 0) return -1;
+        if (n_new > 0) {
+            if (c->grow_pairs(np + n_new) != 0) return -1;
+            hipLaunchKernelGGL(append_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, isnew, pos, I2, J2,
+                               (const int*)c->d_conn.p, (const double*)c->d_rad.p, (const double*)c->d_crad.p, c->P,
+                               c->ea, c->eb, np, newkeys);
+            BP_TRY(hipGetLastError());
+            // merged sorted keys
+            DBuf merged;
+            BP_TRY(merged.ensure(sizeof(int64_t) * (size_t)(np + n_new)));
+            hipLaunchKernelGGL(merge_kernel, dim3(nblk(np + n_new)), dim3(256), 0, st, np,
+                               (const int64_t*)c->bp[S_KEYS].p, n_new, (const int64_t*)newkeys,
+                               (int64_t*)merged.p);
+            BP_TRY(hipGetLastError());
+            BP_TRY(hipStreamSynchronize(st));
+            c->bp[S_KEYS].release();
+            c->bp[S_KEYS] = merged;
+            c->n_pairs = np + n_new;
+        }
+    }
+    BP_TRY(hipStreamSynchronize(st));
+    c->pattern_valid = false;
+    c->lp_on = false;
     return 0;
 }
 
@@ -731,63 +822,32 @@ int xk_detect_candidates(xk_ctx* c, const double* coords, double margin, double 
     }
     c->last_cand = n2;
     if (n_candidates) *n_candidates = n2;
+    return merge_candidates(c, I2, J2, n2, st);
+}
 
-    // existing pairs outside the candidates -> INACTIVE
-    const int64_t np = c->n_pairs;
-    if (np > 0) {
-        hipLaunchKernelGGL(deactivate_kernel, dim3(nblk(np)), dim3(256), 0, st, np, c->ea, c->eb, n2, I2, J2,
-                           c->P.status);
-        BP_TRY(hipGetLastError());
+int xk_strategy_detect(xk_ctx* c, const double* coords, double margin, double cell_size, int shared_node_filter,
+                       int64_t* n_pairs, void* stream) {
+    if (!c || !coords || !n_pairs) {
+        set_error("xk_strategy_detect: null argument");
+        return -1;
     }
-    // sorted keys of the existing pairs
-    if (!c->keys_valid) {
-        BP_TRY(c->bp[S_KEYS].ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(np, 1)));
-        if (np > 0) {
-            std::vector<int64_t> hk(np);
-            std::vector<int> a(np), b(np);
-            BP_TRY(hipMemcpy(a.data(), c->ea, 4 * np, hipMemcpyDeviceToHost));
-            BP_TRY(hipMemcpy(b.data(), c->eb, 4 * np, hipMemcpyDeviceToHost));
-            for (int64_t p = 0; p < np; ++p) {
-                int lo_ = std::min(a[p], b[p]), hi_ = std::max(a[p], b[p]);
-                hk[p] = ((int64_t)lo_ << 32) | (int64_t)(uint32_t)hi_;
-            }
-            std::sort(hk.begin(), hk.end());
-            BP_TRY(hipMemcpy(c->bp[S_KEYS].p, hk.data(), 8 * np, hipMemcpyHostToDevice));
-        }
-        c->keys_valid = true;
-    }
-    int64_t n_new = 0;
-    if (n2 > 0) {
-        BP_TRY(c->bp[S_NEWK].ensure(sizeof(int64_t) * (3 * (size_t)n2 + (size_t)np + 2)));
-        int64_t* isnew = (int64_t*)c->bp[S_NEWK].p;
-        int64_t* pos = isnew + n2;
-        int64_t* newkeys = pos + n2;
-        hipLaunchKernelGGL(newflag_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, I2, J2, np,
-                           (const int64_t*)c->bp[S_KEYS].p, isnew);
-        BP_TRY(hipGetLastError());
-        if (scan_exclusive_i64(c, 0, isnew, pos, n2, &n_new, st) != 0) return -1;
-        if (n_new > 0) {
-            if (c->grow_pairs(np + n_new) != 0) return -1;
-            hipLaunchKernelGGL(append_kernel, dim3(nblk(n2)), dim3(256), 0, st, n2, isnew, pos, I2, J2,
-                               (const int*)c->d_conn.p, (const double*)c->d_rad.p, (const double*)c->d_crad.p, c->P,
-                               c->ea, c->eb, np, newkeys);
-            BP_TRY(hipGetLastError());
-            // merged sorted keys
-            DBuf merged;
-            BP_TRY(merged.ensure(sizeof(int64_t) * (size_t)(np + n_new)));
-            hipLaunchKernelGGL(merge_kernel, dim3(nblk(np + n_new)), dim3(256), 0, st, np,
-                               (const int64_t*)c->bp[S_KEYS].p, n_new, (const int64_t*)newkeys,
-                               (int64_t*)merged.p);
-            BP_TRY(hipGetLastError());
-            BP_TRY(hipStreamSynchronize(st));
-            c->bp[S_KEYS].release();
-            c->bp[S_KEYS] = merged;
-            c->n_pairs = np + n_new;
-        }
-    }
+    hipStream_t st = (hipStream_t)stream;
+    if (c->lp_sync(st, false) != 0) return -1;
     BP_TRY(hipStreamSynchronize(st));
-    c->pattern_valid = false;
+    // a new list: every candidate is a new pair with a fresh state
+    c->n_pairs = 0;
+    c->keys_valid = false;
+    c->filtered_valid = false;
     c->lp_on = false;
+    int64_t n = 0;
+    if (run_broadphase(c, coords, margin, cell_size, shared_node_filter ? 2 : 0, nullptr, &n, st) != 0) return -1;
+    const int* I = (const int*)c->bp[S_RAW].p;
+    const int* J = I + c->raw_cap;
+    c->cand_I = I;
+    c->cand_J = J;
+    c->last_cand = n;
+    if (merge_candidates(c, I, J, n, st) != 0) return -1;
+    *n_pairs = c->n_pairs;
     return 0;
 }
 

@@ -66,6 +66,7 @@ struct Ctx {
     DBuf d_sd, d_sg;
     // broadphase scratch and candidate bookkeeping
     DBuf bp[18];
+    DBuf gj;                        // constraint Jacobian: per-pair flags, row lengths, offsets
     int64_t last_cand = 0;
     int64_t raw_n = 0, raw_cap = 1;
     double last_cell = 0.0;
