@@ -291,9 +291,15 @@ int xk_lp_features(xk_ctx* ctx, const double* coords_ref, const double* u, const
  * assembly to the watch list until the next xk_lp_verify / xk_lp_off, pattern
  * rebuild or pair-list change.  Synchronous. */
 int xk_lp_select(xk_ctx* ctx, int64_t* n_watch, void* stream);
-/* Verification pass: exact narrowphase (geometry, active set frozen) over the pruned
- * pairs at x = coords_ref + u, n_miss = #(pruned pairs with gap <= g_on);
- * switches the watch list off.  Synchronous. */
+/* Verification pass: the pruned pairs get the narrowphase and frame transport
+ * of every fused assembly since xk_lp_select, in order (each assembly over the
+ * watch list records its translational displacement, up to 64 of them; the
+ * frame transport depends on the path, so this leaves every pair's state
+ * bitwise as the full path's assemblies would), active set frozen;
+ * n_miss = #(pruned pairs with gap <= g_on at the last geometry), plus 1 if
+ * more than 64 assemblies ran (the caller then re-runs the call on the full
+ * path).  coords_ref is the reference coordinates (device, n_nodes x 3); u
+ * is not read.  Switches the watch list off.  Synchronous. */
 int xk_lp_verify(xk_ctx* ctx, const double* coords_ref, const double* u, double g_on, int64_t* n_miss,
                 void* stream);
 int xk_lp_off(xk_ctx* ctx);

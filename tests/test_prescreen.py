@@ -88,6 +88,20 @@ def training_set(runs):
     return np.concatenate(Xs), np.concatenate(ys)
 
 
+PAIR_FIELDS = ("elem_a", "elem_b", "s", "t", "gap", "normal", "tangent1", "p_n", "k_pen", "k_t", "q_trial_norm",
+               "dissipation", "z_t", "status", "stick")
+
+
+def same_pair_state(ctx_a, ctx_b, mask=None):
+    """Every state field of every pair bitwise equal (frames and friction history included)."""
+    a, b = ctx_a.get_pairs(), ctx_b.get_pairs()
+    for k in PAIR_FIELDS:
+        x, y = a[k], b[k]
+        if mask is not None:
+            x, y = x[mask], y[mask]
+        assert np.array_equal(x, y), k
+
+
 def same_trajectory(a, b):
     assert len(a.calls) == len(b.calls)
     for p, q in zip(a.calls, b.calls):
@@ -98,11 +112,14 @@ def same_trajectory(a, b):
 
 
 def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
-    """Train on features logged from full-path runs of three squeezes, then
-    solve held-out squeezes with the prescreen: identical trajectory bit for
-    bit (attempts, active counts, residual histories, final u), and the share of
-    pairs the narrowphase visits per Newton attempt against the method without
-    it (all candidates)."""
+    """Train on features logged from full-path runs of three frictionless
+    squeezes, then solve held-out squeezes with the prescreen, frictionless and
+    with Coulomb friction mu = 0.3: identical trajectory bit for bit (attempts,
+    active counts, residual histories, final u), every pair's final state bit
+    for bit (the verification pass replays, for the pruned pairs, the frame
+    transport of every assembly of the call), and the share of pairs the
+    narrowphase visits per Newton attempt against the method without it (all
+    candidates)."""
     P = pkg("prescreen")
     W = pkg("workloads")
     train = [run(W.squeeze_case(f), log=True) for f in (80.0, 100.0, 120.0)]
@@ -112,10 +129,13 @@ def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
     hist = net.fit(X, y, epochs=300, lr=3e-3)
     assert hist[-1] < hist[0]
     net.choose_threshold(X, y, margin=1.0)
-    for case in (W.squeeze_case(110.0), W.squeeze_case(100.0, n_wires=19)):
-        full, _ = run(case)
-        lp, _ = run(case, net=net)
+    cases = (W.squeeze_case(110.0), W.squeeze_case(100.0, n_wires=19),
+             W.squeeze_case(60.0, mu=0.3), W.squeeze_case(100.0, n_wires=19, mu=0.3))
+    for case in cases:
+        full, bf = run(case)
+        lp, bl = run(case, net=net)
         same_trajectory(full, lp)
+        same_pair_state(bf.ctx, bl.ctx)   # frames of pruned pairs transported once per assembly, as the full path
         watched = sum(w for w, _ in lp.watch) / sum(n for _, n in lp.watch)
         print(f"watched {watched:.3f} of candidate pairs per attempt, rechecks {lp.n_rechecks} "
               f"of {len(lp.calls)} calls")
@@ -126,7 +146,8 @@ def test_prescreen_keeps_the_trajectory_exact_and_prunes(gpu):
 def test_prescreened_assembly_is_bitwise_the_full_path_at_91_wires(gpu):
     """With the committed weights on the 91-wire workload: R, K_T and the active
     count of one assembly over the compacted watch list equal the full path's
-    bit for bit, and so do the watched pairs' states once written back."""
+    bit for bit, and so does every pair's state after the verification pass
+    (watched pairs written back, pruned pairs' geometry replayed)."""
     wl, C, A, Kc, P = pkg("workloads"), pkg("context"), pkg("assembly"), pkg("contact"), pkg("prescreen")
     case = wl.C3
     st = wl.make_state(case)
@@ -144,13 +165,48 @@ def test_prescreened_assembly_is_bitwise_the_full_path_at_91_wires(gpu):
     assert 0 < n_watch < ctx.n_pairs
     ctx.pairs_snapshot()
     R1, K1, _, n1 = (t.clone() for t in asm.assemble())
-    ctx.lp_off()
+    assert ctx.lp_verify(asm.u) >= 0      # watched pairs written back, pruned pairs' geometry replayed
     lp_state = ctx.get_pairs()
     ctx.pairs_restore()
     R0, K0, _, n0 = (t.clone() for t in asm.assemble())     # full path from the same state
     full_state = ctx.get_pairs()
     assert gpu.equal(R0, R1) and gpu.equal(K0, K1) and int(n0.item()) == int(n1.item())
-    sinc = full_state["status"] != 0
-    for k in ("p_n", "z_t", "status", "gap", "s", "t"):
-        assert np.array_equal(lp_state[k][sinc], full_state[k][sinc]), k
+    for k in PAIR_FIELDS:
+        assert np.array_equal(lp_state[k], full_state[k]), k
     print(f"91 wires: watched {n_watch} of {ctx.n_pairs} pairs")
+
+
+def test_prescreen_at_1000_wires_is_the_full_path(gpu):
+    """The 1000-wire strand with the committed weights, FP32-mixed: three
+    assemblies at different displacements over the watch list (R, K_T and the
+    active count of each bitwise the full path's), then the verification pass;
+    every pair's state equals the full path's after the same three assemblies."""
+    wl, C, A, Kc, P = pkg("workloads"), pkg("context"), pkg("assembly"), pkg("contact"), pkg("prescreen")
+    case = wl.C5
+    st = wl.make_state(case)
+    m = st.mesh
+    ctx = C.DeviceContext(m.node_coords, m.connectivity, C.circle_section(case.wire_radius, case.E, case.nu),
+                          radii=m.radii, precision="mixed")
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu, f_ext=st.f_ext,
+                           fixed_dofs=st.fixed_dofs,
+                           config=Kc.ContactConfig(mu=case.mu, exclude_same_layer=True,
+                                                   elem_layer_map=wl.layer_map(st.elem_layer)))
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.set_prescreen(P.PrescreenNet.load(), elem_layer=st.elem_layer)
+    asm.begin_increment(st.u)
+    n_watch = asm.n_watch
+    print(f"1000 wires: watched {n_watch} of {ctx.n_pairs} pairs", flush=True)
+    assert 0 < n_watch < ctx.n_pairs
+    rng = np.random.default_rng(1001)
+    us = [st.u + rng.normal(0.0, 2e-4 * case.wire_radius, st.u.shape) for _ in range(3)]
+    ctx.pairs_snapshot()
+    lp_out = [tuple(t.clone() for t in asm.assemble(u)) for u in us]
+    assert ctx.lp_verify(asm.u) >= 0
+    lp_state = ctx.get_pairs()
+    ctx.pairs_restore()
+    full_out = [tuple(t.clone() for t in asm.assemble(u)) for u in us]
+    full_state = ctx.get_pairs()
+    for (R1, K1, _, n1), (R0, K0, _, n0) in zip(lp_out, full_out):
+        assert gpu.equal(R0, R1) and gpu.equal(K0, K1) and int(n0.item()) == int(n1.item())
+    for k in PAIR_FIELDS:
+        assert np.array_equal(lp_state[k], full_state[k]), k

@@ -932,6 +932,9 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(hipEventRecord(ev[0], st));
         XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
                             (double*)c->d_sd.p, c->d_sg.p, act, st, nullptr, c->pair_block));
+        if (lp)
+            XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
+                                       st));
         XK_TRY(hipEventRecord(ev[1], st));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipEventRecord(ev[2], st));
@@ -942,6 +945,10 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
                             (double*)c->d_sd.p, c->d_sg.p, act, c->side, nullptr, c->pair_block));
+        // the geometry this assembly gives the pairs it skips, for the verification pass
+        if (lp)
+            XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
+                                       c->side));
         XK_TRY(hipEventRecord(c->ev_join, c->side));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
@@ -1056,6 +1063,9 @@ int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
                            (int*)c->lp_prune.p, st));
     c->n_watch = nw;
     c->n_prune = n - nw;
+    XK_TRY(c->lp_hist.ensure(sizeof(double) * 3 * (size_t)c->n_nodes * Ctx::LP_HIST_CAP));
+    XK_TRY(c->lp_hist_cnt.ensure(sizeof(int)));
+    XK_TRY(launch_fill_i32(1, (int*)c->lp_hist_cnt.p, 0, st));
     if (c->lp_compact(st) != 0) return -1;
     XK_TRY(hipStreamSynchronize(st));
     c->lp_on = true;
@@ -1065,23 +1075,28 @@ int xk_lp_select(xk_ctx* c, int64_t* n_watch, void* stream) {
 
 int xk_lp_verify(xk_ctx* c, const double* coords_ref, const double* u, double g_on, int64_t* n_miss, void* stream) {
     if (!c || !coords_ref) return -1;
+    (void)u;   // the geometries come from the assemblies' history
     hipStream_t st = (hipStream_t)stream;
     int64_t miss = 0;
     const bool was_on = c->lp_on;
     if (c->lp_sync(st, false) != 0) return -1;   // the watched pairs' state back in the pair list
     if (was_on && c->n_prune > 0) {
-        // exact narrowphase (geometry only, active set frozen) over the pruned pairs
-        PairParams pp{};
-        pp.n_sinc = c->n_sinc;
-        XK_TRY(launch_pairs(c->prec, c->n_prune, c->P, coords_ref, u, nullptr, PAIR_GEOM, pp, (double*)c->d_sd.p,
-                            c->d_sg.p, nullptr, st, (const int*)c->lp_prune.p, c->pair_block));
+        // the pruned pairs get the narrowphase and frame transport of every
+        // assembly of the call, in order (active set frozen, as in the call)
+        int n_hist = 0;
+        XK_TRY(hipMemcpyAsync(&n_hist, c->lp_hist_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        XK_TRY(hipStreamSynchronize(st));
+        if (n_hist > Ctx::LP_HIST_CAP) miss = 1;   // history overflow: the call is re-run on the full path
+        XK_TRY(launch_pair_replay(c->n_prune, (const int*)c->lp_prune.p, c->P, coords_ref, (const double*)c->lp_hist.p,
+                                  3 * (int64_t)c->n_nodes, std::min(n_hist, (int)Ctx::LP_HIST_CAP), st));
+        XK_TRY(launch_fill_i32(1, (int*)c->lp_hist_cnt.p, 0, st));
         XK_TRY(c->lp_count.ensure(sizeof(int)));
         XK_TRY(launch_fill_i32(1, (int*)c->lp_count.p, 0, st));
         XK_TRY(launch_lp_miss(c->n_prune, (const int*)c->lp_prune.p, c->P.gap, g_on, (int*)c->lp_count.p, st));
         int h = 0;
         XK_TRY(hipMemcpyAsync(&h, c->lp_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
         XK_TRY(hipStreamSynchronize(st));
-        miss = h;
+        miss += h;
     }
     c->lp_on = false;
     if (n_miss) *n_miss = miss;

@@ -65,6 +65,86 @@ __device__ __forceinline__ V3<double> min_axis(V3<double> v) {
     return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
 }
 
+// Clamped segment-segment closest points (_compute.py:55-151): s, t, the
+// distance and the unit normal (pA - pB) / dist, (0, 0, 1) below 1e-30.
+__device__ __forceinline__ void closest_points(const V3<double>& xA0, const V3<double>& xA1, const V3<double>& xB0,
+                                               const V3<double>& xB1, double& s, double& t, double& dist,
+                                               V3<double>& nrm) {
+    V3<double> dA = xA1 - xA0, dB = xB1 - xB0, w0 = xA0 - xB0;
+    double a = dot(dA, dA), b = dot(dA, dB), c = dot(dB, dB), d = dot(dA, w0), e = dot(dB, w0);
+    double det = a * c - b * b;
+    double ac = a * c;
+    bool par = det < 1e-10 * (ac > 1e-30 ? ac : 1e-30);
+    double s_unc = par ? 0.0 : (b * e - c * d) / det;
+    double t_unc = par ? 0.0 : (a * e - b * d) / det;
+    bool c_ok = c > 1e-10, a_ok = a > 1e-10;
+    double sc = c_ok ? c : 1.0, sa = a_ok ? a : 1.0;
+    s = par ? 0.0 : clip01(s_unc);
+    t = par ? (c_ok ? clip01(e / sc) : 0.0) : clip01(t_unc);
+    bool s_cl = (s_unc < 0.0) || (s_unc > 1.0);
+    bool t_cl = (t_unc < 0.0) || (t_unc > 1.0);
+    if (s_cl && !par) t = c_ok ? clip01((b * s + e) / sc) : 0.0;
+    if (t_cl && !par) {
+        s = a_ok ? clip01((b * t - d) / sa) : 0.0;
+        t = c_ok ? clip01((b * s + e) / sc) : 0.0;
+    }
+    V3<double> diff = (xA0 + s * dA) - (xB0 + t * dB);
+    dist = sqrt(dot(diff, diff));
+    double sd_ = dist > 1e-30 ? dist : 1e-30;
+    nrm = V3<double>{diff.x / sd_, diff.y / sd_, diff.z / sd_};
+    if (dist < 1e-30) nrm = V3<double>{0.0, 0.0, 1.0};
+}
+
+// Contact frame (_compute.py:154-261): n = nrm normalised, t1 = the previous
+// t1 parallel-transported along the minimal rotation pn -> n, projected and
+// normalised; else the previous t1 projected; else the min-|n| axis projected.
+// t2 = n x t1 is never stored.
+__device__ __forceinline__ void transport_frame(const V3<double>& nrm, const V3<double>& pn, const V3<double>& pt1,
+                                                V3<double>& n, V3<double>& t1) {
+    double nn_ = sqrt(dot(nrm, nrm));
+    nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
+    n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
+    bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
+    bool has_n = sqrt(dot(pn, pn)) > 1e-10;
+    bool done = false;
+    if (has_t && has_n) {
+        double on = sqrt(dot(pn, pn));
+        on = on > 1e-30 ? on : 1e-30;
+        V3<double> no{pn.x / on, pn.y / on, pn.z / on};
+        V3<double> v = cross(no, n);
+        double sv = sqrt(dot(v, v));
+        double cv = dot(no, n);
+        V3<double> tt = pt1;
+        if (sv >= 1e-12) {
+            V3<double> vxt = cross(v, pt1);
+            V3<double> vxvxt = cross(v, vxt);
+            double fac = (1.0 - cv) / (sv * sv);
+            tt = (pt1 + vxt) + fac * vxvxt;
+        }
+        V3<double> raw = tt - dot(tt, n) * n;
+        double rn = sqrt(dot(raw, raw));
+        if (rn > 1e-10) {
+            t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+            done = true;
+        }
+    }
+    if (!done && has_t) {
+        V3<double> raw = pt1 - dot(pt1, n) * n;
+        double rn = sqrt(dot(raw, raw));
+        if (rn > 1e-10) {
+            t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+            done = true;
+        }
+    }
+    if (!done) {
+        V3<double> ref = min_axis(n);
+        V3<double> raw = ref - dot(ref, n) * n;
+        double rn = sqrt(dot(raw, raw));
+        rn = rn > 1e-30 ? rn : 1e-30;
+        t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+    }
+}
+
 // COAT: the Kelvin-Voigt coating terms are compiled in (pp.coat_on); the
 // plain instantiation carries none of their registers.
 template <class GT, int BS, bool COAT>
@@ -102,29 +182,9 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
         if (modes & PAIR_GEOM) {
             V3<double> xA0 = node_x(Xr, u, a0), xA1 = node_x(Xr, u, a1);
             V3<double> xB0 = node_x(Xr, u, b0), xB1 = node_x(Xr, u, b1);
-            V3<double> dA = xA1 - xA0, dB = xB1 - xB0, w0 = xA0 - xB0;
-            double a = dot(dA, dA), b = dot(dA, dB), c = dot(dB, dB), d = dot(dA, w0), e = dot(dB, w0);
-            double det = a * c - b * b;
-            double ac = a * c;
-            bool par = det < 1e-10 * (ac > 1e-30 ? ac : 1e-30);
-            double s_unc = par ? 0.0 : (b * e - c * d) / det;
-            double t_unc = par ? 0.0 : (a * e - b * d) / det;
-            bool c_ok = c > 1e-10, a_ok = a > 1e-10;
-            double sc = c_ok ? c : 1.0, sa = a_ok ? a : 1.0;
-            s = par ? 0.0 : clip01(s_unc);
-            t = par ? (c_ok ? clip01(e / sc) : 0.0) : clip01(t_unc);
-            bool s_cl = (s_unc < 0.0) || (s_unc > 1.0);
-            bool t_cl = (t_unc < 0.0) || (t_unc > 1.0);
-            if (s_cl && !par) t = c_ok ? clip01((b * s + e) / sc) : 0.0;
-            if (t_cl && !par) {
-                s = a_ok ? clip01((b * t - d) / sa) : 0.0;
-                t = c_ok ? clip01((b * s + e) / sc) : 0.0;
-            }
-            V3<double> diff = (xA0 + s * dA) - (xB0 + t * dB);
-            double dist = sqrt(dot(diff, diff));
-            double sd_ = dist > 1e-30 ? dist : 1e-30;
-            V3<double> nrm{diff.x / sd_, diff.y / sd_, diff.z / sd_};
-            if (dist < 1e-30) nrm = V3<double>{0.0, 0.0, 1.0};
+            double dist;
+            V3<double> nrm;
+            closest_points(xA0, xA1, xB0, xB1, s, t, dist, nrm);
             if (coat) {
                 const double cra = P.cra[i], crb = P.crb[i];
                 gap = dist - (cra + crb);
@@ -138,49 +198,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 goto done_pair;
             }
 
-            // frame (n, t1); t2 = n x t1 is never stored
-            double nn_ = sqrt(dot(nrm, nrm));
-            nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
-            n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
-            bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
-            bool has_n = sqrt(dot(pn, pn)) > 1e-10;
-            bool done = false;
-            if (has_t && has_n) {
-                double on = sqrt(dot(pn, pn));
-                on = on > 1e-30 ? on : 1e-30;
-                V3<double> no{pn.x / on, pn.y / on, pn.z / on};
-                V3<double> v = cross(no, n);
-                double sv = sqrt(dot(v, v));
-                double cv = dot(no, n);
-                V3<double> tt = pt1;
-                if (sv >= 1e-12) {
-                    V3<double> vxt = cross(v, pt1);
-                    V3<double> vxvxt = cross(v, vxt);
-                    double fac = (1.0 - cv) / (sv * sv);
-                    tt = (pt1 + vxt) + fac * vxvxt;
-                }
-                V3<double> raw = tt - dot(tt, n) * n;
-                double rn = sqrt(dot(raw, raw));
-                if (rn > 1e-10) {
-                    t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
-                    done = true;
-                }
-            }
-            if (!done && has_t) {
-                V3<double> raw = pt1 - dot(pt1, n) * n;
-                double rn = sqrt(dot(raw, raw));
-                if (rn > 1e-10) {
-                    t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
-                    done = true;
-                }
-            }
-            if (!done) {
-                V3<double> ref = min_axis(n);
-                V3<double> raw = ref - dot(ref, n) * n;
-                double rn = sqrt(dot(raw, raw));
-                rn = rn > 1e-30 ? rn : 1e-30;
-                t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
-            }
+            transport_frame(nrm, pn, pt1, n, t1);
             __builtin_nontemporal_store(s, &P.s[i]);
             __builtin_nontemporal_store(t, &P.t[i]);
             __builtin_nontemporal_store(gap, &P.gap[i]);
@@ -450,6 +468,55 @@ done_pair:
             act_part[lb] = c;
         }
     }
+}
+
+// Replay, for pairs the fused assembly did not visit, of the geometry it
+// would have given them: hist holds the translational displacement of each of
+// the n_hist assemblies of a Newton call (3 values per node, assembly k at
+// hist + k * stride).  Each pair of idx gets the narrowphase and the frame
+// transport of every assembly in order, its state carried in registers, and
+// the last one stored -- bitwise what one PAIR_GEOM pass per assembly leaves
+// on a pair that contributes nothing (status INACTIVE, p_n = 0; no coating).
+__global__ void __launch_bounds__(256)
+pair_replay_kernel(int64_t n_idx, const int* __restrict__ idx, PairSoA P, const double* __restrict__ Xr,
+                   const double* __restrict__ hist, int64_t stride, int n_hist) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_idx) return;
+    const int64_t i = idx[w];
+    const int64_t cap = P.cap;
+    const int a0 = P.na0[i], a1 = P.na1[i], b0 = P.nb0[i], b1 = P.nb1[i];
+    const double ra = P.ra[i], rb = P.rb[i];
+    V3<double> pt1{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
+    V3<double> pn{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
+    double s = 0.0, t = 0.0, gap = 0.0;
+    for (int k = 0; k < n_hist; ++k) {
+        const double* h = hist + (size_t)k * stride;
+        auto x_of = [&](int q) {
+            return V3<double>{Xr[3 * q] + h[3 * (size_t)q], Xr[3 * q + 1] + h[3 * (size_t)q + 1],
+                              Xr[3 * q + 2] + h[3 * (size_t)q + 2]};
+        };
+        double dist;
+        V3<double> nrm, n, t1;
+        closest_points(x_of(a0), x_of(a1), x_of(b0), x_of(b1), s, t, dist, nrm);
+        gap = dist - (ra + rb);
+        transport_frame(nrm, pn, pt1, n, t1);
+        pn = n;
+        pt1 = t1;
+    }
+    if (n_hist <= 0) return;
+    P.s[i] = s;
+    P.t[i] = t;
+    P.gap[i] = gap;
+    P.n[i] = pn.x; P.n[cap + i] = pn.y; P.n[2 * cap + i] = pn.z;
+    P.t1[i] = pt1.x; P.t1[cap + i] = pt1.y; P.t1[2 * cap + i] = pt1.z;
+}
+
+hipError_t launch_pair_replay(int64_t n_idx, const int* idx, const PairSoA& P, const double* Xr, const double* hist,
+                              int64_t stride, int n_hist, hipStream_t st) {
+    if (n_idx <= 0 || n_hist <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_replay_kernel, dim3((unsigned)((n_idx + 255) / 256)), dim3(256), 0, st, n_idx, idx, P,
+                       Xr, hist, stride, n_hist);
+    return hipGetLastError();
 }
 
 int pair_blocks(int64_t n_pairs, int bs) { return (int)((n_pairs + bs - 1) / bs); }

@@ -365,6 +365,25 @@ __global__ void lp_adam_kernel(float* __restrict__ prm, const float* __restrict_
     prm[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
 }
 
+// ------------------------------------------------------------ assembly history
+// Each fused assembly over the watch list appends the translational
+// displacement it was given (3 values per node) to slot *count of hist, if
+// the slot exists; a one-lane kernel then advances *count.  The verification
+// pass replays these geometries for the pruned pairs.
+__global__ void lp_hist_push_kernel(int64_t n3, const double* __restrict__ u, double* __restrict__ hist,
+                                    int64_t stride, int cap, const int* __restrict__ count) {
+    const int slot = *count;
+    if (slot >= cap) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n3) return;
+    const int64_t node = q / 3, c = q - 3 * node;
+    hist[(size_t)slot * stride + q] = u[6 * node + c];
+}
+
+__global__ void lp_hist_bump_kernel(int* __restrict__ count) {
+    if (threadIdx.x == 0) *count += 1;
+}
+
 // ------------------------------------------------------------ launchers
 int lp_n_params() { return LP_NPARAM; }
 
@@ -411,6 +430,17 @@ hipError_t launch_lp_split(int64_t n, const int64_t* keep, const int64_t* pos, i
 hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g_on, int* count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(lp_miss_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, idx, gap, g_on, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st) {
+    const int64_t n3 = 3 * (int64_t)n_nodes;
+    if (n3 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lp_hist_push_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, n3, u, hist, n3,
+                       cap, (const int*)count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(lp_hist_bump_kernel, dim3(1), dim3(64), 0, st, count);
     return hipGetLastError();
 }
 

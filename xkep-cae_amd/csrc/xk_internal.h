@@ -129,6 +129,10 @@ struct Ctx {
     float lp_thr = 0.f;
     bool lp_ready = false;
     DBuf lp_feat, lp_gprev, lp_keep, lp_pos, lp_watch, lp_prune, lp_count;
+    // translational displacement of every assembly over the watch list within a
+    // Newton call (LP_HIST_CAP slots of 3 n_nodes doubles) and the slot counter
+    DBuf lp_hist, lp_hist_cnt;
+    static constexpr int LP_HIST_CAP = 64;
     int64_t lp_gprev_n = 0, lp_gprev_cap = 0;
     int64_t n_watch = 0, n_prune = 0;
     bool lp_on = false;          // the fused assembly runs the pair kernel over the watch list
@@ -233,6 +237,9 @@ hipError_t launch_lp_infer_f32(int64_t n, const float* feat, const float* prm, f
 hipError_t launch_lp_split(int64_t n, const int64_t* keep, const int64_t* pos, int* watch, int* prune,
                            hipStream_t st);
 hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g_on, int* count, hipStream_t st);
+hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st);
+hipError_t launch_pair_replay(int64_t n_idx, const int* idx, const PairSoA& P, const double* Xr, const double* hist,
+                              int64_t stride, int n_hist, hipStream_t st);
 hipError_t launch_lp_grad(int64_t n, const float* X, const float* y, const float* prm, float pos_w, float* partial,
                           float* grad, hipStream_t st);
 hipError_t launch_permute(int esz, int64_t n, int comps, const void* a, int64_t sa, void* b, int64_t sb,
