@@ -160,3 +160,37 @@ def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fix
         assert b.graph_attempts_captured > 0
     tol = 1e-8 if precision == "fp64" else 1e-4
     assert np.linalg.norm(res.u - g["u_final"]) <= tol * np.linalg.norm(g["u_final"])
+
+
+@pytest.mark.gpu
+def test_mixed_and_fp64_take_the_same_newton_trajectory_at_91_wires(gpu):
+    """The 91-wire strand of the benchmark (r = 0.4, L = 80, 150 elements per
+    wire, lay angle 12 deg) squeezed by 0.1 per outer wire with Coulomb
+    friction mu = 0.3, solved with the Newton attempts on the device and
+    replayed from HIP graphs, FP64 and FP32-mixed: every call has the same time
+    window, attempt count, convergence, divergence flag and active count, and
+    the per-attempt residuals agree to 1e-3 (the mixed tangent only steers the
+    iteration).  With tests/golden/newton_91wire_mu03.npz present (the
+    reference's own solver on this case), both also reproduce its trajectory."""
+    import os
+
+    TP, W = pkg("train_prescreen"), pkg("workloads")
+    sq = W.squeeze_case(0.1, n_wires=91, n_elems=150, length=80.0, radius=0.4, lay_angle_deg=12.0, mu=0.3)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"91-wire frictional squeeze, {prec}, device attempts with graphs", flush=True)
+        res, b = TP.run_case(sq, precision=prec, device_solve=True, use_graph=True)
+        assert b.graph_error is None, b.graph_error
+        out[prec] = res
+    a, m = out["fp64"], out["mixed"]
+    assert len(a.calls) == len(m.calls) and len(a.calls) >= 3
+    for p, q in zip(a.calls, m.calls):
+        assert (p.load_frac, p.n_attempts, p.n_active, p.converged, p.diverged) == \
+            (q.load_frac, q.n_attempts, q.n_active, q.converged, q.diverged)
+        np.testing.assert_allclose(q.res_history, p.res_history, rtol=1e-3, atol=1e-6 * p.res_history[0])
+    assert any(c.n_active > 0 for c in a.calls)       # friction engaged
+    ref = os.path.join(os.path.dirname(__file__), "golden", "newton_91wire_mu03.npz")
+    if os.path.exists(ref):
+        g = golden("newton_91wire_mu03.npz")
+        check_trajectory(g, a, "fp64")
+        check_trajectory(g, m, "mixed")

@@ -263,13 +263,14 @@ static void launch_gather_bs(int nb, const GatherArgs& g, const void* Ke, const 
                        (KT*)K);
 }
 
+// K of type T; the pairs' tangent records are FP64 in both forms
 template <class T>
 static hipError_t launch_gather_t(int nb, const GatherArgs& g, const void* Ke, const void* sg, void* K,
                                   hipStream_t st) {
     switch (g.block) {
-        case 128: launch_gather_bs<T, T, 128>(nb, g, Ke, sg, K, st); break;
-        case 256: launch_gather_bs<T, T, 256>(nb, g, Ke, sg, K, st); break;
-        case 512: launch_gather_bs<T, T, 512>(nb, g, Ke, sg, K, st); break;
+        case 128: launch_gather_bs<T, double, 128>(nb, g, Ke, sg, K, st); break;
+        case 256: launch_gather_bs<T, double, 256>(nb, g, Ke, sg, K, st); break;
+        case 512: launch_gather_bs<T, double, 512>(nb, g, Ke, sg, K, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -186,7 +186,7 @@ int Ctx::build_pattern() {
     if (direct && !host_build) {
         if (build_pattern_gpu(nullptr) != 0) return -1;
         XK_TRY(d_sd.ensure((size_t)std::max(n_sinc, 1) * SD_REC * sizeof(double)));
-        XK_TRY(d_sg.ensure((size_t)std::max(n_sinc, 1) * SG_REC * (prec == 0 ? 8 : 4)));
+        XK_TRY(d_sg.ensure((size_t)std::max(n_sinc, 1) * SG_REC * sizeof(double)));
         pattern_valid = true;
         return 0;
     }
@@ -318,7 +318,7 @@ int Ctx::build_pattern() {
         n_cdesc = (int)(desc.size() / 4);
     }
     XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * SD_REC * sizeof(double)));
-    XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * (prec == 0 ? 8 : 4)));
+    XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * sizeof(double)));
     n_sinc = ns;
     n_blocks = nblk;
     host_pattern_valid = true;
@@ -376,9 +376,9 @@ GatherArgs Ctx::gather_args(int node_mode) const {
 }
 
 hipError_t Ctx::launch_elements(const double* u, bool want_f, void* K, double mass_coef, const double* M,
-                                hipStream_t st) {
+                                hipStream_t st, int kprec) {
     if (direct)
-        return launch_beam_direct(prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
+        return launch_beam_direct(kprec >= 0 ? kprec : prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
                                   (const int8_t*)d_eflag.p, (const int*)d_eslot.p, (const int*)d_gptr.p,
                                   mass_coef, M, want_f ? (double*)d_fint.p : nullptr, K, (int*)d_flags.p, st);
     return launch_beam(prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,

@@ -550,13 +550,16 @@ static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, cons
     return hipGetLastError();
 }
 
+// The tangent records are FP64 in both precision forms: the device linear
+// solve factors the pairs' blocks from them, and the FP32 rounding of contact
+// blocks changes the Newton trajectory of frictional strands (measured at 91
+// wires); the mixed form rounds them to FP32 only where the gather writes K.
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
                         int* act_part, hipStream_t st, const int* idx, int bs) {
+    (void)prec;
     if (n_pairs <= 0) return hipSuccess;
-    if (prec == 0)
-        return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (double*)sg, act_part, st, idx);
-    return launch_pairs_t<float>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (float*)sg, act_part, st, idx);
+    return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (double*)sg, act_part, st, idx);
 }
 
 }  // namespace xk

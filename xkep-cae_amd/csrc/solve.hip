@@ -780,7 +780,7 @@ int Ctx::solve_setup() {
     SV_TRY(sv_upload(sv[SV_POS], pos));
     SV_TRY(sv_upload(sv[SV_CSLOT], cslot));
     const size_t nd = 6 * (size_t)n_nodes;
-    SV_TRY(sv[SV_KB].ensure((size_t)std::max(n_blocks, 1) * 9 * (prec == 0 ? 8 : 4)));
+    SV_TRY(sv[SV_KB].ensure((size_t)std::max(n_blocks, 1) * 9 * sizeof(double)));
     SV_TRY(sv[SV_AB].ensure(sizeof(double) * 108 * (size_t)n_nodes));
     SV_TRY(sv[SV_LF].ensure(sizeof(double) * 36 * (size_t)n_nodes));
     SV_TRY(sv[SV_SINV].ensure(sizeof(double) * 36 * (size_t)n_nodes));
@@ -869,27 +869,23 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     SV_RB(rocblas_set_stream((rocblas_handle)sv_handle, st));
     const int ns = n_sinc;
     const int nd = 6 * n_nodes;
-    // beam-only tangent with the mass term, then the chain factors
-    SV_TRY(launch_elements(u, false, sv[SV_KB].p, mass_coef, M, st));
-    if (prec == 0)
-        hipLaunchKernelGGL(sv_extract_kernel<double>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
-                           (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
-                           (const double*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
-    else
-        hipLaunchKernelGGL(sv_extract_kernel<float>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
-                           (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
-                           (const float*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
+    // beam-only tangent with the mass term, then the chain factors.  In both
+    // precision forms the solve factors the FP64 element tangent and the FP64
+    // pair records: the Newton step of a frictional strand is sensitive to
+    // the FP32 rounding of the tangent (the 91-wire squeeze takes another
+    // trajectory with it), so the mixed form's FP32 K is an output for
+    // consumers of the assembled matrix, not the operator this solve inverts.
+    SV_TRY(launch_elements(u, false, sv[SV_KB].p, mass_coef, M, st, 0));
+    hipLaunchKernelGGL(sv_extract_kernel<double>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+                       (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
+                       (const double*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
     SV_TRY(hipGetLastError());
     hipLaunchKernelGGL(sv_factor_kernel, dim3(n_chains), dim3(64), 0, st, (const int*)d_cstart.p,
                        (const double*)sv[SV_AB].p, (double*)sv[SV_LF].p, (double*)sv[SV_SINV].p, (int*)d_flags.p);
     SV_TRY(hipGetLastError());
     if (ns > 0) {
-        if (prec == 0)
-            hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
-                               (const double*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
-        else
-            hipLaunchKernelGGL(sv_slot_pack_kernel<float>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
-                               (const float*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
+        hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
+                           (const double*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
         SV_TRY(hipGetLastError());
         hipLaunchKernelGGL(sv_compliance_kernel, dim3(n_chains), dim3(256), 0, st, (const int*)d_cstart.p,
                            (const int*)sv[SV_CN].p, (const int*)sv[SV_CNODE].p, (const int*)sv[SV_CHCN0].p,

@@ -201,8 +201,9 @@ struct Ctx {
     GatherArgs gather_args(int node_mode) const;
     // element kernel of the context's path (direct: K gets the beam blocks
     // and the mass term; staged: element records)
+    // kprec: precision of K (-1: the context's; 0 FP64 for the solve's beam rows)
     hipError_t launch_elements(const double* u, bool want_f, void* K, double mass_coef, const double* M,
-                               hipStream_t st);
+                               hipStream_t st, int kprec = -1);
 };
 
 // launchers (one per translation unit)
