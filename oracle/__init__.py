@@ -24,7 +24,12 @@ coating   Kelvin-Voigt coating force (dashpot), stiffness, coating Coulomb
           friction and its stiffness
 assembly  per-iteration composition R = f_int - (f_c + f_fric) - f_ext,
           K_T = K_beam + K_c - K_fric, Generalized-alpha effective terms
-newton    restated dynamic Newton driver (host spsolve) for Newton-count checks
+mlp       the learned prescreen's network (forward, loss, gradient, Adam)
+
+The oracle has no Newton driver of its own: the Newton-count checks run the
+product's driver (``xkep-cae_amd/solver.py``) over the oracle's assembly
+(``tests/_oracle_backend.py``) against the reference solver's own
+trajectories (``tests/golden/newton_*.npz``).
 
 Pinning status of each function is listed in ``PINNING`` below and checked by
 ``tests/test_oracle_*.py``.
@@ -50,6 +55,15 @@ PINNING = {
     "force signs, dashpot rate, k_eff = k + c/dt, symmetry, PSD, Coulomb limit, history reset), node "
     "forces and return mapping in closed form, K_coat = FD of the coating force along a normal approach",
     "contact.update_geometry (coating)": "core gap, compression and activation in closed form",
+    "coating.friction_stiffness_blocks": "central differences of the coating friction force at a fixed "
+    "frame, in stick and in slip",
+    "contact.strategy_detect / strategy_update_geometry": "golden vectors from the reference's "
+    "ContactGeometryStrategy classes (shared-node filter on and off, with and without a config, coating config)",
+    "contact.constraint_jacobian_ptp": "golden vectors from the reference's build_constraint_jacobian; central "
+    "differences of the gaps; a hand-made pair with dropped entries and an inactive pair",
     "assembly.*": "golden vectors of R and K_T from the reference on a 7-wire state",
-    "newton.*": "Newton-iteration counts and active-set trajectory from the reference",
+    "driver (tests/_oracle_backend.py + solver.py)": "Newton-iteration counts, cutbacks, divergence flags and "
+    "active counts of the reference's own solver: frictionless, and Coulomb mu = 0.3 under two loads",
+    "mlp.*": "BF16 rounding against torch, gradient against torch autograd and central differences, Adam "
+    "against its closed form",
 }

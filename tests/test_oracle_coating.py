@@ -211,3 +211,36 @@ def test_no_coating_strategy_returns_zeros():
     assert np.array_equal(nc.friction_forces([], x, cfg, u, u), np.zeros(30))
     assert nc.friction_stiffness([], x, cfg, 30).nnz == 0
     assert nc.process(None) is None
+
+
+@pytest.mark.parametrize("regime", ["stick", "slip"])
+def test_friction_stiffness_is_the_derivative_of_the_friction_force(regime):
+    """At a fixed frame (s, t, t1, t2) and a fixed history before the
+    evaluation, the coating friction force is a function of u alone; its
+    Jacobian over the 12 translational DOFs of the pair's nodes, by central
+    differences, equals the coating friction stiffness (w_k w_l) T D T^T with D
+    from the evaluated state -- k_t I in stick, (mu p_n / |q_trial|) k_t
+    (I - q^ q^T) in slip.  A wrong sign, frame or weight fails this."""
+    s, t = 0.3, 0.7
+    z = (100.0, -50.0) if regime == "stick" else (3600.0, 900.0)
+    p0 = one_pair(0.01, s=s, t=t, z=z, stick=True)
+    c = cfg(mu=0.3)                                   # k_t = 5e5, mu p_n = 3000
+    rng = np.random.default_rng(5 if regime == "stick" else 6)
+    u = np.zeros(NDOF)
+    dofs = (np.arange(4)[:, None] * 6 + np.arange(3)[None, :]).ravel()
+    u[dofs] = rng.normal(0.0, 4e-4, 12)
+    f, p = oc.friction_forces(p0, c, u, np.zeros(NDOF), NDOF)
+    assert bool(p.coating_stick[0]) == (regime == "stick")
+    m, b = oc.friction_stiffness_blocks(p, c)
+    K = ocon.blocks_to_csr(p, m, b, NDOF).toarray()[np.ix_(dofs, dofs)]
+    h = 1e-8
+    J = np.zeros((12, 12))
+    for j, d in enumerate(dofs):
+        up, um = u.copy(), u.copy()
+        up[d] += h
+        um[d] -= h
+        fp = oc.friction_forces(p0, c, up, np.zeros(NDOF), NDOF)[0]
+        fm = oc.friction_forces(p0, c, um, np.zeros(NDOF), NDOF)[0]
+        J[:, j] = (fp - fm)[dofs] / (2 * h)
+    np.testing.assert_allclose(K, J, rtol=1e-6, atol=1e-6 * np.abs(K).max())
+    assert np.abs(K).max() > 1e3

@@ -77,7 +77,7 @@ class StrandAssembly:
     def set_prescreen(self, net, *, elem_layer=None, layer_scale: float = 0.125):
         """Run the per-iteration pair kernel over the learned watch list (None: all pairs)."""
         self.prescreen = net
-        self.lp_layer = elem_layer if elem_layer is not None else self._layer
+        self.lp_layer = self._layer_dev(elem_layer if elem_layer is not None else self._layer)
         self.lp_scale = layer_scale
         if net is not None:
             self.ctx.lp_set(net.host_params(), net.threshold)
@@ -87,8 +87,12 @@ class StrandAssembly:
         Newton call) for every call, as training data."""
         self.lp_log = [] if on else None
         if elem_layer is not None or self.lp_layer is None:
-            self.lp_layer = elem_layer if elem_layer is not None else self._layer
+            self.lp_layer = self._layer_dev(elem_layer if elem_layer is not None else self._layer)
         self.lp_scale = layer_scale
+
+    def _layer_dev(self, layer):
+        """Per-element layer ids as a device int32 tensor (converted once, read every increment)."""
+        return None if layer is None else self.ctx.to_dev(layer, self.torch.int32)
 
     def set_u(self, u):
         self.u.copy_(self.ctx.to_dev(u))

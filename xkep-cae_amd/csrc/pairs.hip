@@ -488,20 +488,38 @@ pair_replay_kernel(int64_t n_idx, const int* __restrict__ idx, PairSoA P, const 
     const double ra = P.ra[i], rb = P.rb[i];
     V3<double> pt1{P.t1[i], P.t1[cap + i], P.t1[2 * cap + i]};
     V3<double> pn{P.n[i], P.n[cap + i], P.n[2 * cap + i]};
+    // reference positions stay in registers; each assembly's displacements are
+    // loaded one step ahead, so their latency overlaps the previous geometry
+    const int q[4] = {a0, a1, b0, b1};
+    double X[12], cur[12], nxt[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) X[3 * k + c] = Xr[3 * q[k] + c];
+    auto load = [&](int step, double* dst) {
+        const double* h = hist + (size_t)step * stride;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dst[3 * k + c] = h[3 * (size_t)q[k] + c];
+    };
+    if (n_hist > 0) load(0, cur);
     double s = 0.0, t = 0.0, gap = 0.0;
     for (int k = 0; k < n_hist; ++k) {
-        const double* h = hist + (size_t)k * stride;
-        auto x_of = [&](int q) {
-            return V3<double>{Xr[3 * q] + h[3 * (size_t)q], Xr[3 * q + 1] + h[3 * (size_t)q + 1],
-                              Xr[3 * q + 2] + h[3 * (size_t)q + 2]};
-        };
+        if (k + 1 < n_hist) load(k + 1, nxt);
+        V3<double> x[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            x[m] = V3<double>{X[3 * m] + cur[3 * m], X[3 * m + 1] + cur[3 * m + 1], X[3 * m + 2] + cur[3 * m + 2]};
         double dist;
         V3<double> nrm, n, t1;
-        closest_points(x_of(a0), x_of(a1), x_of(b0), x_of(b1), s, t, dist, nrm);
+        closest_points(x[0], x[1], x[2], x[3], s, t, dist, nrm);
         gap = dist - (ra + rb);
         transport_frame(nrm, pn, pt1, n, t1);
         pn = n;
         pt1 = t1;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) cur[m] = nxt[m];
     }
     if (n_hist <= 0) return;
     P.s[i] = s;
