@@ -23,12 +23,19 @@ from . import beam, coating, contact
 
 
 class GeneralizedAlpha:
-    """Chung-Hulbert Generalized-alpha with a lumped (diagonal) mass
-    (time_integration/strategy.py:96-232)."""
+    """Chung-Hulbert Generalized-alpha (time_integration/strategy.py:96-232):
+    a lumped mass given as its diagonal, or any mass matrix, and an optional
+    damping matrix C (K_eff += (1 - alpha_f) c1 C, R_eff += C v_mid with
+    c1 = gamma / (beta dt), v_mid = (1 - alpha_f) v + alpha_f v_old)."""
 
-    def __init__(self, M_diag: np.ndarray, rho_inf: float = 0.9):
-        self.Md = np.asarray(M_diag, float)
-        self.M = sp.diags(self.Md, format="csr")
+    def __init__(self, M_diag, rho_inf: float = 0.9, *, damping_matrix=None):
+        if sp.issparse(M_diag) or np.ndim(M_diag) == 2:
+            self.M = sp.csr_matrix(M_diag)
+            self.Md = self.M.diagonal()
+        else:
+            self.Md = np.asarray(M_diag, float)
+            self.M = sp.diags(self.Md, format="csr")
+        self.C = None if damping_matrix is None else sp.csr_matrix(damping_matrix)
         self.rho_inf = rho_inf
         self.alpha_m = (2.0 * rho_inf - 1.0) / (rho_inf + 1.0)
         self.alpha_f = rho_inf / (rho_inf + 1.0)
@@ -62,11 +69,17 @@ class GeneralizedAlpha:
     def effective_stiffness(self, K, dt):
         if dt < 1e-30:
             return K
-        return K + (1.0 - self.alpha_m) * self.c0(dt) * self.M
+        K_eff = K + (1.0 - self.alpha_m) * self.c0(dt) * self.M
+        if self.C is not None:
+            K_eff = K_eff + (1.0 - self.alpha_f) * (self.gamma / (self.beta * dt)) * self.C
+        return K_eff
 
     def effective_residual(self, R, dt):
         a_mid = (1.0 - self.alpha_m) * self.acc + self.alpha_m * self._acc_old
-        return R.copy() + self.M @ a_mid
+        R_eff = R.copy() + self.M @ a_mid
+        if self.C is not None:
+            R_eff += self.C @ ((1.0 - self.alpha_f) * self.vel + self.alpha_f * self._vel_old)
+        return R_eff
 
     def checkpoint(self):
         self._ckpt = (self.vel.copy(), self.acc.copy())

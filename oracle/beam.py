@@ -426,3 +426,45 @@ def ke_global_linear(X2: np.ndarray, sec: Section) -> np.ndarray:
     for i in range(4):
         T[3 * i : 3 * i + 3, 3 * i : 3 * i + 3] = R
     return T.T @ ke_local(sec, np.array([L]))[0] @ T
+
+
+def consistent_mass(X, conn, rho: float, sec, v_ref=None) -> sp.csr_matrix:
+    """``assemble_mass(rho, lumped=False)`` (elements/_beam_assembler.py:148-196)
+    with ``timo_beam3d_mass_local`` / ``timo_beam3d_mass_global``
+    (_beam_cr.py:212-321): per element T^T Me T with the local axes of
+    ``_build_local_axes`` (:59-84), entries |val| <= 1e-30 dropped, COO -> CSR."""
+    X = np.asarray(X, float)
+    conn = np.asarray(conn, np.int64)
+    rows, cols, vals = [], [], []
+    for n1, n2 in conn:
+        d = X[n2] - X[n1]
+        L = float(np.linalg.norm(d))
+        ex = d / L
+        if v_ref is None:
+            a = np.abs(ex)
+            vr = np.eye(3)[0 if (a[0] <= a[1] and a[0] <= a[2]) else (1 if a[1] <= a[2] else 2)]
+        else:
+            vr = np.asarray(v_ref, float)
+        ez = np.cross(ex, vr)
+        ez = ez / np.linalg.norm(ez)
+        R = np.stack([ex, np.cross(ez, ex), ez])
+        m = rho * sec.A * L
+        Me = np.zeros((12, 12))
+        Me[0, 0] = Me[6, 6] = m / 3.0
+        Me[0, 6] = Me[6, 0] = m / 6.0
+        mt = rho * (sec.Iy + sec.Iz) * L
+        Me[3, 3] = Me[9, 9] = mt / 3.0
+        Me[3, 9] = Me[9, 3] = mt / 6.0
+        B = (m / 420.0) * np.array([[156.0, 22.0 * L, 54.0, -13.0 * L], [22.0 * L, 4.0 * L**2, 13.0 * L, -3.0 * L**2],
+                                    [54.0, 13.0 * L, 156.0, -22.0 * L], [-13.0 * L, -3.0 * L**2, -22.0 * L, 4.0 * L**2]])
+        for idx, sg in (([1, 5, 7, 11], np.ones(4)), ([2, 4, 8, 10], np.array([1.0, -1.0, 1.0, -1.0]))):
+            Me[np.ix_(idx, idx)] = B * np.outer(sg, sg)
+        T = np.kron(np.eye(4), R)
+        Mg = T.T @ Me @ T
+        ed = np.r_[6 * n1 + np.arange(6), 6 * n2 + np.arange(6)]
+        ii, jj = np.nonzero(np.abs(Mg) > 1e-30)
+        rows.append(ed[ii])
+        cols.append(ed[jj])
+        vals.append(Mg[ii, jj])
+    n = 6 * len(X)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))

@@ -15,7 +15,7 @@ running it can pin, and the strand generator's layouts (angle offsets, lay
 radii, numbering) over its branches.
 
 Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
-        [element|assembly|newton|newton_friction|newton_91|strategy|mesh ...]
+        [element|assembly|newton|newton_friction|newton_91|strategy|dynamics|mesh ...]
 """
 
 from __future__ import annotations
@@ -373,6 +373,47 @@ def strategy_fixture():
     np.savez_compressed(os.path.join(HERE, "strategy_7wire.npz"), **out)
 
 
+def dynamics_fixture():
+    """The reference's consistent mass (``assemble_mass(lumped=False)``, with the
+    automatic reference axis and with v_ref = (0, 1, 0)) on a small 7-wire strand,
+    and two Generalized-alpha steps with that mass and a Rayleigh damping matrix
+    C = a M + b K(0): predict, correct, effective residual and stiffness."""
+    m = mesh_mod.make_strand(7, 1.0, 12.0, 8, lay_angle_deg=10.0)
+    sec = section(1.0)
+    args = (m.node_coords, m.connectivity, sec["E"], sec["G"], sec["A"], sec["Iy"], sec["Iz"], sec["J"],
+            sec["kappa_y"], sec["kappa_z"])
+    asm = ULCRBeamAssembler(*args)
+    out = {"X": m.node_coords, "conn": m.connectivity.astype(np.int64)}
+    M = asm.assemble_mass(RHO, lumped=False)
+    out.update(csr_arrays("M", M))
+    out.update(csr_arrays("Mv", ULCRBeamAssembler(*args, v_ref=np.array([0.0, 1.0, 0.0])).assemble_mass(
+        RHO, lumped=False)))
+    ndof = 6 * m.n_nodes
+    K0 = asm.assemble_tangent(np.zeros(ndof))
+    a, b = 30.0, 2e-5
+    C = (a * M + b * K0).tocsr()
+    out.update(csr_arrays("C", C))
+    out.update(csr_arrays("K0", K0))
+    ga = GeneralizedAlphaProcess(M, damping_matrix=C, rho_inf=0.8)
+    rng = np.random.default_rng(41)
+    dt = 2e-5
+    u = np.zeros(ndof)
+    for k in range(2):
+        up = ga.predict(u, dt)
+        u = up + rng.normal(0.0, 1e-4, ndof)
+        R = rng.normal(0.0, 1.0, ndof)
+        ga.correct(u, np.zeros(ndof), dt)
+        out[f"s{k}_u_pred"] = up
+        out[f"s{k}_u"] = u.copy()
+        out[f"s{k}_R"] = R
+        out[f"s{k}_R_eff"] = ga.effective_residual(R, dt)
+        out.update(csr_arrays(f"s{k}_K_eff", ga.effective_stiffness(K0, dt)))
+        out[f"s{k}_vel"] = ga.vel.copy()
+        out[f"s{k}_acc"] = ga.acc.copy()
+    out["params"] = np.array([RHO, 0.8, dt, a, b])
+    np.savez_compressed(os.path.join(HERE, "dynamics_7wire.npz"), **out)
+
+
 # strand generator configurations: (n_strands, wire_diameter, pitch, length, n_elems, keyword arguments)
 MESH_CASES = [
     (7, 2.0, 40.0, 40.0, 40, {}),
@@ -430,7 +471,7 @@ def newton_91wire_fixture():
 
 FIXTURES = {"element": element_fixture, "assembly": assembly_fixture, "newton": newton_fixture,
             "newton_friction": newton_friction_fixtures, "newton_91": newton_91wire_fixture,
-            "strategy": strategy_fixture, "mesh": mesh_fixture}
+            "strategy": strategy_fixture, "dynamics": dynamics_fixture, "mesh": mesh_fixture}
 
 if __name__ == "__main__":
     for name in sys.argv[1:] or list(FIXTURES):

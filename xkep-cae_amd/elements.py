@@ -150,7 +150,60 @@ class ULCRBeamAssembler:
                 np.add.at(diag, c[:, k] * 6 + j, d6[:, j])
         return diag
 
+    def consistent_mass(self, rho: float) -> sp.csr_matrix:
+        """Consistent mass (``assemble_mass(lumped=False)``, _beam_assembler.py:148-196,
+        with timo_beam3d_mass_local / _global, _beam_cr.py:212-321): per element the
+        local matrix -- axial and torsional m/3, m/6 couplings, the cubic bending
+        blocks m/420 [156, 22L, 54, -13L; ...] in both bending planes (signs
+        alternating in x-z) -- rotated by the element's local axes (reference axis
+        v_ref, or the axis of least |e_x| component, first on ties), entries
+        |val| <= 1e-30 dropped, summed into a CSR matrix.  Set-up data, built
+        on the host once per mesh."""
+        c = np.asarray(self.connectivity, np.int64)
+        X = self.coords_ref
+        d = X[c[:, 1]] - X[c[:, 0]]
+        L = np.linalg.norm(d, axis=1)
+        ex = d / L[:, None]
+        if self.v_ref is None:
+            a = np.abs(ex)
+            k = np.where((a[:, 0] <= a[:, 1]) & (a[:, 0] <= a[:, 2]), 0, np.where(a[:, 1] <= a[:, 2], 1, 2))
+            vr = np.eye(3)[k]
+        else:
+            vr = np.broadcast_to(np.asarray(self.v_ref, float), ex.shape)
+        ez = np.cross(ex, vr)
+        nz = np.linalg.norm(ez, axis=1)
+        if np.any(nz < 1e-10):
+            raise ValueError("reference vector parallel to a beam axis")
+        ez = ez / nz[:, None]
+        R = np.stack([ex, np.cross(ez, ex), ez], axis=1)            # rows e_x, e_y, e_z
+        ne = len(c)
+        m = rho * self.A * L
+        mt = rho * (self.Iy + self.Iz) * L
+        Me = np.zeros((ne, 12, 12))
+        for i, j, w in ((0, 0, 1 / 3), (6, 6, 1 / 3), (0, 6, 1 / 6), (6, 0, 1 / 6)):
+            Me[:, i, j] = w * m
+        for i, j, w in ((3, 3, 1 / 3), (9, 9, 1 / 3), (3, 9, 1 / 6), (9, 3, 1 / 6)):
+            Me[:, i, j] = w * mt
+        q = m / 420.0
+        Lc = [np.ones(ne), L, np.ones(ne), L]
+        bend = np.array([[156.0, 22.0, 54.0, -13.0], [22.0, 4.0, 13.0, -3.0],
+                         [54.0, 13.0, 156.0, -22.0], [-13.0, -3.0, -22.0, 4.0]])
+        for idx, sg in (([1, 5, 7, 11], (1.0, 1.0, 1.0, 1.0)), ([2, 4, 8, 10], (1.0, -1.0, 1.0, -1.0))):
+            for a_ in range(4):
+                for b_ in range(4):
+                    Me[:, idx[a_], idx[b_]] = (q * bend[a_, b_] * Lc[a_] * Lc[b_]) * (sg[a_] * sg[b_])
+        M4 = Me.reshape(ne, 4, 3, 4, 3)
+        Mg = np.einsum("eki,eakbl,elj->eaibj", R, M4, R).reshape(ne, 12, 12)
+        ed = np.concatenate([6 * c[:, :1] + np.arange(6), 6 * c[:, 1:] + np.arange(6)], axis=1)
+        keep = np.abs(Mg) > 1e-30
+        rows = np.broadcast_to(ed[:, :, None], Mg.shape)[keep]
+        cols = np.broadcast_to(ed[:, None, :], Mg.shape)[keep]
+        return sp.csr_matrix((Mg[keep], (rows, cols)), shape=(self.ndof, self.ndof))
+
     def assemble_mass(self, rho: float, *, lumped: bool = True):
+        """``assemble_mass`` (_beam_assembler.py:148-196): the lumped diagonal or the
+        consistent matrix.  The fused assembly and the device solve take the lumped
+        (diagonal) mass; the consistent one serves host-side consumers."""
         if not lumped:
-            raise NotImplementedError("only the lumped mass of the dynamic strand path is provided")
+            return self.consistent_mass(rho)
         return sp.diags(self.lumped_mass_diag(rho), format="csr")

@@ -18,17 +18,26 @@ from .context import require_gpu
 
 
 class GeneralizedAlphaProcess:
+    """``mass_matrix``: a lumped mass as its diagonal (1-D) or any mass matrix;
+    ``damping_matrix``: optional C.  The fused assembly and the device solve
+    take a diagonal mass without damping (``dyn_struct`` refuses anything else);
+    the Strategy methods ``effective_residual`` / ``effective_stiffness`` take
+    any M and C, as the reference's (time_integration/strategy.py:178-216)."""
+
     def __init__(self, mass_matrix, *, damping_matrix=None, rho_inf: float = 0.9):
-        if damping_matrix is not None:
-            raise NotImplementedError("the strand path runs without a damping matrix")
         torch = require_gpu()
         self.torch = torch
-        Md = mass_matrix.diagonal() if sp.issparse(mass_matrix) else np.asarray(mass_matrix, float)
-        if Md.ndim == 2:
-            Md = np.diag(Md)
+        if sp.issparse(mass_matrix) or np.ndim(mass_matrix) == 2:
+            M = sp.csr_matrix(mass_matrix)
+            Md = M.diagonal()
+            self.M = M
+            self.lumped = M.count_nonzero() == np.count_nonzero(Md)
+        else:
+            Md = np.asarray(mass_matrix, float)
+            self.M = sp.diags(Md, format="csr")
+            self.lumped = True
         self.Md = np.ascontiguousarray(Md, dtype=float)
-        self.M = sp.diags(self.Md, format="csr")
-        self.C = None
+        self.C = None if damping_matrix is None else sp.csr_matrix(damping_matrix)
         self.rho_inf = rho_inf
         self.alpha_m = (2.0 * rho_inf - 1.0) / (rho_inf + 1.0)
         self.alpha_f = rho_inf / (rho_inf + 1.0)
@@ -81,13 +90,23 @@ class GeneralizedAlphaProcess:
         self.d_vel.copy_(self.d_v_pred + dt * self.gamma * self.d_acc)
 
     def effective_stiffness(self, K, dt: float):
+        """K + (1 - alpha_m) c0 M + (1 - alpha_f) c1 C, c0 = 1 / (beta dt^2), c1 = gamma / (beta dt)."""
         if dt < 1e-30:
             return K
-        return K + (1.0 - self.alpha_m) * (1.0 / (self.beta * dt**2)) * self.M
+        K_eff = K + (1.0 - self.alpha_m) * (1.0 / (self.beta * dt**2)) * self.M
+        if self.C is not None:
+            K_eff = K_eff + (1.0 - self.alpha_f) * (self.gamma / (self.beta * dt)) * self.C
+        return K_eff
 
     def effective_residual(self, R, dt: float):
-        a_mid = (1.0 - self.alpha_m) * self.d_acc + self.alpha_m * self.d_acc_old
-        return np.asarray(R, float) + self.Md * a_mid.cpu().numpy()
+        """R + M a_mid + C v_mid with a_mid = (1 - alpha_m) a + alpha_m a_old and
+        v_mid = (1 - alpha_f) v + alpha_f v_old."""
+        a_mid = ((1.0 - self.alpha_m) * self.d_acc + self.alpha_m * self.d_acc_old).cpu().numpy()
+        R_eff = np.asarray(R, float) + (self.Md * a_mid if self.lumped else self.M @ a_mid)
+        if self.C is not None:
+            v_mid = ((1.0 - self.alpha_f) * self.d_vel + self.alpha_f * self.d_vel_old).cpu().numpy()
+            R_eff = R_eff + self.C @ v_mid
+        return R_eff
 
     def checkpoint(self) -> None:
         self._ckpt = (self.d_vel.clone(), self.d_acc.clone())
@@ -98,7 +117,10 @@ class GeneralizedAlphaProcess:
             self.d_acc.copy_(self._ckpt[1])
 
     def dyn_struct(self, dt: float) -> L.XkDyn:
-        """Pointers of the device state for the fused assembly kernels."""
+        """Pointers of the device state for the fused assembly kernels (a lumped
+        mass without damping: the kernels apply M as a diagonal)."""
+        if not self.lumped or self.C is not None:
+            raise ValueError("the fused assembly takes a lumped (diagonal) mass and no damping matrix")
         return L.XkDyn(M_diag=self.d_M.data_ptr(), u_pred=self.d_u_pred.data_ptr(), v_pred=self.d_v_pred.data_ptr(),
                        acc_old=self.d_acc_old.data_ptr(), acc=self.d_acc.data_ptr(), vel=self.d_vel.data_ptr(),
                        dt=dt, alpha_m=self.alpha_m, beta=self.beta, gamma=self.gamma)
