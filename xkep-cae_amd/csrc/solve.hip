@@ -37,10 +37,18 @@ namespace xk {
 enum SolveBuf {
     SV_KB, SV_AB, SV_LF, SV_SINV, SV_SLIST, SV_SN, SV_SG, SV_SC, SV_POS, SV_CSLOT, SV_CN, SV_CNODE, SV_CHM,
     SV_CHCN0, SV_CHYOFF, SV_CHCOFF, SV_Y, SV_C, SV_S, SV_IPIV, SV_INFO, SV_W, SV_B, SV_X, SV_R, SV_T, SV_RED,
-    SV_PERM, SV_W2, SV_COUNT
+    SV_PERM, SV_W2, SV_PIV32, SV_COUNT
 };
 
 // ------------------------------------------------------------------ kernels
+
+// 32-bit pivots and info of one LU widened into the 64-bit buffers
+__global__ void sv_widen_kernel(int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
+                                const int* __restrict__ info32, int64_t* __restrict__ info64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) piv64[i] = piv32[i];
+    if (i == 0) *info64 = *info32;
+}
 
 __device__ __forceinline__ bool sv_link(const int8_t* nflag, int n, int n_nodes) {
     return n >= 0 && n < n_nodes && (nflag[n] & 2);
@@ -791,6 +799,7 @@ int Ctx::solve_setup() {
     SV_TRY(sv[SV_S].ensure(sizeof(double) * (size_t)std::max<int64_t>(stot, 1)));
     SV_TRY(sv[SV_IPIV].ensure(sizeof(int64_t) * 3 * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_INFO].ensure(sizeof(int64_t) * (size_t)std::max(ncomp, 1)));
+    SV_TRY(sv[SV_PIV32].ensure(sizeof(int) * (3 * (size_t)std::max(ns, 1) + (size_t)std::max(ncomp, 1))));
     SV_TRY(sv[SV_W].ensure(sizeof(double) * 3 * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_W2].ensure(sizeof(double) * 3 * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_PERM].ensure(sizeof(int) * 3 * (size_t)std::max(ns, 1)));
@@ -905,8 +914,20 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                                (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_C].p, S);
             SV_TRY(hipGetLastError());
             const int64_t n = 3 * (int64_t)Pc;
-            SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
-                                      (int64_t*)sv[SV_INFO].p + c));
+            if (n <= 32768) {
+                // 32-bit interface (its element offsets are 32-bit: n * n must stay far below
+                // 2^31, hence n <= 32768), pivots and info widened to the 64-bit buffers afterwards
+                int* piv32 = (int*)sv[SV_PIV32].p + 3 * sv_comp_pos[c];
+                int* info32 = (int*)sv[SV_PIV32].p + 3 * (size_t)std::max(ns, 1) + c;
+                SV_RB(rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, S, (rocblas_int)n, piv32, info32));
+                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, (const int*)piv32,
+                                   (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], (const int*)info32,
+                                   (int64_t*)sv[SV_INFO].p + c);
+                SV_TRY(hipGetLastError());
+            } else {
+                SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
+                                          (int64_t*)sv[SV_INFO].p + c));
+            }
         }
         hipLaunchKernelGGL(sv_info_kernel, dim3(sv_grid((int64_t)sv_comp_n.size())), dim3(256), 0, st,
                            (int)sv_comp_n.size(), (const int64_t*)sv[SV_INFO].p, (int*)d_flags.p);
