@@ -234,7 +234,10 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
  * K = A_b + sum_p U_p G_p U_p^T: the beam part A_b = K_beam + (1 - alpha_m) c0 M
  * (the element kernel re-run, factored along each chain) and the pairs'
  * staged 3x3 tangent blocks G_p with U_p = c_p (x) I3 (Woodbury, a dense
- * capacitance matrix per contact component, rocSOLVER LU); in the FP64 form
+ * capacitance matrix per contact component, LU-factored without row
+ * interchanges -- blocked, DGEMM trailing updates -- or, when that meets a
+ * zero pivot or leaves an eager refinement above tol, by rocSOLVER's
+ * partially pivoted getrf for the rest of the pattern; see xk_solve_set_lu); in the FP64 form
  * this equals the assembled K up to rounding, in the mixed form K holds the
  * same sum rounded to FP32.  The result is refined against that operator:
  * with tol <= 0 exactly max_refine steps, stream-ordered; with tol > 0 until
@@ -248,6 +251,13 @@ int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,
 int xk_linear_solve(xk_ctx* ctx, const double* u, const uint8_t* fixed_mask, const xk_dyn* dyn,
                     const double* R, double* du, int max_refine, double tol, double* res_rel,
                     int32_t* n_refine, void* stream);
+/* Capacitance LU of xk_linear_solve.  kind 0 (default): without row
+ * interchanges, falling back to partial pivoting as described there; 1:
+ * partial pivoting (rocSOLVER getrf) always; -1: leave unchanged (query).
+ * *pivoting (nullable) = 1 if the next solve on the current pattern uses
+ * partial pivoting; *fallbacks (nullable) = solves that fell back so far.
+ * No counterpart in the reference (SciPy's SuperLU chooses its own pivots). */
+int xk_solve_set_lu(xk_ctx* ctx, int kind, int32_t* pivoting, int64_t* fallbacks);
 /* Line search, update and convergence norms of the attempt on the device,
  * replacing LineSearchUpdateProcess (contact/solver/_newton_steps.py:396-434)
  * with _ncp_line_search (contact/solver/_utils.py:31-61) and the

@@ -135,6 +135,32 @@ def test_device_solve_matches_superlu_at_91_wires(gpu, precision, layers):
     assert out["nact"] > 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_capacitance_lu_without_interchanges_matches_partial_pivoting(gpu, precision):
+    """The capacitance LU without row interchanges (the default) and
+    rocSOLVER's partially pivoted getrf on the same 91-wire attempt: both meet
+    the residual and SuperLU bars, their solutions agree within the SuperLU
+    bar, and the default needed no fallback to pivoting on this pattern."""
+    wl, Kc = pkg("workloads"), pkg("contact")
+    case = wl.C3
+    st = wl.make_state(case)
+    m = st.mesh
+    cfg = Kc.ContactConfig(mu=case.mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(st.elem_layer))
+    ctx, asm = strand_assembly(m.node_coords, m.connectivity, m.radii, case, st, precision, cfg)
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm.begin_increment(st.u)
+    sols = {}
+    for kind in (1, 0):
+        ctx.solve_lu(kind)
+        print(f"capacitance LU kind {kind} ({precision})", flush=True)
+        out = solve_case(ctx, asm, st.fixed_dofs, precision)
+        pivoting, fallbacks = ctx.solve_lu()
+        assert pivoting == (kind == 1)
+        assert fallbacks == 0
+        sols[kind] = out
+    assert sols[0]["err_w"] < SOL_TOL and sols[1]["err_w"] < SOL_TOL
+
+
 def test_device_solve_on_batched_instances(gpu):
     wl = pkg("workloads")
     case = wl.C2

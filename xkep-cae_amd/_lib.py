@@ -66,6 +66,7 @@ SIGNATURES = {
     "xk_linear_solve": (C.c_int, [vp, vp, vp, C.POINTER(XkDyn), vp, vp, C.c_int, C.c_double, vp,
                                   C.POINTER(C.c_int32), vp]),
     "xk_solve_buffer": (C.c_int, [vp, C.c_int, vp, C.c_int64, c_int64_p]),
+    "xk_solve_set_lu": (C.c_int, [vp, C.c_int, c_int32_p, c_int64_p]),
     "xk_newton_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, vp]),
     "xk_status": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "xk_profile_begin": (C.c_int, [vp, C.c_int]),

@@ -425,6 +425,16 @@ class DeviceContext:
             float(mu_eff), L.ptr(R), L.ptr(K_vals), self._cap(K_vals), L.ptr(res_norm2), L.ptr(n_active),
             self.stream), "xk_assemble_iteration")
 
+    def solve_lu(self, kind: int | None = None) -> tuple[bool, int]:
+        """Capacitance LU of the device solve: kind 0 = without row
+        interchanges, partial pivoting after a failure (default); 1 = partial
+        pivoting always; None = unchanged.  Returns (the current pattern's
+        solves pivot, solves that fell back so far)."""
+        piv, fb = C.c_int32(), C.c_int64()
+        L.check(L.lib().xk_solve_set_lu(self.h, -1 if kind is None else int(kind), C.byref(piv), C.byref(fb)),
+                "xk_solve_set_lu")
+        return bool(piv.value), int(fb.value)
+
     def linear_solve(self, u, fixed_mask, R, du, dyn: "L.XkDyn | None" = None, *, max_refine: int = 16,
                      tol: float = 1e-11, res_rel=None) -> int:
         """du = -(D K D + I - D)^-1 D R on the device (LinearSolveProcess,
@@ -461,7 +471,7 @@ class DeviceContext:
 
     SOLVE_BUFFERS = ("Kb", "Ab", "Lf", "Sinv", "slist", "slot_nodes", "slot_G", "slot_c", "pos", "cslot", "cn",
                      "cnode", "ch_m", "ch_cn0", "ch_yoff", "ch_coff", "Y", "C", "S", "ipiv", "info", "W", "b", "x",
-                     "r", "t", "red", "perm", "W2")
+                     "r", "t", "red", "perm", "W2", "piv32")
 
     def solve_buffer(self, name: str, dtype) -> np.ndarray:
         """Host copy of one of the linear solve's device buffers (tests)."""

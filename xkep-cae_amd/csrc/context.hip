@@ -830,6 +830,17 @@ int xk_linear_solve(xk_ctx* c, const double* u, const uint8_t* fixed_mask, const
                            (hipStream_t)stream);
 }
 
+int xk_solve_set_lu(xk_ctx* c, int kind, int32_t* pivoting, int64_t* fallbacks) {
+    if (!c || kind < -1 || kind > 1) {
+        set_error("xk_solve_set_lu: kind must be -1 (query), 0 or 1");
+        return -1;
+    }
+    if (kind >= 0) c->sv_lu_kind = kind;
+    if (pivoting) *pivoting = (c->sv_lu_kind == 1 || c->sv_gepp) ? 1 : 0;
+    if (fallbacks) *fallbacks = c->sv_fallbacks;
+    return 0;
+}
+
 int xk_pattern_info(xk_ctx* c, int64_t* nnz, int64_t* n_blocks, int64_t* n_contrib) {
     if (!c) return -1;
     if (c->ensure_pattern() != 0) return -1;

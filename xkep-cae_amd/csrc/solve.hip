@@ -15,7 +15,10 @@
 // Gauss-Jordan with partial pivoting); the capacitance matrix
 // I + G U~^T A~_b^-1 U~ (3 rows per contributing pair) is dense within a
 // connected component of the contact graph (pairs sharing a chain) and is
-// LU-factored per component by rocSOLVER.  Penalty springs that tie the soft,
+// LU-factored per component: blocked, without row interchanges, the trailing
+// updates by rocBLAS DGEMM (rocSOLVER's partially pivoted getrf when that
+// meets a zero pivot or its refinement stalls above the tolerance); the
+// triangular solves run in blocks of TB rows, one launch per block.  Penalty springs that tie the soft,
 // mass-dominated modes of whole wires make the capacitance ill-conditioned
 // (condition numbers near 1e12 on strands): the result's residual is
 // U (S w - t), so it carries the capacitance solve's backward error times
@@ -29,6 +32,7 @@
 #include "xk_internal.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <numeric>
 #include <rocsolver/rocsolver.h>
 
@@ -339,6 +343,318 @@ __global__ void sv_info_kernel(int nc, const int64_t* __restrict__ info, int* __
     if (c < nc && info[c] != 0) atomicOr(flags, XK_FLAG_SINGULAR);
 }
 
+// ------------------------------------------------------- capacitance LU
+// Blocked right-looking LU without row interchanges of one column-major
+// n x n capacitance block (leading dimension n), panels of LU_NB columns.
+// Per panel one launch: every workgroup factors the w x w diagonal block in
+// LDS (the same arithmetic in each, so the copies agree bitwise; workgroup 0
+// stores it), then forms either LU_NB rows of L21 = A21 U11^-1 or LU_NB
+// columns of U12 = L11^-1 A12 in an LDS tile; rocBLAS's DGEMM then applies
+// A22 -= L21 U12 on the matrix cores.  Inside the launch the diagonal block
+// and the tiles are processed in sub-blocks of LU_SB: the sub-block's
+// factorization or triangular solve runs with one row (or column) per lane in
+// registers, the rest of the block is updated by 3 x 3 / 4 x 3 register tiles
+// (the code stays a few thousand instructions: the launch is not bound by
+// instruction fetch).  Divisions by a pivot are products with its reciprocal
+// (formed once per pivot), which keeps the divisions off the dependent chains.
+//
+// Without interchanges the pivots are the Schur complements' diagonals in
+// order.  On the strands' capacitance blocks (far from diagonally dominant,
+// condition numbers near 1e12) this LU had growth max|U| / max|C| = 25
+// against 1.7 for partial pivoting, and one refinement step brought both to
+// the same residual (measured at 91 wires); the solve refines against the
+// assembled operator in any case.  A pivot that is zero or not finite is
+// reported in info (first such column, 1-based), and the caller then
+// factors the block again with partial pivoting.
+constexpr int LU_NB = 64;   // panel width = tile size
+constexpr int LU_SB = 16;   // sub-block inside the panel
+
+// A double of another lane of the wave (lane uniform)
+__device__ __forceinline__ double sv_readlane(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_row_wg, double* __restrict__ A,
+                                                          int64_t* __restrict__ info) {
+    __shared__ double D[LU_NB][LU_NB + 1];
+    __shared__ double X[LU_NB][LU_NB + 1];
+    __shared__ double invd[LU_NB];   // reciprocals of the pivots
+    __shared__ int s_bad;
+    const int w = min(LU_NB, n - j0);
+    const int tid = threadIdx.x;
+    const size_t ld = (size_t)n;
+    constexpr int CP = LU_NB * LU_NB / 256;   // tile entries per lane (all loads issued before the stores)
+#pragma unroll
+    for (int it = 0; it < CP; ++it) {
+        const int k = tid + 256 * it, r = k & (LU_NB - 1), q = k / LU_NB;
+        D[r][q] = (r < w && q < w) ? A[(size_t)(j0 + r) + (size_t)(j0 + q) * ld] : 0.0;
+    }
+    if (tid == 0) s_bad = -1;
+    __syncthreads();
+    // ---- diagonal block
+    for (int o = 0; o < w; o += LU_SB) {
+        const int bw = min(LU_SB, w - o);
+        if (tid < 64) {
+            // (a) the bw x bw sub-block: lane i holds row o + i, the pivot row
+            // of step c broadcast from lane c
+            const int i = tid;
+            double v[LU_SB];
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) v[k] = (i < bw && k < bw) ? D[o + i][o + k] : 0.0;
+#pragma unroll
+            for (int c = 0; c < LU_SB; ++c) {
+                if (c < bw) {   // uniform
+                    const double p = sv_readlane(v[c], c);
+                    const double ip = 1.0 / p;
+                    if (i == 0) {
+                        if (s_bad < 0 && !(fabs(p) > 0.0 && fabs(p) <= DBL_MAX)) s_bad = o + c;
+                        invd[o + c] = ip;
+                    }
+                    const bool below = i > c && i < bw;
+                    const double l = below ? v[c] * ip : 0.0;   // rows at or above c: unchanged
+#pragma unroll
+                    for (int k = c + 1; k < LU_SB; ++k) v[k] = fma(-l, sv_readlane(v[k], c), v[k]);
+                    if (below) v[c] = l;
+                }
+            }
+            if (i < bw)
+#pragma unroll
+                for (int k = 0; k < LU_SB; ++k)
+                    if (k < bw) D[o + i][o + k] = v[k];
+        }
+        __syncthreads();
+        const int R = w - o - LU_SB;   // rows / columns after the sub-block
+        if (R <= 0) break;             // uniform
+        if (tid < R) {
+            // (b) L part of the rows below: x U_kk^-1
+            const int r = o + LU_SB + tid;
+            double x[LU_SB];
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) x[k] = D[r][o + k];
+#pragma unroll
+            for (int c = 0; c < LU_SB; ++c) {
+                x[c] = x[c] * invd[o + c];
+#pragma unroll
+                for (int k = c + 1; k < LU_SB; ++k) x[k] = fma(-x[c], D[o + c][o + k], x[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) D[r][o + k] = x[k];
+        } else if (tid >= 64 && tid < 64 + R) {
+            // (b) U part of the columns right of it: L_kk^-1 y
+            const int q = o + LU_SB + (tid - 64);
+            double y[LU_SB];
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) y[k] = D[o + k][q];
+#pragma unroll
+            for (int c = 0; c < LU_SB; ++c)
+#pragma unroll
+                for (int k = c + 1; k < LU_SB; ++k) y[k] = fma(-D[o + k][o + c], y[c], y[k]);
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) D[o + k][q] = y[k];
+        }
+        __syncthreads();
+        {
+            // (c) trailing update of the R x R rest: 3 x 3 tiles (R <= 48)
+            const int bi = tid / 16, bj = tid % 16;
+            const int r0 = o + LU_SB + 3 * bi, q0 = o + LU_SB + 3 * bj;
+            if (3 * bi < R && 3 * bj < R) {
+                // rows / columns past R (a partial tile) read as zero, are not stored
+                bool ra[3], cb[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    ra[a] = 3 * bi + a < R;
+                    cb[a] = 3 * bj + a < R;
+                }
+                double acc[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) acc[a][b] = (ra[a] && cb[b]) ? D[r0 + a][q0 + b] : 0.0;
+#pragma unroll
+                for (int k = 0; k < LU_SB; ++k) {
+                    double lk[3], uk[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) lk[a] = ra[a] ? D[r0 + a][o + k] : 0.0;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) uk[b] = cb[b] ? D[o + k][q0 + b] : 0.0;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) acc[a][b] = fma(-lk[a], uk[b], acc[a][b]);
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        if (ra[a] && cb[b]) D[r0 + a][q0 + b] = acc[a][b];
+            }
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0) {
+#pragma unroll
+        for (int it = 0; it < CP; ++it) {
+            const int k = tid + 256 * it, r = k & (LU_NB - 1), q = k / LU_NB;
+            if (r < w && q < w) A[(size_t)(j0 + r) + (size_t)(j0 + q) * ld] = D[r][q];
+        }
+        if (tid == 0) {
+            int64_t v = j0 == 0 ? 0 : *info;
+            if (v == 0 && s_bad >= 0) v = j0 + s_bad + 1;
+            *info = v;
+        }
+    }
+    const int rest = n - j0 - w;
+    if (rest <= 0) return;
+    if ((int)blockIdx.x < n_row_wg) {
+        // ---- LU_NB rows of L21: X U11^-1, X[i][k] = A[row0 + i, j0 + k]
+        const int row0 = j0 + w + blockIdx.x * LU_NB;
+        const int mr = min(LU_NB, n - row0);
+#pragma unroll
+        for (int it = 0; it < CP; ++it) {
+            const int k = tid + 256 * it, i = k & (LU_NB - 1), q = k / LU_NB;
+            X[i][q] = (i < mr && q < w) ? A[(size_t)(row0 + i) + (size_t)(j0 + q) * ld] : 0.0;
+        }
+        __syncthreads();
+        for (int o = 0; o < w; o += LU_SB) {
+            const int bw = min(LU_SB, w - o);
+            if (tid < LU_NB) {
+                double x[LU_SB];
+#pragma unroll
+                for (int k = 0; k < LU_SB; ++k) x[k] = X[tid][o + k];
+#pragma unroll
+                for (int c = 0; c < LU_SB; ++c) {
+                    if (c < bw) {
+                        x[c] = x[c] * invd[o + c];
+#pragma unroll
+                        for (int k = c + 1; k < LU_SB; ++k)
+                            if (k < bw) x[k] = fma(-x[c], D[o + c][o + k], x[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < LU_SB; ++k) X[tid][o + k] = x[k];
+            }
+            __syncthreads();
+            const int R = w - o - LU_SB;
+            if (R <= 0) break;
+            {
+                // X[:, o + 16 ..] -= X[:, o .. o + 16) U[o .. o + 16), o + 16 ..]: 4 x 3 tiles
+                const int bi = tid / 16, bj = tid % 16;
+                const int i0 = 4 * bi, q0 = o + LU_SB + 3 * bj;
+                if (3 * bj < R) {
+                    bool cb[3];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) cb[b] = 3 * bj + b < R;
+                    double acc[4][3];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) acc[a][b] = cb[b] ? X[i0 + a][q0 + b] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < LU_SB; ++k) {
+                        double xk[4], uk[3];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) xk[a] = X[i0 + a][o + k];
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) uk[b] = cb[b] ? D[o + k][q0 + b] : 0.0;
+#pragma unroll
+                        for (int a = 0; a < 4; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) acc[a][b] = fma(-xk[a], uk[b], acc[a][b]);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b)
+                            if (cb[b]) X[i0 + a][q0 + b] = acc[a][b];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int it = 0; it < CP; ++it) {
+            const int k = tid + 256 * it, i = k & (LU_NB - 1), q = k / LU_NB;
+            if (i < mr && q < w) A[(size_t)(row0 + i) + (size_t)(j0 + q) * ld] = X[i][q];
+        }
+        return;
+    }
+    // ---- LU_NB columns of U12: L11^-1 Y, X[k][j] = A[j0 + k, col0 + j]
+    const int col0 = j0 + w + (blockIdx.x - n_row_wg) * LU_NB;
+    const int nc = min(LU_NB, n - col0);
+#pragma unroll
+    for (int it = 0; it < CP; ++it) {
+        const int k = tid + 256 * it, q = k & (LU_NB - 1), j = k / LU_NB;
+        X[q][j] = (q < w && j < nc) ? A[(size_t)(j0 + q) + (size_t)(col0 + j) * ld] : 0.0;
+    }
+    __syncthreads();
+    for (int o = 0; o < w; o += LU_SB) {
+        const int bw = min(LU_SB, w - o);
+        if (tid < LU_NB) {
+            double y[LU_SB];
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) y[k] = X[o + k][tid];
+#pragma unroll
+            for (int c = 0; c < LU_SB; ++c) {
+                if (c < bw) {
+#pragma unroll
+                    for (int k = c + 1; k < LU_SB; ++k)
+                        if (k < bw) y[k] = fma(-D[o + k][o + c], y[c], y[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < LU_SB; ++k) X[o + k][tid] = y[k];
+        }
+        __syncthreads();
+        const int R = w - o - LU_SB;
+        if (R <= 0) break;
+        {
+            // X[o + 16 .., :] -= L[o + 16 .., o .. o + 16) X[o .. o + 16), :]: 3 x 4 tiles
+            const int bi = tid % 16, bj = tid / 16;
+            const int r0 = o + LU_SB + 3 * bi, j0t = 4 * bj;
+            if (3 * bi < R) {
+                bool ra[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) ra[a] = 3 * bi + a < R;
+                double acc[3][4];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = ra[a] ? X[r0 + a][j0t + b] : 0.0;
+#pragma unroll
+                for (int k = 0; k < LU_SB; ++k) {
+                    double lk[3], yk[4];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) lk[a] = ra[a] ? D[r0 + a][o + k] : 0.0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) yk[b] = X[o + k][j0t + b];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) acc[a][b] = fma(-lk[a], yk[b], acc[a][b]);
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if (ra[a]) X[r0 + a][j0t + b] = acc[a][b];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int it = 0; it < CP; ++it) {
+        const int k = tid + 256 * it, q = k & (LU_NB - 1), j = k / LU_NB;
+        if (q < w && j < nc) A[(size_t)(j0 + q) + (size_t)(col0 + j) * ld] = X[q][j];
+    }
+}
+
+// pivots of an LU without interchanges: row i stays row i (LAPACK, 1-based)
+__global__ void sv_ipiv_identity_kernel(int64_t n, int64_t* __restrict__ ipiv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ipiv[i] = i + 1;
+}
+
 // x <- A~_b^-1 x along every chain: 8 lanes per chain (lanes 0..5 hold the
 // node's six components), 8 chains per wavefront; rows of the 6x6 blocks are
 // combined across the chain's lanes by shuffles.
@@ -484,72 +800,68 @@ __global__ void sv_permute_kernel(int n, const int* __restrict__ perm, const dou
     if (i < n) y[i] = t[perm[i]];
 }
 
-// unit lower triangle: step s (-1: only the diagonal solve of block 0)
+
+// Unit lower triangle, step s: every workgroup (one wave) solves the
+// diagonal block s of y (its rows hold the final right-hand side: no later
+// write touches them) by forward substitution with the solved values
+// broadcast from lane to lane; workgroup 0 stores the block in z; every
+// workgroup then subtracts L[i, block s] z_s from its rows i of block
+// s + 1 + blockIdx.x.
 __global__ __launch_bounds__(TB) void sv_lower_step_kernel(int n, const double* __restrict__ LU,
-                                                           double* __restrict__ y, int s) {
-    __shared__ double ys[TB];
+                                                           double* __restrict__ y, double* __restrict__ z, int s) {
+    __shared__ double zs[TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
-    const int nb = (n + TB - 1) / TB;
-    if (s >= 0) {
-        const int k0 = s * TB, k1 = min(n, k0 + TB);
-        if (k0 + t < k1) ys[t] = y[k0 + t];
-        __syncthreads();
-        const int i = (s + 1) * TB + blockIdx.x * TB + t;
-        if (i < n) y[i] = sv_block_dot(y[i], LU, ld, i, k0, k1 - k0, ys);
-    }
-    if (blockIdx.x != 0 || s + 1 >= nb) return;
-    // diagonal solve of block s + 1 (its rows were updated by this workgroup)
-    __syncthreads();
-    const int r0 = (s + 1) * TB, m = min(TB, n - r0);
+    const int k0 = s * TB, m = min(TB, n - k0);
     double row[TB];
 #pragma unroll
-    for (int k = 0; k < TB; ++k) row[k] = (t < m && k < t) ? LU[(r0 + t) + (size_t)(r0 + k) * ld] : 0.0;
-    double v = t < m ? y[r0 + t] : 0.0;
+    for (int k = 0; k < TB; ++k) row[k] = (t < m && k < t) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
+    double v = t < m ? y[k0 + t] : 0.0;
 #pragma unroll
-    for (int k = 0; k < TB; ++k) {
-        if (k >= m) continue;   // uniform; keeps the loop unrolled (row[] stays in registers)
-        if (t == k) ys[k] = v;
-        __syncthreads();
-        if (t > k) v = fma(-row[k], ys[k], v);
-    }
-    if (t < m) y[r0 + t] = v;
+    for (int k = 0; k < TB; ++k) v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k >= t
+    zs[t] = v;
+    if (blockIdx.x == 0 && t < m) z[k0 + t] = v;
+    __syncthreads();
+    const int i = (s + 1 + blockIdx.x) * TB + t;
+    if (i < n) y[i] = sv_block_dot(y[i], LU, ld, i, k0, m, zs);
 }
 
-// upper triangle with its diagonal: step s (nb: only the diagonal solve of the
-// last block); updates the rows above block s, workgroup 0 solves block s - 1
+// Upper triangle with its diagonal, step s (from the last block down): every
+// workgroup solves the diagonal block s of z by back substitution (workgroup 0
+// stores it in x), then subtracts U[i, block s] x_s from its rows i of block
+// s - 1 - blockIdx.x.
 __global__ __launch_bounds__(TB) void sv_upper_step_kernel(int n, const double* __restrict__ LU,
-                                                           double* __restrict__ y, int s) {
-    __shared__ double ys[TB];
+                                                           double* __restrict__ z, double* __restrict__ x, int s) {
+    __shared__ double xs[TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
-    const int nb = (n + TB - 1) / TB;
-    if (s < nb) {
-        const int k0 = s * TB, k1 = min(n, k0 + TB);
-        if (k0 + t < k1) ys[t] = y[k0 + t];
-        __syncthreads();
-        // rows above block s, from the row block just above it downwards
-        const int i = (s - 1) * TB - blockIdx.x * TB + t;
-        if (i >= 0 && i < k0) y[i] = sv_block_dot(y[i], LU, ld, i, k0, k1 - k0, ys);
-    }
-    if (blockIdx.x != 0 || s - 1 < 0) return;
-    __syncthreads();
-    const int r0 = (s - 1) * TB, m = min(TB, n - r0);
+    const int k0 = s * TB, m = min(TB, n - k0);
     double row[TB];
 #pragma unroll
-    for (int k = 0; k < TB; ++k) row[k] = (t < m && k >= t && k < m) ? LU[(r0 + t) + (size_t)(r0 + k) * ld] : 0.0;
-    double v = t < m ? y[r0 + t] : 0.0;
+    for (int k = 0; k < TB; ++k)
+        row[k] = (t < m && k > t && k < m) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
+    const double idg = 1.0 / (t < m ? LU[(size_t)(k0 + t) + (size_t)(k0 + t) * ld] : 1.0);
+    double v = t < m ? z[k0 + t] : 0.0;
 #pragma unroll
     for (int k = TB - 1; k >= 0; --k) {
-        if (k >= m) continue;
-        if (t == k) {
-            v = v / row[k];
-            ys[k] = v;
+        if (k < m) {   // uniform
+            if (t == k) v = v * idg;
+            v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k <= t
         }
-        __syncthreads();
-        if (t < k) v = fma(-row[k], ys[k], v);
     }
-    if (t < m) y[r0 + t] = v;
+    xs[t] = v;
+    if (blockIdx.x == 0 && t < m) x[k0 + t] = v;
+    __syncthreads();
+    const int b = s - 1 - (int)blockIdx.x;
+    if (b < 0) return;
+    const int i = b * TB + t;
+    z[i] = sv_block_dot(z[i], LU, ld, i, k0, m, xs);
+}
+
+// perm[i] = i (a block factored without interchanges)
+__global__ void sv_perm_identity_kernel(int n, int* __restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = i;
 }
 
 // W[3 pos(q) + r] = (G_q U~_q^T y)_r
@@ -771,6 +1083,7 @@ int Ctx::solve_setup() {
     sv_comp_soff.assign(ncomp + 1, 0);
     for (int c = 0; c < ncomp; ++c) sv_comp_soff[c + 1] = sv_comp_soff[c] + 9 * (int64_t)sv_comp_n[c] * sv_comp_n[c];
     const int64_t stot = sv_comp_soff[ncomp];
+    sv_comp_piv.assign(ncomp, 0);
     if ((double)stot * 8.0 > 64.0 * (1ull << 30)) {
         set_error("linear solve: the contact capacitance matrices exceed 64 GiB");
         return -1;
@@ -811,6 +1124,7 @@ int Ctx::solve_setup() {
         sv_handle = h;
     }
     solve_ready = true;
+    sv_gepp = false;   // a new pattern starts with the LU without interchanges
     return 0;
 }
 
@@ -839,16 +1153,19 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
         const int n = 3 * sv_comp_n[c];
         const int nb = (n + TB - 1) / TB;
         const double* LU = (const double*)sv[SV_S].p + sv_comp_soff[c];
+        // permuted right-hand side in W2, the lower solve into W (its
+        // projection is consumed), the upper solve back into W2
         double* y = W2 + 3 * sv_comp_pos[c];
+        double* z = W + 3 * sv_comp_pos[c];
         hipLaunchKernelGGL(sv_permute_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n,
-                           (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)(W + 3 * sv_comp_pos[c]), y);
+                           (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)z, y);
         SV_TRY(hipGetLastError());
-        for (int s = -1; s <= nb - 2; ++s) {
-            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(s < 0 ? 1 : nb - s - 1), dim3(TB), 0, st, n, LU, y, s);
+        for (int s = 0; s < nb; ++s) {
+            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, n, LU, y, z, s);
             SV_TRY(hipGetLastError());
         }
-        for (int s = nb; s >= 1; --s) {
-            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(s == nb ? 1 : s), dim3(TB), 0, st, n, LU, y, s);
+        for (int s = nb - 1; s >= 0; --s) {
+            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(std::max(1, s)), dim3(TB), 0, st, n, LU, z, y, s);
             SV_TRY(hipGetLastError());
         }
     }
@@ -860,6 +1177,109 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
                        (const double*)sv[SV_SINV].p, x);
     SV_TRY(hipGetLastError());
+    return 0;
+}
+
+// Every component's capacitance block assembled from the compliance columns
+// and LU-factored in place: without row interchanges (gepp false, blocks up to
+// LU_MAX rows; info[c] = first bad pivot, no flag raised: the caller decides)
+// or by rocSOLVER's partially pivoted getrf (a zero pivot raises
+// XK_FLAG_SINGULAR); then the pivots as one permutation per component.
+// Stream-ordered (capturable) except for the host permutation of blocks
+// beyond PERM_MAX rows.
+constexpr int64_t LU_MAX = 32768;   // rocBLAS's 32-bit offsets: n * n < 2^31
+
+int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
+    const int ns = n_sinc;
+    const uint8_t* fixed = sv_fixed;
+    rocblas_handle h = (rocblas_handle)sv_handle;
+    for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+        const int Pc = sv_comp_n[c];
+        double* S = (double*)sv[SV_S].p + sv_comp_soff[c];
+        hipLaunchKernelGGL(sv_capacitance_kernel, dim3(sv_grid(Pc, 64), Pc), dim3(64), 0, st, Pc,
+                           (const int*)sv[SV_CSLOT].p + sv_comp_pos[c], (const int4*)sv[SV_SN].p,
+                           (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, (const int*)d_node_chain.p,
+                           (const int*)sv[SV_CN].p, (const int*)sv[SV_CHCN0].p, (const int*)sv[SV_CHM].p,
+                           (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_C].p, S);
+        SV_TRY(hipGetLastError());
+        const int64_t n = 3 * (int64_t)Pc;
+        int64_t* ipiv = (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c];
+        int64_t* info = (int64_t*)sv[SV_INFO].p + c;
+        if (!gepp && n <= LU_MAX) {
+            const double one = 1.0, minus_one = -1.0;
+            for (int64_t j0 = 0; j0 < n; j0 += LU_NB) {
+                const int w = (int)std::min<int64_t>(LU_NB, n - j0);
+                const int64_t rest = n - j0 - w;
+                const int nrw = (int)((rest + LU_NB - 1) / LU_NB);
+                const int ncw = nrw;
+                hipLaunchKernelGGL(sv_lu_panel_kernel, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st, (int)n,
+                                   (int)j0, nrw, S, info);
+                SV_TRY(hipGetLastError());
+                if (rest > 0) {
+                    const size_t ld = (size_t)n;
+                    SV_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)rest,
+                                        (rocblas_int)rest, (rocblas_int)w, &minus_one, S + (j0 + w) + j0 * ld,
+                                        (rocblas_int)n, S + j0 + (j0 + w) * ld, (rocblas_int)n, &one,
+                                        S + (j0 + w) + (j0 + w) * ld, (rocblas_int)n));
+                }
+            }
+            hipLaunchKernelGGL(sv_ipiv_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, ipiv);
+            SV_TRY(hipGetLastError());
+            sv_comp_piv[c] = 0;
+        } else {
+            if (n <= 32768) {
+                // 32-bit interface (its element offsets are 32-bit: n * n must stay far below
+                // 2^31, hence n <= 32768), pivots and info widened to the 64-bit buffers afterwards
+                int* piv32 = (int*)sv[SV_PIV32].p + 3 * sv_comp_pos[c];
+                int* info32 = (int*)sv[SV_PIV32].p + 3 * (size_t)std::max(ns, 1) + c;
+                SV_RB(rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, S, (rocblas_int)n, piv32, info32));
+                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, (const int*)piv32, ipiv,
+                                   (const int*)info32, info);
+                SV_TRY(hipGetLastError());
+            } else {
+                SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, ipiv, info));
+            }
+            hipLaunchKernelGGL(sv_info_kernel, dim3(1), dim3(64), 0, st, 1, (const int64_t*)info, (int*)d_flags.p);
+            SV_TRY(hipGetLastError());
+            sv_comp_piv[c] = 1;
+        }
+    }
+    // the row interchanges of every component as one permutation: on the
+    // device when every block fits the permutation kernel, else on the host
+    bool on_device = true;
+    for (size_t c = 0; c < sv_comp_n.size(); ++c) on_device &= !sv_comp_piv[c] || 3 * sv_comp_n[c] <= PERM_MAX;
+    if (on_device) {
+        for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+            const int n = 3 * sv_comp_n[c];
+            int* perm = (int*)sv[SV_PERM].p + 3 * sv_comp_pos[c];
+            if (sv_comp_piv[c])
+                hipLaunchKernelGGL(sv_perm_kernel, dim3(1), dim3(256), 0, st, n,
+                                   (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], perm, (int*)d_flags.p);
+            else
+                hipLaunchKernelGGL(sv_perm_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, perm);
+            SV_TRY(hipGetLastError());
+        }
+        return 0;
+    }
+    const size_t np3 = 3 * (size_t)ns;
+    piv_h.resize(np3);
+    perm_h.resize(np3);
+    SV_TRY(hipMemcpyAsync(piv_h.data(), sv[SV_IPIV].p, sizeof(int64_t) * np3, hipMemcpyDeviceToHost, st));
+    SV_TRY(hipStreamSynchronize(st));
+    for (size_t c = 0; c < sv_comp_n.size(); ++c) {
+        const int64_t o = 3 * sv_comp_pos[c], n = 3 * (int64_t)sv_comp_n[c];
+        int* p = perm_h.data() + o;
+        for (int64_t i = 0; i < n; ++i) p[i] = (int)i;
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t q = piv_h[o + k] - 1;
+            if (q < 0 || q >= n) {
+                set_error("linear solve: pivot index out of range");
+                return -1;
+            }
+            std::swap(p[k], p[q]);
+        }
+    }
+    SV_TRY(hipMemcpyAsync(sv[SV_PERM].p, perm_h.data(), sizeof(int) * np3, hipMemcpyHostToDevice, st));
     return 0;
 }
 
@@ -892,6 +1312,9 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     hipLaunchKernelGGL(sv_factor_kernel, dim3(n_chains), dim3(64), 0, st, (const int*)d_cstart.p,
                        (const double*)sv[SV_AB].p, (double*)sv[SV_LF].p, (double*)sv[SV_SINV].p, (int*)d_flags.p);
     SV_TRY(hipGetLastError());
+    // capacitance LU: without row interchanges unless partial pivoting was
+    // asked for or this pattern has already needed it
+    bool gepp = sv_gepp || sv_lu_kind == 1;
     if (ns > 0) {
         hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
                            (const double*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
@@ -903,67 +1326,22 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                            (const double*)sv[SV_LF].p, (const double*)sv[SV_SINV].p, (double*)sv[SV_Y].p,
                            (double*)sv[SV_C].p);
         SV_TRY(hipGetLastError());
-        rocblas_handle h = (rocblas_handle)sv_handle;
-        for (size_t c = 0; c < sv_comp_n.size(); ++c) {
-            const int Pc = sv_comp_n[c];
-            double* S = (double*)sv[SV_S].p + sv_comp_soff[c];
-            hipLaunchKernelGGL(sv_capacitance_kernel, dim3(sv_grid(Pc, 64), Pc), dim3(64), 0, st, Pc,
-                               (const int*)sv[SV_CSLOT].p + sv_comp_pos[c], (const int4*)sv[SV_SN].p,
-                               (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, (const int*)d_node_chain.p,
-                               (const int*)sv[SV_CN].p, (const int*)sv[SV_CHCN0].p, (const int*)sv[SV_CHM].p,
-                               (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_C].p, S);
-            SV_TRY(hipGetLastError());
-            const int64_t n = 3 * (int64_t)Pc;
-            if (n <= 32768) {
-                // 32-bit interface (its element offsets are 32-bit: n * n must stay far below
-                // 2^31, hence n <= 32768), pivots and info widened to the 64-bit buffers afterwards
-                int* piv32 = (int*)sv[SV_PIV32].p + 3 * sv_comp_pos[c];
-                int* info32 = (int*)sv[SV_PIV32].p + 3 * (size_t)std::max(ns, 1) + c;
-                SV_RB(rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, S, (rocblas_int)n, piv32, info32));
-                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, (const int*)piv32,
-                                   (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], (const int*)info32,
-                                   (int64_t*)sv[SV_INFO].p + c);
-                SV_TRY(hipGetLastError());
-            } else {
-                SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
-                                          (int64_t*)sv[SV_INFO].p + c));
+        if (sv_factor_capacitance(gepp, st) != 0) return -1;
+        if (!gepp && tol > 0.0) {
+            // eager solve: a zero or non-finite pivot of the LU without
+            // interchanges is read back at once and the blocks factored again
+            // with partial pivoting
+            const size_t nc = sv_comp_n.size();
+            info_h.resize(nc);
+            SV_TRY(hipMemcpyAsync(info_h.data(), sv[SV_INFO].p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, st));
+            SV_TRY(hipStreamSynchronize(st));
+            bool bad = false;
+            for (int64_t v : info_h) bad |= v != 0;
+            if (bad) {
+                gepp = sv_gepp = true;
+                ++sv_fallbacks;
+                if (sv_factor_capacitance(true, st) != 0) return -1;
             }
-        }
-        hipLaunchKernelGGL(sv_info_kernel, dim3(sv_grid((int64_t)sv_comp_n.size())), dim3(256), 0, st,
-                           (int)sv_comp_n.size(), (const int64_t*)sv[SV_INFO].p, (int*)d_flags.p);
-        SV_TRY(hipGetLastError());
-        // the row interchanges of every component as one permutation: on the
-        // device when every block fits the permutation kernel, else on the host
-        bool on_device = true;
-        for (int nc : sv_comp_n) on_device &= 3 * nc <= PERM_MAX;
-        if (on_device) {
-            for (size_t c = 0; c < sv_comp_n.size(); ++c) {
-                hipLaunchKernelGGL(sv_perm_kernel, dim3(1), dim3(256), 0, st, 3 * sv_comp_n[c],
-                                   (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c],
-                                   (int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (int*)d_flags.p);
-                SV_TRY(hipGetLastError());
-            }
-        }
-        const size_t np3 = 3 * (size_t)ns;
-        if (!on_device) {
-        piv_h.resize(np3);
-        perm_h.resize(np3);
-        SV_TRY(hipMemcpyAsync(piv_h.data(), sv[SV_IPIV].p, sizeof(int64_t) * np3, hipMemcpyDeviceToHost, st));
-        SV_TRY(hipStreamSynchronize(st));
-        for (size_t c = 0; c < sv_comp_n.size(); ++c) {
-            const int64_t o = 3 * sv_comp_pos[c], n = 3 * (int64_t)sv_comp_n[c];
-            int* p = perm_h.data() + o;
-            for (int64_t i = 0; i < n; ++i) p[i] = (int)i;
-            for (int64_t k = 0; k < n; ++k) {
-                const int64_t q = piv_h[o + k] - 1;
-                if (q < 0 || q >= n) {
-                    set_error("linear solve: pivot index out of range");
-                    return -1;
-                }
-                std::swap(p[k], p[q]);
-            }
-        }
-        SV_TRY(hipMemcpyAsync(sv[SV_PERM].p, perm_h.data(), sizeof(int) * np3, hipMemcpyHostToDevice, st));
         }
     }
     double* b = (double*)sv[SV_B].p;
@@ -999,26 +1377,53 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     double* red = (double*)sv[SV_RED].p;
     int steps = 0;
     bool have_r = false;
-    double prev = HUGE_VAL;
-    for (; steps < refine; ++steps) {
-        if (residual() != 0) return -1;
+    double rel_last = HUGE_VAL;   // last relative residual read on the host (tol > 0)
+    auto refine_loop = [&]() -> int {
+        have_r = false;
+        double prev = HUGE_VAL;
+        for (int k = 0; k < refine; ++k) {
+            if (residual() != 0) return -1;
+            if (tol > 0.0) {
+                double rel = 0.0;
+                hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r,
+                                   (const double*)b, red);
+                SV_TRY(hipGetLastError());
+                SV_TRY(hipMemcpyAsync(&rel, red, sizeof(double), hipMemcpyDeviceToHost, st));
+                SV_TRY(hipStreamSynchronize(st));
+                rel_last = rel;
+                // done, at the rounding floor (a step no longer halves the
+                // residual), or not finite
+                if (rel <= tol || rel > 0.5 * prev || !(rel <= DBL_MAX)) {
+                    have_r = true;
+                    return 0;
+                }
+                prev = rel;
+            }
+            if (solve_apply(r, x, st) != 0) return -1;
+            hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, (const double*)x, du);
+            SV_TRY(hipGetLastError());
+            ++steps;
+        }
         if (tol > 0.0) {
-            double rel = 0.0;
+            if (residual() != 0) return -1;
             hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r,
                                (const double*)b, red);
             SV_TRY(hipGetLastError());
-            SV_TRY(hipMemcpyAsync(&rel, red, sizeof(double), hipMemcpyDeviceToHost, st));
+            SV_TRY(hipMemcpyAsync(&rel_last, red, sizeof(double), hipMemcpyDeviceToHost, st));
             SV_TRY(hipStreamSynchronize(st));
-            // done, or at the rounding floor (a step no longer halves the residual)
-            if (rel <= tol || rel > 0.5 * prev) {
-                have_r = true;
-                break;
-            }
-            prev = rel;
+            have_r = true;
         }
-        if (solve_apply(r, x, st) != 0) return -1;
-        hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, (const double*)x, du);
-        SV_TRY(hipGetLastError());
+        return 0;
+    };
+    if (refine_loop() != 0) return -1;
+    if (ns > 0 && !gepp && tol > 0.0 && !(rel_last <= tol)) {
+        // the LU without interchanges left the refinement above tol: factor
+        // with partial pivoting (kept for the rest of this pattern) and solve again
+        gepp = sv_gepp = true;
+        ++sv_fallbacks;
+        if (sv_factor_capacitance(true, st) != 0) return -1;
+        if (solve_apply(b, du, st) != 0) return -1;
+        if (refine_loop() != 0) return -1;
     }
     if (n_refine) *n_refine = steps;
     if (res_rel) {

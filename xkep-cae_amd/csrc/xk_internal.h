@@ -99,10 +99,15 @@ struct Ctx {
     std::vector<DBuf> sv;
     std::vector<int> sv_comp_n;                    // contributing pairs per component
     std::vector<int64_t> sv_comp_pos, sv_comp_soff;  // first position / capacitance offset per component
+    std::vector<char> sv_comp_piv;                  // per component: its LU pivots rows (partial pivoting)
     void* sv_handle = nullptr;                     // rocblas handle
     const uint8_t* sv_fixed = nullptr;
     std::vector<int64_t> piv_h;   // host copies of the capacitance pivots and their permutation
     std::vector<int> perm_h;
+    std::vector<int64_t> info_h;
+    int sv_lu_kind = 0;        // capacitance LU: 0 without interchanges (pivoting on failure), 1 partial pivoting
+    bool sv_gepp = false;      // this pattern's solves use partial pivoting (after a failure of the other)
+    int64_t sv_fallbacks = 0;  // solves that fell back to partial pivoting
     // per-element friction coefficients (xk_ctx_set_friction)
     DBuf d_mu_elem;
     bool has_mu_elem = false;
@@ -151,6 +156,7 @@ struct Ctx {
     int build_pattern();
     int solve_setup();
     int solve_apply(const double* r, double* x, hipStream_t st);
+    int sv_factor_capacitance(bool gepp, hipStream_t st);
     int linear_solve(const double* u, const uint8_t* fixed, double mass_coef, const double* M, const double* R,
                      double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st);
     void release_solver();

@@ -575,7 +575,7 @@ class DeviceBackend:
             else:
                 rel = graphs[3]
                 graphs[1].replay()
-                if float(rel.item()) > self.solve_tol:
+                if not float(rel.item()) <= self.solve_tol:   # NaN included
                     # the captured solve's fixed refinement fell short: solve this attempt eagerly
                     self.eager_resolves += 1
                     rel = torch.zeros(1, dtype=torch.float64, device=ctx.dev)
