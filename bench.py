@@ -313,6 +313,31 @@ def coated_assembly(case, states, torch, calls=50):
     return out
 
 
+def mixed_vs_fp64(case, states, torch):
+    """The FP32-mixed assembly against the FP64 plain one on the same batch and
+    state (one fresh assembly each, the same pattern): relative residual and
+    tangent errors, the pairs' statuses (the active set) and the active count."""
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctx, asm, _ = setup(case, states, prec, torch)
+        R, K, _, nact = asm.assemble()
+        torch.cuda.synchronize()
+        res[prec] = (R.cpu().numpy().astype(np.float64), K.cpu().numpy().astype(np.float64),
+                     ctx.get_pairs()["status"].copy(), int(nact.item()))
+        del ctx, asm, R, K
+        torch.cuda.empty_cache()
+    (R64, K64, s64, n64), (R32, K32, s32, n32) = res["fp64"], res["mixed"]
+    scale = np.maximum(np.abs(K64), 1e-30 * np.abs(K64).max())
+    out = {"R_rel_err": float(np.linalg.norm(R32 - R64) / np.linalg.norm(R64)),
+           "K_rel_err_frobenius": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
+           "K_max_entry_rel_err": float(np.max(np.abs(K32 - K64) / scale)),
+           "active_set_equal": bool(np.array_equal(s64, s32)), "n_active_fp64": n64, "n_active_mixed": n32,
+           "note": "FP32-mixed vs FP64 plain, one assembly of the headline batch at the same state; "
+                   "the FP64 form equals the FP64 oracle within 1e-10 (tests)"}
+    log(f"[bench] mixed vs fp64: {out}")
+    return out
+
+
 def newton_attempt(case, torch, wl, reps=5):
     """One Newton attempt at 91 wires: the fused assembly plus the device
     linear solve (xk_linear_solve, refined to 1e-11) against the same assembly
@@ -521,6 +546,7 @@ def main():
                   tune=not args.no_autotune)
     del plain_b["asm"], plain_b["ctx"]
     torch.cuda.empty_cache()
+    mixed_err = mixed_vs_fp64(case, states, torch)
 
     extra = {} if args.no_extra else side_measurements(batched, torch, wl, args)
     nnz_beam = beam_nnz(batched["ctx"].conn_np)
@@ -580,6 +606,7 @@ def main():
         "speedup_vs_fp64_gpu_note": "FP32-mixed against FP64 plain on the same batch and launches",
         "fp64_batched": {"value": round(plain_b["value"], 1), "us_per_step": round(plain_b["ms"] * 1e3, 2),
                          "stages_us": np.round(plain_b["stages"] * 1e3, 2).tolist()},
+        "mixed_vs_fp64": mixed_err,
         "speedup_vs_fp64_gpu_single_instance": round(value / plain["value"], 3),
         "speedup_vs_fp64_gpu_single_instance_note": "batched FP32-mixed against one unbatched FP64 instance "
                                                     "(batching and precision change together)",

@@ -65,11 +65,7 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
             }
             if (g.node_mode != 0 && d < 3 && g.L.pptr) {
                 const int q0 = g.L.pptr[i], q1 = g.L.pptr[i + 1];
-                for (int q = q0; q < q1; ++q) {
-                    const int code = g.L.psrc[q];
-                    const double* rec = g.sd + SD_REC * (size_t)(code >> 2);
-                    fp += rec[code & 3] * rec[4 + d];
-                }
+                for (int q = q0; q < q1; ++q) fp += g.nf[3 * (size_t)q + d];
             }
             double r;
             if (g.node_mode == 0) {

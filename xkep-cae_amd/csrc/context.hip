@@ -185,7 +185,7 @@ int Ctx::build_pattern() {
     tangent_staged = false;
     if (direct && !host_build) {
         if (build_pattern_gpu(nullptr) != 0) return -1;
-        XK_TRY(d_sd.ensure((size_t)std::max(n_sinc, 1) * SD_REC * sizeof(double)));
+        XK_TRY(d_nf.ensure((size_t)std::max(n_sinc, 1) * 12 * sizeof(double)));
         XK_TRY(d_sg.ensure((size_t)std::max(n_sinc, 1) * SG_REC * sizeof(double)));
         pattern_valid = true;
         return 0;
@@ -296,6 +296,11 @@ int Ctx::build_pattern() {
     XK_TRY(upload(d_blk_psrc, psrc));
     XK_TRY(upload(d_node_pptr, nptr));
     XK_TRY(upload(d_node_psrc, nsrc));
+    {
+        std::vector<int> qpos(std::max<size_t>(nsrc.size(), 1), 0);
+        for (size_t q = 0; q < nsrc.size(); ++q) qpos[(size_t)nsrc[q]] = (int)q;
+        XK_TRY(upload(d_node_qpos, qpos));
+    }
     if (np) XK_TRY(hipMemcpy(P.sinc, sinc_of.data(), np * 4, hipMemcpyHostToDevice));
     if (direct) {
         // element -> block slots, and the blocks the gather must still visit
@@ -317,7 +322,7 @@ int Ctx::build_pattern() {
         XK_TRY(upload(d_cdesc, desc));
         n_cdesc = (int)(desc.size() / 4);
     }
-    XK_TRY(d_sd.ensure((size_t)std::max(ns, 1) * SD_REC * sizeof(double)));
+    XK_TRY(d_nf.ensure((size_t)std::max(ns, 1) * 12 * sizeof(double)));
     XK_TRY(d_sg.ensure((size_t)std::max(ns, 1) * SG_REC * sizeof(double)));
     n_sinc = ns;
     n_blocks = nblk;
@@ -364,7 +369,7 @@ GatherArgs Ctx::gather_args(int node_mode) const {
     g.L = node_lists();
     g.B = block_lists();
     g.fe = (const double*)d_fe.p;
-    g.sd = (const double*)d_sd.p;
+    g.nf = (const double*)d_nf.p;
     g.flags = (int*)d_flags.p;
     if (direct) {
         g.fint = (const double*)d_fint.p;
@@ -755,6 +760,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     }
     if ((modes & phys) && c->ensure_pattern() != 0) return -1;
     PairParams pp;
+    pp.nqpos = (const int*)c->d_node_qpos.p;
     pp.k_pen = k_pen;
     pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
     pp.k_t_ratio = k_t_ratio;
@@ -766,7 +772,7 @@ int xk_pairs_run(xk_ctx* c, int modes, const double* coords_ref, const double* u
     pp.allow_deact = allow_deactivation;
     pp.n_sinc = c->n_sinc;
     c->coat_params(pp);
-    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_sd.p,
+    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, coords_ref, u, u_ref, modes, pp, (double*)c->d_nf.p,
                         c->d_sg.p, nullptr, (hipStream_t)stream, nullptr, c->pair_block));
     // a new compression can give an INACTIVE pair a coating contribution: the
     // pattern is rebuilt after every geometry update with the coating on
@@ -884,6 +890,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         return -1;
     }
     PairParams pp;
+    pp.nqpos = (const int*)c->d_node_qpos.p;
     pp.k_pen = k_pen;
     pp.delta_h = smoothing_delta > 0.0 ? k_pen / smoothing_delta : 0.0;
     pp.k_t_ratio = k_t_ratio;
@@ -942,7 +949,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         // profiled: the three stages in sequence on the caller's stream
         XK_TRY(hipEventRecord(ev[0], st));
         XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, st, nullptr, c->pair_block));
+                            (double*)c->d_nf.p, c->d_sg.p, act, st, nullptr, c->pair_block));
         if (lp)
             XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
                                        st));
@@ -955,7 +962,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         XK_TRY(hipEventRecord(c->ev_fork, st));
         XK_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         XK_TRY(launch_pairs(c->prec, n_work, PS, (const double*)c->d_X.p, u, u_ref, modes, pp,
-                            (double*)c->d_sd.p, c->d_sg.p, act, c->side, nullptr, c->pair_block));
+                            (double*)c->d_nf.p, c->d_sg.p, act, c->side, nullptr, c->pair_block));
         // the geometry this assembly gives the pairs it skips, for the verification pass
         if (lp)
             XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
@@ -1280,9 +1287,10 @@ int xk_coating_probe(xk_ctx* c, const double* u, int64_t* n_unplaced, void* stre
     hipStream_t st = (hipStream_t)stream;
     if (c->lp_sync(st, false) != 0) return -1;
     PairParams pp;
+    pp.nqpos = (const int*)c->d_node_qpos.p;
     c->coat_params(pp);
     XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, nullptr, PAIR_GEOM | PAIR_PROBE,
-                        pp, (double*)c->d_sd.p, c->d_sg.p, nullptr, st, nullptr, c->pair_block));
+                        pp, (double*)c->d_nf.p, c->d_sg.p, nullptr, st, nullptr, c->pair_block));
     XK_TRY(c->d_coat_cnt.ensure(sizeof(int)));
     XK_TRY(hipMemsetAsync(c->d_coat_cnt.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(coat_unplaced_kernel, dim3((unsigned)((c->n_pairs + 255) / 256)), dim3(256), 0, st,

@@ -150,7 +150,7 @@ __device__ __forceinline__ void transport_frame(const V3<double>& nrm, const V3<
 template <class GT, int BS, bool COAT>
 __global__ void __launch_bounds__(BS)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
-            const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ sd,
+            const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ nf,
             GT* __restrict__ sg, int* __restrict__ act_part, const int* __restrict__ idx) {
     const int lb = xcd_logical_block();
     const int n_blk = (int)((n_pairs + BS - 1) / BS);
@@ -320,14 +320,15 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 }
             }
             if (modes & (PAIR_HUBER | PAIR_FRIC | PAIR_COAT_F | PAIR_COAT_FRIC)) {
-                double* rec = sd + SD_REC * (size_t)si;
-                rec[0] = 1.0 - s;
-                rec[1] = s;
-                rec[2] = -(1.0 - t);
-                rec[3] = -t;
-                rec[4] = F.x;
-                rec[5] = F.y;
-                rec[6] = F.z;
+                // c_k F at node k, where the gather's node lists read it
+                const double ck[4] = {1.0 - s, s, -(1.0 - t), -t};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    double* o = nf + 3 * (size_t)pp.nqpos[4 * (size_t)si + k];
+                    o[0] = ck[k] * F.x;
+                    o[1] = ck[k] * F.y;
+                    o[2] = ck[k] * F.z;
+                }
             }
             if (modes & (PAIR_TAN_C | PAIR_TAN_F | PAIR_COAT_K | PAIR_COAT_KF)) {
                 double G[9];
@@ -439,11 +440,6 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                         }
                     }
                 }
-                double* rec = sd + SD_REC * (size_t)si;
-                rec[0] = 1.0 - s;
-                rec[1] = s;
-                rec[2] = -(1.0 - t);
-                rec[3] = -t;
                 GT* grec = sg + SG_REC * (size_t)si;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) grec[k] = GT(G[k]);
@@ -543,26 +539,26 @@ bool pair_block_ok(int bs) { return bs == 64 || bs == 128 || bs == 256 || bs == 
 
 template <class GT, int BS>
 static void launch_pairs_bs(int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u, const double* u_ref,
-                            int modes, const PairParams& pp, double* sd, GT* sg, int* act_part, hipStream_t st,
+                            int modes, const PairParams& pp, double* nf, GT* sg, int* act_part, hipStream_t st,
                             const int* idx) {
     dim3 blk(BS), grd((unsigned)xcd_padded(pair_blocks(n_pairs, BS)));
     if (pp.coat_on)
-        hipLaunchKernelGGL((pair_kernel<GT, BS, true>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg,
+        hipLaunchKernelGGL((pair_kernel<GT, BS, true>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg,
                            act_part, idx);
     else
-        hipLaunchKernelGGL((pair_kernel<GT, BS, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg,
+        hipLaunchKernelGGL((pair_kernel<GT, BS, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg,
                            act_part, idx);
 }
 
 template <class GT>
 static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
-                                 const double* u_ref, int modes, const PairParams& pp, double* sd, GT* sg,
+                                 const double* u_ref, int modes, const PairParams& pp, double* nf, GT* sg,
                                  int* act_part, hipStream_t st, const int* idx) {
     switch (bs) {
-        case 64: launch_pairs_bs<GT, 64>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
-        case 128: launch_pairs_bs<GT, 128>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
-        case 256: launch_pairs_bs<GT, 256>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
-        case 512: launch_pairs_bs<GT, 512>(n_pairs, P, Xr, u, u_ref, modes, pp, sd, sg, act_part, st, idx); break;
+        case 64: launch_pairs_bs<GT, 64>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
+        case 128: launch_pairs_bs<GT, 128>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
+        case 256: launch_pairs_bs<GT, 256>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
+        case 512: launch_pairs_bs<GT, 512>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -573,11 +569,11 @@ static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, cons
 // blocks changes the Newton trajectory of frictional strands (measured at 91
 // wires); the mixed form rounds them to FP32 only where the gather writes K.
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
-                        const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
+                        const double* u_ref, int modes, const PairParams& pp, double* nf, void* sg,
                         int* act_part, hipStream_t st, const int* idx, int bs) {
     (void)prec;
     if (n_pairs <= 0) return hipSuccess;
-    return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, sd, (double*)sg, act_part, st, idx);
+    return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, nf, (double*)sg, act_part, st, idx);
 }
 
 }  // namespace xk

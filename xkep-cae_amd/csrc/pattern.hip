@@ -188,6 +188,12 @@ __global__ void pat_list_kernel(int64_t m, const uint64_t* __restrict__ s, int64
     atomicAdd((unsigned long long*)&cnt[(int64_t)(s[j] >> 32)], 1ull);
 }
 
+// qpos[src[q]] = q: where each (slot, k) contribution sits in the node lists
+__global__ void pat_inverse_kernel(int64_t m, const int* __restrict__ src, int* __restrict__ qpos) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) qpos[src[q]] = (int)q;
+}
+
 __global__ void pat_i64_to_i32_kernel(int64_t n, const int64_t* __restrict__ a, int64_t last, int* __restrict__ b) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (int)a[i];
@@ -369,9 +375,13 @@ int Ctx::build_pattern_gpu(hipStream_t st) {
     // node lists
     if (ns > 0) PT_TRY(sort_u64(this, keys3, sorted, 4 * ns, 32 + bits_of((uint64_t)n_nodes), st));
     PT_TRY(hipMemsetAsync(cnt, 0, sizeof(int64_t) * ((size_t)n_nodes + 1), st));
+    PT_TRY(d_node_qpos.ensure(sizeof(int) * (size_t)std::max<int64_t>(4 * ns, 1)));
     if (ns > 0) {
         hipLaunchKernelGGL(pat_list_kernel, dim3(grid(4 * ns)), dim3(256), 0, st, 4 * ns, sorted, cnt,
                            (int*)d_node_psrc.p);
+        PT_TRY(hipGetLastError());
+        hipLaunchKernelGGL(pat_inverse_kernel, dim3(grid(4 * ns)), dim3(256), 0, st, 4 * ns,
+                           (const int*)d_node_psrc.p, (int*)d_node_qpos.p);
         PT_TRY(hipGetLastError());
     }
     if (scan_exclusive_i64(this, 0, cnt, cpos, n_nodes, &tot, st) != 0) return -1;

@@ -142,12 +142,12 @@ struct BeamConst {
     double vref[3];
 };
 
-// Per contributing pair, the pair kernel stages one FP64 force record
-// [c0, c1, c2, c3, Fx, Fy, Fz, pad] and one tangent record
-// [G00 .. G22, c0, c1, c2, c3, pad x3] (FP64 or FP32; 16 entries, so a record
-// is one or two aligned 64-byte segments); element records are 12 FP64 forces
+// Per contributing pair, the pair kernel stages its force at its four nodes,
+// c_k F (3 FP64 values each), in node order -- at the position of (pair slot,
+// k) in the node contribution lists, so the gather sums contiguous runs --
+// and one tangent record [G00 .. G22, c0, c1, c2, c3, pad x3] (FP64; 16
+// entries, two aligned 64-byte segments); element records are 12 FP64 forces
 // and 144 tangent values.
-constexpr int SD_REC = 8;
 constexpr int SG_REC = 16;
 constexpr int SG_C = 9;   // offset of c[4] in the tangent record
 

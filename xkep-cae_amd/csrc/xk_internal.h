@@ -63,7 +63,8 @@ struct Ctx {
     std::vector<DBuf> pat;  // scratch of the device pattern build
     DBuf pat_tmp;           // radix-sort scratch
     DBuf d_gptr, d_gcol, d_brow, d_blk_bptr, d_blk_bsrc, d_blk_pptr, d_blk_psrc, d_node_pptr, d_node_psrc;
-    DBuf d_sd, d_sg;
+    DBuf d_nf, d_sg;
+    DBuf d_node_qpos;   // position of (slot, k) in the node lists (inverse of d_node_psrc)
     // broadphase scratch and candidate bookkeeping
     DBuf bp[18];
     DBuf gj;                        // constraint Jacobian: per-pair flags, row lengths, offsets
@@ -173,7 +174,7 @@ struct Ctx {
     void release_all() {
         DBuf* all[] = {&d_X, &d_conn, &d_rad, &d_node_bptr, &d_node_bsrc, &d_fe, &d_Ke, &d_flags, &d_partial,
                        &d_gptr, &d_gcol, &d_brow, &d_blk_bptr, &d_blk_bsrc, &d_blk_pptr, &d_blk_psrc,
-                       &d_node_pptr, &d_node_psrc, &d_sd, &d_sg};
+                       &d_node_pptr, &d_node_psrc, &d_nf, &d_sg, &d_node_qpos};
         for (DBuf* b : all) b->release();
         for (auto& b : pb) b.release();
         for (auto& b : bp) b.release();
@@ -224,7 +225,7 @@ hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const doub
 // idx (nullable): the kernel visits pairs idx[0 .. n_pairs) instead of 0 .. n_pairs;
 // bs: workgroup size (64, 128, 256 or 512), act_part holds pair_blocks(n, bs) counts
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
-                        const double* u_ref, int modes, const PairParams& pp, double* sd, void* sg,
+                        const double* u_ref, int modes, const PairParams& pp, double* nf, void* sg,
                         int* act_part, hipStream_t st, const int* idx = nullptr, int bs = 256);
 bool pair_block_ok(int bs);
 bool gather_block_ok(int bs);

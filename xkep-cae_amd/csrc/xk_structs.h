@@ -42,6 +42,8 @@ struct PairParams {
     int coat_on = 0;
     double coat_k = 0.0, coat_c = 0.0, coat_mu = 0.0, coat_kt = 0.0, coat_dt = 0.0;
     int* flags = nullptr;   // status flags (FLAG_COAT_UNPLACED)
+    // position of (slot s, node k) in the node contribution lists: nqpos[4 s + k]
+    const int* nqpos = nullptr;
 };
 
 struct NodeLists {
@@ -80,7 +82,7 @@ struct GatherArgs {
     int direct;
     const int4* cdesc;
     int n_cdesc;
-    const double* sd;  // pair c[4], F[3] staging
+    const double* nf;  // pair force staging: c_k F (3 values) per node-list entry
     const double* u;
     const double* f_ext;
     const uint8_t* fixed;
