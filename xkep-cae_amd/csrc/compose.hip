@@ -64,8 +64,15 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                 }
             }
             if (g.node_mode != 0 && d < 3 && g.L.pptr) {
+                // four loads in flight per step, summed in list order
                 const int q0 = g.L.pptr[i], q1 = g.L.pptr[i + 1];
-                for (int q = q0; q < q1; ++q) fp += g.nf[3 * (size_t)q + d];
+                int q = q0;
+                for (; q + 4 <= q1; q += 4) {
+                    const double a0 = g.nf[3 * (size_t)q + d], a1 = g.nf[3 * (size_t)(q + 1) + d];
+                    const double a2 = g.nf[3 * (size_t)(q + 2) + d], a3 = g.nf[3 * (size_t)(q + 3) + d];
+                    fp = (((fp + a0) + a1) + a2) + a3;
+                }
+                for (; q < q1; ++q) fp += g.nf[3 * (size_t)q + d];
             }
             double r;
             if (g.node_mode == 0) {
@@ -113,7 +120,27 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
 #pragma unroll
                 for (int k = 0; k < 9; ++k) v[k] = A(0);
             }
-            for (int q = q0; q < q1; ++q) {
+            // two contributions per step: both codes, then both records, in
+            // flight together; accumulated in list order
+            int q = q0;
+            for (; q + 2 <= q1; q += 2) {
+                const int c0 = g.B.psrc[q], c1 = g.B.psrc[q + 1];
+                const GT* r0 = sg + SG_REC * (size_t)(c0 >> 4);
+                const GT* r1 = sg + SG_REC * (size_t)(c1 >> 4);
+                GT x0[9], x1[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    x0[k] = r0[k];
+                    x1[k] = r1[k];
+                }
+                const A cc0 = A(r0[SG_C + ((c0 >> 2) & 3)]) * A(r0[SG_C + (c0 & 3)]);
+                const A cc1 = A(r1[SG_C + ((c1 >> 2) & 3)]) * A(r1[SG_C + (c1 & 3)]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[k] += cc0 * A(x0[k]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[k] += cc1 * A(x1[k]);
+            }
+            if (q < q1) {
                 const int code = g.B.psrc[q];
                 const GT* r = sg + SG_REC * (size_t)(code >> 4);
                 const A cc = A(r[SG_C + ((code >> 2) & 3)]) * A(r[SG_C + (code & 3)]);
