@@ -355,7 +355,8 @@ __global__ void sv_info_kernel(int nc, const int64_t* __restrict__ info, int* __
 // factorization or triangular solve runs with one row (or column) per lane in
 // registers, the rest of the block is updated by 3 x 3 / 4 x 3 register tiles
 // (the code stays a few thousand instructions: the launch is not bound by
-// instruction fetch).  Divisions by a pivot are products with its reciprocal
+// instruction fetch; full panels use a variant whose unrolled steps carry no
+// bound checks).  Divisions by a pivot are products with its reciprocal
 // (formed once per pivot), which keeps the divisions off the dependent chains.
 //
 // Without interchanges the pivots are the Schur complements' diagonals in
@@ -376,13 +377,16 @@ __device__ __forceinline__ double sv_readlane(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
+// FULL: the panel holds LU_NB columns (every panel but a last partial one):
+// the sub-block bounds are constants and the unrolled steps carry no guards.
+template <bool FULL>
 __global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_row_wg, double* __restrict__ A,
                                                           int64_t* __restrict__ info) {
     __shared__ double D[LU_NB][LU_NB + 1];
     __shared__ double X[LU_NB][LU_NB + 1];
     __shared__ double invd[LU_NB];   // reciprocals of the pivots
     __shared__ int s_bad;
-    const int w = min(LU_NB, n - j0);
+    const int w = FULL ? LU_NB : min(LU_NB, n - j0);
     const int tid = threadIdx.x;
     const size_t ld = (size_t)n;
     constexpr int CP = LU_NB * LU_NB / 256;   // tile entries per lane (all loads issued before the stores)
@@ -394,8 +398,9 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_r
     if (tid == 0) s_bad = -1;
     __syncthreads();
     // ---- diagonal block
+#pragma unroll 1
     for (int o = 0; o < w; o += LU_SB) {
-        const int bw = min(LU_SB, w - o);
+        const int bw = FULL ? LU_SB : min(LU_SB, w - o);
         if (tid < 64) {
             // (a) the bw x bw sub-block: lane i holds row o + i, the pivot row
             // of step c broadcast from lane c
@@ -517,8 +522,9 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_r
             X[i][q] = (i < mr && q < w) ? A[(size_t)(row0 + i) + (size_t)(j0 + q) * ld] : 0.0;
         }
         __syncthreads();
+#pragma unroll 1
         for (int o = 0; o < w; o += LU_SB) {
-            const int bw = min(LU_SB, w - o);
+            const int bw = FULL ? LU_SB : min(LU_SB, w - o);
             if (tid < LU_NB) {
                 double x[LU_SB];
 #pragma unroll
@@ -588,8 +594,9 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_r
         X[q][j] = (q < w && j < nc) ? A[(size_t)(j0 + q) + (size_t)(col0 + j) * ld] : 0.0;
     }
     __syncthreads();
+#pragma unroll 1
     for (int o = 0; o < w; o += LU_SB) {
-        const int bw = min(LU_SB, w - o);
+        const int bw = FULL ? LU_SB : min(LU_SB, w - o);
         if (tid < LU_NB) {
             double y[LU_SB];
 #pragma unroll
@@ -1212,8 +1219,12 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                 const int64_t rest = n - j0 - w;
                 const int nrw = (int)((rest + LU_NB - 1) / LU_NB);
                 const int ncw = nrw;
-                hipLaunchKernelGGL(sv_lu_panel_kernel, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st, (int)n,
-                                   (int)j0, nrw, S, info);
+                if (w == LU_NB)
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<true>, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st,
+                                       (int)n, (int)j0, nrw, S, info);
+                else
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<false>, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st,
+                                       (int)n, (int)j0, nrw, S, info);
                 SV_TRY(hipGetLastError());
                 if (rest > 0) {
                     const size_t ld = (size_t)n;
