@@ -54,8 +54,9 @@ __device__ __forceinline__ M3<double> rotvec_to_rotmat(V3<double> th, HalfAngle*
         sinc = 0.5 - h2 / 48.0;
         ch = 1.0 - h2 / 2.0;
     } else {
-        sinc = sin(half) / ang;
-        ch = cos(half);
+        double sh;
+        sincos(half, &sh, &ch);   // one argument reduction for both
+        sinc = sh / ang;
     }
     if (ha) *ha = HalfAngle{ang, sinc * ang, ch};
     double w = ch, x = sinc * th.x, y = sinc * th.y, z = sinc * th.z;
