@@ -147,10 +147,15 @@ def profile_stages(asm, ctx, torch, calls):
 
 
 STAGES = ["pairs", "elements", "gather", "finish"]
-# kernel-name prefixes of each stage in the FP32-mixed form (the template
-# arguments after the precision are the tuned workgroup sizes)
-KERNELS = {"pairs": "xk::pair_kernel<double", "elements": "xk::beam_direct_kernel<float",
-           "gather": "xk::gather_kernel<float, float, double", "finish": "xk::finish_kernel"}
+
+
+def stage_kernels(tuning) -> dict:
+    """Kernel names of each stage in the FP32-mixed form, with the tuned
+    workgroup sizes as they appear in the rocprofv3 summaries."""
+    return {"pairs": f"xk::pair_kernel<double, {tuning['pair_block']}, false>",
+            "elements": "xk::beam_direct_kernel<float>",
+            "gather": f"xk::gather_kernel<float, float, double, {tuning['gather_block']}>",
+            "finish": "xk::finish_kernel"}
 TRAFFIC_TABLE = os.path.join(HERE, "profiles", "assembly_kernels_91wire_hbm_traffic.md")
 
 
@@ -327,10 +332,9 @@ def mixed_vs_fp64(case, states, torch):
         del ctx, asm, R, K
         torch.cuda.empty_cache()
     (R64, K64, s64, n64), (R32, K32, s32, n32) = res["fp64"], res["mixed"]
-    scale = np.maximum(np.abs(K64), 1e-30 * np.abs(K64).max())
     out = {"R_rel_err": float(np.linalg.norm(R32 - R64) / np.linalg.norm(R64)),
            "K_rel_err_frobenius": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
-           "K_max_entry_rel_err": float(np.max(np.abs(K32 - K64) / scale)),
+           "K_max_abs_err_over_max_K": float(np.max(np.abs(K32 - K64)) / np.abs(K64).max()),
            "active_set_equal": bool(np.array_equal(s64, s32)), "n_active_fp64": n64, "n_active_mixed": n32,
            "note": "FP32-mixed vs FP64 plain, one assembly of the headline batch at the same state; "
                    "the FP64 form equals the FP64 oracle within 1e-10 (tests)"}
@@ -554,9 +558,10 @@ def main():
     dom = int(np.argmax(batched["stages"]))
     dom_ms = float(batched["stages"][dom])
     achieved = sb[STAGES[dom]] / (dom_ms * 1e-3) / 1e9
+    kernels = stage_kernels(batched["ctx"].get_tuning())
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes(KERNELS[STAGES[dom]]),
-            "kernel": KERNELS[STAGES[dom]], "algorithmic_bytes": int(sb[STAGES[dom]]),
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes(kernels[STAGES[dom]]),
+            "kernel": kernels[STAGES[dom]], "algorithmic_bytes": int(sb[STAGES[dom]]),
             "kernel_us": round(dom_ms * 1e3, 2),
             "all_kernels": {k: {"us": round(float(t) * 1e3, 2), "algorithmic_bytes": int(sb[k]),
                                 "GB_s": round(sb[k] / (float(t) * 1e-3) / 1e9, 1)}
