@@ -1213,26 +1213,43 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
         int64_t* ipiv = (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c];
         int64_t* info = (int64_t*)sv[SV_INFO].p + c;
         if (!gepp && n <= LU_MAX) {
+            // Outer panels of two LU_NB panels: P0 is factored and updates only
+            // P1's columns and P1's rows right of it (thin DGEMMs); P1 is factored;
+            // one rank-2 LU_NB DGEMM then updates the trailing matrix with both, so
+            // the trailing matrix streams through HBM once per 2 LU_NB columns.
             const double one = 1.0, minus_one = -1.0;
-            for (int64_t j0 = 0; j0 < n; j0 += LU_NB) {
+            const size_t ld = (size_t)n;
+            auto panel = [&](int64_t j0) -> int {
                 const int w = (int)std::min<int64_t>(LU_NB, n - j0);
                 const int64_t rest = n - j0 - w;
                 const int nrw = (int)((rest + LU_NB - 1) / LU_NB);
-                const int ncw = nrw;
                 if (w == LU_NB)
-                    hipLaunchKernelGGL(sv_lu_panel_kernel<true>, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st,
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<true>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st,
                                        (int)n, (int)j0, nrw, S, info);
                 else
-                    hipLaunchKernelGGL(sv_lu_panel_kernel<false>, dim3(std::max(1, nrw + ncw)), dim3(256), 0, st,
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<false>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st,
                                        (int)n, (int)j0, nrw, S, info);
                 SV_TRY(hipGetLastError());
-                if (rest > 0) {
-                    const size_t ld = (size_t)n;
-                    SV_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)rest,
-                                        (rocblas_int)rest, (rocblas_int)w, &minus_one, S + (j0 + w) + j0 * ld,
-                                        (rocblas_int)n, S + j0 + (j0 + w) * ld, (rocblas_int)n, &one,
-                                        S + (j0 + w) + (j0 + w) * ld, (rocblas_int)n));
-                }
+                return 0;
+            };
+            // C (m x q) -= A (m x k) B (k x q), all in S
+            auto update = [&](int64_t m, int64_t q, int64_t k, const double* A_, const double* B_, double* C_) -> int {
+                if (m <= 0 || q <= 0) return 0;
+                SV_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m,
+                                    (rocblas_int)q, (rocblas_int)k, &minus_one, A_, (rocblas_int)n, B_,
+                                    (rocblas_int)n, &one, C_, (rocblas_int)n));
+                return 0;
+            };
+            for (int64_t J = 0; J < n; J += 2 * LU_NB) {
+                if (panel(J) != 0) return -1;
+                const int64_t w0 = std::min<int64_t>(LU_NB, n - J), j1 = J + w0;
+                if (j1 >= n) break;
+                const int64_t w1 = std::min<int64_t>(LU_NB, n - j1), j2 = j1 + w1;
+                if (update(n - j1, w1, w0, S + j1 + J * ld, S + J + j1 * ld, S + j1 + j1 * ld) != 0) return -1;
+                if (update(w1, n - j2, w0, S + j1 + J * ld, S + J + j2 * ld, S + j1 + j2 * ld) != 0) return -1;
+                if (panel(j1) != 0) return -1;
+                if (update(n - j2, n - j2, w0 + w1, S + j2 + J * ld, S + J + j2 * ld, S + j2 + j2 * ld) != 0)
+                    return -1;
             }
             hipLaunchKernelGGL(sv_ipiv_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, ipiv);
             SV_TRY(hipGetLastError());
