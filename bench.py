@@ -38,11 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(case, states, precision, torch, batch=None, mu_elem=None, coating=None):
+def setup(case, states, precision, torch, batch=None, mu_elem=None, coating=None, layer_filter=True):
     """Context + assembly for one instance, a concatenated batch of instances
     of one case, or a prepared uneven batch (per-element friction); ``coating``
     (config fields, core radius fraction, coat_dt) switches the Kelvin-Voigt
-    coating terms on."""
+    coating terms on; ``layer_filter`` False keeps every candidate pair of the
+    broadphase (no same-layer exclusion, no layer map)."""
     wl = importlib.import_module("xkep-cae_amd.workloads")
     C = importlib.import_module("xkep-cae_amd.context")
     A = importlib.import_module("xkep-cae_amd.assembly")
@@ -64,7 +65,8 @@ def setup(case, states, precision, torch, batch=None, mu_elem=None, coating=None
         layer = b.elem_layer
     mu = case.mu if mu_elem is None else 1.0
     coat_kw, core, coat_dt = ({}, None, 0.0) if coating is None else coating
-    cfg = K.ContactConfig(mu=mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer), **coat_kw)
+    cfg = (K.ContactConfig(mu=mu, exclude_same_layer=True, elem_layer_map=wl.layer_map(layer), **coat_kw)
+           if layer_filter else K.ContactConfig(mu=mu, exclude_same_layer=False, **coat_kw))
     sec = C.circle_section(case.wire_radius, case.E, case.nu)
     ctx = C.DeviceContext(X, conn, sec, radii=radii, precision=precision)
     if mu_elem is not None:
@@ -187,6 +189,25 @@ def stage_bytes(ctx, asm, n_pair, precision, nnz_beam):
         "gather": nn * 10 * 48 + (nnz - nnz_beam) * s_k,
         "finish": (6 * nn + 255) // 256 * 8 + (n_pair + 255) // 256 * 4,
     }
+
+
+FLOPS_TABLE = os.path.join(HERE, "profiles", "assembly_kernels_91wire_flops.md")
+
+
+def kernel_mflop(kernel: str):
+    """Floating-point work per dispatch (MFLOP, FP64 + FP32) of the kernel whose
+    name starts with `kernel`, at the largest grid of the committed rocprofv3
+    instruction-count summary, or None."""
+    if not os.path.exists(FLOPS_TABLE):
+        return None
+    best = None
+    for line in open(FLOPS_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 14 and cells[0].startswith(kernel) and cells[1].isdigit():
+            grid, mf = int(cells[1]), float(cells[13])
+            if best is None or grid > best[0]:
+                best = (grid, mf)
+    return None if best is None else best[1]
 
 
 def traffic_bytes(kernel: str):
@@ -487,6 +508,21 @@ def side_measurements(batched, torch, wl, args):
     torch.cuda.empty_cache()
     out["coated_assembly"] = coated_assembly(wl.C3, batched["states"], torch)
     out["newton_attempt_91wire"] = newton_attempt(wl.C3, torch, wl)
+    # the same batch with every candidate pair of the broadphase (no same-layer exclusion)
+    ctx_u, asm_u, _ = setup(wl.C3, batched["states"], "mixed", torch, layer_filter=False)
+    asm_u.assemble()
+    torch.cuda.synchronize()
+    st_u = profile_stages(asm_u, ctx_u, torch, 20)
+    ms_u, _ = time_graph(asm_u, torch, steps, args.warmup)
+    n_u = ctx_u.n_elems + ctx_u.n_pairs
+    out["unfiltered_candidates"] = {"n_elem": ctx_u.n_elems, "n_pair": ctx_u.n_pairs,
+                                    "us_per_step": round(ms_u * 1e3, 2), "value": round(n_u / (ms_u * 1e-3), 1),
+                                    "stages_us": np.round(st_u * 1e3, 2).tolist(),
+                                    "workload": f"the same {args.batch} x 91-wire batch without the same-layer "
+                                                "exclusion: every broadphase candidate pair is a pair"}
+    log(f"[bench] unfiltered candidates x{args.batch}: {out['unfiltered_candidates']}")
+    del ctx_u, asm_u
+    torch.cuda.empty_cache()
     return out
 
 
@@ -566,6 +602,14 @@ def main():
             "all_kernels": {k: {"us": round(float(t) * 1e3, 2), "algorithmic_bytes": int(sb[k]),
                                 "GB_s": round(sb[k] / (float(t) * 1e-3) / 1e9, 1)}
                             for k, t in zip(STAGES, batched["stages"])}}
+    # floating-point work per stage (measured instruction counts, committed summary)
+    flop_base = {"pairs": "xk::pair_kernel<double", "elements": "xk::beam_direct_kernel<float>",
+                 "gather": "xk::gather_kernel<float, float, double", "finish": "xk::finish_kernel"}
+    for k, t in zip(STAGES, batched["stages"]):
+        mf = kernel_mflop(flop_base[k])
+        if mf is not None:
+            roof["all_kernels"][k].update(mflop=mf, TFLOP_s=round(mf * 1e6 / (float(t) * 1e-3) / 1e12, 2),
+                                          flop_per_byte=round(mf * 1e6 / max(sb[k], 1), 2))
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (FP64 NumPy oracle, 1 thread)")
