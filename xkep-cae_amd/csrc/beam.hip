@@ -132,7 +132,15 @@ __device__ __forceinline__ V3<double> rotmat_to_rotvec(const M3<double>& R, Half
 // and, for the inverse, 1/a^2 - (1 + cos a)/(2 a sin a) = 1/a^2 - cos(a/2)/(2 a sin(a/2)),
 // by their Taylor series below a = 1/2 (truncation < 1e-12) where the closed
 // forms cancel in FP32; the reference's own constants below a = 1e-10.
-__device__ __forceinline__ M3<float> tangent_op_f32(V3<double> thd, const HalfAngle& h, bool inverse) {
+// The operator is I + c1 S(th) + c2 S(th)^2; its coefficients are formed where
+// the rotation maps give the half angle (so the FP64 half-angle values are not
+// kept until the tangent), the matrix at the tangent.
+struct TanOpF {
+    V3<float> th;
+    float c1, c2;
+};
+
+__device__ __forceinline__ TanOpF tangent_coef_f32(V3<double> thd, const HalfAngle& h, bool inverse) {
     const float a = (float)h.a, s = (float)h.s, c = (float)h.c;
     const float a2 = a * a;
     float c1, c2;
@@ -154,12 +162,15 @@ __device__ __forceinline__ M3<float> tangent_op_f32(V3<double> thd, const HalfAn
                                                                                      a2 * (1.0f / 47900160.0f))));
         else c2 = 1.0f / a2 - c / (2.0f * a * s);
     }
-    V3<float> th = cvt<float>(thd);
-    M3<float> S = skew(th);
+    return TanOpF{cvt<float>(thd), c1, c2};
+}
+
+__device__ __forceinline__ M3<float> tangent_mat_f32(const TanOpF& o) {
+    M3<float> S = skew(o.th);
     M3<float> S2 = mul(S, S);
     M3<float> r;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r.m[k] = ((k % 4 == 0) ? 1.0f : 0.0f) + c1 * S.m[k] + c2 * S2.m[k];
+    for (int k = 0; k < 9; ++k) r.m[k] = ((k % 4 == 0) ? 1.0f : 0.0f) + o.c1 * S.m[k] + o.c2 * S2.m[k];
     return r;
 }
 
@@ -184,15 +195,54 @@ __device__ __forceinline__ M3<T> tangent_op(V3<double> thd, bool inverse) {
     return r;
 }
 
+// The symmetrised element tangent Ks = (K + K^T) / 2 in 3x3 group blocks
+// (b, c) over {node-1 translations, node-1 rotations, node-2 translations,
+// node-2 rotations}.  The node-2 translation rows are the negated node-1
+// translation rows (translational equilibrium of the element: f_2 = -f_1 for
+// the axial and shear forces, and the B-operator rows of node 2 are those of
+// node 1 negated), and column group 2 is column group 0 negated (the
+// [x, y, -x, z] form), so every block is +-(a block over groups {0, 1, 3}),
+// possibly transposed:
+//   Ks(2, c) = -Ks(0, c) for c != 2, Ks(b, 2) = -Ks(b, 0) for b != 2,
+//   Ks(2, 2) = Ks(0, 0), Ks(c, b) = Ks(b, c)^T.
+// Six blocks carry everything: the symmetric diagonals (0,0), (1,1), (3,3)
+// (upper triangles, 6 values each) and (0,1), (0,3), (1,3) (9 values each).
+template <class T>
+struct KSym {
+    T d0[6], d1[6], d3[6];
+    T o01[9], o03[9], o13[9];
+};
+
+// Upper-triangle index of (i, j) in a symmetric 3x3 block.
+__device__ __forceinline__ constexpr int sym_idx(int i, int j) {
+    return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
+}
+
+// Entry (i, j) of block (b, c) of Ks.
+template <class T>
+__device__ __forceinline__ T ksv(const KSym<T>& S, int b, int c, int i, int j) {
+    const bool neg = (b == 2) != (c == 2);
+    int bb = b == 2 ? 0 : b, cc = c == 2 ? 0 : c;
+    if (bb > cc) {
+        const int t = bb; bb = cc; cc = t;
+        const int u = i; i = j; j = u;
+    }
+    T v;
+    if (bb == cc) v = (bb == 0 ? S.d0 : (bb == 1 ? S.d1 : S.d3))[sym_idx(i, j)];
+    else if (bb == 0) v = (cc == 1 ? S.o01 : S.o03)[3 * i + j];
+    else v = S.o13[3 * i + j];
+    return neg ? -v : v;
+}
+
 constexpr int BEAM_BLOCK = 64;
 
-// Element e: global-frame force fe[12] (FP64) and, when want_K, the three 3x3
-// matrices per 3-row block of the unsymmetrised tangent (precision TT).
-template <class TT>
+// Element e: global-frame force fe[12] (FP64), handed to on_force(fe) as soon
+// as it is formed (so its registers are free during the tangent), and, when
+// want_K, the symmetrised tangent S (precision TT).
+template <class TT, class OnForce>
 __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, const double* __restrict__ u,
-                                          const int* __restrict__ conn, const BeamConst& bc, double* fe,
-                                          M3<TT>* Xb, M3<TT>* Yb, M3<TT>* Zb, bool want_K,
-                                          int* __restrict__ flags) {
+                                          const int* __restrict__ conn, const BeamConst& bc, OnForce&& on_force,
+                                          KSym<TT>& S, bool want_K, int* __restrict__ flags) {
     const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
     V3<double> P1{X[3 * n1], X[3 * n1 + 1], X[3 * n1 + 2]};
     V3<double> P2{X[3 * n2], X[3 * n2 + 1], X[3 * n2 + 2]};
@@ -285,6 +335,14 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     M3<double> Rd2 = mulNT(mul(Rcr, rotvec_to_rotmat(r2, &hr2)), R0);
     V3<double> th1 = rotmat_to_rotvec<FAST>(Rd1, &ht1);
     V3<double> th2 = rotmat_to_rotvec<FAST>(Rd2, &ht2);
+    // FP32 tangent operators' coefficients, formed while the half angles are at hand
+    TanOpF tos1{}, tos2{}, toi1{}, toi2{};
+    if constexpr (FAST) {
+        tos1 = tangent_coef_f32(r1, hr1, false);
+        tos2 = tangent_coef_f32(r2, hr2, false);
+        toi1 = tangent_coef_f32(th1, ht1, true);
+        toi2 = tangent_coef_f32(th2, ht2, true);
+    }
     double dax = Ld - L0;
 
     // local Timoshenko stiffness coefficients (_beam_cr.py:911-998)
@@ -342,12 +400,16 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     fcr[4] = m44 * th1.y + m410 * th2.y;
     fcr[10] = m410 * th1.y + m44 * th2.y;
 
+    {
+        double fe[12];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
-        fe[3 * b] = fb.x;
-        fe[3 * b + 1] = fb.y;
-        fe[3 * b + 2] = fb.z;
+        for (int b = 0; b < 4; ++b) {
+            V3<double> fb = mulTv(Rcr, V3<double>{fcr[3 * b], fcr[3 * b + 1], fcr[3 * b + 2]});
+            fe[3 * b] = fb.x;
+            fe[3 * b + 1] = fb.y;
+            fe[3 * b + 2] = fb.z;
+        }
+        on_force(fe);
     }
     if (!want_K) return;
 
@@ -362,10 +424,10 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     }
     M3<T> Ti1, Ti2, Ts1, Ts2;
     if constexpr (std::is_same<T, float>::value) {
-        Ti1 = tangent_op_f32(th1, ht1, true);
-        Ti2 = tangent_op_f32(th2, ht2, true);
-        Ts1 = tangent_op_f32(r1, hr1, false);
-        Ts2 = tangent_op_f32(r2, hr2, false);
+        Ti1 = tangent_mat_f32(toi1);
+        Ti2 = tangent_mat_f32(toi2);
+        Ts1 = tangent_mat_f32(tos1);
+        Ts2 = tangent_mat_f32(tos2);
     } else {
         Ti1 = tangent_op<T>(th1, true);
         Ti2 = tangent_op<T>(th2, true);
@@ -381,35 +443,33 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
     const T tea = T(ea), tgj = T(gj), tk15 = T(k15), tk55 = T(k55), tk511 = T(k511);
     const T tm15 = T(m15), tm44 = T(m44), tm410 = T(m410);
 
-    // (Ke B) rows as x / y / z parts; KBx[r] etc. are 3-vectors
+    // (Ke B) rows as x / y / z parts; KBx[r] etc. are 3-vectors.  Rows 6-8 (node-2
+    // translations) are rows 0-2 negated and are not formed.
     T KBx[12][3], KBy[12][3], KBz[12][3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         T exj = j == 0 ? ex.x : (j == 1 ? ex.y : ex.z);
-        // axial rows 0 / 6: B[6] = [-e_x, 0, e_x, 0]
+        // axial row 0: B[6] = [-e_x, 0, e_x, 0]
         KBx[0][j] = tea * exj;  KBy[0][j] = T(0); KBz[0][j] = T(0);
-        KBx[6][j] = -tea * exj; KBy[6][j] = T(0); KBz[6][j] = T(0);
         // torsion rows 3 / 9
         KBx[3][j] = tgj * P1m(0, j) - tgj * P2m(0, j);
         KBy[3][j] = tgj * Q1m(0, j);
         KBz[3][j] = -tgj * Q2m(0, j);
         KBx[9][j] = -KBx[3][j]; KBy[9][j] = -KBy[3][j]; KBz[9][j] = -KBz[3][j];
-        // bending x-y plane rows 1, 5, 7, 11 (rotation component z = row 2 of P/Q)
+        // bending x-y plane rows 1, 5, 11 (rotation component z = row 2 of P/Q)
         KBx[1][j] = tk15 * P1m(2, j) + tk15 * P2m(2, j);
         KBy[1][j] = tk15 * Q1m(2, j);
         KBz[1][j] = tk15 * Q2m(2, j);
-        KBx[7][j] = -KBx[1][j]; KBy[7][j] = -KBy[1][j]; KBz[7][j] = -KBz[1][j];
         KBx[5][j] = tk55 * P1m(2, j) + tk511 * P2m(2, j);
         KBy[5][j] = tk55 * Q1m(2, j);
         KBz[5][j] = tk511 * Q2m(2, j);
         KBx[11][j] = tk511 * P1m(2, j) + tk55 * P2m(2, j);
         KBy[11][j] = tk511 * Q1m(2, j);
         KBz[11][j] = tk55 * Q2m(2, j);
-        // bending x-z plane rows 2, 4, 8, 10 (rotation component y = row 1)
+        // bending x-z plane rows 2, 4, 10 (rotation component y = row 1)
         KBx[2][j] = -tm15 * P1m(1, j) - tm15 * P2m(1, j);
         KBy[2][j] = -tm15 * Q1m(1, j);
         KBz[2][j] = -tm15 * Q2m(1, j);
-        KBx[8][j] = -KBx[2][j]; KBy[8][j] = -KBy[2][j]; KBz[8][j] = -KBz[2][j];
         KBx[4][j] = tm44 * P1m(1, j) + tm410 * P2m(1, j);
         KBy[4][j] = tm44 * Q1m(1, j);
         KBz[4][j] = tm410 * Q2m(1, j);
@@ -418,9 +478,9 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
         KBz[10][j] = tm44 * Q2m(1, j);
     }
 
-    // block rows b: X_b = Rc^T KBx_b + Rc^T S(f_b) dpsi, Y_b = Rc^T KBy_b, Z_b = Rc^T KBz_b
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    // block row b of K: X_b = Rc^T KBx_b + Rc^T S(f_b) dpsi, Y_b = Rc^T KBy_b,
+    // Z_b = Rc^T KBz_b; K(b, .) = [X_b, Y_b, -X_b, Z_b]
+    auto block_row = [&](int b, M3<T>& Xo, M3<T>& Yo, M3<T>& Zo) {
         M3<T> Ax, Ay, Az;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -431,24 +491,40 @@ __device__ __forceinline__ void beam_core(int e, const double* __restrict__ X, c
                 Az.m[3 * i + j] = KBz[3 * b + i][j];
             }
         M3<T> W = mul(mulTN(Rc, skew(V3<T>{T(fcr[3 * b]), T(fcr[3 * b + 1]), T(fcr[3 * b + 2])})), dpsi);
-        Xb[b] = mulTN(Rc, Ax);
+        Xo = mulTN(Rc, Ax);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Xb[b].m[k] = Xb[b].m[k] + W.m[k];
-        Yb[b] = mulTN(Rc, Ay);
-        Zb[b] = mulTN(Rc, Az);
-    }
-
-}
-
-// Symmetrised 3x3 group block Ks(b,c) = (K(b,c) + K(c,b)^T) / 2 of the element
-// tangent, K(b,0) = X_b, K(b,1) = Y_b, K(b,2) = -X_b, K(b,3) = Z_b.
-template <class T>
-__device__ __forceinline__ T ks(const M3<T>* Xb, const M3<T>* Yb, const M3<T>* Zb, int b, int c, int i, int j) {
-    const M3<T>& Kbc = (c == 0 || c == 2) ? Xb[b] : (c == 1 ? Yb[b] : Zb[b]);
-    const T sbc = (c == 2) ? T(-1) : T(1);
-    const M3<T>& Kcb = (b == 0 || b == 2) ? Xb[c] : (b == 1 ? Yb[c] : Zb[c]);
-    const T scb = (b == 2) ? T(-1) : T(1);
-    return T(0.5) * (sbc * Kbc(i, j) + scb * Kcb(j, i));
+        for (int k = 0; k < 9; ++k) Xo.m[k] = Xo.m[k] + W.m[k];
+        Yo = mulTN(Rc, Ay);
+        Zo = mulTN(Rc, Az);
+    };
+    // Ks(b, c)(i, j) = (K(b, c)(i, j) + K(c, b)(j, i)) / 2 over groups {0, 1, 3},
+    // each block folded as soon as both of its sources exist
+    M3<T> X0, Y0, Z0, X1, Y1, Z1, X3, Y3, Z3;
+    block_row(0, X0, Y0, Z0);
+    block_row(1, X1, Y1, Z1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            S.d0[sym_idx(i, j)] = T(0.5) * (X0(i, j) + X0(j, i));
+            S.d1[sym_idx(i, j)] = T(0.5) * (Y1(i, j) + Y1(j, i));
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S.o01[3 * i + j] = T(0.5) * (Y0(i, j) + X1(j, i));
+    block_row(3, X3, Y3, Z3);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            S.o03[3 * i + j] = T(0.5) * (Z0(i, j) + X3(j, i));
+            S.o13[3 * i + j] = T(0.5) * (Z1(i, j) + Y3(j, i));
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) S.d3[sym_idx(i, j)] = T(0.5) * (Z3(i, j) + Z3(j, i));
 }
 
 // Staged path: per-element records (any connectivity).
@@ -462,13 +538,13 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
     const int e_lane = blockIdx.x * BEAM_BLOCK + threadIdx.x;
     const bool live = e_lane < n_elems;
     const int e = live ? e_lane : n_elems - 1;
-    double fe[12];
-    M3<TT> Xb[4], Yb[4], Zb[4];
-    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, Ke_stage != nullptr, flags);
-    if (fe_stage && live) {
+    KSym<TT> S;
+    beam_core<TT>(e, X, u, conn, bc, [&](const double* fe) {
+        if (fe_stage && live) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) fe_stage[12 * (size_t)e + k] = fe[k];
-    }
+            for (int k = 0; k < 12; ++k) fe_stage[12 * (size_t)e + k] = fe[k];
+        }
+    }, S, Ke_stage != nullptr, flags);
     if (!Ke_stage) return;  // uniform across the launch
     // block row b of the 144-value record = one 36-value chunk, through LDS
     __shared__ TT tile[BEAM_BLOCK * 36];
@@ -481,7 +557,7 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) tile[threadIdx.x * 36 + c * 9 + 3 * i + j] = ks(Xb, Yb, Zb, b, c, i, j);
+                for (int j = 0; j < 3; ++j) tile[threadIdx.x * 36 + c * 9 + 3 * i + j] = ksv(S, b, c, i, j);
         __syncthreads();
         TT* dst = Ke_stage + (size_t)e0 * 144 + 36 * b;
         for (int k = threadIdx.x; k < cnt * 36; k += BEAM_BLOCK) {
@@ -500,21 +576,32 @@ beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict_
 // The tangent rows of a node n of such a chain hold its beam entries as one
 // contiguous run of the scalar CSR row: column groups 2(n-1) .. 2(n+1)+1, 18
 // entries (12 at a chain end).  A workgroup builds the runs of the rows of
-// its owned nodes in LDS (one row group -- translations, then rotations -- at
-// a time): each lane writes its element's node-2 rows, then adds its node-1
-// rows into the previous lane's image (element order, so the sums are fixed),
-// and the workgroup stores the runs with consecutive lanes on consecutive
-// addresses.  The first node of a chain is written by its element directly.
+// its owned nodes in LDS, IMG_ROWS scalar rows per lane at a time: each lane
+// writes its element's node-2 rows, then adds its node-1 rows into the previous
+// lane's image (element order, so the sums are fixed), and the workgroup stores
+// the runs with consecutive lanes on consecutive addresses.  The first node of
+// a chain is written by its element directly.
+//
+// Two waves per SIMD: the FP32 form needs about 170 registers, the FP64 form
+// is held to 256 (a few values spill to scratch; at its natural 340 it would
+// run one wave per SIMD and take 97 instead of 81 us on 16 x 91 wires).
 constexpr int DIRECT_OWN = BEAM_BLOCK - 1;
-constexpr int RUN = 18;   // entries of an interior row run
+constexpr int RUN = 18;       // entries of an interior row run
 
 template <class TT>
-__global__ void __launch_bounds__(BEAM_BLOCK)
+__global__ void __launch_bounds__(BEAM_BLOCK, 2)
 beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
                    const int* __restrict__ conn, BeamConst bc, const int8_t* __restrict__ eflag,
                    const int* __restrict__ eslot, const int* __restrict__ gptr, double mass_coef,
                    const double* __restrict__ M, double* __restrict__ fint, TT* __restrict__ K,
                    int* __restrict__ flags) {
+    // scalar rows per lane in one LDS image: a whole row group (13.8 KB) in
+    // FP32, one row (9.2 KB) in FP64, where three would take 29 KB and leave
+    // room for one wave per SIMD
+    constexpr int IMG_ROWS = sizeof(TT) == 4 ? 3 : 1;
+    // node 2's mass terms: loaded before the element math in FP32, after it in
+    // FP64, whose registers are full during the math
+    constexpr bool MASS_EARLY = sizeof(TT) == 4;
     const int l = threadIdx.x;
     const int e_lane = blockIdx.x * DIRECT_OWN + l;
     const bool live = e_lane < n_elems;
@@ -523,123 +610,59 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
     const bool prv = live && (fl & 1), nxt = (fl & 2) != 0;
     const bool own = live && l < DIRECT_OWN;
     const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
-    // Row offsets, block slots and mass terms of the rows this lane writes,
-    // loaded before the element math so that their latency hides behind it
-    // (loaded at the write-out they cost one dependent round trip per row).
-    // Index 0/1 of gq/bq/mq: node 2's rows, then the chain-start rows of node 1.
+    // Row offsets and block slots of the rows this lane writes, loaded before
+    // the element math so that their latency hides behind it (loaded at the
+    // write-out they cost one dependent round trip per row).  Index 0/1 of
+    // gq/bq: node 2's rows, then the chain-start rows of node 1 (whose mass
+    // terms are loaded where they are used: one lane in 150).
     int gq[2][3] = {}, bq[2][2] = {};
-    double mq[2][6] = {};
+    double mq[6] = {};
     const bool first = own && !prv;
     if (K) {   // uniform
 #pragma unroll
         for (int k = 0; k < 3; ++k) gq[0][k] = gptr[2 * n2 + k];
         bq[0][0] = eslot[16 * (size_t)e + 8];
         bq[0][1] = eslot[16 * (size_t)e + 12];
-        if (mass_coef != 0.0) {
+        if (MASS_EARLY && mass_coef != 0.0) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) mq[0][k] = M[6 * (size_t)n2 + k];
+            for (int k = 0; k < 6; ++k) mq[k] = M[6 * (size_t)n2 + k];
         }
         if (first) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) gq[1][k] = gptr[2 * n1 + k];
             bq[1][0] = eslot[16 * (size_t)e];
             bq[1][1] = eslot[16 * (size_t)e + 4];
-            if (mass_coef != 0.0) {
+        }
+    }
+    // LDS: the node-1 force parts first, then the row image in the same bytes
+    constexpr int IMG_BYTES = BEAM_BLOCK * IMG_ROWS * RUN * (int)sizeof(TT);
+    constexpr int F_BYTES = BEAM_BLOCK * 6 * (int)sizeof(double);
+    __shared__ __align__(16) unsigned char smem[IMG_BYTES > F_BYTES ? IMG_BYTES : F_BYTES];
+    __shared__ int2 dst[BEAM_BLOCK * IMG_ROWS];   // per image row: CSR offset (-1: not owned), run length
+    double* fsh = reinterpret_cast<double*>(smem);
+    KSym<TT> S;
+    beam_core<TT>(e, X, u, conn, bc, [&](const double* fe) {
+        // node-1 force part for the previous lane
 #pragma unroll
-                for (int k = 0; k < 6; ++k) mq[1][k] = M[6 * (size_t)n1 + k];
+        for (int d = 0; d < 6; ++d) fsh[l * 6 + d] = fe[d];
+        __syncthreads();
+        if (fint && own) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) fint[6 * (size_t)n2 + d] = nxt ? fe[6 + d] + fsh[(l + 1) * 6 + d] : fe[6 + d];
+            if (!prv) {
+#pragma unroll
+                for (int d = 0; d < 6; ++d) fint[6 * (size_t)n1 + d] = fe[d];
             }
         }
-    }
-    double fe[12];
-    M3<TT> Xb[4], Yb[4], Zb[4];
-    beam_core<TT>(e, X, u, conn, bc, fe, Xb, Yb, Zb, K != nullptr, flags);
-    // node-1 force part for the previous lane
-    __shared__ double fsh[BEAM_BLOCK * 6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) fsh[l * 6 + d] = fe[d];
-    __syncthreads();
-    if (fint && own) {
-#pragma unroll
-        for (int d = 0; d < 6; ++d) fint[6 * (size_t)n2 + d] = nxt ? fe[6 + d] + fsh[(l + 1) * 6 + d] : fe[6 + d];
-        if (!prv) {
-#pragma unroll
-            for (int d = 0; d < 6; ++d) fint[6 * (size_t)n1 + d] = fe[d];
-        }
-    }
+    }, S, K != nullptr, flags);
     if (!K) return;  // uniform across the launch
+    __syncthreads();   // every lane has read the force parts before the image overwrites them
 
-    __shared__ TT img[BEAM_BLOCK * 3 * RUN];   // lane l: 3 rows of its node-2 row group
-    __shared__ int2 dst[BEAM_BLOCK * 3];       // per image row: CSR offset (-1: not owned), run length
+    // lumped-mass terms of node 2's diagonal entries
+    TT mt[6];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        // own node-2 rows: columns [node 1 | node 2 | next element's node 2]
-        {
-            const int r0 = gq[0][p], deg = gq[0][p + 1] - r0;
-            const int b0 = bq[0][p];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                TT* row = img + (l * 3 + i) * RUN;
-                const TT m = mass_coef != 0.0 ? TT(mass_coef * mq[0][3 * p + i]) : TT(0);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        TT v = ks(Xb, Yb, Zb, 2 + p, c, i, j);
-                        if (c == 2 + p && j == i) v += m;
-                        row[3 * c + j] = v;
-                    }
-#pragma unroll
-                for (int k = 12; k < RUN; ++k) row[k] = TT(0);
-                dst[l * 3 + i] = int2{own ? 9 * r0 + i * 3 * deg + 3 * (b0 - r0) : -1, nxt ? RUN : 12};
-            }
-        }
-        __syncthreads();
-        // node-1 rows into the previous lane's node-2 rows (same node)
-        if (prv && l > 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                TT* row = img + ((l - 1) * 3 + i) * RUN + 6;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) row[3 * c + j] += ks(Xb, Yb, Zb, p, c, i, j);
-            }
-        }
-        __syncthreads();
-        // entry t = l + 64 it of the image: row rr = t / RUN, column k = t % RUN.
-        // The image holds BEAM_BLOCK * 3 rows, so every t < BEAM_BLOCK * 3 * RUN
-        // reads inside it; rows of the last lane (not owned) carry offset -1.
-        // Each group of U entries reads its offsets and values first and stores
-        // after, so the LDS reads of the group overlap instead of each store
-        // waiting on its own chain of reads.
-        {
-            constexpr int NIT = 3 * RUN;   // BEAM_BLOCK * 3 * RUN / BEAM_BLOCK
-            constexpr int U = 6;
-            static_assert(NIT % U == 0, "store loop unroll");
-            constexpr int STEP_R = BEAM_BLOCK / RUN, STEP_K = BEAM_BLOCK % RUN;
-            int rr = l / RUN, k = l % RUN, t = l;
-#pragma unroll 1
-            for (int it0 = 0; it0 < NIT; it0 += U) {
-                int2 dd[U];
-                TT vv[U];
-                int kk[U];
-#pragma unroll
-                for (int j = 0; j < U; ++j) {
-                    dd[j] = dst[rr];
-                    vv[j] = img[t];
-                    kk[j] = k;
-                    t += BEAM_BLOCK;
-                    rr += STEP_R;
-                    k += STEP_K;
-                    if (k >= RUN) { k -= RUN; ++rr; }
-                }
-#pragma unroll
-                for (int j = 0; j < U; ++j)
-                    if ((dd[j].x >= 0) & (kk[j] < dd[j].y)) K[(size_t)dd[j].x + kk[j]] = vv[j];
-            }
-        }
-        __syncthreads();
-    }
+    for (int k = 0; k < 6; ++k)
+        mt[k] = mass_coef != 0.0 ? TT(mass_coef * (MASS_EARLY ? mq[k] : M[6 * (size_t)n2 + k])) : TT(0);
     // first node of a chain: its rows hold this element's node-1 blocks only
     if (first) {
 #pragma unroll
@@ -649,18 +672,92 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const size_t d = 9 * (size_t)r0 + (size_t)i * 3 * deg + 3 * (size_t)(b0 - r0);
-                const TT m = mass_coef != 0.0 ? TT(mass_coef * mq[1][3 * p + i]) : TT(0);
+                const TT m = mass_coef != 0.0 ? TT(mass_coef * M[6 * (size_t)n1 + 3 * p + i]) : TT(0);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
-                        TT v = ks(Xb, Yb, Zb, p, c, i, j);
+                        TT v = ksv(S, p, c, i, j);
                         if (c == p && j == i) v += m;
                         K[d + 3 * c + j] = v;
                     }
             }
         }
     }
+    TT* img = reinterpret_cast<TT*>(smem);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int r0 = gq[0][p], deg = gq[0][p + 1] - r0;
+        const int b0 = bq[0][p];
+#pragma unroll
+        for (int i0 = 0; i0 < 3; i0 += IMG_ROWS) {
+            // own node-2 rows: columns [node 1 | node 2 | next element's node 2]
+#pragma unroll
+            for (int ii = 0; ii < IMG_ROWS; ++ii) {
+                const int i = i0 + ii;
+                TT* row = img + (l * IMG_ROWS + ii) * RUN;
+                const TT m = mt[3 * p + i];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        TT v = ksv(S, 2 + p, c, i, j);
+                        if (c == 2 + p && j == i) v += m;
+                        row[3 * c + j] = v;
+                    }
+#pragma unroll
+                for (int k = 12; k < RUN; ++k) row[k] = TT(0);
+                dst[l * IMG_ROWS + ii] = int2{own ? 9 * r0 + i * 3 * deg + 3 * (b0 - r0) : -1, nxt ? RUN : 12};
+            }
+            __syncthreads();
+            // node-1 rows into the previous lane's node-2 rows (same node)
+            if (prv && l > 0) {
+#pragma unroll
+                for (int ii = 0; ii < IMG_ROWS; ++ii) {
+                    TT* row = img + ((l - 1) * IMG_ROWS + ii) * RUN + 6;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) row[3 * c + j] += ksv(S, p, c, i0 + ii, j);
+                }
+            }
+            __syncthreads();
+            // entry t = l + 64 it of the image: row rr = t / RUN, column k = t % RUN.
+            // The image holds BEAM_BLOCK * IMG_ROWS rows, so every t < BEAM_BLOCK *
+            // IMG_ROWS * RUN reads inside it; rows of the last lane (not owned)
+            // carry offset -1.  Each group of U entries reads its offsets and
+            // values first and stores after, so the LDS reads of the group
+            // overlap instead of each store waiting on its own chain of reads.
+            {
+                constexpr int NIT = IMG_ROWS * RUN;   // BEAM_BLOCK * IMG_ROWS * RUN / BEAM_BLOCK
+                constexpr int U = 6;
+                static_assert(NIT % U == 0, "store loop unroll");
+                constexpr int STEP_R = BEAM_BLOCK / RUN, STEP_K = BEAM_BLOCK % RUN;
+                int rr = l / RUN, k = l % RUN, t = l;
+#pragma unroll 1
+                for (int it0 = 0; it0 < NIT; it0 += U) {
+                    int2 dd[U];
+                    TT vv[U];
+                    int kk[U];
+#pragma unroll
+                    for (int j = 0; j < U; ++j) {
+                        dd[j] = dst[rr];
+                        vv[j] = img[t];
+                        kk[j] = k;
+                        t += BEAM_BLOCK;
+                        rr += STEP_R;
+                        k += STEP_K;
+                        if (k >= RUN) { k -= RUN; ++rr; }
+                    }
+#pragma unroll
+                    for (int j = 0; j < U; ++j)
+                        if ((dd[j].x >= 0) & (kk[j] < dd[j].y)) K[(size_t)dd[j].x + kk[j]] = vv[j];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
 }
 
 }  // namespace xk
