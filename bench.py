@@ -416,17 +416,20 @@ def newton_attempt(case, torch, wl, reps=5):
 def newton_loop(torch, wl, force=0.1):
     """A whole dynamic solve of the 91-wire strand geometry (radial squeeze of
     the outer wires, workloads.squeeze_case, Coulomb mu 0.3, FP64): Newton
-    attempts with host SuperLU, on the device, and on the device with the
-    attempts after each increment's first replayed from HIP graphs.  The
-    three trajectories (attempts and active counts per call) must agree."""
+    attempts with host SuperLU, on the device (convergence and divergence
+    tests on the device, attempts enqueued back to back), the same with the
+    attempts after each increment's first replayed from HIP graphs, and on the
+    device with the tests on the host (a host read after each attempt's
+    stages).  The trajectories (attempts and active counts per call) must
+    agree."""
     TP = importlib.import_module("xkep-cae_amd.train_prescreen")
     sq = wl.squeeze_case(force, n_wires=91, n_elems=150, length=80.0, radius=0.4, lay_angle_deg=12.0, mu=0.3)
     out, traj = {}, {}
-    for mode in ("device_graph", "device", "host_superlu"):
+    for mode in ("device_graph", "device", "device_hostctl", "host_superlu"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res, b = TP.run_case(sq, precision="fp64", device_solve=mode != "host_superlu",
-                             use_graph=mode == "device_graph")
+                             use_graph=mode == "device_graph", device_control=mode != "device_hostctl")
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         att = sum(c.n_attempts for c in res.calls)
@@ -435,7 +438,8 @@ def newton_loop(torch, wl, force=0.1):
         if mode == "device_graph":
             out[mode]["graphs_captured"] = b.graph_attempts_captured
         log(f"[bench] Newton loop 91 wires {mode}: {out[mode]}")
-    out["trajectories_equal"] = traj["device_graph"] == traj["device"] == traj["host_superlu"]
+    out["trajectories_equal"] = (traj["device_graph"] == traj["device"] == traj["device_hostctl"]
+                                 == traj["host_superlu"])
     out["calls"] = traj["device"]
     out["workload"] = (f"91-wire strand (150 el/wire, r 0.4, L 80), dynamic radial squeeze {force} N per outer wire, "
                        "mu 0.3, 4 increments x up to 30 attempts (the reference's line search halves to 1/32)")

@@ -278,6 +278,72 @@ int xk_newton_update(xk_ctx* ctx, double* u, double* du, const double* R, const 
                      int line_search_steps, double diverge_factor, double du_norm_cap, double* status,
                      void* stream);
 
+/* ---- Device Newton control: the tests of NewtonDynamicProcess.process
+ * (contact/solver/_newton_dynamic.py:160-389) and ConvergenceCheckProcess
+ * (contact/solver/_newton_steps.py:182-243) evaluated on the device, so that
+ * the attempts of a call can be enqueued back to back with no host read in
+ * between.  The control record `ctl` (device doubles, XK_NC_HIST +
+ * XK_NC_HSTRIDE * hist_cap) holds the call's state and history.  While a
+ * record is attached to the context (xk_newton_control_attach), every kernel
+ * of an attempt -- fused assembly, pair force, linear solve, line search and
+ * update -- first reads the state word and does nothing once the call has
+ * ended (or while an attempt waits on the host), so attempts enqueued past
+ * the end are no-ops.  rocBLAS's trailing updates of the capacitance LU are
+ * the one exception: they run on stale scratch in a no-op attempt.  The host
+ * detaches the record for an eager re-solve (XK_NC_ASSIST). */
+enum {
+    XK_NC_STATE = 0,     /* XK_NC_RUNNING .. XK_NC_ASSIST */
+    XK_NC_ATT = 1,       /* attempts whose force test ran */
+    XK_NC_REASON = 2,    /* XK_NCR_* bits of a stopped call */
+    XK_NC_NACT = 3,      /* n_active of the last attempt */
+    XK_NC_RATIO = 4,     /* ||R|| / f_ref of the last attempt */
+    XK_NC_DU = 5, XK_NC_U = 6, XK_NC_ENERGY = 7,   /* ||du||, ||u||, |du . R| of the last update */
+    XK_NC_REL = 8,       /* relative residual of the last linear solve (xk_linear_solve's res_rel) */
+    XK_NC_ALPHA = 9,     /* step length of the last update */
+    XK_NC_CONSEC = 10, XK_NC_PREV = 11, XK_NC_FREF = 12, XK_NC_EREF = 13, XK_NC_SOLVE_OK = 14,
+    XK_NC_RGATE = 15,    /* device-decided refinement: 0 while steps remain */
+    XK_NC_TOL_F = 16, XK_NC_TOL_D = 17, XK_NC_FEXT = 18, XK_NC_DYNREF = 19, XK_NC_WIN = 20,
+    XK_NC_MAXATT = 21, XK_NC_STOL = 22, XK_NC_HCAP = 23,
+    XK_NC_RPREV = 24,    /* last refinement residual that did not stop it */
+    XK_NC_RSTEPS = 25,   /* refinement steps of the last solve */
+    XK_NC_HIST = 32,     /* attempt k: [ratio, n_active, solve residual, alpha] at XK_NC_HIST + 4 k */
+    XK_NC_HSTRIDE = 4
+};
+enum { XK_NC_RUNNING = 0, XK_NC_CONVERGED = 1, XK_NC_DIVERGED = 2, XK_NC_STOPPED = 3,
+       XK_NC_ASSIST = 4 };  /* the attempt's solve missed solve_tol: the host re-solves it eagerly */
+enum { XK_NCR_NONFINITE = 1, XK_NCR_DEGENERATE = 2, XK_NCR_SINGULAR = 4, XK_NCR_SOLVE = 8, XK_NCR_LIMIT = 16 };
+/* Reset `ctl` for a new call (stream-ordered) and clear the device status
+ * word.  f_ref of the force test: with dynamic_ref the call's first residual
+ * norm (if > 1e-30), else f_ext_ref_norm (_newton_steps.py:194-213);
+ * divergence: divergence_window consecutive ratio increases by more than 1 %
+ * unless the ratio is below 0.01, or a ratio above 100 from the 11th attempt
+ * on (_newton_dynamic.py:258-285); displacement ||du|| / ||u|| < tol_disp and
+ * energy |du . R| / (that of the first update) < 1e-10 after the update
+ * (_newton_steps.py:215-243); at most max_attempts attempts; an attempt whose
+ * solve leaves a relative residual above solve_tol goes to XK_NC_ASSIST. */
+int xk_newton_control_begin(xk_ctx* ctx, double* ctl, int hist_cap, double tol_force, double tol_disp,
+                            double f_ext_ref_norm, int dynamic_ref, int divergence_window, int max_attempts,
+                            double solve_tol, void* stream);
+/* Attach `ctl` to the context (null detaches).  While attached,
+ * xk_newton_update also applies the solve check before the line search
+ * (XK_FLAG_SINGULAR: stopped; residual above solve_tol: XK_NC_ASSIST) and the
+ * displacement, energy and attempt-limit tests after the update, and
+ * xk_linear_solve with tol > 0 decides on the device when its refinement
+ * stops (same rule as its host loop; the steps after the stop are no-ops, no
+ * host read, capturable; the LU without interchanges is not re-checked:
+ * a solve it leaves above tol goes to XK_NC_ASSIST). */
+int xk_newton_control_attach(xk_ctx* ctx, double* ctl);
+/* Force and divergence tests of the attempt after its assembly, on the
+ * attached record: res_norm2 and n_active are xk_assemble_iteration's; the
+ * device status word stops the call on XK_FLAG_DEGENERATE (checked first, as
+ * the reference raises there) or XK_FLAG_NONFINITE (after the tests, as a
+ * failed linear solve, _newton_steps.py:360-366). */
+int xk_newton_control_force(xk_ctx* ctx, const double* res_norm2, const int32_t* n_active, void* stream);
+/* Host decision on the attached record after an eager re-solve: sets the
+ * state, ORs `reason` into XK_NC_REASON, and with XK_NC_RUNNING accepts the
+ * attempt's solve (the next update skips the solve check once). */
+int xk_newton_control_set(xk_ctx* ctx, int state, int reason, void* stream);
+
 /* Copy of one of the linear solve's internal device buffers (index `which`,
  * layouts in csrc/solve.hip) for tests; *size_bytes receives its size.
  * Synchronous.  No counterpart in the reference. */

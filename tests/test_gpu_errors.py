@@ -55,11 +55,13 @@ def test_broadphase_refuses_non_finite_coordinates(gpu, bad):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("device_solve", [True, False])
-def test_nan_displacement_flags_and_fails_the_call(gpu, device_solve):
+@pytest.mark.parametrize("mode", ["device_control", "device_hostctl", "host_superlu"])
+def test_nan_displacement_flags_and_fails_the_call(gpu, mode):
     S = pkg("solver")
     case, st, ctx = _strand()
+    device_solve = mode != "host_superlu"
     b = _backend(case, st, ctx, device_solve=device_solve)
+    b.device_control = mode == "device_control"
     u0 = np.zeros(ctx.ndof)
     b.detect(u0)
     b.update_geometry(u0)
@@ -67,11 +69,13 @@ def test_nan_displacement_flags_and_fails_the_call(gpu, device_solve):
     free_rot = 6 * 7 + 4            # a rotation of a free node: only the element kernel sees it
     assert free_rot not in set(st.fixed_dofs.tolist())
     u[free_rot] = np.nan
-    print(f"NaN displacement, device_solve={device_solve}", flush=True)
+    print(f"NaN displacement, {mode}", flush=True)
     cfg = S.SolverConfig(max_nr_attempts=10, tol_force=1e-8)
     out = S.newton_dynamic(b, u, u0.copy(), st.f_ext, st.fixed_dofs, 1e-4, 1.0, False, cfg)
     assert not out.converged and out.n_attempts == 1
     assert b.flags & S.FLAG_NONFINITE
+    if mode == "device_control":
+        assert b.control_reasons == [1]   # XK_NCR_NONFINITE
 
 
 @pytest.mark.gpu
@@ -126,4 +130,5 @@ def test_singular_tangent_flags_and_fails_the_call(gpu):
                            S.SolverConfig(max_nr_attempts=5))
     assert not out.converged and out.n_attempts == 1
     assert b.flags & S.FLAG_SINGULAR
+    assert b.control_reasons == [4]   # XK_NCR_SINGULAR: stopped on the device after the solve
     assert np.all(u == 0.0)    # no update after the failed solve

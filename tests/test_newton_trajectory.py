@@ -130,13 +130,16 @@ def test_newton_trajectory_matches_reference_on_oracle_assembly(fixture):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("fixture", FIXTURES)
-@pytest.mark.parametrize("mode", ["host_superlu", "device_solve", "device_graph"])
+@pytest.mark.parametrize("mode", ["host_superlu", "device_hostctl", "device_graph_hostctl", "device_solve",
+                                  "device_graph"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fixture):
-    """The device assembly under the driver: with SuperLU on the host, with the
+    """The device assembly under the driver: with SuperLU on the host; with the
     Newton attempts on the device (device linear solve, line search and
-    updates in HBM), and with the attempts after each increment's first as
-    replays of two captured HIP graphs."""
+    updates in HBM) and the convergence and divergence tests on the host
+    (``*_hostctl``: a host read after each attempt's stages) or on the device
+    (attempts enqueued back to back); eagerly or, after each increment's first
+    attempt, as replays of captured HIP graphs."""
     device_solve = mode != "host_superlu"
     g, prm, cfg = case(fixture)
     S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
@@ -148,16 +151,19 @@ def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fix
     b = S.DeviceBackend(ctx, M_diag=g["M_diag"], rho_inf=prm["rho_inf"], k_pen=prm["k_pen"],
                         smoothing_delta=prm["smoothing_delta"], mu=prm["mu"], k_t_ratio=prm["k_t_ratio"],
                         fixed_dofs=g["fixed"], config=kcfg, device_solve=device_solve,
-                        use_graph=mode == "device_graph")
+                        use_graph=mode.startswith("device_graph"), device_control=not mode.endswith("hostctl"))
     res = S.ContactFrictionSolver(b, fixed_dofs=g["fixed"], f_ext_total=g["f_ext"], t_total=prm["t_total"],
                                   config=cfg).solve()
     check_trajectory(g, res, precision)
     if device_solve:
         rel = [float(r.item()) for r in b.solve_residuals]
         assert len(rel) > 0 and max(rel) < 1e-10, max(rel)
-    if mode == "device_graph":
+    if mode.startswith("device_graph"):
         assert b.graph_error is None, b.graph_error
         assert b.graph_attempts_captured > 0
+    if device_solve and not mode.endswith("hostctl"):
+        # every call ended on the device by a test of the driver, never by a solve failure
+        assert all((r & ~16) == 0 for r in b.control_reasons), b.control_reasons
     tol = 1e-8 if precision == "fp64" else 1e-4
     assert np.linalg.norm(res.u - g["u_final"]) <= tol * np.linalg.norm(g["u_final"])
 

@@ -26,6 +26,11 @@ PREC = {"fp64": 0, "mixed": 1}
 OPT_STAGED = 256
 OPT_HOST_PATTERN = 512
 FLAG_COAT_UNPLACED = 8
+# device Newton control record (XK_NC_* of include/xkamd.h)
+NC_STATE, NC_ATT, NC_REASON, NC_NACT, NC_RATIO, NC_DU, NC_U, NC_ENERGY, NC_REL, NC_ALPHA = range(10)
+NC_HIST, NC_HSTRIDE = 32, 4
+NC_RUNNING, NC_CONVERGED, NC_DIVERGED, NC_STOPPED, NC_ASSIST = range(5)
+NCR_NONFINITE, NCR_DEGENERATE, NCR_SINGULAR, NCR_SOLVE, NCR_LIMIT = 1, 2, 4, 8, 16
 
 
 class XkTuning(C.Structure):
@@ -68,6 +73,11 @@ SIGNATURES = {
     "xk_solve_buffer": (C.c_int, [vp, C.c_int, vp, C.c_int64, c_int64_p]),
     "xk_solve_set_lu": (C.c_int, [vp, C.c_int, c_int32_p, c_int64_p]),
     "xk_newton_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, vp]),
+    "xk_newton_control_begin": (C.c_int, [vp, vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, C.c_double, vp]),
+    "xk_newton_control_attach": (C.c_int, [vp, vp]),
+    "xk_newton_control_force": (C.c_int, [vp, vp, vp, vp]),
+    "xk_newton_control_set": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     "xk_status": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "xk_profile_begin": (C.c_int, [vp, C.c_int]),
     "xk_profile_end": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),

@@ -469,6 +469,35 @@ class DeviceContext:
             L.ptr(res_norm2), int(line_search_steps), float(diverge_factor), float(du_norm_cap), L.ptr(status),
             self.stream), "xk_newton_update")
 
+    # ---- device Newton control (the driver's tests on the device)
+    def newton_control_begin(self, ctl, *, hist_cap: int, tol_force: float, tol_disp: float,
+                             f_ext_ref_norm: float, dynamic_ref: bool, divergence_window: int, max_attempts: int,
+                             solve_tol: float):
+        """Reset the control record ``ctl`` (device float64, NC_HIST +
+        NC_HSTRIDE * hist_cap) for a new Newton call and clear the device
+        status word; stream-ordered."""
+        if ctl.dtype != self.torch.float64 or not ctl.is_cuda or ctl.numel() < L.NC_HIST + L.NC_HSTRIDE * hist_cap:
+            raise ValueError("ctl must be a device float64 tensor of NC_HIST + NC_HSTRIDE * hist_cap entries")
+        L.check(L.lib().xk_newton_control_begin(
+            self.h, L.ptr(ctl), int(hist_cap), float(tol_force), float(tol_disp), float(f_ext_ref_norm),
+            int(bool(dynamic_ref)), int(divergence_window), int(max_attempts), float(solve_tol), self.stream),
+            "xk_newton_control_begin")
+
+    def newton_control_attach(self, ctl):
+        """Attach ``ctl`` (None detaches): the attempt kernels skip once its call has ended."""
+        L.check(L.lib().xk_newton_control_attach(self.h, L.ptr(ctl) if ctl is not None else None),
+                "xk_newton_control_attach")
+
+    def newton_control_force(self, res_norm2, n_active):
+        """Force and divergence tests of the attempt, on the attached record."""
+        L.check(L.lib().xk_newton_control_force(self.h, L.ptr(res_norm2), L.ptr(n_active), self.stream),
+                "xk_newton_control_force")
+
+    def newton_control_set(self, state: int, reason: int = 0):
+        """Host decision after an eager re-solve (XK_NC_RUNNING accepts the attempt's solve)."""
+        L.check(L.lib().xk_newton_control_set(self.h, int(state), int(reason), self.stream),
+                "xk_newton_control_set")
+
     SOLVE_BUFFERS = ("Kb", "Ab", "Lf", "Sinv", "slist", "slot_nodes", "slot_G", "slot_c", "pos", "cslot", "cn",
                      "cnode", "ch_m", "ch_cn0", "ch_yoff", "ch_coff", "Y", "C", "S", "ipiv", "info", "W", "b", "x",
                      "r", "t", "red", "perm", "W2", "piv32")

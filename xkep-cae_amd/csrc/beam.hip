@@ -532,7 +532,8 @@ template <class TT>
 __global__ void __launch_bounds__(BEAM_BLOCK)
 beam_kernel(int n_elems, const double* __restrict__ X, const double* __restrict__ u,
             const int* __restrict__ conn, BeamConst bc, double* __restrict__ fe_stage,
-            TT* __restrict__ Ke_stage, int* __restrict__ flags) {
+            TT* __restrict__ Ke_stage, int* __restrict__ flags, const double* __restrict__ stop) {
+    if (!nc_go(stop)) return;   // the Newton call this attempt belongs to has ended
     // tail lanes recompute the last element (nothing of theirs is stored) so
     // that every lane reaches the LDS barriers of the tangent write-out
     const int e_lane = blockIdx.x * BEAM_BLOCK + threadIdx.x;
@@ -594,7 +595,8 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
                    const int* __restrict__ conn, BeamConst bc, const int8_t* __restrict__ eflag,
                    const int* __restrict__ eslot, const int* __restrict__ gptr, double mass_coef,
                    const double* __restrict__ M, double* __restrict__ fint, TT* __restrict__ K,
-                   int* __restrict__ flags) {
+                   int* __restrict__ flags, const double* __restrict__ stop) {
+    if (!nc_go(stop)) return;   // the Newton call this attempt belongs to has ended
     // scalar rows per lane in one LDS image: a whole row group (13.8 KB) in
     // FP32, one row (9.2 KB) in FP64, where three would take 29 KB and leave
     // room for one wave per SIMD
@@ -766,30 +768,30 @@ beam_direct_kernel(int n_elems, const double* __restrict__ X, const double* __re
 namespace xk {
 hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, const int* conn,
                        const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags,
-                       hipStream_t st) {
+                       hipStream_t st, const double* stop) {
     if (n_elems <= 0) return hipSuccess;
     dim3 blk(BEAM_BLOCK), grd((n_elems + BEAM_BLOCK - 1) / BEAM_BLOCK);
     if (prec == 0)
         hipLaunchKernelGGL(beam_kernel<double>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
-                           (double*)Ke_stage, flags);
+                           (double*)Ke_stage, flags, stop);
     else
         hipLaunchKernelGGL(beam_kernel<float>, grd, blk, 0, st, n_elems, X, u, conn, bc, fe_stage,
-                           (float*)Ke_stage, flags);
+                           (float*)Ke_stage, flags, stop);
     return hipGetLastError();
 }
 
 hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const double* u, const int* conn,
                               const BeamConst& bc, const int8_t* eflag, const int* eslot, const int* gptr,
                               double mass_coef, const double* M, double* f_int, void* K, int* flags,
-                              hipStream_t st) {
+                              hipStream_t st, const double* stop) {
     if (n_elems <= 0) return hipSuccess;
     dim3 blk(BEAM_BLOCK), grd((n_elems + DIRECT_OWN - 1) / DIRECT_OWN);
     if (prec == 0)
         hipLaunchKernelGGL(beam_direct_kernel<double>, grd, blk, 0, st, n_elems, X, u, conn, bc, eflag, eslot,
-                           gptr, mass_coef, M, f_int, (double*)K, flags);
+                           gptr, mass_coef, M, f_int, (double*)K, flags, stop);
     else
         hipLaunchKernelGGL(beam_direct_kernel<float>, grd, blk, 0, st, n_elems, X, u, conn, bc, eflag, eslot,
-                           gptr, mass_coef, M, f_int, (float*)K, flags);
+                           gptr, mass_coef, M, f_int, (float*)K, flags, stop);
     return hipGetLastError();
 }
 }  // namespace xk

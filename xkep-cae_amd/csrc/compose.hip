@@ -46,6 +46,7 @@ __device__ __forceinline__ T wave_sum(T v) {
 template <class KT, class ST, class GT, int GB>
 __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __restrict__ Ke,
                                                      const GT* __restrict__ sg, KT* __restrict__ K) {
+    if (!nc_go(g.stop)) return;   // the Newton call this attempt belongs to has ended
     __shared__ double red[GB / 64];
     double ss = 0.0;
     const bool node_block = (int)blockIdx.x < g.node_blocks;
@@ -218,7 +219,9 @@ constexpr int FB = 1024;  // finish kernel lanes
 
 __global__ void __launch_bounds__(FB) finish_kernel(int n_part, const double* __restrict__ partial,
                                                     int n_act, const int* __restrict__ act_part,
-                                                    double* __restrict__ res_norm2, int* __restrict__ n_active) {
+                                                    double* __restrict__ res_norm2, int* __restrict__ n_active,
+                                                    const double* __restrict__ stop) {
+    if (!nc_go(stop)) return;
     __shared__ double red[FB / 64];
     __shared__ int ired[FB / 64];
     // each lane sums its strided entries in index order; one chunk issues all its
@@ -312,7 +315,7 @@ hipError_t launch_gather(int prec, GatherArgs g, const void* Ke, const void* sg,
     if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
     if (g.partial && (g.res_norm2 || g.n_active))
         hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(FB), 0, st, g.node_blocks, (const double*)g.partial,
-                           g.n_act_part, g.act_part, g.res_norm2, g.n_active);
+                           g.n_act_part, g.act_part, g.res_norm2, g.n_active, g.stop);
     return hipGetLastError();
 }
 

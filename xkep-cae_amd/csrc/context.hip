@@ -371,6 +371,7 @@ GatherArgs Ctx::gather_args(int node_mode) const {
     g.fe = (const double*)d_fe.p;
     g.nf = (const double*)d_nf.p;
     g.flags = (int*)d_flags.p;
+    g.stop = nstop;
     if (direct) {
         g.fint = (const double*)d_fint.p;
         g.direct = 1;
@@ -385,9 +386,10 @@ hipError_t Ctx::launch_elements(const double* u, bool want_f, void* K, double ma
     if (direct)
         return launch_beam_direct(kprec >= 0 ? kprec : prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
                                   (const int8_t*)d_eflag.p, (const int*)d_eslot.p, (const int*)d_gptr.p,
-                                  mass_coef, M, want_f ? (double*)d_fint.p : nullptr, K, (int*)d_flags.p, st);
+                                  mass_coef, M, want_f ? (double*)d_fint.p : nullptr, K, (int*)d_flags.p, st,
+                                  nstop);
     return launch_beam(prec, n_elems, (const double*)d_X.p, u, (const int*)d_conn.p, bc,
-                       want_f ? (double*)d_fe.p : nullptr, K ? d_Ke.p : nullptr, (int*)d_flags.p, st);
+                       want_f ? (double*)d_fe.p : nullptr, K ? d_Ke.p : nullptr, (int*)d_flags.p, st, nstop);
 }
 
 }  // namespace xk
@@ -815,6 +817,42 @@ int xk_newton_update(xk_ctx* c, double* u, double* du, const double* R, const do
                             du_norm_cap, status, (hipStream_t)stream);
 }
 
+int xk_newton_control_begin(xk_ctx* c, double* ctl, int hist_cap, double tol_force, double tol_disp,
+                            double f_ext_ref_norm, int dynamic_ref, int divergence_window, int max_attempts,
+                            double solve_tol, void* stream) {
+    if (!c || !ctl) {
+        set_error("xk_newton_control_begin: null argument");
+        return -1;
+    }
+    return c->newton_control_begin(ctl, hist_cap, tol_force, tol_disp, f_ext_ref_norm, dynamic_ref,
+                                   divergence_window, max_attempts, solve_tol, (hipStream_t)stream);
+}
+
+int xk_newton_control_attach(xk_ctx* c, double* ctl) {
+    if (!c) {
+        set_error("xk_newton_control_attach: null context");
+        return -1;
+    }
+    c->nstop = ctl;
+    return 0;
+}
+
+int xk_newton_control_force(xk_ctx* c, const double* res_norm2, const int32_t* n_active, void* stream) {
+    if (!c || !res_norm2 || !n_active) {
+        set_error("xk_newton_control_force: null argument");
+        return -1;
+    }
+    return c->newton_control_force(res_norm2, n_active, (hipStream_t)stream);
+}
+
+int xk_newton_control_set(xk_ctx* c, int state, int reason, void* stream) {
+    if (!c) {
+        set_error("xk_newton_control_set: null context");
+        return -1;
+    }
+    return c->newton_control_set(state, reason, (hipStream_t)stream);
+}
+
 int xk_solve_buffer(xk_ctx* c, int which, void* host, int64_t nbytes, int64_t* size) {
     return c ? c->solve_buffer(which, host, nbytes, size) : -1;
 }
@@ -901,6 +939,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     pp.g_off = 0.0;
     pp.allow_deact = 0;
     pp.n_sinc = c->n_sinc;
+    pp.stop = c->nstop;
     c->coat_params(pp);
     if (c->coat_on && c->lp_on) {
         set_error("xk_assemble_iteration: the learned prescreen is not available with the coating on");
@@ -952,7 +991,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
                             (double*)c->d_nf.p, c->d_sg.p, act, st, nullptr, c->pair_block));
         if (lp)
             XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
-                                       st));
+                                       st, c->nstop));
         XK_TRY(hipEventRecord(ev[1], st));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipEventRecord(ev[2], st));
@@ -966,7 +1005,7 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
         // the geometry this assembly gives the pairs it skips, for the verification pass
         if (lp)
             XK_TRY(launch_lp_hist_push(c->n_nodes, u, (double*)c->lp_hist.p, Ctx::LP_HIST_CAP, (int*)c->lp_hist_cnt.p,
-                                       c->side));
+                                       c->side, c->nstop));
         XK_TRY(hipEventRecord(c->ev_join, c->side));
         XK_TRY(c->launch_elements(u, true, K_vals, emass, g.M, st));
         XK_TRY(hipStreamWaitEvent(st, c->ev_join, 0));

@@ -32,8 +32,9 @@ constexpr int NT_MAX_LS = 16;
 // uniform layout of the status buffer (doubles)
 enum { NS_ALPHA = 0, NS_DU = 1, NS_U = 2, NS_ENERGY = 3, NS_LS = 4, NS_NORM0 = 8 };
 
-__global__ void nt_axpy_kernel(int n, const double* __restrict__ u, const double* __restrict__ du, int k,
-                               double* __restrict__ out) {
+__global__ void nt_axpy_kernel(const double* __restrict__ stop, int n, const double* __restrict__ u,
+                               const double* __restrict__ du, int k, double* __restrict__ out) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = u[i] + ldexp(1.0, -k) * du[i];
 }
@@ -51,11 +52,13 @@ __device__ __forceinline__ double nt_block_sum(double v, double* red) {
 }
 
 // ||f_int - f_contact - f_ext|| over free dofs -> status[NS_NORM0 + k] (inf if non-finite)
-__global__ __launch_bounds__(NR) void nt_trial_norm_kernel(int n, const double* __restrict__ f_int,
+__global__ __launch_bounds__(NR) void nt_trial_norm_kernel(const double* __restrict__ stop, int n,
+                                                           const double* __restrict__ f_int,
                                                            const double* __restrict__ f_contact,
                                                            const double* __restrict__ f_ext,
                                                            const uint8_t* __restrict__ fixed, int k,
                                                            double* __restrict__ status) {
+    if (!nc_go(stop)) return;
     __shared__ double red[NR];
     double s = 0.0;
     int bad = 0;
@@ -71,11 +74,13 @@ __global__ __launch_bounds__(NR) void nt_trial_norm_kernel(int n, const double* 
 }
 
 // alpha from the candidates' norms, then the cap on ||alpha du||
-__global__ __launch_bounds__(NR) void nt_select_kernel(int n, int max_steps, double diverge_factor, double cap,
+__global__ __launch_bounds__(NR) void nt_select_kernel(const double* __restrict__ stop, int n, int max_steps,
+                                                       double diverge_factor, double cap,
                                                        const double* __restrict__ res_norm2,
                                                        const double* __restrict__ u,
                                                        const double* __restrict__ du,
                                                        double* __restrict__ status) {
+    if (!nc_go(stop)) return;
     __shared__ double red[NR];
     double alpha = 1.0;
     int used = max_steps;
@@ -107,8 +112,9 @@ __global__ __launch_bounds__(NR) void nt_select_kernel(int n, int max_steps, dou
 }
 
 // du *= alpha; u += du
-__global__ void nt_update_kernel(int n, double* __restrict__ u, double* __restrict__ du,
-                                 const double* __restrict__ status) {
+__global__ void nt_update_kernel(const double* __restrict__ stop, int n, double* __restrict__ u,
+                                 double* __restrict__ du, const double* __restrict__ status) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double d = status[NS_ALPHA] * du[i];
@@ -116,9 +122,10 @@ __global__ void nt_update_kernel(int n, double* __restrict__ u, double* __restri
     u[i] += d;
 }
 
-__global__ __launch_bounds__(NR) void nt_norms_kernel(int n, const double* __restrict__ u,
-                                                      const double* __restrict__ du, const double* __restrict__ R,
-                                                      double* __restrict__ status) {
+__global__ __launch_bounds__(NR) void nt_norms_kernel(const double* __restrict__ stop, int n,
+                                                      const double* __restrict__ u, const double* __restrict__ du,
+                                                      const double* __restrict__ R, double* __restrict__ status) {
+    if (!nc_go(stop)) return;
     __shared__ double red[NR];
     double sd = 0.0, su = 0.0, se = 0.0;
     for (int i = threadIdx.x; i < n; i += NR) {
@@ -136,6 +143,169 @@ __global__ __launch_bounds__(NR) void nt_norms_kernel(int n, const double* __res
 
 static unsigned nt_grid(int n) { return (unsigned)std::max(1, (n + 255) / 256); }
 
+// ------------------------------------------------------------ device control
+//
+// The control record of a call (XK_NC_* layout of include/xkamd.h) is read
+// and written by single-lane kernels in stream order, so every test sees the
+// values of the attempt before it; the arithmetic (sqrt, division, the 1.01
+// and 0.01 / 100 thresholds, NaN comparisons) is the host driver's, so the
+// device takes the same branch for the same numbers.
+
+static_assert(XK_NC_RUNNING == 0, "nc_go (xk_common.h) tests this value");
+
+__global__ void nc_begin_kernel(double* __restrict__ c, int hcap, double tol_f, double tol_d, double fext,
+                                int dynref, int win, int maxatt, double stol, int* __restrict__ flags) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < XK_NC_HIST; ++k) c[k] = 0.0;
+    c[XK_NC_STATE] = XK_NC_RUNNING;
+    c[XK_NC_PREV] = HUGE_VAL;      // prev_ratio = inf
+    c[XK_NC_EREF] = -1.0;          // energy_ref not set
+    c[XK_NC_TOL_F] = tol_f;
+    c[XK_NC_TOL_D] = tol_d;
+    c[XK_NC_FEXT] = fext;
+    c[XK_NC_DYNREF] = dynref;
+    c[XK_NC_WIN] = win;
+    c[XK_NC_MAXATT] = maxatt;
+    c[XK_NC_STOL] = stol;
+    c[XK_NC_HCAP] = hcap;
+    *flags = 0;
+}
+
+__device__ __forceinline__ void nc_stop(double* c, int state, int reason) {
+    c[XK_NC_STATE] = state;
+    c[XK_NC_REASON] = (double)((int)c[XK_NC_REASON] | reason);
+}
+
+// ConvergenceCheckProcess, force part (_newton_steps.py:194-213), and the
+// divergence tests (_newton_dynamic.py:258-285), after the attempt's assembly
+__global__ void nc_force_kernel(double* __restrict__ c, const double* __restrict__ res_norm2,
+                                const int* __restrict__ n_active, const int* __restrict__ flags) {
+    if (threadIdx.x != 0 || c[XK_NC_STATE] != XK_NC_RUNNING) return;
+    const int att = (int)c[XK_NC_ATT];
+    const int fl = *flags;
+    const double res = sqrt(*res_norm2);
+    const bool dyn = c[XK_NC_DYNREF] != 0.0;
+    const double incr = c[XK_NC_FREF];
+    double f_ref = incr > 1e-30 ? incr : c[XK_NC_FEXT];
+    if (dyn && att == 0 && res > 1e-30) f_ref = res;
+    if (att == 0 && dyn) c[XK_NC_FREF] = f_ref;
+    const double ratio = res / f_ref;
+    const int na = *n_active;
+    c[XK_NC_RATIO] = ratio;
+    c[XK_NC_NACT] = na;
+    if (att < (int)c[XK_NC_HCAP]) {
+        double* h = c + XK_NC_HIST + XK_NC_HSTRIDE * att;
+        h[0] = ratio;
+        h[1] = na;
+        h[2] = 0.0;
+        h[3] = 0.0;
+    }
+    c[XK_NC_ATT] = att + 1;
+    // a degenerate element: the reference raises in the element routine
+    if (fl & XK_FLAG_DEGENERATE) { nc_stop(c, XK_NC_STOPPED, XK_NCR_DEGENERATE); return; }
+    if (ratio < c[XK_NC_TOL_F]) { c[XK_NC_STATE] = XK_NC_CONVERGED; return; }
+    int consec = (int)c[XK_NC_CONSEC];
+    if (att > 0 && ratio > c[XK_NC_PREV] * 1.01) consec += 1;
+    else consec = 0;
+    c[XK_NC_CONSEC] = consec;
+    c[XK_NC_PREV] = ratio;
+    if ((consec >= (int)c[XK_NC_WIN] && !(ratio < 0.01)) || (att >= 10 && ratio > 100.0)) {
+        c[XK_NC_STATE] = XK_NC_DIVERGED;
+        return;
+    }
+    // a non-finite system has no solution: LinearSolveProcess success=False (_newton_steps.py:360-366)
+    if ((fl & XK_FLAG_NONFINITE) || !isfinite(res)) nc_stop(c, XK_NC_STOPPED, XK_NCR_NONFINITE);
+}
+
+// after the linear solve, before the line search: a zero pivot stops the call;
+// a residual above the tolerance hands the attempt to the host (XK_NC_ASSIST),
+// unless the host has just accepted this attempt's solve
+__global__ void nc_solve_kernel(double* __restrict__ c, const int* __restrict__ flags) {
+    if (threadIdx.x != 0 || c[XK_NC_STATE] != XK_NC_RUNNING) return;
+    const int att = (int)c[XK_NC_ATT] - 1;
+    if (att >= 0 && att < (int)c[XK_NC_HCAP]) c[XK_NC_HIST + XK_NC_HSTRIDE * att + 2] = c[XK_NC_REL];
+    if (c[XK_NC_SOLVE_OK] != 0.0) {
+        c[XK_NC_SOLVE_OK] = 0.0;
+        return;
+    }
+    if (*flags & XK_FLAG_SINGULAR) { nc_stop(c, XK_NC_STOPPED, XK_NCR_SINGULAR); return; }
+    if (!(c[XK_NC_REL] <= c[XK_NC_STOL])) c[XK_NC_STATE] = XK_NC_ASSIST;   // NaN included
+}
+
+// ConvergenceCheckProcess, displacement and energy part (_newton_steps.py:215-243),
+// after the update, and the attempt limit
+__global__ void nc_after_kernel(double* __restrict__ c, const double* __restrict__ status) {
+    if (threadIdx.x != 0 || c[XK_NC_STATE] != XK_NC_RUNNING) return;
+    const double dn = status[NS_DU], un = status[NS_U], en = status[NS_ENERGY];
+    c[XK_NC_DU] = dn;
+    c[XK_NC_U] = un;
+    c[XK_NC_ENERGY] = en;
+    c[XK_NC_ALPHA] = status[NS_ALPHA];
+    const int att = (int)c[XK_NC_ATT] - 1;
+    if (att >= 0 && att < (int)c[XK_NC_HCAP]) c[XK_NC_HIST + XK_NC_HSTRIDE * att + 3] = status[NS_ALPHA];
+    if (un > 1e-30 && dn / un < c[XK_NC_TOL_D]) { c[XK_NC_STATE] = XK_NC_CONVERGED; return; }
+    double eref = c[XK_NC_EREF];
+    if (eref < 0.0) {
+        eref = en > 1e-30 ? en : 1.0;
+        c[XK_NC_EREF] = eref;
+    }
+    if (eref > 1e-30 && en / eref < 1e-10) { c[XK_NC_STATE] = XK_NC_CONVERGED; return; }
+    if (att + 1 >= (int)c[XK_NC_MAXATT]) nc_stop(c, XK_NC_STOPPED, XK_NCR_LIMIT);
+}
+
+__global__ void nc_set_kernel(double* __restrict__ c, int state, int reason) {
+    if (threadIdx.x != 0) return;
+    c[XK_NC_STATE] = state;
+    c[XK_NC_REASON] = (double)((int)c[XK_NC_REASON] | reason);
+    if (state == XK_NC_RUNNING) c[XK_NC_SOLVE_OK] = 1.0;
+}
+
+int Ctx::newton_control_begin(double* ctl, int hist_cap, double tol_force, double tol_disp, double f_ext_ref_norm,
+                              int dynamic_ref, int divergence_window, int max_attempts, double solve_tol,
+                              hipStream_t st) {
+    if (hist_cap < 0 || max_attempts < 1) {
+        set_error("xk_newton_control_begin: invalid history capacity or attempt limit");
+        return -1;
+    }
+    hipLaunchKernelGGL(nc_begin_kernel, dim3(1), dim3(64), 0, st, ctl, hist_cap, tol_force, tol_disp,
+                       f_ext_ref_norm, dynamic_ref, divergence_window, max_attempts, solve_tol, (int*)d_flags.p);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("xk_newton_control_begin: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+int Ctx::newton_control_force(const double* res_norm2, const int* n_active, hipStream_t st) {
+    if (!nstop) {
+        set_error("xk_newton_control_force: no control record attached");
+        return -1;
+    }
+    hipLaunchKernelGGL(nc_force_kernel, dim3(1), dim3(64), 0, st, nstop, res_norm2, n_active,
+                       (const int*)d_flags.p);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("xk_newton_control_force: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+int Ctx::newton_control_set(int state, int reason, hipStream_t st) {
+    if (!nstop || state < XK_NC_RUNNING || state > XK_NC_ASSIST) {
+        set_error("xk_newton_control_set: no control record attached or invalid state");
+        return -1;
+    }
+    hipLaunchKernelGGL(nc_set_kernel, dim3(1), dim3(64), 0, st, nstop, state, reason);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("xk_newton_control_set: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
 int Ctx::newton_update(double* u, double* du, const double* R, const double* f_contact, const double* f_ext,
                        const uint8_t* fixed, const double* res_norm2, int max_steps, double diverge_factor,
                        double cap, double* status, hipStream_t st) {
@@ -152,9 +322,11 @@ int Ctx::newton_update(double* u, double* du, const double* R, const double* f_c
         set_error("xk_newton_update: out of device memory");
         return -1;
     }
+    const double* stop = nstop;
+    if (nstop) hipLaunchKernelGGL(nc_solve_kernel, dim3(1), dim3(64), 0, st, nstop, (const int*)d_flags.p);
     double* ut = (double*)nt_try.p;
     for (int k = 0; k < max_steps; ++k) {
-        hipLaunchKernelGGL(nt_axpy_kernel, dim3(nt_grid(n)), dim3(256), 0, st, n, (const double*)u,
+        hipLaunchKernelGGL(nt_axpy_kernel, dim3(nt_grid(n)), dim3(256), 0, st, stop, n, (const double*)u,
                            (const double*)du, k, ut);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = launch_elements(ut, true, nullptr, 0.0, nullptr, st);
@@ -162,13 +334,16 @@ int Ctx::newton_update(double* u, double* du, const double* R, const double* f_c
             set_error(std::string("xk_newton_update: ") + hipGetErrorString(e));
             return -1;
         }
-        hipLaunchKernelGGL(nt_trial_norm_kernel, dim3(1), dim3(NR), 0, st, n, (const double*)d_fint.p, f_contact,
-                           f_ext, fixed, k, status);
+        hipLaunchKernelGGL(nt_trial_norm_kernel, dim3(1), dim3(NR), 0, st, stop, n, (const double*)d_fint.p,
+                           f_contact, f_ext, fixed, k, status);
     }
-    hipLaunchKernelGGL(nt_select_kernel, dim3(1), dim3(NR), 0, st, n, max_steps, diverge_factor, cap, res_norm2,
-                       (const double*)u, (const double*)du, status);
-    hipLaunchKernelGGL(nt_update_kernel, dim3(nt_grid(n)), dim3(256), 0, st, n, u, du, (const double*)status);
-    hipLaunchKernelGGL(nt_norms_kernel, dim3(1), dim3(NR), 0, st, n, (const double*)u, (const double*)du, R, status);
+    hipLaunchKernelGGL(nt_select_kernel, dim3(1), dim3(NR), 0, st, stop, n, max_steps, diverge_factor, cap,
+                       res_norm2, (const double*)u, (const double*)du, status);
+    hipLaunchKernelGGL(nt_update_kernel, dim3(nt_grid(n)), dim3(256), 0, st, stop, n, u, du,
+                       (const double*)status);
+    hipLaunchKernelGGL(nt_norms_kernel, dim3(1), dim3(NR), 0, st, stop, n, (const double*)u, (const double*)du, R,
+                       status);
+    if (nstop) hipLaunchKernelGGL(nc_after_kernel, dim3(1), dim3(64), 0, st, nstop, (const double*)status);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(std::string("xk_newton_update: ") + hipGetErrorString(e));

@@ -152,6 +152,9 @@ __global__ void __launch_bounds__(BS)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ nf,
             GT* __restrict__ sg, int* __restrict__ act_part, const int* __restrict__ idx) {
+    // the Newton call this attempt belongs to has ended (or waits on the host):
+    // the friction history must not move
+    if (!nc_go(pp.stop)) return;
     const int lb = xcd_logical_block();
     const int n_blk = (int)((n_pairs + BS - 1) / BS);
     if (lb >= n_blk) return;  // padding workgroup (uniform over the workgroup)

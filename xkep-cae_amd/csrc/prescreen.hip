@@ -371,7 +371,9 @@ __global__ void lp_adam_kernel(float* __restrict__ prm, const float* __restrict_
 // the slot exists; a one-lane kernel then advances *count.  The verification
 // pass replays these geometries for the pruned pairs.
 __global__ void lp_hist_push_kernel(int64_t n3, const double* __restrict__ u, double* __restrict__ hist,
-                                    int64_t stride, int cap, const int* __restrict__ count) {
+                                    int64_t stride, int cap, const int* __restrict__ count,
+                                    const double* __restrict__ stop) {
+    if (!nc_go(stop)) return;   // no assembly ran: nothing to record
     const int slot = *count;
     if (slot >= cap) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,8 +382,8 @@ __global__ void lp_hist_push_kernel(int64_t n3, const double* __restrict__ u, do
     hist[(size_t)slot * stride + q] = u[6 * node + c];
 }
 
-__global__ void lp_hist_bump_kernel(int* __restrict__ count) {
-    if (threadIdx.x == 0) *count += 1;
+__global__ void lp_hist_bump_kernel(int* __restrict__ count, const double* __restrict__ stop) {
+    if (threadIdx.x == 0 && nc_go(stop)) *count += 1;
 }
 
 // ------------------------------------------------------------ launchers
@@ -433,14 +435,15 @@ hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g
     return hipGetLastError();
 }
 
-hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st) {
+hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st,
+                               const double* stop) {
     const int64_t n3 = 3 * (int64_t)n_nodes;
     if (n3 <= 0) return hipSuccess;
     hipLaunchKernelGGL(lp_hist_push_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, n3, u, hist, n3,
-                       cap, (const int*)count);
+                       cap, (const int*)count, stop);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(lp_hist_bump_kernel, dim3(1), dim3(64), 0, st, count);
+    hipLaunchKernelGGL(lp_hist_bump_kernel, dim3(1), dim3(64), 0, st, count, stop);
     return hipGetLastError();
 }
 

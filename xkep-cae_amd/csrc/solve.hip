@@ -45,10 +45,15 @@ enum SolveBuf {
 };
 
 // ------------------------------------------------------------------ kernels
+//
+// Every kernel takes the device Newton control state word first (`stop`,
+// null outside a device-controlled call) and does nothing once the call it
+// belongs to has ended: a solve enqueued past the end of a call is a no-op.
 
 // 32-bit pivots and info of one LU widened into the 64-bit buffers
-__global__ void sv_widen_kernel(int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
+__global__ void sv_widen_kernel(const double* __restrict__ stop, int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
                                 const int* __restrict__ info32, int64_t* __restrict__ info64) {
+    if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) piv64[i] = piv32[i];
     if (i == 0) *info64 = *info32;
@@ -71,9 +76,10 @@ __device__ __forceinline__ int sv_find(const int* __restrict__ gptr, const int* 
 // Per node n, its 6 scalar rows over the columns of nodes n-1, n, n+1
 // (6 x 18, FP64), masked: a fixed row or column becomes the identity's.
 template <class KT>
-__global__ void sv_extract_kernel(int n_nodes, const int8_t* __restrict__ nflag, const int* __restrict__ gptr,
+__global__ void sv_extract_kernel(const double* __restrict__ stop, int n_nodes, const int8_t* __restrict__ nflag, const int* __restrict__ gptr,
                                   const int* __restrict__ gcol, const KT* __restrict__ Kb,
                                   const uint8_t* __restrict__ fixed, double* __restrict__ Ab) {
+    if (!nc_go(stop)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * n_nodes) return;
     const int n = t / 6, r = t - 6 * (t / 6);
@@ -111,9 +117,10 @@ __global__ void sv_extract_kernel(int n_nodes, const int8_t* __restrict__ nflag,
 
 // Block LU along each chain: S_n0 = D_n0, L_n = F_{n-1} S_{n-1}^-1,
 // S_n = D_n - L_n E_{n-1}; stores L_n and S_n^-1.  One workgroup per chain.
-__global__ __launch_bounds__(64) void sv_factor_kernel(const int* __restrict__ cstart,
+__global__ __launch_bounds__(64) void sv_factor_kernel(const double* __restrict__ stop, const int* __restrict__ cstart,
                                                        const double* __restrict__ Ab, double* __restrict__ Lf,
                                                        double* __restrict__ Sinv, int* __restrict__ flags) {
+    if (!nc_go(stop)) return;
     const int c = blockIdx.x;
     const int n0 = cstart[c], n1 = cstart[c + 1];
     __shared__ double M[6][13];
@@ -192,12 +199,14 @@ __global__ __launch_bounds__(64) void sv_factor_kernel(const int* __restrict__ c
 }
 
 // slot -> pair
-__global__ void sv_slist_kernel(int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
+__global__ void sv_slist_kernel(const double* __restrict__ stop, int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
+    if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_pairs && sinc[i] >= 0) slist[sinc[i]] = (int)i;
 }
 
-__global__ void sv_slot_nodes_kernel(int ns, const int* __restrict__ slist, PairSoA P, int4* __restrict__ sn) {
+__global__ void sv_slot_nodes_kernel(const double* __restrict__ stop, int ns, const int* __restrict__ slist, PairSoA P, int4* __restrict__ sn) {
+    if (!nc_go(stop)) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ns) return;
     const int i = slist[q];
@@ -206,8 +215,9 @@ __global__ void sv_slot_nodes_kernel(int ns, const int* __restrict__ slist, Pair
 
 // the pair's staged tangent record -> G (3x3) and c (4) in FP64
 template <class GT>
-__global__ void sv_slot_pack_kernel(int ns, const GT* __restrict__ sg, double* __restrict__ sG,
+__global__ void sv_slot_pack_kernel(const double* __restrict__ stop, int ns, const GT* __restrict__ sg, double* __restrict__ sG,
                                     double* __restrict__ sc) {
+    if (!nc_go(stop)) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ns) return;
     const GT* r = sg + SG_REC * (size_t)q;
@@ -221,11 +231,12 @@ __global__ void sv_slot_pack_kernel(int ns, const GT* __restrict__ sg, double* _
 // C_w[(k, a), (l, b)] = (A~_b^-1)[(node k, a), (node l, b)].  One workgroup per
 // chain, one lane per right-hand side e_(l, b); forward sweep into Y, backward
 // sweep reading it.
-__global__ __launch_bounds__(256) void sv_compliance_kernel(
+__global__ __launch_bounds__(256) void sv_compliance_kernel(const double* __restrict__ stop, 
     const int* __restrict__ cstart, const int* __restrict__ cn, const int* __restrict__ cnode,
     const int* __restrict__ ch_cn0, const int* __restrict__ ch_m, const int64_t* __restrict__ ch_yoff,
     const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed, const double* __restrict__ Ab,
     const double* __restrict__ Lf, const double* __restrict__ Sinv, double* __restrict__ Y, double* __restrict__ C) {
+    if (!nc_go(stop)) return;
     const int c = blockIdx.x;
     const int m = ch_m[c];
     if (m == 0) return;
@@ -291,12 +302,13 @@ __global__ __launch_bounds__(256) void sv_compliance_kernel(
 
 // Capacitance block (i, q) of one component, column-major with ld = 3 Pc:
 // I delta_iq + G_i sum_{k,l} c_ik c_ql M_k C[node_k(i), node_l(q)] M_l.
-__global__ void sv_capacitance_kernel(int Pc, const int* __restrict__ cslot, const int4* __restrict__ sn,
+__global__ void sv_capacitance_kernel(const double* __restrict__ stop, int Pc, const int* __restrict__ cslot, const int4* __restrict__ sn,
                                       const double* __restrict__ sG, const double* __restrict__ sc,
                                       const int* __restrict__ node_chain, const int* __restrict__ cn,
                                       const int* __restrict__ ch_cn0, const int* __restrict__ ch_m,
                                       const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed,
                                       const double* __restrict__ C, double* __restrict__ S) {
+    if (!nc_go(stop)) return;
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     const int lq = blockIdx.y;
     if (li >= Pc) return;
@@ -338,7 +350,8 @@ __global__ void sv_capacitance_kernel(int Pc, const int* __restrict__ cslot, con
         }
 }
 
-__global__ void sv_info_kernel(int nc, const int64_t* __restrict__ info, int* __restrict__ flags) {
+__global__ void sv_info_kernel(const double* __restrict__ stop, int nc, const int64_t* __restrict__ info, int* __restrict__ flags) {
+    if (!nc_go(stop)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < nc && info[c] != 0) atomicOr(flags, XK_FLAG_SINGULAR);
 }
@@ -380,8 +393,9 @@ __device__ __forceinline__ double sv_readlane(double v, int lane) {
 // FULL: the panel holds LU_NB columns (every panel but a last partial one):
 // the sub-block bounds are constants and the unrolled steps carry no guards.
 template <bool FULL>
-__global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_row_wg, double* __restrict__ A,
+__global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restrict__ stop, int n, int j0, int n_row_wg, double* __restrict__ A,
                                                           int64_t* __restrict__ info) {
+    if (!nc_go(stop)) return;
     __shared__ double D[LU_NB][LU_NB + 1];
     __shared__ double X[LU_NB][LU_NB + 1];
     __shared__ double invd[LU_NB];   // reciprocals of the pivots
@@ -657,7 +671,8 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(int n, int j0, int n_r
 }
 
 // pivots of an LU without interchanges: row i stays row i (LAPACK, 1-based)
-__global__ void sv_ipiv_identity_kernel(int64_t n, int64_t* __restrict__ ipiv) {
+__global__ void sv_ipiv_identity_kernel(const double* __restrict__ stop, int64_t n, int64_t* __restrict__ ipiv) {
+    if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) ipiv[i] = i + 1;
 }
@@ -665,10 +680,11 @@ __global__ void sv_ipiv_identity_kernel(int64_t n, int64_t* __restrict__ ipiv) {
 // x <- A~_b^-1 x along every chain: 8 lanes per chain (lanes 0..5 hold the
 // node's six components), 8 chains per wavefront; rows of the 6x6 blocks are
 // combined across the chain's lanes by shuffles.
-__global__ __launch_bounds__(64) void sv_chain_solve_kernel(int n_chains, const int* __restrict__ cstart,
+__global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __restrict__ stop, int n_chains, const int* __restrict__ cstart,
                                                             const double* __restrict__ Ab,
                                                             const double* __restrict__ Lf,
                                                             const double* __restrict__ Sinv, double* __restrict__ x) {
+    if (!nc_go(stop)) return;
     const int lane = threadIdx.x, base = lane & ~7, a = lane & 7;
     const int c = blockIdx.x * 8 + (lane >> 3);
     if (c >= n_chains || a >= 6) return;
@@ -774,8 +790,9 @@ __device__ __forceinline__ double sv_block_dot(double acc, const double* __restr
 // needs no host round trip (and can be captured in a graph).
 constexpr int PERM_MAX = 16384;
 
-__global__ __launch_bounds__(256) void sv_perm_kernel(int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
+__global__ __launch_bounds__(256) void sv_perm_kernel(const double* __restrict__ stop, int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
                                                       int* __restrict__ flags) {
+    if (!nc_go(stop)) return;
     __shared__ int p[PERM_MAX];
     __shared__ int q[PERM_MAX];
     bool bad = false;
@@ -801,8 +818,9 @@ __global__ __launch_bounds__(256) void sv_perm_kernel(int n, const int64_t* __re
 }
 
 // y[i] = t[perm[i]]
-__global__ void sv_permute_kernel(int n, const int* __restrict__ perm, const double* __restrict__ t,
+__global__ void sv_permute_kernel(const double* __restrict__ stop, int n, const int* __restrict__ perm, const double* __restrict__ t,
                                   double* __restrict__ y) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = t[perm[i]];
 }
@@ -814,8 +832,9 @@ __global__ void sv_permute_kernel(int n, const int* __restrict__ perm, const dou
 // broadcast from lane to lane; workgroup 0 stores the block in z; every
 // workgroup then subtracts L[i, block s] z_s from its rows i of block
 // s + 1 + blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_lower_step_kernel(int n, const double* __restrict__ LU,
+__global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop, int n, const double* __restrict__ LU,
                                                            double* __restrict__ y, double* __restrict__ z, int s) {
+    if (!nc_go(stop)) return;
     __shared__ double zs[TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
@@ -837,8 +856,9 @@ __global__ __launch_bounds__(TB) void sv_lower_step_kernel(int n, const double* 
 // workgroup solves the diagonal block s of z by back substitution (workgroup 0
 // stores it in x), then subtracts U[i, block s] x_s from its rows i of block
 // s - 1 - blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_upper_step_kernel(int n, const double* __restrict__ LU,
+__global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop, int n, const double* __restrict__ LU,
                                                            double* __restrict__ z, double* __restrict__ x, int s) {
+    if (!nc_go(stop)) return;
     __shared__ double xs[TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
@@ -866,16 +886,18 @@ __global__ __launch_bounds__(TB) void sv_upper_step_kernel(int n, const double* 
 }
 
 // perm[i] = i (a block factored without interchanges)
-__global__ void sv_perm_identity_kernel(int n, int* __restrict__ perm) {
+__global__ void sv_perm_identity_kernel(const double* __restrict__ stop, int n, int* __restrict__ perm) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) perm[i] = i;
 }
 
 // W[3 pos(q) + r] = (G_q U~_q^T y)_r
-__global__ void sv_project_kernel(int ns, const int* __restrict__ pos, const int4* __restrict__ sn,
+__global__ void sv_project_kernel(const double* __restrict__ stop, int ns, const int* __restrict__ pos, const int4* __restrict__ sn,
                                   const double* __restrict__ sG, const double* __restrict__ sc,
                                   const uint8_t* __restrict__ fixed, const double* __restrict__ y,
                                   double* __restrict__ W) {
+    if (!nc_go(stop)) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ns) return;
     const int4 n4 = sn[q];
@@ -896,10 +918,11 @@ __global__ void sv_project_kernel(int ns, const int* __restrict__ pos, const int
 }
 
 // out = r - U~ W  (node lists hold the codes 4 slot + k of the pairs at each node)
-__global__ void sv_lift_kernel(int n_nodes, const int* __restrict__ pptr, const int* __restrict__ psrc,
+__global__ void sv_lift_kernel(const double* __restrict__ stop, int n_nodes, const int* __restrict__ pptr, const int* __restrict__ psrc,
                                const int* __restrict__ pos, const double* __restrict__ sc,
                                const uint8_t* __restrict__ fixed, const double* __restrict__ r,
                                const double* __restrict__ W, double* __restrict__ out) {
+    if (!nc_go(stop)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * n_nodes) return;
     const int n = t / 6, a = t - 6 * (t / 6);
@@ -915,8 +938,9 @@ __global__ void sv_lift_kernel(int n_nodes, const int* __restrict__ pptr, const 
 
 // out = b - A~_b x from the chain blocks (one lane per scalar row); the
 // contact part U~ G U~^T x is subtracted by sv_project_kernel + sv_lift_kernel
-__global__ void sv_chain_residual_kernel(int n_nodes, const double* __restrict__ Ab, const double* __restrict__ b,
+__global__ void sv_chain_residual_kernel(const double* __restrict__ stop, int n_nodes, const double* __restrict__ Ab, const double* __restrict__ b,
                                          const double* __restrict__ x, double* __restrict__ out) {
+    if (!nc_go(stop)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * n_nodes) return;
     const int n = t / 6, r = t - 6 * (t / 6);
@@ -927,20 +951,23 @@ __global__ void sv_chain_residual_kernel(int n_nodes, const double* __restrict__
     out[t] = b[t] - v;
 }
 
-__global__ void sv_rhs_kernel(int ndof, const double* __restrict__ R, const uint8_t* __restrict__ fixed,
+__global__ void sv_rhs_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ R, const uint8_t* __restrict__ fixed,
                               double* __restrict__ b) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ndof) b[i] = fixed[i] ? 0.0 : -R[i];
 }
 
-__global__ void sv_axpy_kernel(int ndof, const double* __restrict__ dx, double* __restrict__ x) {
+__global__ void sv_axpy_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ dx, double* __restrict__ x) {
+    if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ndof) x[i] += dx[i];
 }
 
 // rel = ||r|| / ||b|| (one workgroup; fixed order, so deterministic)
-__global__ __launch_bounds__(1024) void sv_relnorm_kernel(int ndof, const double* __restrict__ r,
+__global__ __launch_bounds__(1024) void sv_relnorm_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ r,
                                                           const double* __restrict__ b, double* __restrict__ rel) {
+    if (!nc_go(stop)) return;
     __shared__ double s0[1024], s1[1024];
     double a = 0.0, c = 0.0;
     for (int i = threadIdx.x; i < ndof; i += 1024) {
@@ -961,6 +988,36 @@ __global__ __launch_bounds__(1024) void sv_relnorm_kernel(int ndof, const double
 }
 
 // ------------------------------------------------------------------ host
+
+// Device-decided refinement (a control record attached, tol > 0): the gate
+// word XK_NC_RGATE of the record is 0 while steps remain to be taken; the
+// test after each step's residual norm closes it at the tolerance, at the
+// rounding floor (a step no longer halves the residual) or on a non-finite
+// norm -- the host loop's rule -- and the closed gate turns the remaining
+// steps' kernels into no-ops.  A call that has ended never opens it.
+__global__ void sv_refine_begin_kernel(double* __restrict__ c) {
+    if (threadIdx.x != 0) return;
+    c[XK_NC_RGATE] = c[XK_NC_STATE] == XK_NC_RUNNING ? 0.0 : 1.0;
+    c[XK_NC_RPREV] = HUGE_VAL;
+    c[XK_NC_RSTEPS] = 0.0;
+}
+
+__global__ void sv_refine_test_kernel(double* __restrict__ c, const double* __restrict__ red, double tol) {
+    if (threadIdx.x != 0 || c[XK_NC_RGATE] != 0.0) return;
+    const double rel = *red;
+    if (rel <= tol || rel > 0.5 * c[XK_NC_RPREV] || !(rel <= DBL_MAX)) {
+        c[XK_NC_RGATE] = 1.0;
+        return;
+    }
+    c[XK_NC_RPREV] = rel;
+    c[XK_NC_RSTEPS] += 1.0;
+}
+
+__global__ void sv_refine_end_kernel(const double* __restrict__ c, const double* __restrict__ red,
+                                     double* __restrict__ res_rel) {
+    if (threadIdx.x != 0 || c[XK_NC_STATE] != XK_NC_RUNNING) return;
+    if (res_rel) *res_rel = *red;
+}
 
 static unsigned sv_grid(int64_t n, int bs = 256) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
 
@@ -1017,10 +1074,10 @@ int Ctx::solve_setup() {
     SV_TRY(sv[SV_SLIST].ensure(sizeof(int) * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_SN].ensure(sizeof(int4) * (size_t)std::max(ns, 1)));
     if (ns > 0) {
-        hipLaunchKernelGGL(sv_slist_kernel, dim3(sv_grid(n_pairs)), dim3(256), 0, 0, n_pairs, (const int*)P.sinc,
+        hipLaunchKernelGGL(sv_slist_kernel, dim3(sv_grid(n_pairs)), dim3(256), 0, 0, nullptr, n_pairs, (const int*)P.sinc,
                            (int*)sv[SV_SLIST].p);
         SV_TRY(hipGetLastError());
-        hipLaunchKernelGGL(sv_slot_nodes_kernel, dim3(sv_grid(ns)), dim3(256), 0, 0, ns, (const int*)sv[SV_SLIST].p,
+        hipLaunchKernelGGL(sv_slot_nodes_kernel, dim3(sv_grid(ns)), dim3(256), 0, 0, nullptr, ns, (const int*)sv[SV_SLIST].p,
                            P, (int4*)sv[SV_SN].p);
         SV_TRY(hipGetLastError());
     }
@@ -1136,14 +1193,14 @@ int Ctx::solve_setup() {
 }
 
 // x <- A~^-1 r (Woodbury), r and x device vectors of 6 n_nodes; T scratch
-int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
+int Ctx::solve_apply(const double* r, double* x, hipStream_t st, const double* stop) {
     const int ns = n_sinc;
     const int nd = 6 * n_nodes;
     double* T = (double*)sv[SV_T].p;
     double* W = (double*)sv[SV_W].p;
     const uint8_t* fixed = sv_fixed;
     SV_TRY(hipMemcpyAsync(T, r, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, n_chains,
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, stop, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
                        (const double*)sv[SV_SINV].p, T);
     SV_TRY(hipGetLastError());
@@ -1151,7 +1208,7 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
         SV_TRY(hipMemcpyAsync(x, T, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
         return 0;
     }
-    hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, ns, (const int*)sv[SV_POS].p,
+    hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, stop, ns, (const int*)sv[SV_POS].p,
                        (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, fixed,
                        (const double*)T, W);
     SV_TRY(hipGetLastError());
@@ -1164,23 +1221,23 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st) {
         // projection is consumed), the upper solve back into W2
         double* y = W2 + 3 * sv_comp_pos[c];
         double* z = W + 3 * sv_comp_pos[c];
-        hipLaunchKernelGGL(sv_permute_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n,
+        hipLaunchKernelGGL(sv_permute_kernel, dim3(sv_grid(n)), dim3(256), 0, st, stop, n,
                            (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)z, y);
         SV_TRY(hipGetLastError());
         for (int s = 0; s < nb; ++s) {
-            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, n, LU, y, z, s);
+            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, stop, n, LU, y, z, s);
             SV_TRY(hipGetLastError());
         }
         for (int s = nb - 1; s >= 0; --s) {
-            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(std::max(1, s)), dim3(TB), 0, st, n, LU, z, y, s);
+            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(std::max(1, s)), dim3(TB), 0, st, stop, n, LU, z, y, s);
             SV_TRY(hipGetLastError());
         }
     }
-    hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes, (const int*)d_node_pptr.p,
+    hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, stop, n_nodes, (const int*)d_node_pptr.p,
                        (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p, fixed, r,
                        (const double*)W2, x);
     SV_TRY(hipGetLastError());
-    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, n_chains,
+    hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, stop, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
                        (const double*)sv[SV_SINV].p, x);
     SV_TRY(hipGetLastError());
@@ -1203,7 +1260,7 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
     for (size_t c = 0; c < sv_comp_n.size(); ++c) {
         const int Pc = sv_comp_n[c];
         double* S = (double*)sv[SV_S].p + sv_comp_soff[c];
-        hipLaunchKernelGGL(sv_capacitance_kernel, dim3(sv_grid(Pc, 64), Pc), dim3(64), 0, st, Pc,
+        hipLaunchKernelGGL(sv_capacitance_kernel, dim3(sv_grid(Pc, 64), Pc), dim3(64), 0, st, nstop, Pc,
                            (const int*)sv[SV_CSLOT].p + sv_comp_pos[c], (const int4*)sv[SV_SN].p,
                            (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, (const int*)d_node_chain.p,
                            (const int*)sv[SV_CN].p, (const int*)sv[SV_CHCN0].p, (const int*)sv[SV_CHM].p,
@@ -1224,10 +1281,10 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                 const int64_t rest = n - j0 - w;
                 const int nrw = (int)((rest + LU_NB - 1) / LU_NB);
                 if (w == LU_NB)
-                    hipLaunchKernelGGL(sv_lu_panel_kernel<true>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st,
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<true>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st, nstop,
                                        (int)n, (int)j0, nrw, S, info);
                 else
-                    hipLaunchKernelGGL(sv_lu_panel_kernel<false>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st,
+                    hipLaunchKernelGGL(sv_lu_panel_kernel<false>, dim3(std::max(1, 2 * nrw)), dim3(256), 0, st, nstop,
                                        (int)n, (int)j0, nrw, S, info);
                 SV_TRY(hipGetLastError());
                 return 0;
@@ -1251,7 +1308,7 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                 if (update(n - j2, n - j2, w0 + w1, S + j2 + J * ld, S + J + j2 * ld, S + j2 + j2 * ld) != 0)
                     return -1;
             }
-            hipLaunchKernelGGL(sv_ipiv_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, ipiv);
+            hipLaunchKernelGGL(sv_ipiv_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, ipiv);
             SV_TRY(hipGetLastError());
             sv_comp_piv[c] = 0;
         } else {
@@ -1261,13 +1318,13 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                 int* piv32 = (int*)sv[SV_PIV32].p + 3 * sv_comp_pos[c];
                 int* info32 = (int*)sv[SV_PIV32].p + 3 * (size_t)std::max(ns, 1) + c;
                 SV_RB(rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, S, (rocblas_int)n, piv32, info32));
-                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, (const int*)piv32, ipiv,
+                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, (const int*)piv32, ipiv,
                                    (const int*)info32, info);
                 SV_TRY(hipGetLastError());
             } else {
                 SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, ipiv, info));
             }
-            hipLaunchKernelGGL(sv_info_kernel, dim3(1), dim3(64), 0, st, 1, (const int64_t*)info, (int*)d_flags.p);
+            hipLaunchKernelGGL(sv_info_kernel, dim3(1), dim3(64), 0, st, nstop, 1, (const int64_t*)info, (int*)d_flags.p);
             SV_TRY(hipGetLastError());
             sv_comp_piv[c] = 1;
         }
@@ -1281,10 +1338,10 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
             const int n = 3 * sv_comp_n[c];
             int* perm = (int*)sv[SV_PERM].p + 3 * sv_comp_pos[c];
             if (sv_comp_piv[c])
-                hipLaunchKernelGGL(sv_perm_kernel, dim3(1), dim3(256), 0, st, n,
+                hipLaunchKernelGGL(sv_perm_kernel, dim3(1), dim3(256), 0, st, nstop, n,
                                    (const int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c], perm, (int*)d_flags.p);
             else
-                hipLaunchKernelGGL(sv_perm_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, n, perm);
+                hipLaunchKernelGGL(sv_perm_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, perm);
             SV_TRY(hipGetLastError());
         }
         return 0;
@@ -1293,7 +1350,12 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
     piv_h.resize(np3);
     perm_h.resize(np3);
     SV_TRY(hipMemcpyAsync(piv_h.data(), sv[SV_IPIV].p, sizeof(int64_t) * np3, hipMemcpyDeviceToHost, st));
+    double state = 0.0;
+    if (nstop) SV_TRY(hipMemcpyAsync(&state, nstop, sizeof(double), hipMemcpyDeviceToHost, st));
     SV_TRY(hipStreamSynchronize(st));
+    // a solve enqueued past the end of a device-controlled call factored stale
+    // scratch: its pivots mean nothing and nothing reads the permutation
+    if (state != XK_NC_RUNNING) return 0;
     for (size_t c = 0; c < sv_comp_n.size(); ++c) {
         const int64_t o = 3 * sv_comp_pos[c], n = 3 * (int64_t)sv_comp_n[c];
         int* p = perm_h.data() + o;
@@ -1323,6 +1385,11 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     }
     if (!solve_ready && solve_setup() != 0) return -1;
     sv_fixed = fixed;
+    // with a device Newton control record attached and tol > 0 the refinement
+    // stops on the device (no host read): each step is gated by a word of the
+    // record that the step's norm test sets, so the steps after the stop are
+    // no-ops; a solve left above tol is then the control's to hand to the host
+    const bool dev_refine = nstop != nullptr && tol > 0.0;
     SV_RB(rocblas_set_stream((rocblas_handle)sv_handle, st));
     const int ns = n_sinc;
     const int nd = 6 * n_nodes;
@@ -1333,21 +1400,21 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     // trajectory with it), so the mixed form's FP32 K is an output for
     // consumers of the assembled matrix, not the operator this solve inverts.
     SV_TRY(launch_elements(u, false, sv[SV_KB].p, mass_coef, M, st, 0));
-    hipLaunchKernelGGL(sv_extract_kernel<double>, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+    hipLaunchKernelGGL(sv_extract_kernel<double>, dim3(sv_grid(nd)), dim3(256), 0, st, nstop, n_nodes,
                        (const int8_t*)d_nflag.p, (const int*)d_gptr.p, (const int*)d_gcol.p,
                        (const double*)sv[SV_KB].p, fixed, (double*)sv[SV_AB].p);
     SV_TRY(hipGetLastError());
-    hipLaunchKernelGGL(sv_factor_kernel, dim3(n_chains), dim3(64), 0, st, (const int*)d_cstart.p,
+    hipLaunchKernelGGL(sv_factor_kernel, dim3(n_chains), dim3(64), 0, st, nstop, (const int*)d_cstart.p,
                        (const double*)sv[SV_AB].p, (double*)sv[SV_LF].p, (double*)sv[SV_SINV].p, (int*)d_flags.p);
     SV_TRY(hipGetLastError());
     // capacitance LU: without row interchanges unless partial pivoting was
     // asked for or this pattern has already needed it
     bool gepp = sv_gepp || sv_lu_kind == 1;
     if (ns > 0) {
-        hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, ns,
+        hipLaunchKernelGGL(sv_slot_pack_kernel<double>, dim3(sv_grid(ns)), dim3(256), 0, st, nstop, ns,
                            (const double*)d_sg.p, (double*)sv[SV_SG].p, (double*)sv[SV_SC].p);
         SV_TRY(hipGetLastError());
-        hipLaunchKernelGGL(sv_compliance_kernel, dim3(n_chains), dim3(256), 0, st, (const int*)d_cstart.p,
+        hipLaunchKernelGGL(sv_compliance_kernel, dim3(n_chains), dim3(256), 0, st, nstop, (const int*)d_cstart.p,
                            (const int*)sv[SV_CN].p, (const int*)sv[SV_CNODE].p, (const int*)sv[SV_CHCN0].p,
                            (const int*)sv[SV_CHM].p, (const int64_t*)sv[SV_CHYOFF].p,
                            (const int64_t*)sv[SV_CHCOFF].p, fixed, (const double*)sv[SV_AB].p,
@@ -1355,7 +1422,7 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                            (double*)sv[SV_C].p);
         SV_TRY(hipGetLastError());
         if (sv_factor_capacitance(gepp, st) != 0) return -1;
-        if (!gepp && tol > 0.0) {
+        if (!gepp && tol > 0.0 && !dev_refine) {
             // eager solve: a zero or non-finite pivot of the LU without
             // interchanges is read back at once and the blocks factored again
             // with partial pivoting
@@ -1375,25 +1442,26 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     double* b = (double*)sv[SV_B].p;
     double* r = (double*)sv[SV_R].p;
     double* x = (double*)sv[SV_X].p;
-    hipLaunchKernelGGL(sv_rhs_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, R, fixed, b);
+    hipLaunchKernelGGL(sv_rhs_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nstop, nd, R, fixed, b);
     SV_TRY(hipGetLastError());
-    if (solve_apply(b, du, st) != 0) return -1;
+    if (solve_apply(b, du, st, nstop) != 0) return -1;
     // iterative refinement against the assembled K: a fixed number of steps
     // (tol <= 0, stream-ordered) or until ||b - A du|| <= tol ||b|| (one
     // host read of the norm per step)
     // residual of the operator the solve inverts, A~_b + U~ G U~^T (in the
     // FP64 form equal to the assembled K up to rounding; in the mixed form K
     // accumulates its contact blocks in FP32 and differs by that rounding)
+    const double* gate = dev_refine ? nstop + XK_NC_RGATE : nstop;
     auto residual = [&]() -> int {
-        hipLaunchKernelGGL(sv_chain_residual_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes,
+        hipLaunchKernelGGL(sv_chain_residual_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, n_nodes,
                            (const double*)sv[SV_AB].p, (const double*)b, (const double*)du, x);
         SV_TRY(hipGetLastError());
         if (ns > 0) {
-            hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, ns, (const int*)sv[SV_POS].p,
+            hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, gate, ns, (const int*)sv[SV_POS].p,
                                (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p,
                                fixed, (const double*)du, (double*)sv[SV_W].p);
             SV_TRY(hipGetLastError());
-            hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, n_nodes, (const int*)d_node_pptr.p,
+            hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, n_nodes, (const int*)d_node_pptr.p,
                                (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p,
                                fixed, (const double*)x, (const double*)sv[SV_W].p, r);
         } else {
@@ -1409,11 +1477,20 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     auto refine_loop = [&]() -> int {
         have_r = false;
         double prev = HUGE_VAL;
+        if (dev_refine) {
+            hipLaunchKernelGGL(sv_refine_begin_kernel, dim3(1), dim3(64), 0, st, nstop);
+            SV_TRY(hipGetLastError());
+        }
         for (int k = 0; k < refine; ++k) {
             if (residual() != 0) return -1;
-            if (tol > 0.0) {
+            if (dev_refine) {
+                hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, gate, nd, (const double*)r,
+                                   (const double*)b, red);
+                hipLaunchKernelGGL(sv_refine_test_kernel, dim3(1), dim3(64), 0, st, nstop, (const double*)red, tol);
+                SV_TRY(hipGetLastError());
+            } else if (tol > 0.0) {
                 double rel = 0.0;
-                hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r,
+                hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nstop, nd, (const double*)r,
                                    (const double*)b, red);
                 SV_TRY(hipGetLastError());
                 SV_TRY(hipMemcpyAsync(&rel, red, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1427,14 +1504,21 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                 }
                 prev = rel;
             }
-            if (solve_apply(r, x, st) != 0) return -1;
-            hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, nd, (const double*)x, du);
+            if (solve_apply(r, x, st, gate) != 0) return -1;
+            hipLaunchKernelGGL(sv_axpy_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, nd, (const double*)x, du);
             SV_TRY(hipGetLastError());
             ++steps;
         }
-        if (tol > 0.0) {
+        if (dev_refine) {
+            // all steps taken without a stop: the residual of the last one
             if (residual() != 0) return -1;
-            hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r,
+            hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, gate, nd, (const double*)r,
+                               (const double*)b, red);
+            SV_TRY(hipGetLastError());
+            have_r = true;
+        } else if (tol > 0.0) {
+            if (residual() != 0) return -1;
+            hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nstop, nd, (const double*)r,
                                (const double*)b, red);
             SV_TRY(hipGetLastError());
             SV_TRY(hipMemcpyAsync(&rel_last, red, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1444,19 +1528,26 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
         return 0;
     };
     if (refine_loop() != 0) return -1;
-    if (ns > 0 && !gepp && tol > 0.0 && !(rel_last <= tol)) {
+    if (ns > 0 && !gepp && tol > 0.0 && !dev_refine && !(rel_last <= tol)) {
         // the LU without interchanges left the refinement above tol: factor
         // with partial pivoting (kept for the rest of this pattern) and solve again
         gepp = sv_gepp = true;
         ++sv_fallbacks;
         if (sv_factor_capacitance(true, st) != 0) return -1;
-        if (solve_apply(b, du, st) != 0) return -1;
+        if (solve_apply(b, du, st, nstop) != 0) return -1;
         if (refine_loop() != 0) return -1;
+    }
+    if (dev_refine) {
+        // the relative residual where the refinement stopped (red) -> res_rel
+        hipLaunchKernelGGL(sv_refine_end_kernel, dim3(1), dim3(64), 0, st, nstop, (const double*)red, res_rel);
+        SV_TRY(hipGetLastError());
+        if (n_refine) *n_refine = -1;   // decided on the device: XK_NC_RSTEPS
+        return 0;
     }
     if (n_refine) *n_refine = steps;
     if (res_rel) {
         if (!have_r && residual() != 0) return -1;
-        hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nd, (const double*)r, (const double*)b,
+        hipLaunchKernelGGL(sv_relnorm_kernel, dim3(1), dim3(1024), 0, st, nstop, nd, (const double*)r, (const double*)b,
                            res_rel);
         SV_TRY(hipGetLastError());
     }

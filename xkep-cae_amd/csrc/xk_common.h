@@ -163,4 +163,11 @@ __device__ __forceinline__ int xcd_logical_block() {
     return ((int)blockIdx.x % N_XCD) * per + (int)blockIdx.x / N_XCD;
 }
 
+// Device Newton control: the state word of the control record of the call
+// in progress (null: no call attached).  The kernels of an attempt run only
+// while the call runs (state 0); an attempt enqueued after the call ended,
+// or while the host re-solves an attempt, is a no-op.  The value is uniform
+// over the grid.
+__device__ __forceinline__ bool nc_go(const double* stop) { return !stop || *stop == 0.0; }
+
 }  // namespace xk

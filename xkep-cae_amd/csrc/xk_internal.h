@@ -156,12 +156,20 @@ struct Ctx {
     void bind_pairs();
     int build_pattern();
     int solve_setup();
-    int solve_apply(const double* r, double* x, hipStream_t st);
+    int solve_apply(const double* r, double* x, hipStream_t st, const double* stop);
     int sv_factor_capacitance(bool gepp, hipStream_t st);
     int linear_solve(const double* u, const uint8_t* fixed, double mass_coef, const double* M, const double* R,
                      double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st);
     void release_solver();
     DBuf nt_try;   // trial displacement of the device line search
+    // device Newton control record of the call in progress (xk_newton_control_attach;
+    // null: none): the kernels of an attempt read its state word and skip once the call ended
+    double* nstop = nullptr;
+    int newton_control_begin(double* ctl, int hist_cap, double tol_force, double tol_disp, double f_ext_ref_norm,
+                             int dynamic_ref, int divergence_window, int max_attempts, double solve_tol,
+                             hipStream_t st);
+    int newton_control_force(const double* res_norm2, const int* n_active, hipStream_t st);
+    int newton_control_set(int state, int reason, hipStream_t st);
     int newton_update(double* u, double* du, const double* R, const double* f_contact, const double* f_ext,
                       const uint8_t* fixed, const double* res_norm2, int max_steps, double diverge_factor,
                       double cap, double* status, hipStream_t st);
@@ -215,13 +223,14 @@ struct Ctx {
 
 // launchers (one per translation unit)
 hipError_t launch_beam(int prec, int n_elems, const double* X, const double* u, const int* conn,
-                       const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags, hipStream_t st);
+                       const BeamConst& bc, double* fe_stage, void* Ke_stage, int* flags, hipStream_t st,
+                       const double* stop = nullptr);
 // direct variant: writes f_int (nullable) and the beam blocks of K (nullable,
 // pattern order, mass_coef * M added on diagonal blocks) without staging
 hipError_t launch_beam_direct(int prec, int n_elems, const double* X, const double* u, const int* conn,
                               const BeamConst& bc, const int8_t* eflag, const int* eslot, const int* gptr,
                               double mass_coef, const double* M, double* f_int, void* K, int* flags,
-                              hipStream_t st);
+                              hipStream_t st, const double* stop = nullptr);
 // idx (nullable): the kernel visits pairs idx[0 .. n_pairs) instead of 0 .. n_pairs;
 // bs: workgroup size (64, 128, 256 or 512), act_part holds pair_blocks(n, bs) counts
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
@@ -245,7 +254,8 @@ hipError_t launch_lp_infer_f32(int64_t n, const float* feat, const float* prm, f
 hipError_t launch_lp_split(int64_t n, const int64_t* keep, const int64_t* pos, int* watch, int* prune,
                            hipStream_t st);
 hipError_t launch_lp_miss(int64_t n, const int* idx, const double* gap, double g_on, int* count, hipStream_t st);
-hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st);
+hipError_t launch_lp_hist_push(int n_nodes, const double* u, double* hist, int cap, int* count, hipStream_t st,
+                               const double* stop = nullptr);
 hipError_t launch_pair_replay(int64_t n_idx, const int* idx, const PairSoA& P, const double* Xr, const double* hist,
                               int64_t stride, int n_hist, hipStream_t st);
 hipError_t launch_lp_grad(int64_t n, const float* X, const float* y, const float* prm, float pos_w, float* partial,

@@ -44,6 +44,8 @@ struct PairParams {
     int* flags = nullptr;   // status flags (FLAG_COAT_UNPLACED)
     // position of (slot s, node k) in the node contribution lists: nqpos[4 s + k]
     const int* nqpos = nullptr;
+    // device Newton control state word (nullable): the kernel does nothing unless it is 0
+    const double* stop = nullptr;
 };
 
 struct NodeLists {
@@ -98,6 +100,7 @@ struct GatherArgs {
     const int* act_part;   // per-workgroup active counts of the pair kernel
     int n_act_part;
     int* n_active;
+    const double* stop;    // device Newton control state word (nullable): skip once the call ended
 };
 
 }  // namespace xk

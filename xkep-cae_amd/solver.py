@@ -37,6 +37,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import scipy.sparse as sp
 
+from . import _lib as L
+
 # device status word bits (XK_FLAG_* of include/xkamd.h)
 FLAG_NONFINITE, FLAG_DEGENERATE, FLAG_SINGULAR = 1, 2, 4
 
@@ -419,13 +421,16 @@ class DeviceBackend:
 
     def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
                  k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False,
-                 use_graph: bool = False, core_radii=None):
+                 use_graph: bool = False, core_radii=None, device_control: bool = True):
         from .assembly import StrandAssembly
         from .time_integration import GeneralizedAlphaProcess
 
         self.ctx = ctx
         self.torch = ctx.torch
         self.device_newton = bool(device_solve)
+        # device Newton attempts: the driver's tests on the device (attempts enqueued
+        # back to back) or on the host (a host read after each attempt's stages)
+        self.device_control = bool(device_control)
         self.solve_residuals = []   # device solve: relative residual of every linear solve
         self.solve_tol = 1e-11      # device solve: target relative residual (SuperLU reaches about 1e-12)
         self.max_refine = 16
@@ -442,6 +447,7 @@ class DeviceBackend:
         self.flags = 0              # every XK_FLAG_* bit the device status word has reported
         self.solve_fail_tol = 1e-6  # a solve whose refined residual stays above this counts as failed
         self.eager_resolves = 0     # graph attempts whose fixed refinement missed solve_tol, re-solved eagerly
+        self.control_reasons = []   # device-controlled calls: XK_NCR_* bits of each call's end
 
     def _read_flags(self) -> int:
         """Read (and clear) the device status word, keep its bits, and raise as
@@ -508,6 +514,8 @@ class DeviceBackend:
         refinement steps) + line search + update.  ``u`` (host) is updated in
         place at the end."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
+        if self.device_control and not asm.coating:
+            return self._newton_controlled(u, u_ref, f_ext, dt, f_ext_ref_norm, dynamic_ref, cfg)
         if dt != self._dt:
             asm.set_dt(dt)
             self._dt = dt
@@ -607,6 +615,143 @@ class DeviceBackend:
                     self.graph_attempts_captured += 1
         u[:] = ud.cpu().numpy()
         return out
+
+    def _newton_controlled(self, u, u_ref, f_ext, dt, f_ext_ref_norm, dynamic_ref, cfg) -> NewtonOutcome:
+        """``newton_dynamic`` with its tests on the device (xk_newton_control_*).
+        Each attempt -- assembly, pair force, force and divergence tests,
+        linear solve (``graph_refine`` fixed refinement steps), solve check,
+        line search, update, displacement / energy tests and the attempt limit
+        -- is enqueued without a host read; the refinement of the solve
+        stops on the device (up to ``graph_refine`` steps); the host keeps one
+        attempt queued ahead and reads each attempt's control record
+        afterwards from pinned memory, so the device never waits on it.
+        Every attempt kernel skips once the call has ended, so the attempt
+        enqueued past the end changes nothing.  An attempt whose solve missed
+        ``solve_tol`` is handed to the host (XK_NC_ASSIST), which re-solves it
+        eagerly as the host-tested path does.  With ``use_graph`` the attempts
+        after each increment's first replay one captured graph.  The coating's
+        per-assembly pattern probe needs the host, so a coated call takes the
+        host-tested path."""
+        torch, asm, ctx = self.torch, self.asm, self.ctx
+        if dt != self._dt:
+            asm.set_dt(dt)
+            self._dt = dt
+        asm.set_u(u)
+        asm.set_u_ref(u_ref)
+        asm.set_f_ext(f_ext)
+        ud = asm.u
+        du = torch.empty_like(ud)
+        status = torch.zeros(24, dtype=torch.float64, device=ctx.dev)
+        ls_steps = cfg.line_search_max_steps if cfg.use_line_search else 0
+        cap = int(cfg.max_nr_attempts)
+        ctl = torch.zeros(L.NC_HIST + L.NC_HSTRIDE * cap, dtype=torch.float64, device=ctx.dev)
+        rel = ctl[L.NC_REL:L.NC_REL + 1]   # the solve writes its residual into the record
+        pin = torch.zeros((2, L.NC_HIST), dtype=torch.float64, pin_memory=True)
+        evs = [torch.cuda.Event(), torch.cuda.Event()]
+        graph_ok = self.use_graph and asm.prescreen is None
+        graph = None
+
+        def solve(refine, tol):
+            ctx.linear_solve(ud, asm.fixed_mask, asm.R, du, asm._dyn, max_refine=refine, tol=tol, res_rel=rel)
+
+        def update():
+            ctx.newton_update(ud, du, asm.R, self._f, asm.f_ext, asm.fixed_mask, asm.res2, status,
+                              line_search_steps=ls_steps, diverge_factor=3.0, du_norm_cap=cfg.du_norm_cap)
+
+        def attempt():
+            asm.assemble()
+            ctx.pairs_gather(f=self._f)
+            ctx.newton_control_force(asm.res2, asm.n_active)
+            solve(self.graph_refine, self.solve_tol)   # refinement stopped on the device
+            update()
+
+        def post(slot):
+            pin[slot].copy_(ctl[:L.NC_HIST], non_blocking=True)
+            evs[slot].record()
+
+        self._read_flags()   # flags of the increment's set-up (raises on a degenerate element)
+        ctx.newton_control_begin(ctl, hist_cap=cap, tol_force=cfg.tol_force, tol_disp=cfg.tol_disp,
+                                 f_ext_ref_norm=f_ext_ref_norm, dynamic_ref=dynamic_ref,
+                                 divergence_window=cfg.divergence_window, max_attempts=cap,
+                                 solve_tol=self.solve_tol)
+        ctx.newton_control_attach(ctl)
+        try:
+            k = 0               # next attempt to enqueue
+            pending = deque()   # attempts enqueued and not yet read
+            while True:
+                if k < cap:
+                    if graph is not None:
+                        graph.replay()
+                    else:
+                        attempt()
+                    post(k % 2)
+                    pending.append(k)
+                    k += 1
+                    if graph is None and graph_ok:
+                        graph = self._capture_controlled(attempt)   # recorded, not run
+                        graph_ok = graph is not None
+                        self.graph_attempts_captured += graph is not None
+                    if len(pending) < 2 and k < cap:
+                        continue   # keep one attempt queued ahead of the reads
+                if not pending:
+                    break
+                p = pending.popleft()
+                evs[p % 2].synchronize()
+                state = int(pin[p % 2, L.NC_STATE])
+                if state == L.NC_RUNNING:
+                    continue
+                if state != L.NC_ASSIST:
+                    break
+                # the attempt's fixed refinement missed solve_tol: the attempt enqueued
+                # after it was a no-op; re-solve this one eagerly, then update
+                torch.cuda.synchronize()
+                pending.clear()
+                k = int(pin[p % 2, L.NC_ATT])
+                self.eager_resolves += 1
+                ctx.newton_control_attach(None)   # the host's solve: norms read per step, LU fallback
+                solve(self.max_refine, self.solve_tol)
+                ctx.newton_control_attach(ctl)
+                rel_v = float(rel.item())
+                fl = self._read_flags()
+                if fl & FLAG_SINGULAR or not rel_v <= self.solve_fail_tol:
+                    # zero pivot or no accurate solution: the solve failed (LinearSolveProcess success=False)
+                    ctx.newton_control_set(L.NC_STOPPED, L.NCR_SINGULAR if fl & FLAG_SINGULAR else L.NCR_SOLVE)
+                    break
+                ctx.newton_control_set(L.NC_RUNNING)
+                update()
+                post(p % 2)
+                pending.append(p)
+                graph = None   # a fallback to partial pivoting changes the captured solve
+                graph_ok = self.use_graph and asm.prescreen is None
+            torch.cuda.synchronize()
+        finally:
+            ctx.newton_control_attach(None)
+        c = ctl.cpu().numpy()
+        self._read_flags()   # raises on a degenerate element, as the reference does
+        att = int(c[L.NC_ATT])
+        h = c[L.NC_HIST:L.NC_HIST + L.NC_HSTRIDE * att].reshape(att, L.NC_HSTRIDE)
+        state = int(c[L.NC_STATE])
+        out = NewtonOutcome(state == L.NC_CONVERGED, att, int(c[L.NC_NACT]), state == L.NC_DIVERGED, self._f,
+                            [float(x) for x in h[:, 0]], [int(x) for x in h[:, 1]])
+        self.solve_residuals.extend(np.float64(x) for x in h[:, 2] if not np.isnan(x))
+        self.control_reasons.append(int(c[L.NC_REASON]))
+        u[:] = ud.cpu().numpy()
+        return out
+
+    def _capture_controlled(self, attempt):
+        """One graph of a whole device-controlled attempt, recorded without being
+        run (the eager attempt before it has set the solve up); None if capture
+        is not possible, the reason kept in ``graph_error``."""
+        torch = self.torch
+        try:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                attempt()
+        except Exception as e:  # noqa: BLE001 - any capture failure falls back to eager attempts
+            self.graph_error = repr(e)
+            torch.cuda.synchronize()
+            return None
+        return g
 
     def _capture_attempt(self, solve, update):
         """G1 (assembly + pair force), G2 (solve with graph_refine fixed

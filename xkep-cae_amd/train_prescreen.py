@@ -32,10 +32,11 @@ TRAIN_RUNS = [(f, nw) for f in (60.0, 80.0, 100.0, 120.0, 140.0) for nw in (7, 1
 
 
 def run_case(case, *, precision: str = "fp64", net=None, log: bool = False, device_solve: bool = False,
-             use_graph: bool = False):
+             use_graph: bool = False, device_control: bool = True):
     """Solve one squeeze case on the device; returns (SolverResult, backend).
     device_solve / use_graph: the Newton attempts on the device (linear solve,
-    line search, update), after each increment's first as HIP-graph replays."""
+    line search, update), after each increment's first as HIP-graph replays;
+    device_control: their convergence and divergence tests on the device too."""
     r = float(case["radii"][0])
     sec = C.circle_section(r, W.E_STEEL, W.NU_STEEL)
     ctx = C.DeviceContext(case["X"], case["conn"], sec, radii=case["radii"], precision=precision)
@@ -45,7 +46,7 @@ def run_case(case, *, precision: str = "fp64", net=None, log: bool = False, devi
                            elem_layer_map=W.layer_map(case["layer"]))
     b = S.DeviceBackend(ctx, M_diag=Md, rho_inf=case["rho_inf"], k_pen=case["k_pen"], smoothing_delta=case["sd"],
                         mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg,
-                        device_solve=device_solve, use_graph=use_graph)
+                        device_solve=device_solve, use_graph=use_graph, device_control=device_control)
     if net is not None:
         b.use_prescreen(net, elem_layer=case["layer"])
     if log:
