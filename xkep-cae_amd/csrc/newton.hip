@@ -51,26 +51,72 @@ __device__ __forceinline__ double nt_block_sum(double v, double* red) {
     return s;
 }
 
-// ||f_int - f_contact - f_ext|| over free dofs -> status[NS_NORM0 + k] (inf if non-finite)
-__global__ __launch_bounds__(NR) void nt_trial_norm_kernel(const double* __restrict__ stop, int n,
-                                                           const double* __restrict__ f_int,
-                                                           const double* __restrict__ f_contact,
-                                                           const double* __restrict__ f_ext,
-                                                           const uint8_t* __restrict__ fixed, int k,
-                                                           double* __restrict__ status) {
-    if (!nc_go(stop)) return;
-    __shared__ double red[NR];
-    double s = 0.0;
-    int bad = 0;
-    for (int i = threadIdx.x; i < n; i += NR) {
-        double r = (f_int[i] - f_contact[i]) - f_ext[i];
-        if (fixed[i]) r = 0.0;
-        if (!isfinite(r)) bad = 1;
-        s = fma(r, r, s);
+// Norms over the ndof vectors in two launches: NP_BLOCK lanes per workgroup,
+// each workgroup's partial sums (in lane order, then a fixed tree) to part[],
+// then one workgroup folding the partials in index order -- deterministic,
+// no atomics, and spread over the chip instead of one workgroup.
+constexpr int NP_BLOCK = 256;
+constexpr int NP_PER_LANE = 4;
+static int np_blocks(int n) { return std::max(1, (n + NP_BLOCK * NP_PER_LANE - 1) / (NP_BLOCK * NP_PER_LANE)); }
+
+// sums of v[0..NV) over the workgroup (lane order, then a fixed tree), valid in lane 0
+template <int NV>
+__device__ __forceinline__ void np_block_sum(double (&v)[NV]) {
+    __shared__ double red[NV][NP_BLOCK];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = NP_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
     }
-    const double tot = nt_block_sum(s, red);
-    const double nb = nt_block_sum((double)bad, red);
-    if (threadIdx.x == 0) status[NS_NORM0 + k] = nb > 0.0 ? HUGE_VAL : sqrt(tot);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = red[q][0];
+}
+
+// partials part[q * gridDim + block] of sum r^2 (q = 0) and #(non-finite r)
+// (q = 1), r = f_int - f_contact - f_ext on free dofs
+__global__ __launch_bounds__(NP_BLOCK) void nt_trial_part_kernel(const double* __restrict__ stop, int n,
+                                                                 const double* __restrict__ f_int,
+                                                                 const double* __restrict__ f_contact,
+                                                                 const double* __restrict__ f_ext,
+                                                                 const uint8_t* __restrict__ fixed,
+                                                                 double* __restrict__ part) {
+    if (!nc_go(stop)) return;
+    double v[2] = {0.0, 0.0};
+    const int i0 = blockIdx.x * NP_BLOCK * NP_PER_LANE + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NP_PER_LANE; ++q) {
+        const int i = i0 + q * NP_BLOCK;
+        if (i < n) {
+            double r = (f_int[i] - f_contact[i]) - f_ext[i];
+            if (fixed[i]) r = 0.0;
+            if (!isfinite(r)) v[1] += 1.0;
+            v[0] = fma(r, r, v[0]);
+        }
+    }
+    np_block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// ||r|| of candidate k from the nb partials -> status[NS_NORM0 + k] (inf if
+// any entry is non-finite)
+__global__ __launch_bounds__(NP_BLOCK) void nt_trial_norm_kernel(const double* __restrict__ stop, int nb,
+                                                                 const double* __restrict__ part, int k,
+                                                                 double* __restrict__ status) {
+    if (!nc_go(stop)) return;
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += NP_BLOCK) {
+        v[0] += part[b];
+        v[1] += part[nb + b];
+    }
+    np_block_sum<2>(v);
+    if (threadIdx.x == 0) status[NS_NORM0 + k] = v[1] > 0.0 ? HUGE_VAL : sqrt(v[0]);
 }
 
 // alpha from the candidates' norms, then the cap on ||alpha du||
@@ -122,22 +168,44 @@ __global__ void nt_update_kernel(const double* __restrict__ stop, int n, double*
     u[i] += d;
 }
 
-__global__ __launch_bounds__(NR) void nt_norms_kernel(const double* __restrict__ stop, int n,
-                                                      const double* __restrict__ u, const double* __restrict__ du,
-                                                      const double* __restrict__ R, double* __restrict__ status) {
+// partials of ||du||^2, ||u||^2 and du . R
+__global__ __launch_bounds__(NP_BLOCK) void nt_norms_part_kernel(const double* __restrict__ stop, int n,
+                                                                 const double* __restrict__ u,
+                                                                 const double* __restrict__ du,
+                                                                 const double* __restrict__ R,
+                                                                 double* __restrict__ part) {
     if (!nc_go(stop)) return;
-    __shared__ double red[NR];
-    double sd = 0.0, su = 0.0, se = 0.0;
-    for (int i = threadIdx.x; i < n; i += NR) {
-        sd = fma(du[i], du[i], sd);
-        su = fma(u[i], u[i], su);
-        se = fma(du[i], R[i], se);
+    double v[3] = {0.0, 0.0, 0.0};
+    const int i0 = blockIdx.x * NP_BLOCK * NP_PER_LANE + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NP_PER_LANE; ++q) {
+        const int i = i0 + q * NP_BLOCK;
+        if (i < n) {
+            v[0] = fma(du[i], du[i], v[0]);
+            v[1] = fma(u[i], u[i], v[1]);
+            v[2] = fma(du[i], R[i], v[2]);
+        }
     }
-    const double a = nt_block_sum(sd, red), b = nt_block_sum(su, red), c = nt_block_sum(se, red);
+    np_block_sum<3>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) part[q * gridDim.x + blockIdx.x] = v[q];
+}
+
+// ||du||, ||u||, |du . R| from the nb partials
+__global__ __launch_bounds__(NP_BLOCK) void nt_norms_kernel(const double* __restrict__ stop, int nb,
+                                                            const double* __restrict__ part,
+                                                            double* __restrict__ status) {
+    if (!nc_go(stop)) return;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += NP_BLOCK)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[q] += part[q * nb + b];
+    np_block_sum<3>(v);
     if (threadIdx.x == 0) {
-        status[NS_DU] = sqrt(a);
-        status[NS_U] = sqrt(b);
-        status[NS_ENERGY] = fabs(c);
+        status[NS_DU] = sqrt(v[0]);
+        status[NS_U] = sqrt(v[1]);
+        status[NS_ENERGY] = fabs(v[2]);
     }
 }
 
@@ -318,10 +386,13 @@ int Ctx::newton_update(double* u, double* du, const double* R, const double* f_c
         return -1;
     }
     const int n = 6 * n_nodes;
-    if (nt_try.ensure(sizeof(double) * (size_t)n) != hipSuccess) {
+    const int nb = np_blocks(n);
+    if (nt_try.ensure(sizeof(double) * (size_t)n) != hipSuccess ||
+        nt_part.ensure(sizeof(double) * 3 * (size_t)nb) != hipSuccess) {
         set_error("xk_newton_update: out of device memory");
         return -1;
     }
+    double* part = (double*)nt_part.p;
     const double* stop = nstop;
     if (nstop) hipLaunchKernelGGL(nc_solve_kernel, dim3(1), dim3(64), 0, st, nstop, (const int*)d_flags.p);
     double* ut = (double*)nt_try.p;
@@ -334,15 +405,18 @@ int Ctx::newton_update(double* u, double* du, const double* R, const double* f_c
             set_error(std::string("xk_newton_update: ") + hipGetErrorString(e));
             return -1;
         }
-        hipLaunchKernelGGL(nt_trial_norm_kernel, dim3(1), dim3(NR), 0, st, stop, n, (const double*)d_fint.p,
-                           f_contact, f_ext, fixed, k, status);
+        hipLaunchKernelGGL(nt_trial_part_kernel, dim3(nb), dim3(NP_BLOCK), 0, st, stop, n, (const double*)d_fint.p,
+                           f_contact, f_ext, fixed, part);
+        hipLaunchKernelGGL(nt_trial_norm_kernel, dim3(1), dim3(NP_BLOCK), 0, st, stop, nb, (const double*)part, k,
+                           status);
     }
     hipLaunchKernelGGL(nt_select_kernel, dim3(1), dim3(NR), 0, st, stop, n, max_steps, diverge_factor, cap,
                        res_norm2, (const double*)u, (const double*)du, status);
     hipLaunchKernelGGL(nt_update_kernel, dim3(nt_grid(n)), dim3(256), 0, st, stop, n, u, du,
                        (const double*)status);
-    hipLaunchKernelGGL(nt_norms_kernel, dim3(1), dim3(NR), 0, st, stop, n, (const double*)u, (const double*)du, R,
-                       status);
+    hipLaunchKernelGGL(nt_norms_part_kernel, dim3(nb), dim3(NP_BLOCK), 0, st, stop, n, (const double*)u,
+                       (const double*)du, R, part);
+    hipLaunchKernelGGL(nt_norms_kernel, dim3(1), dim3(NP_BLOCK), 0, st, stop, nb, (const double*)part, status);
     if (nstop) hipLaunchKernelGGL(nc_after_kernel, dim3(1), dim3(64), 0, st, nstop, (const double*)status);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -116,89 +116,91 @@ __global__ void sv_extract_kernel(const double* __restrict__ stop, int n_nodes, 
 }
 
 // Block LU along each chain: S_n0 = D_n0, L_n = F_{n-1} S_{n-1}^-1,
-// S_n = D_n - L_n E_{n-1}; stores L_n and S_n^-1.  One workgroup per chain.
+// S_n = D_n - L_n E_{n-1}; stores L_n and S_n^-1 (Gauss-Jordan with partial
+// pivoting, the first of the largest |entries| on ties).  One wavefront per
+// chain: lane 6 i + j holds entry (i, j) of the node's 6x6 blocks, the block
+// products and the elimination exchange entries by lane shuffles (no LDS, no
+// barriers), and the next node's three entries per lane are loaded before
+// the current node's elimination so that the loads overlap it.
 __global__ __launch_bounds__(64) void sv_factor_kernel(const double* __restrict__ stop, const int* __restrict__ cstart,
                                                        const double* __restrict__ Ab, double* __restrict__ Lf,
                                                        double* __restrict__ Sinv, int* __restrict__ flags) {
     if (!nc_go(stop)) return;
     const int c = blockIdx.x;
     const int n0 = cstart[c], n1 = cstart[c + 1];
-    __shared__ double M[6][13];
-    __shared__ double Pv[36];
-    __shared__ double T[36];
-    __shared__ int piv;
-    const int l = threadIdx.x, i = l / 6, j = l - 6 * (l / 6);
+    const int l = threadIdx.x;
+    const bool act = l < 36;
+    const int i = act ? l / 6 : 0, j = act ? l - 6 * (l / 6) : 0;
+    // node n's row i: [F (node n - 1) | D (node n) | E (node n + 1)], entry j of each
+    auto load = [&](int n, double& D, double& F, double& E) {
+        const double* a = Ab + 108 * (size_t)n + 18 * i;
+        F = a[j];
+        D = a[6 + j];
+        E = a[12 + j];
+    };
+    double D, F, E, Dn = 0.0, Fn = 0.0, En = 0.0;
+    load(n0, D, F, E);
+    double P = 0.0, Ep = 0.0;   // S_{n-1}^-1 and E_{n-1}, entry (i, j)
     bool singular = false;
     for (int n = n0; n < n1; ++n) {
-        const double* a = Ab + 108 * (size_t)n;
-        double s = 0.0;
-        if (l < 36) {
-            s = a[18 * i + 6 + j];
-            if (n > n0) {
-                double t = 0.0;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) t = fma(a[18 * i + k], Pv[6 * k + j], t);
-                T[l] = t;
-            }
-        }
-        __syncthreads();
-        if (n > n0 && l < 36) {
-            const double* e = Ab + 108 * (size_t)(n - 1);
+        if (n + 1 < n1) load(n + 1, Dn, Fn, En);
+        double s = D;
+        if (n > n0) {
             double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) t = fma(T[6 * i + k], e[18 * k + 12 + j], t);
-            s -= t;
-            Lf[36 * (size_t)n + l] = T[l];
+            for (int k = 0; k < 6; ++k) t = fma(__shfl(F, 6 * i + k, 64), __shfl(P, 6 * k + j, 64), t);
+            double u = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) u = fma(__shfl(t, 6 * i + k, 64), __shfl(Ep, 6 * k + j, 64), u);
+            s -= u;
+            if (act) Lf[36 * (size_t)n + l] = t;
         }
-        if (l < 36) {
-            M[i][j] = s;
-            M[i][6 + j] = i == j ? 1.0 : 0.0;
-        }
-        __syncthreads();
+        // Gauss-Jordan on [S | I]: m holds S's entry, y the identity's
+        double m = s, y = i == j ? 1.0 : 0.0;
+#pragma unroll
         for (int k = 0; k < 6; ++k) {
-            if (l == 0) {
-                int p = k;
-                double best = fabs(M[k][k]);
-                for (int r = k + 1; r < 6; ++r)
-                    if (fabs(M[r][k]) > best) {
-                        best = fabs(M[r][k]);
-                        p = r;
-                    }
-                piv = p;
+            int p = k;
+            double best = fabs(__shfl(m, 7 * k, 64));
+#pragma unroll
+            for (int r = k + 1; r < 6; ++r) {
+                const double v = fabs(__shfl(m, 6 * r + k, 64));
+                if (v > best) {
+                    best = v;
+                    p = r;
+                }
             }
-            __syncthreads();
-            const int p = piv;
-            if (p != k && l < 12) {
-                const double tmp = M[k][l];
-                M[k][l] = M[p][l];
-                M[p][l] = tmp;
+            const double mk = __shfl(m, 6 * k + j, 64), mp = __shfl(m, 6 * p + j, 64);
+            const double yk = __shfl(y, 6 * k + j, 64), yp = __shfl(y, 6 * p + j, 64);
+            if (i == k) {
+                m = mp;
+                y = yp;
+            } else if (i == p) {
+                m = mk;
+                y = yk;
             }
-            __syncthreads();
-            const double d = M[k][k];
+            const double d = __shfl(m, 7 * k, 64);
             if (d == 0.0) singular = true;
-            __syncthreads();
-            if (l < 12) M[k][l] = M[k][l] / d;
-            __syncthreads();
-            double f0 = 0.0, f1 = 0.0;
-            const int t0 = l, t1 = l + 64;
-            if (t0 < 72) f0 = M[t0 / 12][k];
-            if (t1 < 72) f1 = M[t1 / 12][k];
-            __syncthreads();
-            if (t0 < 72 && t0 / 12 != k) M[t0 / 12][t0 % 12] -= f0 * M[k][t0 % 12];
-            if (t1 < 72 && t1 / 12 != k) M[t1 / 12][t1 % 12] -= f1 * M[k][t1 % 12];
-            __syncthreads();
+            if (i == k) {
+                m = m / d;
+                y = y / d;
+            }
+            const double f = __shfl(m, 6 * i + k, 64);
+            const double pm = __shfl(m, 6 * k + j, 64), py = __shfl(y, 6 * k + j, 64);
+            if (i != k) {
+                m -= f * pm;
+                y -= f * py;
+            }
         }
-        if (l < 36) {
-            const double v = M[i][6 + j];
-            Sinv[36 * (size_t)n + l] = v;
-            Pv[l] = v;
-        }
-        __syncthreads();
+        if (act) Sinv[36 * (size_t)n + l] = y;
+        P = y;
+        Ep = E;
+        D = Dn;
+        F = Fn;
+        E = En;
     }
     if (singular && l == 0) atomicOr(flags, XK_FLAG_SINGULAR);
 }
 
-// slot -> pair
 __global__ void sv_slist_kernel(const double* __restrict__ stop, int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
     if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

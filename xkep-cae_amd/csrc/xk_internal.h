@@ -162,6 +162,7 @@ struct Ctx {
                      double* du, int refine, double tol, double* res_rel, int* n_refine, hipStream_t st);
     void release_solver();
     DBuf nt_try;   // trial displacement of the device line search
+    DBuf nt_part;  // per-workgroup partial sums of the line search's norms
     // device Newton control record of the call in progress (xk_newton_control_attach;
     // null: none): the kernels of an attempt read its state word and skip once the call ended
     double* nstop = nullptr;
@@ -194,6 +195,7 @@ struct Ctx {
         d_cstart.release();
         d_node_chain.release();
         nt_try.release();
+        nt_part.release();
         release_solver();
         d_bbeam.release();
         pat_tmp.release();
