@@ -169,6 +169,33 @@ def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fix
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_device_control_hands_missed_solves_to_the_host(gpu, use_graph):
+    """With an unreachable solve tolerance every attempt's solve ends its
+    device refinement at the rounding floor, above the tolerance, and stops in
+    the assist state: the host re-solves it eagerly (refinement to the floor,
+    accepted below 1e-6) and the call goes on; the trajectory is still the
+    reference's (friction, load 60)."""
+    g, prm, cfg = case("newton_7wire_mu03_f60.npz")
+    S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
+    r = float(g["radii"][0])
+    ctx = C.DeviceContext(g["X"], g["conn"], C.circle_section(r, 200e3, 0.3), radii=g["radii"], precision="fp64")
+    kcfg = Kc.ContactConfig(mu=prm["mu"], k_t_ratio=prm["k_t_ratio"], exclude_same_layer=True,
+                            elem_layer_map=wl.layer_map(g["layer"]))
+    b = S.DeviceBackend(ctx, M_diag=g["M_diag"], rho_inf=prm["rho_inf"], k_pen=prm["k_pen"],
+                        smoothing_delta=prm["smoothing_delta"], mu=prm["mu"], k_t_ratio=prm["k_t_ratio"],
+                        fixed_dofs=g["fixed"], config=kcfg, device_solve=True, use_graph=use_graph)
+    b.solve_tol = 1e-30
+    print(f"assisted device control, graphs={use_graph}", flush=True)
+    res = S.ContactFrictionSolver(b, fixed_dofs=g["fixed"], f_ext_total=g["f_ext"], t_total=prm["t_total"],
+                                  config=cfg).solve()
+    check_trajectory(g, res, "fp64")
+    solves = sum(1 for x in b.solve_residuals)
+    assert b.eager_resolves == solves > 0, (b.eager_resolves, solves)
+    assert max(float(x) for x in b.solve_residuals) < 1e-10
+
+
+@pytest.mark.gpu
 def test_mixed_and_fp64_take_the_same_newton_trajectory_at_91_wires(gpu):
     """The 91-wire strand of the benchmark (r = 0.4, L = 80, 150 elements per
     wire, lay angle 12 deg) squeezed by 0.1 per outer wire with Coulomb

@@ -435,7 +435,9 @@ class DeviceBackend:
         self.solve_tol = 1e-11      # device solve: target relative residual (SuperLU reaches about 1e-12)
         self.max_refine = 16
         self.use_graph = bool(device_solve and use_graph)   # attempts after the first as HIP-graph replays
-        self.graph_refine = 3       # fixed refinement steps inside the captured solve
+        self.graph_refine = 3       # fixed refinement steps inside the captured solve (host tests)
+        self.ctl_refine = 2         # most refinement steps of a device-tested attempt (strands need 2);
+                                    # a solve they leave above solve_tol is re-solved by the host
         self.graph_attempts_captured = 0
         self.graph_error = None
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
@@ -622,7 +624,7 @@ class DeviceBackend:
         linear solve (``graph_refine`` fixed refinement steps), solve check,
         line search, update, displacement / energy tests and the attempt limit
         -- is enqueued without a host read; the refinement of the solve
-        stops on the device (up to ``graph_refine`` steps); the host keeps one
+        stops on the device (up to ``ctl_refine`` steps); the host keeps one
         attempt queued ahead and reads each attempt's control record
         afterwards from pinned memory, so the device never waits on it.
         Every attempt kernel skips once the call has ended, so the attempt
@@ -662,7 +664,7 @@ class DeviceBackend:
             asm.assemble()
             ctx.pairs_gather(f=self._f)
             ctx.newton_control_force(asm.res2, asm.n_active)
-            solve(self.graph_refine, self.solve_tol)   # refinement stopped on the device
+            solve(self.ctl_refine, self.solve_tol)   # refinement stopped on the device
             update()
 
         def post(slot):
