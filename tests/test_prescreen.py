@@ -210,3 +210,21 @@ def test_prescreen_at_1000_wires_is_the_full_path(gpu):
         assert gpu.equal(R0, R1) and gpu.equal(K0, K1) and int(n0.item()) == int(n1.item())
     for k in PAIR_FIELDS:
         assert np.array_equal(lp_state[k], full_state[k]), k
+
+
+def test_prescreen_under_device_newton_control(gpu):
+    """The committed network on two frictional squeezes (mu = 0.3) solved with
+    the Newton attempts and the driver's tests on the device (attempts enqueued
+    back to back, the attempt after a call's end a no-op): the trajectory, the
+    final u and every pair's final state equal the full path's under the same
+    device control bit for bit, so the no-op attempts record no assembly for
+    the verification pass's replay."""
+    P, W, TP = pkg("prescreen"), pkg("workloads"), pkg("train_prescreen")
+    net = P.PrescreenNet.load()
+    for case in (W.squeeze_case(60.0, mu=0.3), W.squeeze_case(100.0, n_wires=19, mu=0.3)):
+        print(f"prescreen under device control, {len(case['conn'])} elements", flush=True)
+        full, bf = TP.run_case(case, device_solve=True)
+        lp, bl = TP.run_case(case, net=net, device_solve=True)
+        same_trajectory(full, lp)
+        same_pair_state(bf.ctx, bl.ctx)
+        assert sum(w for w, _ in lp.watch) < sum(n for _, n in lp.watch)

@@ -725,8 +725,8 @@ class DeviceBackend:
                 pending.append(p)
                 graph = None   # a fallback to partial pivoting changes the captured solve
                 graph_ok = self.use_graph and asm.prescreen is None
-            torch.cuda.synchronize()
         finally:
+            torch.cuda.synchronize()   # no enqueued attempt may outlive ctl, also on an exception
             ctx.newton_control_attach(None)
         c = ctl.cpu().numpy()
         self._read_flags()   # raises on a degenerate element, as the reference does
