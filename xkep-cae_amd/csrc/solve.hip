@@ -51,7 +51,8 @@ enum SolveBuf {
 // belongs to has ended: a solve enqueued past the end of a call is a no-op.
 
 // 32-bit pivots and info of one LU widened into the 64-bit buffers
-__global__ void sv_widen_kernel(const double* __restrict__ stop, int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
+__global__ void sv_widen_kernel(const double* __restrict__ stop,
+                                int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
                                 const int* __restrict__ info32, int64_t* __restrict__ info64) {
     if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,7 +77,8 @@ __device__ __forceinline__ int sv_find(const int* __restrict__ gptr, const int* 
 // Per node n, its 6 scalar rows over the columns of nodes n-1, n, n+1
 // (6 x 18, FP64), masked: a fixed row or column becomes the identity's.
 template <class KT>
-__global__ void sv_extract_kernel(const double* __restrict__ stop, int n_nodes, const int8_t* __restrict__ nflag, const int* __restrict__ gptr,
+__global__ void sv_extract_kernel(const double* __restrict__ stop,
+                                  int n_nodes, const int8_t* __restrict__ nflag, const int* __restrict__ gptr,
                                   const int* __restrict__ gcol, const KT* __restrict__ Kb,
                                   const uint8_t* __restrict__ fixed, double* __restrict__ Ab) {
     if (!nc_go(stop)) return;
@@ -201,13 +203,15 @@ __global__ __launch_bounds__(64) void sv_factor_kernel(const double* __restrict_
     if (singular && l == 0) atomicOr(flags, XK_FLAG_SINGULAR);
 }
 
-__global__ void sv_slist_kernel(const double* __restrict__ stop, int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
+__global__ void sv_slist_kernel(const double* __restrict__ stop,
+                                int64_t n_pairs, const int* __restrict__ sinc, int* __restrict__ slist) {
     if (!nc_go(stop)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_pairs && sinc[i] >= 0) slist[sinc[i]] = (int)i;
 }
 
-__global__ void sv_slot_nodes_kernel(const double* __restrict__ stop, int ns, const int* __restrict__ slist, PairSoA P, int4* __restrict__ sn) {
+__global__ void sv_slot_nodes_kernel(const double* __restrict__ stop,
+                                     int ns, const int* __restrict__ slist, PairSoA P, int4* __restrict__ sn) {
     if (!nc_go(stop)) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ns) return;
@@ -217,7 +221,8 @@ __global__ void sv_slot_nodes_kernel(const double* __restrict__ stop, int ns, co
 
 // the pair's staged tangent record -> G (3x3) and c (4) in FP64
 template <class GT>
-__global__ void sv_slot_pack_kernel(const double* __restrict__ stop, int ns, const GT* __restrict__ sg, double* __restrict__ sG,
+__global__ void sv_slot_pack_kernel(const double* __restrict__ stop,
+                                    int ns, const GT* __restrict__ sg, double* __restrict__ sG,
                                     double* __restrict__ sc) {
     if (!nc_go(stop)) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(256) void sv_compliance_kernel(const double* __rest
 
 // Capacitance block (i, q) of one component, column-major with ld = 3 Pc:
 // I delta_iq + G_i sum_{k,l} c_ik c_ql M_k C[node_k(i), node_l(q)] M_l.
-__global__ void sv_capacitance_kernel(const double* __restrict__ stop, int Pc, const int* __restrict__ cslot, const int4* __restrict__ sn,
+__global__ void sv_capacitance_kernel(const double* __restrict__ stop,
+                                      int Pc, const int* __restrict__ cslot, const int4* __restrict__ sn,
                                       const double* __restrict__ sG, const double* __restrict__ sc,
                                       const int* __restrict__ node_chain, const int* __restrict__ cn,
                                       const int* __restrict__ ch_cn0, const int* __restrict__ ch_m,
@@ -352,7 +358,8 @@ __global__ void sv_capacitance_kernel(const double* __restrict__ stop, int Pc, c
         }
 }
 
-__global__ void sv_info_kernel(const double* __restrict__ stop, int nc, const int64_t* __restrict__ info, int* __restrict__ flags) {
+__global__ void sv_info_kernel(const double* __restrict__ stop,
+                               int nc, const int64_t* __restrict__ info, int* __restrict__ flags) {
     if (!nc_go(stop)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < nc && info[c] != 0) atomicOr(flags, XK_FLAG_SINGULAR);
@@ -395,7 +402,8 @@ __device__ __forceinline__ double sv_readlane(double v, int lane) {
 // FULL: the panel holds LU_NB columns (every panel but a last partial one):
 // the sub-block bounds are constants and the unrolled steps carry no guards.
 template <bool FULL>
-__global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restrict__ stop, int n, int j0, int n_row_wg, double* __restrict__ A,
+__global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restrict__ stop,
+                             int n, int j0, int n_row_wg, double* __restrict__ A,
                                                           int64_t* __restrict__ info) {
     if (!nc_go(stop)) return;
     __shared__ double D[LU_NB][LU_NB + 1];
@@ -682,7 +690,8 @@ __global__ void sv_ipiv_identity_kernel(const double* __restrict__ stop, int64_t
 // x <- A~_b^-1 x along every chain: 8 lanes per chain (lanes 0..5 hold the
 // node's six components), 8 chains per wavefront; rows of the 6x6 blocks are
 // combined across the chain's lanes by shuffles.
-__global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __restrict__ stop, int n_chains, const int* __restrict__ cstart,
+__global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __restrict__ stop,
+                             int n_chains, const int* __restrict__ cstart,
                                                             const double* __restrict__ Ab,
                                                             const double* __restrict__ Lf,
                                                             const double* __restrict__ Sinv, double* __restrict__ x) {
@@ -792,7 +801,8 @@ __device__ __forceinline__ double sv_block_dot(double acc, const double* __restr
 // needs no host round trip (and can be captured in a graph).
 constexpr int PERM_MAX = 16384;
 
-__global__ __launch_bounds__(256) void sv_perm_kernel(const double* __restrict__ stop, int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
+__global__ __launch_bounds__(256) void sv_perm_kernel(const double* __restrict__ stop,
+                             int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
                                                       int* __restrict__ flags) {
     if (!nc_go(stop)) return;
     __shared__ int p[PERM_MAX];
@@ -820,7 +830,8 @@ __global__ __launch_bounds__(256) void sv_perm_kernel(const double* __restrict__
 }
 
 // y[i] = t[perm[i]]
-__global__ void sv_permute_kernel(const double* __restrict__ stop, int n, const int* __restrict__ perm, const double* __restrict__ t,
+__global__ void sv_permute_kernel(const double* __restrict__ stop,
+                                  int n, const int* __restrict__ perm, const double* __restrict__ t,
                                   double* __restrict__ y) {
     if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -834,7 +845,8 @@ __global__ void sv_permute_kernel(const double* __restrict__ stop, int n, const 
 // broadcast from lane to lane; workgroup 0 stores the block in z; every
 // workgroup then subtracts L[i, block s] z_s from its rows i of block
 // s + 1 + blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop, int n, const double* __restrict__ LU,
+__global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop,
+                             int n, const double* __restrict__ LU,
                                                            double* __restrict__ y, double* __restrict__ z, int s) {
     if (!nc_go(stop)) return;
     __shared__ double zs[TB];
@@ -858,7 +870,8 @@ __global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restr
 // workgroup solves the diagonal block s of z by back substitution (workgroup 0
 // stores it in x), then subtracts U[i, block s] x_s from its rows i of block
 // s - 1 - blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop, int n, const double* __restrict__ LU,
+__global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop,
+                             int n, const double* __restrict__ LU,
                                                            double* __restrict__ z, double* __restrict__ x, int s) {
     if (!nc_go(stop)) return;
     __shared__ double xs[TB];
@@ -895,7 +908,8 @@ __global__ void sv_perm_identity_kernel(const double* __restrict__ stop, int n, 
 }
 
 // W[3 pos(q) + r] = (G_q U~_q^T y)_r
-__global__ void sv_project_kernel(const double* __restrict__ stop, int ns, const int* __restrict__ pos, const int4* __restrict__ sn,
+__global__ void sv_project_kernel(const double* __restrict__ stop,
+                                  int ns, const int* __restrict__ pos, const int4* __restrict__ sn,
                                   const double* __restrict__ sG, const double* __restrict__ sc,
                                   const uint8_t* __restrict__ fixed, const double* __restrict__ y,
                                   double* __restrict__ W) {
@@ -920,7 +934,8 @@ __global__ void sv_project_kernel(const double* __restrict__ stop, int ns, const
 }
 
 // out = r - U~ W  (node lists hold the codes 4 slot + k of the pairs at each node)
-__global__ void sv_lift_kernel(const double* __restrict__ stop, int n_nodes, const int* __restrict__ pptr, const int* __restrict__ psrc,
+__global__ void sv_lift_kernel(const double* __restrict__ stop,
+                               int n_nodes, const int* __restrict__ pptr, const int* __restrict__ psrc,
                                const int* __restrict__ pos, const double* __restrict__ sc,
                                const uint8_t* __restrict__ fixed, const double* __restrict__ r,
                                const double* __restrict__ W, double* __restrict__ out) {
@@ -940,7 +955,8 @@ __global__ void sv_lift_kernel(const double* __restrict__ stop, int n_nodes, con
 
 // out = b - A~_b x from the chain blocks (one lane per scalar row); the
 // contact part U~ G U~^T x is subtracted by sv_project_kernel + sv_lift_kernel
-__global__ void sv_chain_residual_kernel(const double* __restrict__ stop, int n_nodes, const double* __restrict__ Ab, const double* __restrict__ b,
+__global__ void sv_chain_residual_kernel(const double* __restrict__ stop,
+                                         int n_nodes, const double* __restrict__ Ab, const double* __restrict__ b,
                                          const double* __restrict__ x, double* __restrict__ out) {
     if (!nc_go(stop)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -953,21 +969,24 @@ __global__ void sv_chain_residual_kernel(const double* __restrict__ stop, int n_
     out[t] = b[t] - v;
 }
 
-__global__ void sv_rhs_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ R, const uint8_t* __restrict__ fixed,
+__global__ void sv_rhs_kernel(const double* __restrict__ stop,
+                              int ndof, const double* __restrict__ R, const uint8_t* __restrict__ fixed,
                               double* __restrict__ b) {
     if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ndof) b[i] = fixed[i] ? 0.0 : -R[i];
 }
 
-__global__ void sv_axpy_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ dx, double* __restrict__ x) {
+__global__ void sv_axpy_kernel(const double* __restrict__ stop,
+                               int ndof, const double* __restrict__ dx, double* __restrict__ x) {
     if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ndof) x[i] += dx[i];
 }
 
 // rel = ||r|| / ||b|| (one workgroup; fixed order, so deterministic)
-__global__ __launch_bounds__(1024) void sv_relnorm_kernel(const double* __restrict__ stop, int ndof, const double* __restrict__ r,
+__global__ __launch_bounds__(1024) void sv_relnorm_kernel(const double* __restrict__ stop,
+                             int ndof, const double* __restrict__ r,
                                                           const double* __restrict__ b, double* __restrict__ rel) {
     if (!nc_go(stop)) return;
     __shared__ double s0[1024], s1[1024];
@@ -1076,10 +1095,12 @@ int Ctx::solve_setup() {
     SV_TRY(sv[SV_SLIST].ensure(sizeof(int) * (size_t)std::max(ns, 1)));
     SV_TRY(sv[SV_SN].ensure(sizeof(int4) * (size_t)std::max(ns, 1)));
     if (ns > 0) {
-        hipLaunchKernelGGL(sv_slist_kernel, dim3(sv_grid(n_pairs)), dim3(256), 0, 0, nullptr, n_pairs, (const int*)P.sinc,
+        hipLaunchKernelGGL(sv_slist_kernel, dim3(sv_grid(n_pairs)), dim3(256), 0, 0, nullptr, n_pairs,
+                           (const int*)P.sinc,
                            (int*)sv[SV_SLIST].p);
         SV_TRY(hipGetLastError());
-        hipLaunchKernelGGL(sv_slot_nodes_kernel, dim3(sv_grid(ns)), dim3(256), 0, 0, nullptr, ns, (const int*)sv[SV_SLIST].p,
+        hipLaunchKernelGGL(sv_slot_nodes_kernel, dim3(sv_grid(ns)), dim3(256), 0, 0, nullptr, ns,
+                           (const int*)sv[SV_SLIST].p,
                            P, (int4*)sv[SV_SN].p);
         SV_TRY(hipGetLastError());
     }
@@ -1227,7 +1248,8 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st, const double* s
                            (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)z, y);
         SV_TRY(hipGetLastError());
         for (int s = 0; s < nb; ++s) {
-            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, stop, n, LU, y, z, s);
+            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, stop, n, LU, y,
+                               z, s);
             SV_TRY(hipGetLastError());
         }
         for (int s = nb - 1; s >= 0; --s) {
@@ -1320,13 +1342,15 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                 int* piv32 = (int*)sv[SV_PIV32].p + 3 * sv_comp_pos[c];
                 int* info32 = (int*)sv[SV_PIV32].p + 3 * (size_t)std::max(ns, 1) + c;
                 SV_RB(rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, S, (rocblas_int)n, piv32, info32));
-                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, (const int*)piv32, ipiv,
+                hipLaunchKernelGGL(sv_widen_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, (const int*)piv32,
+                                   ipiv,
                                    (const int*)info32, info);
                 SV_TRY(hipGetLastError());
             } else {
                 SV_RB(rocsolver_dgetrf_64(h, n, n, S, n, ipiv, info));
             }
-            hipLaunchKernelGGL(sv_info_kernel, dim3(1), dim3(64), 0, st, nstop, 1, (const int64_t*)info, (int*)d_flags.p);
+            hipLaunchKernelGGL(sv_info_kernel, dim3(1), dim3(64), 0, st, nstop, 1, (const int64_t*)info,
+                               (int*)d_flags.p);
             SV_TRY(hipGetLastError());
             sv_comp_piv[c] = 1;
         }
@@ -1459,11 +1483,13 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                            (const double*)sv[SV_AB].p, (const double*)b, (const double*)du, x);
         SV_TRY(hipGetLastError());
         if (ns > 0) {
-            hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, gate, ns, (const int*)sv[SV_POS].p,
+            hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, gate, ns,
+                               (const int*)sv[SV_POS].p,
                                (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p,
                                fixed, (const double*)du, (double*)sv[SV_W].p);
             SV_TRY(hipGetLastError());
-            hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, n_nodes, (const int*)d_node_pptr.p,
+            hipLaunchKernelGGL(sv_lift_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, n_nodes,
+                               (const int*)d_node_pptr.p,
                                (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p,
                                fixed, (const double*)x, (const double*)sv[SV_W].p, r);
         } else {
