@@ -238,11 +238,12 @@ __global__ void sv_slot_pack_kernel(const double* __restrict__ stop,
 // C_w[(k, a), (l, b)] = (A~_b^-1)[(node k, a), (node l, b)].  One workgroup per
 // chain, one lane per right-hand side e_(l, b); forward sweep into Y, backward
 // sweep reading it.
-__global__ __launch_bounds__(256) void sv_compliance_kernel(const double* __restrict__ stop, 
-    const int* __restrict__ cstart, const int* __restrict__ cn, const int* __restrict__ cnode,
-    const int* __restrict__ ch_cn0, const int* __restrict__ ch_m, const int64_t* __restrict__ ch_yoff,
-    const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed, const double* __restrict__ Ab,
-    const double* __restrict__ Lf, const double* __restrict__ Sinv, double* __restrict__ Y, double* __restrict__ C) {
+__global__ __launch_bounds__(256) void sv_compliance_kernel(
+    const double* __restrict__ stop, const int* __restrict__ cstart, const int* __restrict__ cn,
+    const int* __restrict__ cnode, const int* __restrict__ ch_cn0, const int* __restrict__ ch_m,
+    const int64_t* __restrict__ ch_yoff, const int64_t* __restrict__ ch_coff, const uint8_t* __restrict__ fixed,
+    const double* __restrict__ Ab, const double* __restrict__ Lf, const double* __restrict__ Sinv,
+    double* __restrict__ Y, double* __restrict__ C) {
     if (!nc_go(stop)) return;
     const int c = blockIdx.x;
     const int m = ch_m[c];
@@ -403,7 +404,7 @@ __device__ __forceinline__ double sv_readlane(double v, int lane) {
 // the sub-block bounds are constants and the unrolled steps carry no guards.
 template <bool FULL>
 __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restrict__ stop,
-                             int n, int j0, int n_row_wg, double* __restrict__ A,
+                                                          int n, int j0, int n_row_wg, double* __restrict__ A,
                                                           int64_t* __restrict__ info) {
     if (!nc_go(stop)) return;
     __shared__ double D[LU_NB][LU_NB + 1];
@@ -687,14 +688,16 @@ __global__ void sv_ipiv_identity_kernel(const double* __restrict__ stop, int64_t
     if (i < n) ipiv[i] = i + 1;
 }
 
-// x <- A~_b^-1 x along every chain: 8 lanes per chain (lanes 0..5 hold the
-// node's six components), 8 chains per wavefront; rows of the 6x6 blocks are
-// combined across the chain's lanes by shuffles.
-__global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __restrict__ stop,
-                             int n_chains, const int* __restrict__ cstart,
+// x <- A~_b^-1 b along every chain (b and x may be the same vector): 8 lanes
+// per chain (lanes 0..5 hold the node's six components), 8 chains per
+// wavefront; rows of the 6x6 blocks are combined across the chain's lanes by
+// shuffles.
+__global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __restrict__ stop, int n_chains,
+                                                            const int* __restrict__ cstart,
                                                             const double* __restrict__ Ab,
                                                             const double* __restrict__ Lf,
-                                                            const double* __restrict__ Sinv, double* __restrict__ x) {
+                                                            const double* __restrict__ Sinv, const double* b,
+                                                            double* x) {
     if (!nc_go(stop)) return;
     const int lane = threadIdx.x, base = lane & ~7, a = lane & 7;
     const int c = blockIdx.x * 8 + (lane >> 3);
@@ -703,12 +706,12 @@ __global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __rest
     // forward: y_n = x_n - L_n y_{n-1}; node n + 1's row and value are loaded
     // before node n's shuffles so that the loads overlap the dependent chain
     double y = 0.0;
-    double xv = x[6 * (size_t)n0 + a];
+    double xv = b[6 * (size_t)n0 + a];
     double L[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int n = n0; n < n1; ++n) {
         double xn = 0.0, Ln[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (n + 1 < n1) {
-            xn = x[6 * (size_t)(n + 1) + a];
+            xn = b[6 * (size_t)(n + 1) + a];
             const double* p = Lf + 36 * (size_t)(n + 1) + 6 * a;
 #pragma unroll
             for (int b = 0; b < 6; ++b) Ln[b] = p[b];
@@ -802,7 +805,7 @@ __device__ __forceinline__ double sv_block_dot(double acc, const double* __restr
 constexpr int PERM_MAX = 16384;
 
 __global__ __launch_bounds__(256) void sv_perm_kernel(const double* __restrict__ stop,
-                             int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
+                                                      int n, const int64_t* __restrict__ ipiv, int* __restrict__ perm,
                                                       int* __restrict__ flags) {
     if (!nc_go(stop)) return;
     __shared__ int p[PERM_MAX];
@@ -846,7 +849,7 @@ __global__ void sv_permute_kernel(const double* __restrict__ stop,
 // workgroup then subtracts L[i, block s] z_s from its rows i of block
 // s + 1 + blockIdx.x.
 __global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop,
-                             int n, const double* __restrict__ LU,
+                                                           int n, const double* __restrict__ LU,
                                                            double* __restrict__ y, double* __restrict__ z, int s) {
     if (!nc_go(stop)) return;
     __shared__ double zs[TB];
@@ -871,7 +874,7 @@ __global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restr
 // stores it in x), then subtracts U[i, block s] x_s from its rows i of block
 // s - 1 - blockIdx.x.
 __global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop,
-                             int n, const double* __restrict__ LU,
+                                                           int n, const double* __restrict__ LU,
                                                            double* __restrict__ z, double* __restrict__ x, int s) {
     if (!nc_go(stop)) return;
     __shared__ double xs[TB];
@@ -986,7 +989,7 @@ __global__ void sv_axpy_kernel(const double* __restrict__ stop,
 
 // rel = ||r|| / ||b|| (one workgroup; fixed order, so deterministic)
 __global__ __launch_bounds__(1024) void sv_relnorm_kernel(const double* __restrict__ stop,
-                             int ndof, const double* __restrict__ r,
+                                                          int ndof, const double* __restrict__ r,
                                                           const double* __restrict__ b, double* __restrict__ rel) {
     if (!nc_go(stop)) return;
     __shared__ double s0[1024], s1[1024];
@@ -1222,15 +1225,13 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st, const double* s
     double* T = (double*)sv[SV_T].p;
     double* W = (double*)sv[SV_W].p;
     const uint8_t* fixed = sv_fixed;
-    SV_TRY(hipMemcpyAsync(T, r, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+    // T = A~_b^-1 r (without contact the result itself)
+    if (ns == 0) T = x;
     hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, stop, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
-                       (const double*)sv[SV_SINV].p, T);
+                       (const double*)sv[SV_SINV].p, r, T);
     SV_TRY(hipGetLastError());
-    if (ns == 0) {
-        SV_TRY(hipMemcpyAsync(x, T, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
-        return 0;
-    }
+    if (ns == 0) return 0;
     hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, stop, ns, (const int*)sv[SV_POS].p,
                        (const int4*)sv[SV_SN].p, (const double*)sv[SV_SG].p, (const double*)sv[SV_SC].p, fixed,
                        (const double*)T, W);
@@ -1263,7 +1264,7 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st, const double* s
     SV_TRY(hipGetLastError());
     hipLaunchKernelGGL(sv_chain_solve_kernel, dim3(sv_grid(n_chains, 8)), dim3(64), 0, st, stop, n_chains,
                        (const int*)d_cstart.p, (const double*)sv[SV_AB].p, (const double*)sv[SV_LF].p,
-                       (const double*)sv[SV_SINV].p, x);
+                       (const double*)sv[SV_SINV].p, x, x);
     SV_TRY(hipGetLastError());
     return 0;
 }
@@ -1479,8 +1480,9 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
     // accumulates its contact blocks in FP32 and differs by that rounding)
     const double* gate = dev_refine ? nstop + XK_NC_RGATE : nstop;
     auto residual = [&]() -> int {
+        // b - A~_b du into x (with contact terms to subtract) or straight into r
         hipLaunchKernelGGL(sv_chain_residual_kernel, dim3(sv_grid(nd)), dim3(256), 0, st, gate, n_nodes,
-                           (const double*)sv[SV_AB].p, (const double*)b, (const double*)du, x);
+                           (const double*)sv[SV_AB].p, (const double*)b, (const double*)du, ns > 0 ? x : r);
         SV_TRY(hipGetLastError());
         if (ns > 0) {
             hipLaunchKernelGGL(sv_project_kernel, dim3(sv_grid(ns)), dim3(256), 0, st, gate, ns,
@@ -1492,8 +1494,6 @@ int Ctx::linear_solve(const double* u, const uint8_t* fixed, double mass_coef, c
                                (const int*)d_node_pptr.p,
                                (const int*)d_node_psrc.p, (const int*)sv[SV_POS].p, (const double*)sv[SV_SC].p,
                                fixed, (const double*)x, (const double*)sv[SV_W].p, r);
-        } else {
-            SV_TRY(hipMemcpyAsync(r, x, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
         }
         SV_TRY(hipGetLastError());
         return 0;
