@@ -842,68 +842,99 @@ __global__ void sv_permute_kernel(const double* __restrict__ stop,
 }
 
 
-// Unit lower triangle, step s: every workgroup (one wave) solves the
-// diagonal block s of y (its rows hold the final right-hand side: no later
-// write touches them) by forward substitution with the solved values
-// broadcast from lane to lane; workgroup 0 stores the block in z; every
-// workgroup then subtracts L[i, block s] z_s from its rows i of block
-// s + 1 + blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop,
-                                                           int n, const double* __restrict__ LU,
-                                                           double* __restrict__ y, double* __restrict__ z, int s) {
+// The triangular solves advance TS blocks per launch.  Unit lower triangle,
+// steps s0 .. s0 + ks - 1: every workgroup (one wave) solves those diagonal
+// blocks of y in order -- each block's rows first take the updates of the
+// blocks solved before it in this launch, then forward substitution with the
+// solved values broadcast from lane to lane -- keeping the solutions in LDS
+// (workgroup 0 stores them in z); every workgroup then subtracts
+// L[i, block s0 + j] z_{s0 + j}, j = 0 .. ks - 1 in order, from its rows i of
+// block s0 + ks + blockIdx.x.  Every row sees the updates in the order of the
+// one-block-per-launch form, so the result does not depend on TS; the
+// redundant diagonal work of the workgroups reads the same blocks from L2,
+// and fewer launches are the saving: up to TS blocks per launch, TS for small
+// blocks (the 91-wire squeeze, 21 blocks: 2.12 -> 2.00 ms per attempt), half of
+// it for large ones, where the redundant diagonal work of many workgroups
+// costs about what the launches save (6,012 rows, 94 blocks: 15.1 -> 14.8 ms
+// per solve with 2, 15.2 with 4).
+constexpr int TS = 4;
+static int sv_tri_steps(int nb) { return nb <= 32 ? TS : TS / 2; }
+
+__global__ __launch_bounds__(TB) void sv_lower_step_kernel(const double* __restrict__ stop, int n,
+                                                           const double* __restrict__ LU,
+                                                           double* __restrict__ y, double* __restrict__ z,
+                                                           int s0, int ks) {
     if (!nc_go(stop)) return;
-    __shared__ double zs[TB];
+    __shared__ double zs[TS][TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
-    const int k0 = s * TB, m = min(TB, n - k0);
-    double row[TB];
+    for (int j = 0; j < ks; ++j) {   // ks <= TS, uniform
+        const int k0 = (s0 + j) * TB, m = min(TB, n - k0);
+        double v = t < m ? y[k0 + t] : 0.0;
+        for (int i = 0; i < j; ++i) v = sv_block_dot(v, LU, ld, t < m ? k0 + t : k0, (s0 + i) * TB, TB, zs[i]);
+        if (t >= m) v = 0.0;
+        double row[TB];
 #pragma unroll
-    for (int k = 0; k < TB; ++k) row[k] = (t < m && k < t) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
-    double v = t < m ? y[k0 + t] : 0.0;
+        for (int k = 0; k < TB; ++k) row[k] = (t < m && k < t) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
 #pragma unroll
-    for (int k = 0; k < TB; ++k) v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k >= t
-    zs[t] = v;
-    if (blockIdx.x == 0 && t < m) z[k0 + t] = v;
-    __syncthreads();
-    const int i = (s + 1 + blockIdx.x) * TB + t;
-    if (i < n) y[i] = sv_block_dot(y[i], LU, ld, i, k0, m, zs);
+        for (int k = 0; k < TB; ++k) v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k >= t
+        zs[j][t] = v;
+        if (blockIdx.x == 0 && t < m) z[k0 + t] = v;
+        __syncthreads();
+    }
+    const int i = (s0 + ks + blockIdx.x) * TB + t;
+    if (i >= n) return;
+    double acc = y[i];
+    for (int j = 0; j < ks; ++j)
+        acc = sv_block_dot(acc, LU, ld, i, (s0 + j) * TB, min(TB, n - (s0 + j) * TB), zs[j]);
+    y[i] = acc;
 }
 
-// Upper triangle with its diagonal, step s (from the last block down): every
-// workgroup solves the diagonal block s of z by back substitution (workgroup 0
-// stores it in x), then subtracts U[i, block s] x_s from its rows i of block
-// s - 1 - blockIdx.x.
-__global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop,
-                                                           int n, const double* __restrict__ LU,
-                                                           double* __restrict__ z, double* __restrict__ x, int s) {
+// Upper triangle with its diagonal, steps s0, s0 - 1, .., s0 - ks + 1 (from
+// the last block down): every workgroup solves those diagonal blocks of z in
+// order by back substitution (each after the updates of the blocks solved
+// before it in this launch; workgroup 0 stores them in x), then subtracts
+// U[i, block s0 - j] x_{s0 - j}, j = 0 .. ks - 1 in order, from its rows i of
+// block s0 - ks - blockIdx.x.
+__global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restrict__ stop, int n,
+                                                           const double* __restrict__ LU,
+                                                           double* __restrict__ z, double* __restrict__ x,
+                                                           int s0, int ks) {
     if (!nc_go(stop)) return;
-    __shared__ double xs[TB];
+    __shared__ double xs[TS][TB];
     const int t = threadIdx.x;
     const size_t ld = (size_t)n;
-    const int k0 = s * TB, m = min(TB, n - k0);
-    double row[TB];
+    for (int j = 0; j < ks; ++j) {
+        const int k0 = (s0 - j) * TB, m = min(TB, n - k0);
+        double v = t < m ? z[k0 + t] : 0.0;
+        for (int i = 0; i < j; ++i)
+            v = sv_block_dot(v, LU, ld, t < m ? k0 + t : k0, (s0 - i) * TB, min(TB, n - (s0 - i) * TB), xs[i]);
+        if (t >= m) v = 0.0;
+        double row[TB];
 #pragma unroll
-    for (int k = 0; k < TB; ++k)
-        row[k] = (t < m && k > t && k < m) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
-    const double idg = 1.0 / (t < m ? LU[(size_t)(k0 + t) + (size_t)(k0 + t) * ld] : 1.0);
-    double v = t < m ? z[k0 + t] : 0.0;
+        for (int k = 0; k < TB; ++k)
+            row[k] = (t < m && k > t && k < m) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
+        const double idg = 1.0 / (t < m ? LU[(size_t)(k0 + t) + (size_t)(k0 + t) * ld] : 1.0);
 #pragma unroll
-    for (int k = TB - 1; k >= 0; --k) {
-        if (k < m) {   // uniform
-            if (t == k) v = v * idg;
-            v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k <= t
+        for (int k = TB - 1; k >= 0; --k) {
+            if (k < m) {   // uniform
+                if (t == k) v = v * idg;
+                v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k <= t
+            }
         }
+        xs[j][t] = v;
+        if (blockIdx.x == 0 && t < m) x[k0 + t] = v;
+        __syncthreads();
     }
-    xs[t] = v;
-    if (blockIdx.x == 0 && t < m) x[k0 + t] = v;
-    __syncthreads();
-    const int b = s - 1 - (int)blockIdx.x;
+    const int b = s0 - ks - (int)blockIdx.x;
     if (b < 0) return;
     const int i = b * TB + t;
-    z[i] = sv_block_dot(z[i], LU, ld, i, k0, m, xs);
+    double acc = z[i];
+    for (int j = 0; j < ks; ++j)
+        acc = sv_block_dot(acc, LU, ld, i, (s0 - j) * TB, min(TB, n - (s0 - j) * TB), xs[j]);
+    z[i] = acc;
 }
 
-// perm[i] = i (a block factored without interchanges)
 __global__ void sv_perm_identity_kernel(const double* __restrict__ stop, int n, int* __restrict__ perm) {
     if (!nc_go(stop)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1248,13 +1279,17 @@ int Ctx::solve_apply(const double* r, double* x, hipStream_t st, const double* s
         hipLaunchKernelGGL(sv_permute_kernel, dim3(sv_grid(n)), dim3(256), 0, st, stop, n,
                            (const int*)sv[SV_PERM].p + 3 * sv_comp_pos[c], (const double*)z, y);
         SV_TRY(hipGetLastError());
-        for (int s = 0; s < nb; ++s) {
-            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - 1)), dim3(TB), 0, st, stop, n, LU, y,
-                               z, s);
+        const int ts = sv_tri_steps(nb);
+        for (int s = 0; s < nb; s += ts) {
+            const int ks = std::min(ts, nb - s);
+            hipLaunchKernelGGL(sv_lower_step_kernel, dim3(std::max(1, nb - s - ks)), dim3(TB), 0, st, stop, n, LU, y,
+                               z, s, ks);
             SV_TRY(hipGetLastError());
         }
-        for (int s = nb - 1; s >= 0; --s) {
-            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(std::max(1, s)), dim3(TB), 0, st, stop, n, LU, z, y, s);
+        for (int s = nb - 1; s >= 0; s -= ts) {
+            const int ks = std::min(ts, s + 1);
+            hipLaunchKernelGGL(sv_upper_step_kernel, dim3(std::max(1, s + 1 - ks)), dim3(TB), 0, st, stop, n, LU, z,
+                               y, s, ks);
             SV_TRY(hipGetLastError());
         }
     }
