@@ -443,7 +443,9 @@ class DeviceContext:
         is <= tol or stops decreasing (tol <= 0: exactly max_refine steps, no
         host read).
         res_rel: optional device float64 scalar receiving the final relative
-        residual.  Returns the refinement steps taken."""
+        residual.  Returns the refinement steps taken (-1 when a device Newton
+        control record is attached and tol > 0: the refinement then stops on
+        the device, without a host read)."""
         for a, nm in ((u, "u"), (fixed_mask, "fixed_mask"), (R, "R"), (du, "du")):
             self._check_vec(a, self.ndof, nm)
         steps = C.c_int32()
