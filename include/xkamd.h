@@ -306,7 +306,8 @@ enum {
     XK_NC_MAXATT = 21, XK_NC_STOL = 22, XK_NC_HCAP = 23,
     XK_NC_RPREV = 24,    /* last refinement residual that did not stop it */
     XK_NC_RSTEPS = 25,   /* refinement steps of the last solve */
-    XK_NC_HIST = 32,     /* attempt k: [ratio, n_active, solve residual, alpha] at XK_NC_HIST + 4 k */
+    XK_NC_HIST = 32,     /* attempt k: [ratio, n_active, solve residual, alpha] at XK_NC_HIST + 4 k
+                            (the last two NaN when the attempt ended before its solve / update) */
     XK_NC_HSTRIDE = 4
 };
 enum { XK_NC_RUNNING = 0, XK_NC_CONVERGED = 1, XK_NC_DIVERGED = 2, XK_NC_STOPPED = 3,

@@ -265,8 +265,8 @@ __global__ void nc_force_kernel(double* __restrict__ c, const double* __restrict
         double* h = c + XK_NC_HIST + XK_NC_HSTRIDE * att;
         h[0] = ratio;
         h[1] = na;
-        h[2] = 0.0;
-        h[3] = 0.0;
+        h[2] = NAN;   // solve residual and step length: NaN until the attempt gets that far
+        h[3] = NAN;
     }
     c[XK_NC_ATT] = att + 1;
     // a degenerate element: the reference raises in the element routine

@@ -621,10 +621,10 @@ class DeviceBackend:
     def _newton_controlled(self, u, u_ref, f_ext, dt, f_ext_ref_norm, dynamic_ref, cfg) -> NewtonOutcome:
         """``newton_dynamic`` with its tests on the device (xk_newton_control_*).
         Each attempt -- assembly, pair force, force and divergence tests,
-        linear solve (``graph_refine`` fixed refinement steps), solve check,
-        line search, update, displacement / energy tests and the attempt limit
-        -- is enqueued without a host read; the refinement of the solve
-        stops on the device (up to ``ctl_refine`` steps); the host keeps one
+        linear solve, solve check, line search, update, displacement / energy
+        tests and the attempt limit -- is enqueued without a host read; the
+        refinement of the solve stops on the device (at most ``ctl_refine``
+        steps); the host keeps one
         attempt queued ahead and reads each attempt's control record
         afterwards from pinned memory, so the device never waits on it.
         Every attempt kernel skips once the call has ended, so the attempt
@@ -704,7 +704,7 @@ class DeviceBackend:
                     continue
                 if state != L.NC_ASSIST:
                     break
-                # the attempt's fixed refinement missed solve_tol: the attempt enqueued
+                # the attempt's device refinement missed solve_tol: the attempt enqueued
                 # after it was a no-op; re-solve this one eagerly, then update
                 torch.cuda.synchronize()
                 pending.clear()
