@@ -50,6 +50,13 @@ enum SolveBuf {
 // null outside a device-controlled call) and does nothing once the call it
 // belongs to has ended: a solve enqueued past the end of a call is a no-op.
 
+// A double of another lane of the wave (lane uniform)
+__device__ __forceinline__ double sv_readlane(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 // 32-bit pivots and info of one LU widened into the 64-bit buffers
 __global__ void sv_widen_kernel(const double* __restrict__ stop,
                                 int64_t n, const int* __restrict__ piv32, int64_t* __restrict__ piv64,
@@ -161,26 +168,29 @@ __global__ __launch_bounds__(64) void sv_factor_kernel(const double* __restrict_
         double m = s, y = i == j ? 1.0 : 0.0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
+            // pivot column entries come from fixed lanes: read lane to scalar
             int p = k;
-            double best = fabs(__shfl(m, 7 * k, 64));
+            double best = fabs(sv_readlane(m, 7 * k));
 #pragma unroll
             for (int r = k + 1; r < 6; ++r) {
-                const double v = fabs(__shfl(m, 6 * r + k, 64));
+                const double v = fabs(sv_readlane(m, 6 * r + k));
                 if (v > best) {
                     best = v;
                     p = r;
                 }
             }
-            const double mk = __shfl(m, 6 * k + j, 64), mp = __shfl(m, 6 * p + j, 64);
-            const double yk = __shfl(y, 6 * k + j, 64), yp = __shfl(y, 6 * p + j, 64);
-            if (i == k) {
-                m = mp;
-                y = yp;
-            } else if (i == p) {
-                m = mk;
-                y = yk;
+            if (p != k) {   // uniform
+                const double mk = __shfl(m, 6 * k + j, 64), mp = __shfl(m, 6 * p + j, 64);
+                const double yk = __shfl(y, 6 * k + j, 64), yp = __shfl(y, 6 * p + j, 64);
+                if (i == k) {
+                    m = mp;
+                    y = yp;
+                } else if (i == p) {
+                    m = mk;
+                    y = yk;
+                }
             }
-            const double d = __shfl(m, 7 * k, 64);
+            const double d = sv_readlane(m, 7 * k);
             if (d == 0.0) singular = true;
             if (i == k) {
                 m = m / d;
@@ -393,12 +403,6 @@ __global__ void sv_info_kernel(const double* __restrict__ stop,
 constexpr int LU_NB = 64;   // panel width = tile size
 constexpr int LU_SB = 16;   // sub-block inside the panel
 
-// A double of another lane of the wave (lane uniform)
-__device__ __forceinline__ double sv_readlane(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
 
 // FULL: the panel holds LU_NB columns (every panel but a last partial one):
 // the sub-block bounds are constants and the unrolled steps carry no guards.
