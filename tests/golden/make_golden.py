@@ -464,8 +464,11 @@ def newton_91wire_fixture():
     """The reference's dynamic solver on the 91-wire strand geometry of the
     benchmark (r = 0.4, L = 80, 150 elements per wire, lay angle 12 deg),
     squeezed by 0.1 per outer wire, Coulomb mu = 0.3: 4 Newton calls, 112
-    attempts, friction engaged on 432 pairs in the last two.  Slow (the
-    reference's per-pair Python loops over about 6e4 candidates)."""
+    attempts, friction engaged on 432 pairs in the last two.  Too slow to run
+    in this container: the first Newton attempt had not finished after ten
+    minutes (the reference's per-pair Python loops over about 6e4
+    candidates); make_oracle_trajectory.py writes the same case's trajectory
+    from the product's driver on the oracle assembly instead."""
     newton_fixture(0.3, 0.1, "newton_91wire_mu03.npz", nw=91, r=0.4, L=80.0, m=150, ang=12.0)
 
 

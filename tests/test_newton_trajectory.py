@@ -24,8 +24,10 @@ from oracle import contact as ocon
 # the linear solve's rounding only; a residual left after a near-exact step
 # (1e-5 of the call's first one) carries that rounding at about 1e-11 of the
 # first residual, hence the absolute floor relative to the call's first entry
-HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3}
-HIST_FLOOR = {"oracle": 1e-9, "fp64": 1e-9, "mixed": 1e-6}
+# (91 wires, FP64: the device solve and SuperLU differ by their rounding on a
+# capacitance of condition near 1e12; measured within 3e-6)
+HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3, "fp64_91": 1e-5}
+HIST_FLOOR = {"oracle": 1e-9, "fp64": 1e-9, "mixed": 1e-6, "fp64_91": 1e-9}
 
 
 FIXTURES = ["newton_7wire.npz", "newton_7wire_mu03_f60.npz", "newton_7wire_mu03_f100.npz"]
@@ -68,6 +70,25 @@ def test_squeeze_workload_is_the_fixture_case():
     assert np.array_equal(c["X"], g["X"]) and np.array_equal(c["fixed"], g["fixed"])
     assert np.allclose(c["f"], g["f_ext"], rtol=0, atol=1e-13)
     assert np.array_equal(c["layer"], g["layer"])
+
+
+def test_91wire_oracle_fixture_is_the_benchmark_squeeze():
+    """The 91-wire oracle-driver fixture was made on workloads.squeeze_case at
+    the benchmark's strand (r = 0.4, L = 80, 150 elements per wire, 12 deg,
+    mu = 0.3, load 0.1); its lumped mass is the drop-in's."""
+    g = golden("newton_91wire_mu03_oracle.npz")
+    W, E = pkg("workloads"), pkg("elements")
+    c = W.squeeze_case(0.1, n_wires=91, n_elems=150, length=80.0, radius=0.4, lay_angle_deg=12.0, mu=0.3)
+    k_pen, mu, t_total, rho_inf, dtmax = g["params"][:5]
+    assert (k_pen, mu, t_total, rho_inf, dtmax) == (c["k_pen"], c["mu"], c["t_total"], c["rho_inf"],
+                                                    c["cfg"]["dt_max_fraction"])
+    assert np.array_equal(c["X"], g["X"]) and np.array_equal(c["fixed"], g["fixed"])
+    assert np.array_equal(c["f"], g["f_ext"]) and np.array_equal(c["layer"], g["layer"])
+    sec = pkg("context").circle_section(0.4, W.E_STEEL, W.NU_STEEL)
+    Md = E.ULCRBeamAssembler(c["X"], c["conn"], sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
+                             sec.kappa_z).lumped_mass_diag(W.RHO_STEEL)
+    np.testing.assert_allclose(Md, g["M_diag"], rtol=1e-14, atol=0.0)
+    assert list(g["call_attempts"]) == [30, 22, 30, 30] and list(g["call_active"]) == [0, 0, 432, 432]
 
 
 def test_adaptive_stepping_schedule():
@@ -200,34 +221,31 @@ def test_device_control_hands_missed_solves_to_the_host(gpu, use_graph):
 
 
 @pytest.mark.gpu
-def test_mixed_and_fp64_take_the_same_newton_trajectory_at_91_wires(gpu):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_newton_trajectory_at_91_wires_matches_the_oracle_driver(gpu, precision):
     """The 91-wire strand of the benchmark (r = 0.4, L = 80, 150 elements per
     wire, lay angle 12 deg) squeezed by 0.1 per outer wire with Coulomb
-    friction mu = 0.3, solved with the Newton attempts on the device and
-    replayed from HIP graphs, FP64 and FP32-mixed: every call has the same time
-    window, attempt count, convergence, divergence flag and active count, and
-    the per-attempt residuals agree to 1e-3 (the mixed tangent only steers the
-    iteration).  With tests/golden/newton_91wire_mu03.npz present (the
-    reference's own solver on this case), both also reproduce its trajectory."""
-    import os
-
-    TP, W = pkg("train_prescreen"), pkg("workloads")
-    sq = W.squeeze_case(0.1, n_wires=91, n_elems=150, length=80.0, radius=0.4, lay_angle_deg=12.0, mu=0.3)
-    out = {}
-    for prec in ("fp64", "mixed"):
-        print(f"91-wire frictional squeeze, {prec}, device attempts with graphs", flush=True)
-        res, b = TP.run_case(sq, precision=prec, device_solve=True, use_graph=True)
-        assert b.graph_error is None, b.graph_error
-        out[prec] = res
-    a, m = out["fp64"], out["mixed"]
-    assert len(a.calls) == len(m.calls) and len(a.calls) >= 3
-    for p, q in zip(a.calls, m.calls):
-        assert (p.load_frac, p.n_attempts, p.n_active, p.converged, p.diverged) == \
-            (q.load_frac, q.n_attempts, q.n_active, q.converged, q.diverged)
-        np.testing.assert_allclose(q.res_history, p.res_history, rtol=1e-3, atol=1e-6 * p.res_history[0])
-    assert any(c.n_active > 0 for c in a.calls)       # friction engaged
-    ref = os.path.join(os.path.dirname(__file__), "golden", "newton_91wire_mu03.npz")
-    if os.path.exists(ref):
-        g = golden("newton_91wire_mu03.npz")
-        check_trajectory(g, a, "fp64")
-        check_trajectory(g, m, "mixed")
+    friction mu = 0.3, solved with the Newton attempts and the driver's tests
+    on the device, replayed from HIP graphs: every call has the time window,
+    attempt count, convergence, divergence flag and active count of the same
+    driver run on the oracle assembly with host SuperLU
+    (tests/golden/newton_91wire_mu03_oracle.npz, written by
+    tests/golden/make_oracle_trajectory.py; the oracle driver reproduces the
+    reference solver's 7-wire trajectories exactly, and the reference's own
+    solver is too slow to run this strand), in FP64 and in FP32-mixed."""
+    g, prm, cfg = case("newton_91wire_mu03_oracle.npz")
+    S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
+    r = float(g["radii"][0])
+    ctx = C.DeviceContext(g["X"], g["conn"], C.circle_section(r, wl.E_STEEL, wl.NU_STEEL), radii=g["radii"],
+                          precision=precision)
+    kcfg = Kc.ContactConfig(mu=prm["mu"], k_t_ratio=prm["k_t_ratio"], exclude_same_layer=True,
+                            elem_layer_map=wl.layer_map(g["layer"]))
+    b = S.DeviceBackend(ctx, M_diag=g["M_diag"], rho_inf=prm["rho_inf"], k_pen=prm["k_pen"],
+                        smoothing_delta=prm["smoothing_delta"], mu=prm["mu"], k_t_ratio=prm["k_t_ratio"],
+                        fixed_dofs=g["fixed"], config=kcfg, device_solve=True, use_graph=True)
+    print(f"91-wire frictional squeeze, {precision}, device attempts and tests, graphs", flush=True)
+    res = S.ContactFrictionSolver(b, fixed_dofs=g["fixed"], f_ext_total=g["f_ext"], t_total=prm["t_total"],
+                                  config=cfg).solve()
+    assert b.graph_error is None, b.graph_error
+    assert any(c.n_active > 0 for c in res.calls)       # friction engaged
+    check_trajectory(g, res, "fp64_91" if precision == "fp64" else precision)
