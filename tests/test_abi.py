@@ -38,3 +38,30 @@ def test_product_package_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", src).replace("oracle/", ""), f
+
+
+def _header_enum(prefix):
+    """name -> value of the XK_<prefix>* enumerators of include/xkamd.h."""
+    txt = open(os.path.join(REPO, "include", "xkamd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"\bXK_(" + prefix + r"[A-Z0-9_]*)\s*=\s*(\d+)", txt)}
+
+
+def test_python_mirror_of_the_control_record_matches_the_header():
+    """The device Newton control record's layout, states and reasons as the
+    Python driver reads them (_lib.NC_*) are the header's XK_NC_* / XK_NCR_*."""
+    L = pkg("_lib")
+    nc = _header_enum("NC_")
+    for name in ("STATE", "ATT", "REASON", "NACT", "RATIO", "DU", "U", "ENERGY", "REL", "ALPHA", "HIST", "HSTRIDE",
+                 "RUNNING", "CONVERGED", "DIVERGED", "STOPPED", "ASSIST"):
+        assert getattr(L, "NC_" + name) == nc["NC_" + name], name
+    ncr = _header_enum("NCR_")
+    for name in ("NONFINITE", "DEGENERATE", "SINGULAR", "SOLVE", "LIMIT"):
+        assert getattr(L, "NCR_" + name) == ncr["NCR_" + name], name
+    # the record's scalar fields fit before the history
+    assert max(v for k, v in nc.items() if k not in ("NC_HIST", "NC_HSTRIDE") and not k.startswith(
+        ("NC_RUNNING", "NC_CONVERGED", "NC_DIVERGED", "NC_STOPPED", "NC_ASSIST"))) < nc["NC_HIST"]
+    flags = _header_enum("FLAG_")
+    S = pkg("solver")
+    assert (S.FLAG_NONFINITE, S.FLAG_DEGENERATE, S.FLAG_SINGULAR) == (
+        flags["FLAG_NONFINITE"], flags["FLAG_DEGENERATE"], flags["FLAG_SINGULAR"])
