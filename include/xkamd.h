@@ -290,9 +290,11 @@ int xk_newton_update(xk_ctx* ctx, double* u, double* du, const double* R, const 
  * ended (or while an attempt waits on the host), so attempts enqueued past
  * the end are no-ops.  rocBLAS's trailing updates of the capacitance LU are
  * the one exception: they run on stale scratch in a no-op attempt.  The host
- * detaches the record for an eager re-solve (XK_NC_ASSIST). */
+ * detaches the record for an eager re-solve (XK_NC_ASSIST), and rebuilds the
+ * tangent pattern when a coated attempt's probe found a pair without a slot
+ * (XK_NC_REBUILD, xk_coating_probe_control). */
 enum {
-    XK_NC_STATE = 0,     /* XK_NC_RUNNING .. XK_NC_ASSIST */
+    XK_NC_STATE = 0,     /* XK_NC_RUNNING .. XK_NC_REBUILD */
     XK_NC_ATT = 1,       /* attempts whose force test ran */
     XK_NC_REASON = 2,    /* XK_NCR_* bits of a stopped call */
     XK_NC_NACT = 3,      /* n_active of the last attempt */
@@ -311,7 +313,9 @@ enum {
     XK_NC_HSTRIDE = 4
 };
 enum { XK_NC_RUNNING = 0, XK_NC_CONVERGED = 1, XK_NC_DIVERGED = 2, XK_NC_STOPPED = 3,
-       XK_NC_ASSIST = 4 };  /* the attempt's solve missed solve_tol: the host re-solves it eagerly */
+       XK_NC_ASSIST = 4,    /* the attempt's solve missed solve_tol: the host re-solves it eagerly */
+       XK_NC_REBUILD = 5 }; /* the attempt's coating probe met a pair without a pattern slot: the
+                               host rebuilds the pattern and enqueues the attempt again */
 enum { XK_NCR_NONFINITE = 1, XK_NCR_DEGENERATE = 2, XK_NCR_SINGULAR = 4, XK_NCR_SOLVE = 8, XK_NCR_LIMIT = 16 };
 /* Reset `ctl` for a new call (stream-ordered) and clear the device status
  * word.  f_ref of the force test: with dynamic_ref the call's first residual
@@ -340,9 +344,10 @@ int xk_newton_control_attach(xk_ctx* ctx, double* ctl);
  * the reference raises there) or XK_FLAG_NONFINITE (after the tests, as a
  * failed linear solve, _newton_steps.py:360-366). */
 int xk_newton_control_force(xk_ctx* ctx, const double* res_norm2, const int32_t* n_active, void* stream);
-/* Host decision on the attached record after an eager re-solve: sets the
- * state, ORs `reason` into XK_NC_REASON, and with XK_NC_RUNNING accepts the
- * attempt's solve (the next update skips the solve check once). */
+/* Host decision on the attached record after an eager re-solve or a pattern
+ * rebuild: sets the state and ORs `reason` into XK_NC_REASON; XK_NC_RUNNING
+ * after XK_NC_ASSIST accepts the attempt's solve (the next update skips the
+ * solve check once). */
 int xk_newton_control_set(xk_ctx* ctx, int state, int reason, void* stream);
 
 /* Copy of one of the linear solve's internal device buffers (index `which`,
@@ -443,6 +448,13 @@ int xk_coating_set_dt(xk_ctx* ctx, double coat_dt);
  * whenever the coating is on.  No counterpart in the reference (whose
  * assembly has no fixed pattern).  Synchronous. */
 int xk_coating_probe(xk_ctx* ctx, const double* u, int64_t* n_unplaced, void* stream);
+/* xk_coating_probe on the attached Newton control record, stream-ordered with
+ * no host read (capturable): a compressed pair without a slot sets the
+ * record's state to XK_NC_REBUILD, so the rest of the attempt is a no-op; the
+ * host then calls xk_coating_probe (which marks the pattern stale), rebuilds
+ * the pattern, sets XK_NC_RUNNING and enqueues the attempt again.  Nothing to
+ * do while the call is not running.  No counterpart in the reference. */
+int xk_coating_probe_control(xk_ctx* ctx, const double* u, void* stream);
 /* End of a converged step: coating_compression_prev <- coating_compression
  * (contact/solver/process.py:539-547).  Stream-ordered. */
 int xk_coating_commit(xk_ctx* ctx, void* stream);

@@ -53,14 +53,14 @@ def test_python_mirror_of_the_control_record_matches_the_header():
     L = pkg("_lib")
     nc = _header_enum("NC_")
     for name in ("STATE", "ATT", "REASON", "NACT", "RATIO", "DU", "U", "ENERGY", "REL", "ALPHA", "HIST", "HSTRIDE",
-                 "RUNNING", "CONVERGED", "DIVERGED", "STOPPED", "ASSIST"):
+                 "RUNNING", "CONVERGED", "DIVERGED", "STOPPED", "ASSIST", "REBUILD"):
         assert getattr(L, "NC_" + name) == nc["NC_" + name], name
     ncr = _header_enum("NCR_")
     for name in ("NONFINITE", "DEGENERATE", "SINGULAR", "SOLVE", "LIMIT"):
         assert getattr(L, "NCR_" + name) == ncr["NCR_" + name], name
     # the record's scalar fields fit before the history
     assert max(v for k, v in nc.items() if k not in ("NC_HIST", "NC_HSTRIDE") and not k.startswith(
-        ("NC_RUNNING", "NC_CONVERGED", "NC_DIVERGED", "NC_STOPPED", "NC_ASSIST"))) < nc["NC_HIST"]
+        ("NC_RUNNING", "NC_CONVERGED", "NC_DIVERGED", "NC_STOPPED", "NC_ASSIST", "NC_REBUILD"))) < nc["NC_HIST"]
     flags = _header_enum("FLAG_")
     S = pkg("solver")
     assert (S.FLAG_NONFINITE, S.FLAG_DEGENERATE, S.FLAG_SINGULAR) == (

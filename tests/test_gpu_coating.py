@@ -258,18 +258,37 @@ def squeeze_solver(coat_k_frac, mu_coat):
     return c, kw, cfg, 0.95 * c["radii"]
 
 
-@pytest.mark.parametrize("mode", ["host_superlu", "device_solve"])
-def test_coating_newton_calls_match_oracle(gpu, mode):
-    """The solver loop with the coating on (no initial-penetration error, the
-    dashpot step per increment, compression committed at every converged
-    step) over the device assembly and over the oracle.  On this squeeze the
-    coating engages where the reference's Newton starts to stall (calls that
-    end at 30 attempts, after which the two runs' rounding differences grow):
-    the calls are compared up to the first converged call that ends with
-    compressed pairs -- same load fractions, attempts, convergence, active
-    counts and residual histories."""
-    from _oracle_backend import OracleBackend
+_ORACLE_SQUEEZE = {}
 
+
+def coated_squeeze_oracle():
+    """The coated squeeze's calls on the oracle assembly (computed once)."""
+    if "res" not in _ORACLE_SQUEEZE:
+        from _oracle_backend import OracleBackend
+
+        S, C, E = pkg("solver"), pkg("context"), pkg("elements")
+        c, kw, cfg, core = squeeze_solver(0.001, 0.2)
+        X, conn, radii = c["X"], c["conn"], c["radii"]
+        lmap = {int(e): int(k) for e, k in enumerate(c["layer"])}
+        sec = C.circle_section(1.0, 200e3, 0.3)
+        Md = E.ULCRBeamAssembler(X, conn, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
+                                 sec.kappa_z).lumped_mass_diag(7.85e-9)
+        osec = obeam.circle_section(1.0, 200e3, 0.3)
+        ocfg = ocon.ContactConfig(mu=c["mu"], exclude_same_layer=True, elem_layer_map=lmap, **kw)
+        ob = OracleBackend(X, conn, radii, osec, M_diag=Md, rho_inf=c["rho_inf"], k_pen=c["k_pen"],
+                           smoothing_delta=c["sd"], mu=c["mu"], k_t_ratio=c["ktr"], cfg=ocfg)
+        ob.fixed = c["fixed"]
+        # the oracle's pairs take the core radii at creation (DetectCandidatesInput.core_radii)
+        ob.detect = lambda u, _ob=ob: setattr(_ob, "pairs", ocon.detect_candidates(
+            _ob.pairs, _ob._x(u), conn, radii, ocfg, core_radii=core)[0])
+        _ORACLE_SQUEEZE["res"] = S.ContactFrictionSolver(ob, fixed_dofs=c["fixed"], f_ext_total=c["f"],
+                                                         t_total=c["t_total"], config=cfg).solve()
+    return _ORACLE_SQUEEZE["res"]
+
+
+def coated_squeeze_device(mode):
+    """The coated squeeze on the device in `mode` -> (result, #compressed pairs at
+    each commit, pattern rebuilds during each Newton call)."""
     S, C, Kc, E = pkg("solver"), pkg("context"), pkg("contact"), pkg("elements")
     c, kw, cfg, core = squeeze_solver(0.001, 0.2)
     X, conn, radii = c["X"], c["conn"], c["radii"]
@@ -281,28 +300,46 @@ def test_coating_newton_calls_match_oracle(gpu, mode):
     kcfg = Kc.ContactConfig(mu=c["mu"], k_t_ratio=c["ktr"], exclude_same_layer=True, elem_layer_map=lmap, **kw)
     b = S.DeviceBackend(ctx, M_diag=Md, rho_inf=c["rho_inf"], k_pen=c["k_pen"], smoothing_delta=c["sd"],
                         mu=c["mu"], k_t_ratio=c["ktr"], fixed_dofs=c["fixed"], config=kcfg,
-                        device_solve=mode == "device_solve", core_radii=core)
-    dev_cc = []
-    commit = b.commit_coating
+                        device_solve=mode != "host_superlu", core_radii=core,
+                        device_control=mode in ("device_ctl", "device_ctl_graph"),
+                        use_graph=mode == "device_ctl_graph")
+    dev_cc, marks = [], []
+    commit, update_geometry = b.commit_coating, b.update_geometry
 
     def dev_commit():
         dev_cc.append(int((ctx.get_coating()["coating_compression"] > 0).sum()))
         commit()
 
-    b.commit_coating = dev_commit
+    def dev_update_geometry(u):   # once per increment, just before its Newton call
+        marks.append(b.asm.n_coat_rebuilds)
+        update_geometry(u)
+
+    b.commit_coating, b.update_geometry = dev_commit, dev_update_geometry
     res = S.ContactFrictionSolver(b, fixed_dofs=c["fixed"], f_ext_total=c["f"], t_total=c["t_total"],
                                   config=cfg).solve()
-    osec = obeam.circle_section(1.0, 200e3, 0.3)
-    ocfg = ocon.ContactConfig(mu=c["mu"], exclude_same_layer=True, elem_layer_map=lmap, **kw)
-    ob = OracleBackend(X, conn, radii, osec, M_diag=Md, rho_inf=c["rho_inf"], k_pen=c["k_pen"],
-                       smoothing_delta=c["sd"], mu=c["mu"], k_t_ratio=c["ktr"], cfg=ocfg)
-    ob.fixed = c["fixed"]
-    # the oracle's pairs take the core radii at creation (DetectCandidatesInput.core_radii)
-    ob.detect = lambda u, _ob=ob: setattr(_ob, "pairs", ocon.detect_candidates(
-        _ob.pairs, _ob._x(u), conn, radii, ocfg, core_radii=core)[0])
-    ref = S.ContactFrictionSolver(ob, fixed_dofs=c["fixed"], f_ext_total=c["f"], t_total=c["t_total"],
-                                  config=cfg).solve()
-    print([(k.load_frac, k.n_attempts, k.n_active, k.converged) for k in res.calls], dev_cc)
+    marks.append(b.asm.n_coat_rebuilds)
+    assert len(marks) == len(res.calls) + 1
+    return res, dev_cc, list(np.diff(marks))
+
+
+@pytest.mark.parametrize("mode", ["host_superlu", "device_hostctl", "device_ctl", "device_ctl_graph"])
+def test_coating_newton_calls_match_oracle(gpu, mode):
+    """The solver loop with the coating on (no initial-penetration error, the
+    dashpot step per increment, compression committed at every converged
+    step) over the device assembly and over the oracle: host SuperLU, device
+    attempts with the tests on the host, and device attempts under device
+    control (eager and from graphs), where a pair that starts pressing its
+    coating within a call stops the attempt at XK_NC_REBUILD and the host
+    rebuilds the pattern.  On this squeeze the coating engages where the
+    reference's Newton starts to stall (calls that end at 30 attempts, after
+    which the two runs' rounding differences grow): the calls are compared up
+    to the first converged call that ends with compressed pairs -- same load
+    fractions, attempts, convergence, active counts and residual histories --
+    and the pattern is rebuilt within those calls as often as by the
+    host-tested attempts."""
+    ref = coated_squeeze_oracle()
+    res, dev_cc, rebuilds = coated_squeeze_device(mode)
+    print([(k.load_frac, k.n_attempts, k.n_active, k.converged) for k in res.calls], dev_cc, rebuilds)
     conv = [k for k, call in enumerate(res.calls) if call.converged]
     n_cmp = next(k for j, k in enumerate(conv) if dev_cc[j] > 0) + 1
     assert n_cmp <= len(ref.calls)
@@ -310,3 +347,7 @@ def test_coating_newton_calls_match_oracle(gpu, mode):
         assert (a.load_frac, a.n_attempts, a.n_active, a.converged) == (r.load_frac, r.n_attempts, r.n_active,
                                                                         r.converged)
         np.testing.assert_allclose(a.res_history, r.res_history, rtol=1e-4, atol=0)
+    assert sum(rebuilds[:n_cmp]) >= 1, "no pair started pressing its coating within a call"
+    if mode in ("device_ctl", "device_ctl_graph"):
+        _, _, host_rebuilds = coated_squeeze_device("device_hostctl")
+        assert rebuilds[:n_cmp] == host_rebuilds[:n_cmp]

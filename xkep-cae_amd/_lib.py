@@ -29,7 +29,7 @@ FLAG_COAT_UNPLACED = 8
 # device Newton control record (XK_NC_* of include/xkamd.h)
 NC_STATE, NC_ATT, NC_REASON, NC_NACT, NC_RATIO, NC_DU, NC_U, NC_ENERGY, NC_REL, NC_ALPHA = range(10)
 NC_HIST, NC_HSTRIDE = 32, 4
-NC_RUNNING, NC_CONVERGED, NC_DIVERGED, NC_STOPPED, NC_ASSIST = range(5)
+NC_RUNNING, NC_CONVERGED, NC_DIVERGED, NC_STOPPED, NC_ASSIST, NC_REBUILD = range(6)
 NCR_NONFINITE, NCR_DEGENERATE, NCR_SINGULAR, NCR_SOLVE, NCR_LIMIT = 1, 2, 4, 8, 16
 
 
@@ -92,6 +92,7 @@ SIGNATURES = {
     "xk_ctx_set_coating": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp]),
     "xk_coating_set_dt": (C.c_int, [vp, C.c_double]),
     "xk_coating_probe": (C.c_int, [vp, vp, c_int64_p, vp]),
+    "xk_coating_probe_control": (C.c_int, [vp, vp, vp]),
     "xk_coating_commit": (C.c_int, [vp, vp]),
     "xk_pairs_get_coating": (C.c_int, [vp] + [vp] * 8),
     "xk_pairs_set_coating": (C.c_int, [vp] + [vp] * 6),

@@ -179,21 +179,37 @@ class StrandAssembly:
             self.ctx.set_coating_state(coat)
         return int(np.sum(gap < 0.0))
 
-    def assemble(self, u=None, *, tangent: bool = True):
-        """Launch one fused assembly; returns device tensors (R, K, ||R||^2, n_active)."""
+    def assemble(self, u=None, *, tangent: bool = True, controlled: bool = False):
+        """Launch one fused assembly; returns device tensors (R, K, ||R||^2, n_active).
+        With the coating on, the pattern probe reads its count on the host, or,
+        with ``controlled`` (a Newton control record attached), leaves its outcome
+        in the record (XK_NC_REBUILD skips the rest of the attempt; the host then
+        calls ``coating_rebuild``), so the assembly needs no host read."""
         if u is not None:
             self.set_u(u)
-        if self.coating and self.ctx.coating_probe(self.u) > 0:
+        if self.coating and not controlled and self.ctx.coating_probe(self.u) > 0:
             self.K = None   # a pair pressing its coating has no slot yet: new pattern
             self.graph = None
             self.n_coat_rebuilds += 1
         if self.K is None:
             self.K = self.ctx.new_K()
+        if self.coating and controlled:
+            self.ctx.coating_probe_control(self.u)
         self.ctx.assemble_iteration(self.u, self.u_ref, self.f_ext, self.fixed_mask, self.R,
                                     self.K if tangent else None, self._dyn, k_pen=self.k_pen,
                                     smoothing_delta=self.smoothing_delta, k_t_ratio=self.k_t_ratio,
                                     mu_eff=self.mu, res_norm2=self.res2, n_active=self.n_active)
         return self.R, (self.K if tangent else None), self.res2, self.n_active
+
+    def coating_rebuild(self):
+        """After a controlled attempt stopped at XK_NC_REBUILD (its probe met a
+        compressed pair without a slot, at the same u): probe on the host, which
+        marks the pattern stale, so that the next assembly rebuilds it."""
+        if self.ctx.coating_probe(self.u) == 0:
+            raise RuntimeError("XK_NC_REBUILD without an unplaced pair at the same displacement")
+        self.K = None
+        self.graph = None
+        self.n_coat_rebuilds += 1
 
     # -------------------------------------------------------------- HIP graph
     def capture(self, *, tangent: bool = True):

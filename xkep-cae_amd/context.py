@@ -254,6 +254,12 @@ class DeviceContext:
             self.invalidate_pattern()   # the host copy of the pattern is stale from here
         return n.value
 
+    def coating_probe_control(self, u):
+        """The probe on the attached Newton control record, with no host read: a
+        pair without a slot moves the record to XK_NC_REBUILD (the rest of the
+        attempt is a no-op); the host then calls ``coating_probe`` and rebuilds."""
+        L.check(L.lib().xk_coating_probe_control(self.h, L.ptr(u), self.stream), "xk_coating_probe_control")
+
     def coating_commit(self):
         L.check(L.lib().xk_coating_commit(self.h, self.stream), "xk_coating_commit")
 
@@ -496,7 +502,8 @@ class DeviceContext:
                 "xk_newton_control_force")
 
     def newton_control_set(self, state: int, reason: int = 0):
-        """Host decision after an eager re-solve (XK_NC_RUNNING accepts the attempt's solve)."""
+        """Host decision after an eager re-solve or a pattern rebuild (XK_NC_RUNNING
+        after XK_NC_ASSIST accepts the attempt's solve)."""
         L.check(L.lib().xk_newton_control_set(self.h, int(state), int(reason), self.stream),
                 "xk_newton_control_set")
 

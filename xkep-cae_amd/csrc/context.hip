@@ -425,6 +425,16 @@ __global__ void coat_unplaced_kernel(int64_t np, const int8_t* __restrict__ cnee
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
 }
 
+// the same test on a running Newton call: a compressed pair without a slot
+// moves the control record to XK_NC_REBUILD (every lane that finds one stores
+// the same value; the attempt's later kernels then skip)
+__global__ void coat_unplaced_ctl_kernel(double* __restrict__ ctl, int64_t np, const int8_t* __restrict__ cneed,
+                                         const int* __restrict__ sinc) {
+    if (!nc_go(ctl)) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np && cneed[p] != 0 && sinc[p] < 0) ctl[XK_NC_STATE] = XK_NC_REBUILD;
+}
+
 }  // namespace xk
 
 extern "C" {
@@ -1340,6 +1350,27 @@ int xk_coating_probe(xk_ctx* c, const double* u, int64_t* n_unplaced, void* stre
     XK_TRY(hipStreamSynchronize(st));
     *n_unplaced = cnt;
     if (cnt > 0) c->pattern_valid = false;   // the next pattern build gives the marked pairs a slot
+    return 0;
+}
+
+int xk_coating_probe_control(xk_ctx* c, const double* u, void* stream) {
+    if (!c) return -1;
+    if (!c->nstop) {
+        set_error("xk_coating_probe_control: no Newton control record attached");
+        return -1;
+    }
+    if (!c->coat_on || c->n_pairs == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (c->lp_sync(st, false) != 0) return -1;
+    PairParams pp;
+    pp.nqpos = (const int*)c->d_node_qpos.p;
+    pp.stop = c->nstop;
+    c->coat_params(pp);
+    XK_TRY(launch_pairs(c->prec, c->n_pairs, c->P, (const double*)c->d_X.p, u, nullptr, PAIR_GEOM | PAIR_PROBE,
+                        pp, (double*)c->d_nf.p, c->d_sg.p, nullptr, st, nullptr, c->pair_block));
+    hipLaunchKernelGGL(coat_unplaced_ctl_kernel, dim3((unsigned)((c->n_pairs + 255) / 256)), dim3(256), 0, st,
+                       c->nstop, c->n_pairs, (const int8_t*)c->P.cneed, (const int*)c->P.sinc);
+    XK_TRY(hipGetLastError());
     return 0;
 }
 

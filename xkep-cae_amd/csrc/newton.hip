@@ -323,9 +323,10 @@ __global__ void nc_after_kernel(double* __restrict__ c, const double* __restrict
 
 __global__ void nc_set_kernel(double* __restrict__ c, int state, int reason) {
     if (threadIdx.x != 0) return;
+    const bool assisted = c[XK_NC_STATE] == XK_NC_ASSIST;
     c[XK_NC_STATE] = state;
     c[XK_NC_REASON] = (double)((int)c[XK_NC_REASON] | reason);
-    if (state == XK_NC_RUNNING) c[XK_NC_SOLVE_OK] = 1.0;
+    if (state == XK_NC_RUNNING && assisted) c[XK_NC_SOLVE_OK] = 1.0;   // the host's re-solve is accepted
 }
 
 int Ctx::newton_control_begin(double* ctl, int hist_cap, double tol_force, double tol_disp, double f_ext_ref_norm,

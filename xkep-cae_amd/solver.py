@@ -516,7 +516,7 @@ class DeviceBackend:
         refinement steps) + line search + update.  ``u`` (host) is updated in
         place at the end."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
-        if self.device_control and not asm.coating:
+        if self.device_control:
             return self._newton_controlled(u, u_ref, f_ext, dt, f_ext_ref_norm, dynamic_ref, cfg)
         if dt != self._dt:
             asm.set_dt(dt)
@@ -630,10 +630,11 @@ class DeviceBackend:
         Every attempt kernel skips once the call has ended, so the attempt
         enqueued past the end changes nothing.  An attempt whose solve missed
         ``solve_tol`` is handed to the host (XK_NC_ASSIST), which re-solves it
-        eagerly as the host-tested path does.  With ``use_graph`` the attempts
-        after each increment's first replay one captured graph.  The coating's
-        per-assembly pattern probe needs the host, so a coated call takes the
-        host-tested path."""
+        eagerly as the host-tested path does.  With the coating on, an attempt
+        whose pattern probe met a compressed pair without a slot stops at
+        XK_NC_REBUILD: the host rebuilds the pattern and enqueues the attempt
+        again.  With ``use_graph`` the attempts after each increment's first
+        (and after a rebuild or an assist) replay one captured graph."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
         if dt != self._dt:
             asm.set_dt(dt)
@@ -661,7 +662,7 @@ class DeviceBackend:
                               line_search_steps=ls_steps, diverge_factor=3.0, du_norm_cap=cfg.du_norm_cap)
 
         def attempt():
-            asm.assemble()
+            asm.assemble(controlled=True)
             ctx.pairs_gather(f=self._f)
             ctx.newton_control_force(asm.res2, asm.n_active)
             solve(self.ctl_refine, self.solve_tol)   # refinement stopped on the device
@@ -680,6 +681,7 @@ class DeviceBackend:
         try:
             k = 0               # next attempt to enqueue
             pending = deque()   # attempts enqueued and not yet read
+            rebuilt_at = -1     # attempt whose pattern was last rebuilt
             while True:
                 if k < cap:
                     if graph is not None:
@@ -701,6 +703,20 @@ class DeviceBackend:
                 evs[p % 2].synchronize()
                 state = int(pin[p % 2, L.NC_STATE])
                 if state == L.NC_RUNNING:
+                    continue
+                if state == L.NC_REBUILD:
+                    # the attempt's coating probe met a pair without a pattern slot: it and the
+                    # attempt enqueued after it were no-ops; rebuild, then enqueue it again
+                    torch.cuda.synchronize()
+                    pending.clear()
+                    k = int(pin[p % 2, L.NC_ATT])
+                    if k == rebuilt_at:
+                        raise RuntimeError("the rebuilt tangent pattern still lacks a coating slot")
+                    rebuilt_at = k
+                    asm.coating_rebuild()
+                    ctx.newton_control_set(L.NC_RUNNING)
+                    graph = None   # the captured attempt holds the old pattern's K
+                    graph_ok = self.use_graph and asm.prescreen is None
                     continue
                 if state != L.NC_ASSIST:
                     break
