@@ -76,9 +76,11 @@ def setup(case, states, precision, torch, batch=None, mu_elem=None, coating=None
     dt = case.dt()
     ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
     ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
+    # the assembly as the Newton driver runs it each attempt: the attempt's acceleration and
+    # velocity are not stored (the driver's correct() at the increment's end sets them)
     asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=mu,
                            f_ext=f_ext, fixed_dofs=fixed, config=cfg, time=ga, dt=dt,
-                           core_radii=None if core is None else core * radii)
+                           core_radii=None if core is None else core * radii, ga_state=False)
     if coating is not None:
         asm.set_coat_dt(coat_dt)
     asm.bench_layer = layer
@@ -177,8 +179,9 @@ def stage_bytes(ctx, asm, n_pair, precision, nnz_beam):
               write s, t, gap, frame, p_n, history, status 132 B);
     elements  per node X 24 + u 48 read, f_int 48 written; per element conn 8;
               the beam values of K written (nnz_beam x s_K);
-    gather    per node u, f_ext, f_int, M, u_pred, v_pred, acc_old read and R,
-              acc, vel written (10 x 48 B); the contact-only values of K written;
+    gather    per node u, f_ext, f_int, M, u_pred, acc_old read and R written
+              (7 x 48 B; with the attempt's GA state stored also v_pred read and
+              acc, vel written, 10 x 48 B); the contact-only values of K written;
     finish    one partial sum per gather workgroup and one count per pair workgroup."""
     nn, ne = ctx.n_nodes, ctx.n_elems
     s_k = 8 if precision == "fp64" else 4
@@ -186,7 +189,7 @@ def stage_bytes(ctx, asm, n_pair, precision, nnz_beam):
     return {
         "pairs": n_pair * 216,
         "elements": nn * (24 + 48 + 48) + ne * 8 + nnz_beam * s_k,
-        "gather": nn * 10 * 48 + (nnz - nnz_beam) * s_k,
+        "gather": nn * (10 if asm.ga_state else 7) * 48 + (nnz - nnz_beam) * s_k,
         "finish": (6 * nn + 255) // 256 * 8 + (n_pair + 255) // 256 * 4,
     }
 

@@ -217,7 +217,10 @@ typedef struct xk_dyn {
  *   narrowphase (frozen active set) -> Huber -> friction -> element f and K ->
  *   R = f_int - (f_c + f_fric) - f_ext, R[fixed] = 0, R += M a_mid,
  *   R[fixed] = 0 -> K = K_beam + K_c - K_fric + (1 - alpha_m) c0 M.
- * fixed_mask (device uint8, ndof); dyn may be NULL (static).  R (device),
+ * fixed_mask (device uint8, ndof); dyn may be NULL (static); dyn->acc and
+ * dyn->vel (the per-attempt GeneralizedAlphaProcess.correct state) may both be
+ * NULL for a caller that corrects at the increment's end (process.py:549-551),
+ * which overwrites them before anything reads them.  R (device),
  * K_vals (device, nullable, K_cap entries), res_norm2 (device double, ||R||^2, nullable),
  * n_active (device int32, #(p_n > 0), nullable). */
 int xk_assemble_iteration(xk_ctx* ctx, const double* u, const double* u_ref,

@@ -147,6 +147,50 @@ def test_fused_newton_assembly_matches_reference_two_increments(gpu, precision, 
         asm.set_u_ref(u)
 
 
+def test_assembly_leaves_the_attempts_ga_state_unless_told_not_to(gpu):
+    """By default the fused assembly leaves the attempt's acceleration and
+    velocity in the time integrator, as the reference's loop does with
+    GeneralizedAlphaProcess.correct before effective_residual
+    (time_integration/strategy.py:178-188, _newton_dynamic.py:190-191):
+    acc = c0 (u - u_pred), vel = v_pred + dt gamma acc.  With ga_state=False
+    (a driver that corrects at the increment's end) it leaves them alone, and
+    R and K are bitwise the same."""
+    a = golden("assembly_7wire.npz")
+    C, K, T, A, E = pkg("context"), pkg("contact"), pkg("time_integration"), pkg("assembly"), pkg("elements")
+    k_pen, sd, mu, rho_inf, dt, rho, ktr = a["params"]
+    r = float(a["radius"])
+    X, conn = a["X"], a["conn"]
+    radii = np.full(len(conn), r)
+    s = section(r)
+    Md = E.ULCRBeamAssembler(X, conn, s.E, s.G, s.A, s.Iy, s.Iz, s.J, s.kappa_y, s.kappa_z).lumped_mass_diag(rho)
+    out = []
+    for ga_state in (True, False):
+        ctx = C.DeviceContext(X, conn, section(r), radii=radii, precision="fp64")
+        mgr = K.ContactManager(ctx, K.ContactConfig(mu=mu, k_t_ratio=ktr))
+        K.DetectCandidatesProcess().process(K.DetectCandidatesInput(mgr, X, conn, radii))
+        ga = T.GeneralizedAlphaProcess(Md, rho_inf=rho_inf)
+        asm = A.StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=sd, k_t_ratio=ktr, mu=mu, f_ext=a["f_ext"],
+                               fixed_dofs=a["fixed"], config=mgr.config, time=ga, dt=dt, ga_state=ga_state)
+        torch = ctx.torch
+        ga.predict_device(torch.zeros(ctx.ndof, dtype=torch.float64, device=ctx.dev), dt)
+        asm.begin_increment(a["inc0_it0_u"])
+        ga.d_acc.fill_(7.0)
+        ga.d_vel.fill_(-7.0)
+        R, Kv, _, _ = asm.assemble(a["inc0_it1_u"])
+        u, up, vp = (x.cpu().numpy() for x in (asm.u, ga.d_u_pred, ga.d_v_pred))
+        acc, vel = ga.d_acc.cpu().numpy(), ga.d_vel.cpu().numpy()
+        if ga_state:
+            c0 = 1.0 / (ga.beta * dt**2)
+            acc_ref = c0 * (u - up)
+            np.testing.assert_allclose(acc, acc_ref, rtol=1e-14, atol=0)
+            np.testing.assert_allclose(vel, vp + dt * ga.gamma * acc_ref, rtol=1e-14, atol=1e-300)
+            assert np.abs(acc_ref).max() > 0
+        else:
+            assert np.all(acc == 7.0) and np.all(vel == -7.0)
+        out.append((R.cpu().numpy(), Kv.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("staged", [False, True])
 def test_strategy_api_matches_oracle_on_37_wires(gpu, staged):
     """Drop-in Processes / Strategies (DetectCandidates, UpdateGeometry, Huber,

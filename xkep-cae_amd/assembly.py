@@ -34,7 +34,8 @@ from .time_integration import GeneralizedAlphaProcess
 class StrandAssembly:
     def __init__(self, ctx: DeviceContext, *, k_pen: float, smoothing_delta: float, k_t_ratio: float = 0.5,
                  mu: float = 0.3, f_ext=None, fixed_dofs=None, config: ContactConfig | None = None,
-                 time: GeneralizedAlphaProcess | None = None, dt: float = 0.0, core_radii=None):
+                 time: GeneralizedAlphaProcess | None = None, dt: float = 0.0, core_radii=None,
+                 ga_state: bool = True):
         torch = ctx.torch
         self.ctx = ctx
         self.torch = torch
@@ -64,7 +65,12 @@ class StrandAssembly:
         self.K = None
         self.time = time
         self.dt = float(dt)
-        self._dyn = time.dyn_struct(self.dt) if time is not None else None
+        # each assembly stores the attempt's acceleration and velocity into the time
+        # integrator (GeneralizedAlphaProcess.correct, as the reference's loop does per
+        # attempt, _newton_dynamic.py:190); a driver that corrects at the increment's
+        # end (process.py:549-551), before anything reads them, passes ga_state=False
+        self.ga_state = bool(ga_state)
+        self._dyn = time.dyn_struct(self.dt, state=self.ga_state) if time is not None else None
         self.graph = None
         # learned prescreen (set_prescreen): network, feature layer ids, logging
         self.prescreen = None
@@ -116,7 +122,7 @@ class StrandAssembly:
     def set_dt(self, dt: float):
         """Time step of the inertia terms (a captured graph is dropped)."""
         self.dt = float(dt)
-        self._dyn = self.time.dyn_struct(self.dt) if self.time is not None else None
+        self._dyn = self.time.dyn_struct(self.dt, state=self.ga_state) if self.time is not None else None
         self.graph = None
 
     def deformed_coords(self, u=None):

@@ -87,9 +87,12 @@ __global__ void __launch_bounds__(GB) gather_kernel(GatherArgs g, const ST* __re
                 if (g.dy.on) {
                     const double ac = g.dy.c0 * (g.u[tid] - g.dy.u_pred[tid]);
                     // acc and vel are state for the increment's end, not read by the
-                    // following kernels: stored non-temporally (cache kept for the inputs)
-                    __builtin_nontemporal_store(ac, &g.dy.acc[tid]);
-                    __builtin_nontemporal_store(g.dy.v_pred[tid] + g.dy.dt * g.dy.gamma * ac, &g.dy.vel[tid]);
+                    // following kernels: stored non-temporally (cache kept for the inputs),
+                    // and not at all when the caller recomputes them there (null acc)
+                    if (g.dy.acc) {
+                        __builtin_nontemporal_store(ac, &g.dy.acc[tid]);
+                        __builtin_nontemporal_store(g.dy.v_pred[tid] + g.dy.dt * g.dy.gamma * ac, &g.dy.vel[tid]);
+                    }
                     const double amid = (1.0 - g.dy.alpha_m) * ac + g.dy.alpha_m * g.dy.acc_old[tid];
                     r = r + g.dy.M[tid] * amid;
                     if (fx) r = 0.0;

@@ -976,6 +976,10 @@ int xk_assemble_iteration(xk_ctx* c, const double* u, const double* u_ref, const
     g.use_beam = 1;
     g.use_pair = 1;
     if (dyn && dyn->dt > 1e-30) {
+        if ((dyn->acc == nullptr) != (dyn->vel == nullptr)) {
+            set_error("xk_assemble_iteration: dyn acc and vel are both set or both null");
+            return -1;
+        }
         DynArgs& dy = g.dy;
         dy.on = 1;
         dy.M = dyn->M_diag; dy.u_pred = dyn->u_pred; dy.v_pred = dyn->v_pred; dy.acc_old = dyn->acc_old;

@@ -441,9 +441,10 @@ class DeviceBackend:
         self.graph_attempts_captured = 0
         self.graph_error = None
         self.ga = GeneralizedAlphaProcess(M_diag, rho_inf=rho_inf)
+        # the attempts' acceleration and velocity are set by correct() at the increment's end
         self.asm = StrandAssembly(ctx, k_pen=k_pen, smoothing_delta=smoothing_delta, k_t_ratio=k_t_ratio, mu=mu,
                                   fixed_dofs=fixed_dofs, config=config, time=self.ga, dt=0.0,
-                                  core_radii=core_radii)
+                                  core_radii=core_radii, ga_state=False)
         self._f = self.torch.zeros(ctx.ndof, dtype=self.torch.float64, device=ctx.dev)
         self._dt = None
         self.flags = 0              # every XK_FLAG_* bit the device status word has reported

@@ -116,11 +116,14 @@ class GeneralizedAlphaProcess:
             self.d_vel.copy_(self._ckpt[0])
             self.d_acc.copy_(self._ckpt[1])
 
-    def dyn_struct(self, dt: float) -> L.XkDyn:
+    def dyn_struct(self, dt: float, *, state: bool = True) -> L.XkDyn:
         """Pointers of the device state for the fused assembly kernels (a lumped
-        mass without damping: the kernels apply M as a diagonal)."""
+        mass without damping: the kernels apply M as a diagonal).  Without
+        ``state`` the assembly leaves acc and vel alone (``correct`` at the
+        increment's end sets them)."""
         if not self.lumped or self.C is not None:
             raise ValueError("the fused assembly takes a lumped (diagonal) mass and no damping matrix")
         return L.XkDyn(M_diag=self.d_M.data_ptr(), u_pred=self.d_u_pred.data_ptr(), v_pred=self.d_v_pred.data_ptr(),
-                       acc_old=self.d_acc_old.data_ptr(), acc=self.d_acc.data_ptr(), vel=self.d_vel.data_ptr(),
+                       acc_old=self.d_acc_old.data_ptr(), acc=self.d_acc.data_ptr() if state else None,
+                       vel=self.d_vel.data_ptr() if state else None,
                        dt=dt, alpha_m=self.alpha_m, beta=self.beta, gamma=self.gamma)
