@@ -14,8 +14,17 @@
 // Every candidate alpha is evaluated (one element force pass and one residual
 // norm each) and the first accepted one is selected on the device: the result
 // is the sequential search's, without a host round trip per candidate.
-// Reductions run in one workgroup in a fixed order (deterministic, no atomics,
-// no memset).
+// The candidates' residual norms and the update's norms are reduced in two
+// launches over the chip, the optional step cap's norms in one workgroup, each
+// in a fixed order (deterministic, no atomics, no memset).
+//
+// The device Newton control (the nc_* kernels) evaluates the driver's tests
+// into a per-call control record: force convergence and divergence after the
+// assembly, the solve check before the line search, displacement and energy
+// convergence and the attempt limit after the update.  Each is a single-lane
+// kernel with the host driver's arithmetic, so the device takes the branches
+// the host would; every attempt kernel of the library reads the record's state
+// word first and does nothing once the call has ended.
 
 #include "../../include/xkamd.h"
 #include "xk_common.h"

@@ -1,7 +1,8 @@
 """Dynamic frictional-contact Newton driver around the device assembly.
 
 Restates, for the strand path (smooth-penalty Huber contact, Coulomb friction,
-Generalized-alpha dynamics, no coating, no prescribed displacements), the
+Generalized-alpha dynamics, optionally the Kelvin-Voigt coating, no prescribed
+displacements), the
 reference's solver loop so that the assembly on the GPU can be driven through
 whole solves and its Newton trajectory compared with the reference's:
 
@@ -24,8 +25,10 @@ the host, as in the reference.  The backend protocol is small so that the same
 driver runs on the device (``DeviceBackend``) or on any other assembly with
 the reference's semantics.  ``DeviceBackend(device_solve=True)`` instead runs
 the Newton attempts of an increment on the device: the assembly, the linear
-solve (``xk_linear_solve``), the line search and the updates stay in HBM and
-the host reads a few scalars per attempt.
+solve (``xk_linear_solve``), the line search and the updates stay in HBM, and
+the driver's convergence and divergence tests run on the device as well
+(``xk_newton_control_*``): attempts are enqueued back to back and the host
+reads each attempt's control record afterwards.
 """
 
 from __future__ import annotations
@@ -416,8 +419,9 @@ class DeviceBackend:
     """The solver backend on one GPU: candidates, geometry, the fused assembly
     (R_eff, K_eff, ||R||^2, #active in one graph-capturable launch sequence),
     the element-only internal force of the line search and the
-    Generalized-alpha state all live on the device; R, K and f_c are copied to
-    the host for the SuperLU solve."""
+    Generalized-alpha state all live on the device.  By default R, K and f_c
+    are copied to the host for the SuperLU solve; with ``device_solve`` whole
+    Newton calls run on the device (``newton``)."""
 
     def __init__(self, ctx, *, M_diag, rho_inf: float, k_pen: float, smoothing_delta: float, mu: float,
                  k_t_ratio: float = 0.5, fixed_dofs=None, config=None, device_solve: bool = False,
@@ -509,13 +513,15 @@ class DeviceBackend:
         """``newton_dynamic`` with the vectors in HBM: the linear solve
         (LinearSolveProcess -> xk_linear_solve) and the line search + update
         (LineSearchUpdateProcess -> xk_newton_update) on the device, with the
-        same control flow, thresholds and order of updates.  The host reads
-        (||R||^2, n_active) after the assembly and (alpha, ||du||, ||u||,
-        |du.R|) after the update of every attempt.  With ``use_graph`` the
-        attempts after the first replay two captured HIP graphs: G1 =
-        assembly + pair force, G2 = linear solve (a fixed number of
-        refinement steps) + line search + update.  ``u`` (host) is updated in
-        place at the end."""
+        same control flow, thresholds and order of updates.  With
+        ``device_control`` (the default) the driver's tests run on the device
+        too (``_newton_controlled``).  Otherwise the host reads (||R||^2,
+        n_active) after the assembly, the solve's residual, and (alpha,
+        ||du||, ||u||, |du.R|) after the update of every attempt; with
+        ``use_graph`` the attempts after the first replay three captured HIP
+        graphs: G1 = assembly + pair force, G2 = linear solve (a fixed number
+        of refinement steps), G3 = line search + update.  ``u`` (host) is
+        updated in place at the end."""
         torch, asm, ctx = self.torch, self.asm, self.ctx
         if self.device_control:
             return self._newton_controlled(u, u_ref, f_ext, dt, f_ext_ref_norm, dynamic_ref, cfg)
