@@ -402,6 +402,7 @@ __global__ void sv_info_kernel(const double* __restrict__ stop,
 // factors the block again with partial pivoting.
 constexpr int LU_NB = 64;   // panel width = tile size
 constexpr int LU_SB = 16;   // sub-block inside the panel
+constexpr int LU_NP = 2;    // panels per outer panel (one trailing update each)
 
 
 // FULL: the panel holds LU_NB columns (every panel but a last partial one):
@@ -913,19 +914,19 @@ __global__ __launch_bounds__(TB) void sv_upper_step_kernel(const double* __restr
         double v = t < m ? z[k0 + t] : 0.0;
         for (int i = 0; i < j; ++i)
             v = sv_block_dot(v, LU, ld, t < m ? k0 + t : k0, (s0 - i) * TB, min(TB, n - (s0 - i) * TB), xs[i]);
-        if (t >= m) v = 0.0;
+        // back substitution on the row scaled by the reciprocal of its
+        // diagonal: x_t = z_t / u_tt - sum_{k > t} (u_tk / u_tt) x_k, so step k
+        // reads lane k's finished value with no multiply in the chain
+        const double idg = 1.0 / (t < m ? LU[(size_t)(k0 + t) + (size_t)(k0 + t) * ld] : 1.0);
+        v = t < m ? v * idg : 0.0;
         double row[TB];
 #pragma unroll
         for (int k = 0; k < TB; ++k)
             row[k] = (t < m && k > t && k < m) ? LU[(size_t)(k0 + t) + (size_t)(k0 + k) * ld] : 0.0;
-        const double idg = 1.0 / (t < m ? LU[(size_t)(k0 + t) + (size_t)(k0 + t) * ld] : 1.0);
+        // (the scaling of row[k] is off the chain through v, and after the loads:
+        // scaling where the masked loads are issued would wait on each load)
 #pragma unroll
-        for (int k = TB - 1; k >= 0; --k) {
-            if (k < m) {   // uniform
-                if (t == k) v = v * idg;
-                v = fma(-row[k], sv_readlane(v, k), v);   // row[k] = 0 for k <= t
-            }
-        }
+        for (int k = TB - 1; k >= 0; --k) v = fma(-(row[k] * idg), sv_readlane(v, k), v);   // row[k] = 0 for k <= t
         xs[j][t] = v;
         if (blockIdx.x == 0 && t < m) x[k0 + t] = v;
         __syncthreads();
@@ -1334,10 +1335,11 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
         int64_t* ipiv = (int64_t*)sv[SV_IPIV].p + 3 * sv_comp_pos[c];
         int64_t* info = (int64_t*)sv[SV_INFO].p + c;
         if (!gepp && n <= LU_MAX) {
-            // Outer panels of two LU_NB panels: P0 is factored and updates only
-            // P1's columns and P1's rows right of it (thin DGEMMs); P1 is factored;
-            // one rank-2 LU_NB DGEMM then updates the trailing matrix with both, so
-            // the trailing matrix streams through HBM once per 2 LU_NB columns.
+            // Outer panels of LU_NP panels of LU_NB columns: each panel first takes
+            // the outer panel's earlier panels on its own columns and rows (thin
+            // DGEMMs), then is factored; one rank-(LU_NP LU_NB) DGEMM then updates the
+            // trailing matrix with the whole outer panel, so the trailing matrix
+            // streams through HBM once per LU_NP LU_NB columns.
             const double one = 1.0, minus_one = -1.0;
             const size_t ld = (size_t)n;
             auto panel = [&](int64_t j0) -> int {
@@ -1361,15 +1363,24 @@ int Ctx::sv_factor_capacitance(bool gepp, hipStream_t st) {
                                     (rocblas_int)n, &one, C_, (rocblas_int)n));
                 return 0;
             };
-            for (int64_t J = 0; J < n; J += 2 * LU_NB) {
-                if (panel(J) != 0) return -1;
-                const int64_t w0 = std::min<int64_t>(LU_NB, n - J), j1 = J + w0;
-                if (j1 >= n) break;
-                const int64_t w1 = std::min<int64_t>(LU_NB, n - j1), j2 = j1 + w1;
-                if (update(n - j1, w1, w0, S + j1 + J * ld, S + J + j1 * ld, S + j1 + j1 * ld) != 0) return -1;
-                if (update(w1, n - j2, w0, S + j1 + J * ld, S + J + j2 * ld, S + j1 + j2 * ld) != 0) return -1;
-                if (panel(j1) != 0) return -1;
-                if (update(n - j2, n - j2, w0 + w1, S + j2 + J * ld, S + J + j2 * ld, S + j2 + j2 * ld) != 0)
+            for (int64_t J = 0; J < n; J += (int64_t)LU_NP * LU_NB) {
+                const int64_t jE = std::min<int64_t>(n, J + (int64_t)LU_NP * LU_NB);
+                for (int64_t jp = J; jp < jE; jp += LU_NB) {
+                    const int64_t jp1 = std::min<int64_t>(jp + LU_NB, n);
+                    if (jp > J) {
+                        // the panel's columns (rows from jp down) and rows (columns right of
+                        // it) take the outer panel's earlier panels, rank jp - J
+                        const int64_t kk = jp - J;
+                        if (update(n - jp, jp1 - jp, kk, S + jp + J * ld, S + J + jp * ld, S + jp + jp * ld) != 0)
+                            return -1;
+                        if (update(jp1 - jp, n - jp1, kk, S + jp + J * ld, S + J + jp1 * ld, S + jp + jp1 * ld) != 0)
+                            return -1;
+                    }
+                    if (panel(jp) != 0) return -1;
+                }
+                if (jE >= n) break;
+                // the trailing matrix, once per outer panel
+                if (update(n - jE, n - jE, jE - J, S + jE + J * ld, S + J + jE * ld, S + jE + jE * ld) != 0)
                     return -1;
             }
             hipLaunchKernelGGL(sv_ipiv_identity_kernel, dim3(sv_grid(n)), dim3(256), 0, st, nstop, n, ipiv);
