@@ -438,15 +438,18 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restri
             double v[LU_SB];
 #pragma unroll
             for (int k = 0; k < LU_SB; ++k) v[k] = (i < bw && k < bw) ? D[o + i][o + k] : 0.0;
+            // lane c keeps pivot c's reciprocal and validity in registers (no LDS
+            // traffic and no branch on the chain from one pivot to the next)
+            double my_ip = 0.0;
+            bool my_bad = false;
 #pragma unroll
             for (int c = 0; c < LU_SB; ++c) {
                 if (c < bw) {   // uniform
                     const double p = sv_readlane(v[c], c);
                     const double ip = 1.0 / p;
-                    if (i == 0) {
-                        if (s_bad < 0 && !(fabs(p) > 0.0 && fabs(p) <= DBL_MAX)) s_bad = o + c;
-                        invd[o + c] = ip;
-                    }
+                    const bool bad = !(fabs(p) > 0.0 && fabs(p) <= DBL_MAX);
+                    my_ip = i == c ? ip : my_ip;
+                    my_bad = i == c ? bad : my_bad;
                     const bool below = i > c && i < bw;
                     const double l = below ? v[c] * ip : 0.0;   // rows at or above c: unchanged
 #pragma unroll
@@ -454,6 +457,9 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restri
                     if (below) v[c] = l;
                 }
             }
+            if (i < bw) invd[o + i] = my_ip;
+            const unsigned long long badm = __ballot(i < bw && my_bad);
+            if (i == 0 && badm != 0ull && s_bad < 0) s_bad = o + __ffsll((long long)badm) - 1;   // the first one
             if (i < bw)
 #pragma unroll
                 for (int k = 0; k < LU_SB; ++k)
