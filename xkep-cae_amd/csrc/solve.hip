@@ -699,6 +699,8 @@ __global__ void sv_ipiv_identity_kernel(const double* __restrict__ stop, int64_t
     if (i < n) ipiv[i] = i + 1;
 }
 
+constexpr int CS_PF = 4;   // nodes in flight in the chain sweeps
+
 // x <- A~_b^-1 b along every chain (b and x may be the same vector): 8 lanes
 // per chain (lanes 0..5 hold the node's six components), 8 chains per
 // wavefront; rows of the 6x6 blocks are combined across the chain's lanes by
@@ -714,74 +716,101 @@ __global__ __launch_bounds__(64) void sv_chain_solve_kernel(const double* __rest
     const int c = blockIdx.x * 8 + (lane >> 3);
     if (c >= n_chains || a >= 6) return;
     const int n0 = cstart[c], n1 = cstart[c + 1];
-    // forward: y_n = x_n - L_n y_{n-1}; node n + 1's row and value are loaded
-    // before node n's shuffles so that the loads overlap the dependent chain
+    // forward: y_n = x_n - L_n y_{n-1}.  The rows and values of the next
+    // CS_PF nodes are in flight while node n's shuffles run (a ring of
+    // registers, slot q holding node n0 + q mod CS_PF): one node ahead left
+    // each step waiting on its loads.
     double y = 0.0;
-    double xv = b[6 * (size_t)n0 + a];
-    double L[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int n = n0; n < n1; ++n) {
-        double xn = 0.0, Ln[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (n + 1 < n1) {
-            xn = b[6 * (size_t)(n + 1) + a];
-            const double* p = Lf + 36 * (size_t)(n + 1) + 6 * a;
+    double xr[CS_PF], Lr[CS_PF][6];
 #pragma unroll
-            for (int b = 0; b < 6; ++b) Ln[b] = p[b];
-        }
-        double v = xv;
-        if (n > n0) {
-            double t = 0.0;
+    for (int q = 0; q < CS_PF; ++q) {
+        const int m = n0 + q;
+        xr[q] = 0.0;
 #pragma unroll
-            for (int b = 0; b < 6; ++b) t = fma(L[b], __shfl(y, base + b, 64), t);
-            v -= t;
-        }
-        y = v;
-        x[6 * (size_t)n + a] = v;
-        xv = xn;
+        for (int b = 0; b < 6; ++b) Lr[q][b] = 0.0;
+        if (m < n1) {
+            xr[q] = b[6 * (size_t)m + a];
+            if (m > n0) {
+                const double* p = Lf + 36 * (size_t)m + 6 * a;
 #pragma unroll
-        for (int b = 0; b < 6; ++b) L[b] = Ln[b];
-    }
-    // backward: x_n = S_n^-1 (y_n - E_n x_{n+1}), prefetched the same way
-    double z = 0.0;
-    double yv = x[6 * (size_t)(n1 - 1) + a];
-    double E[6], Si[6];
-    {
-        const double* pe = Ab + 108 * (size_t)(n1 - 1) + 18 * a + 12;
-        const double* ps = Sinv + 36 * (size_t)(n1 - 1) + 6 * a;
-#pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            E[b] = pe[b];
-            Si[b] = ps[b];
-        }
-    }
-    for (int n = n1 - 1; n >= n0; --n) {
-        double yn = 0.0, En[6], Sn[6];
-        if (n - 1 >= n0) {
-            yn = x[6 * (size_t)(n - 1) + a];
-            const double* pe = Ab + 108 * (size_t)(n - 1) + 18 * a + 12;
-            const double* ps = Sinv + 36 * (size_t)(n - 1) + 6 * a;
-#pragma unroll
-            for (int b = 0; b < 6; ++b) {
-                En[b] = pe[b];
-                Sn[b] = ps[b];
+                for (int b = 0; b < 6; ++b) Lr[q][b] = p[b];
             }
         }
-        double v = yv;
-        if (n < n1 - 1) {
-            double t = 0.0;
+    }
+    for (int n = n0; n < n1; n += CS_PF) {
 #pragma unroll
-            for (int b = 0; b < 6; ++b) t = fma(E[b], __shfl(z, base + b, 64), t);
-            v -= t;
+        for (int q = 0; q < CS_PF; ++q) {
+            const int m = n + q;
+            if (m < n1) {
+                double v = xr[q];
+                if (m > n0) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) t = fma(Lr[q][b], __shfl(y, base + b, 64), t);
+                    v -= t;
+                }
+                y = v;
+                x[6 * (size_t)m + a] = v;
+                const int mf = m + CS_PF;   // refill the slot
+                if (mf < n1) {
+                    xr[q] = b[6 * (size_t)mf + a];
+                    const double* p = Lf + 36 * (size_t)mf + 6 * a;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) Lr[q][b] = p[b];
+                }
+            }
         }
-        double w = 0.0;
+    }
+    // backward: x_n = S_n^-1 (y_n - E_n x_{n+1}), the same ring from the last
+    // node down (slot q holding node n1 - 1 - q mod CS_PF)
+    double z = 0.0;
+    double yr[CS_PF], Er[CS_PF][6], Sr[CS_PF][6];
 #pragma unroll
-        for (int b = 0; b < 6; ++b) w = fma(Si[b], __shfl(v, base + b, 64), w);
-        z = w;
-        x[6 * (size_t)n + a] = w;
-        yv = yn;
+    for (int q = 0; q < CS_PF; ++q) {
+        const int m = n1 - 1 - q;
+        yr[q] = 0.0;
 #pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            E[b] = En[b];
-            Si[b] = Sn[b];
+        for (int b = 0; b < 6; ++b) Er[q][b] = Sr[q][b] = 0.0;
+        if (m >= n0) {
+            yr[q] = x[6 * (size_t)m + a];
+            const double* pe = Ab + 108 * (size_t)m + 18 * a + 12;
+            const double* ps = Sinv + 36 * (size_t)m + 6 * a;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                Er[q][b] = pe[b];
+                Sr[q][b] = ps[b];
+            }
+        }
+    }
+    for (int n = n1 - 1; n >= n0; n -= CS_PF) {
+#pragma unroll
+        for (int q = 0; q < CS_PF; ++q) {
+            const int m = n - q;
+            if (m >= n0) {
+                double v = yr[q];
+                if (m < n1 - 1) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) t = fma(Er[q][b], __shfl(z, base + b, 64), t);
+                    v -= t;
+                }
+                double w = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) w = fma(Sr[q][b], __shfl(v, base + b, 64), w);
+                z = w;
+                x[6 * (size_t)m + a] = w;
+                const int mf = m - CS_PF;
+                if (mf >= n0) {
+                    yr[q] = x[6 * (size_t)mf + a];
+                    const double* pe = Ab + 108 * (size_t)mf + 18 * a + 12;
+                    const double* ps = Sinv + 36 * (size_t)mf + 6 * a;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) {
+                        Er[q][b] = pe[b];
+                        Sr[q][b] = ps[b];
+                    }
+                }
+            }
         }
     }
 }
