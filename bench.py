@@ -449,6 +449,37 @@ def newton_loop(torch, wl, force=0.1):
     return out
 
 
+def coated_newton_loop(torch, wl):
+    """A whole dynamic solve with the Kelvin-Voigt coating on: the 7-wire radial
+    squeeze (workloads.squeeze_case(100), frictionless contact, coating
+    stiffness 0.001 k_pen, coating mu 0.2, core radii 0.95 r), device attempts
+    with the driver's tests on the device (a pair that starts pressing its
+    coating without a pattern slot stops the attempt for a host rebuild), the
+    same from graphs, and with the tests on the host (a host read per attempt
+    and per assembly's pattern probe)."""
+    TP = importlib.import_module("xkep-cae_amd.train_prescreen")
+    sq = wl.squeeze_case(100.0)
+    coat = dict(coating_stiffness=0.001 * sq["k_pen"], coating_damping=0.0, coating_mu=0.2, coating_k_t_ratio=0.5)
+    out, traj = {}, {}
+    for mode in ("device", "device_graph", "device_hostctl"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, b = TP.run_case(sq, precision="fp64", device_solve=True, use_graph=mode == "device_graph",
+                             device_control=mode != "device_hostctl", coating=coat, core_radii=0.95 * sq["radii"])
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        att = sum(c.n_attempts for c in res.calls)
+        traj[mode] = [(c.n_attempts, c.n_active, c.converged) for c in res.calls]
+        out[mode] = {"seconds": round(t, 3), "attempts": att, "ms_per_attempt": round(1e3 * t / max(att, 1), 2),
+                     "pattern_rebuilds": b.asm.n_coat_rebuilds}
+        log(f"[bench] coated Newton loop 7 wires {mode}: {out[mode]}")
+    out["trajectories_equal"] = traj["device"] == traj["device_graph"] == traj["device_hostctl"]
+    out["calls"] = len(traj["device"])
+    out["workload"] = ("7-wire strand, dynamic radial squeeze 100 N per outer wire, Kelvin-Voigt coating "
+                       "(k 0.001 k_pen, coating mu 0.2, core radii 0.95 r), FP64 device attempts")
+    return out
+
+
 def side_measurements(batched, torch, wl, args):
     """Beside the headline: the learned prescreen on the batched 91-wire
     workload, the uneven 64-instance batch, and the 1000-wire strand (FP32-mixed,
@@ -458,6 +489,7 @@ def side_measurements(batched, torch, wl, args):
     steps = max(20, args.steps // 4)
     loop = newton_loop(torch, wl)
     out["newton_loop_91wire"] = loop
+    out["coated_newton_loop_7wire"] = coated_newton_loop(torch, wl)
     att_per_call = loop["device"]["attempts"] / max(len(loop["calls"]), 1)
     P = importlib.import_module("xkep-cae_amd.prescreen")
     if os.path.exists(P.WEIGHTS):
@@ -573,7 +605,8 @@ def main():
     case = wl.C3
     if args.newton_only:
         print(json.dumps({"newton_attempt_91wire": newton_attempt(case, torch, wl),
-                          "newton_loop_91wire": newton_loop(torch, wl)}), flush=True)
+                          "newton_loop_91wire": newton_loop(torch, wl),
+                          "coated_newton_loop_7wire": coated_newton_loop(torch, wl)}), flush=True)
         return
     st = wl.make_state(case)
     if args.coated_only:

@@ -32,21 +32,23 @@ TRAIN_RUNS = [(f, nw) for f in (60.0, 80.0, 100.0, 120.0, 140.0) for nw in (7, 1
 
 
 def run_case(case, *, precision: str = "fp64", net=None, log: bool = False, device_solve: bool = False,
-             use_graph: bool = False, device_control: bool = True):
+             use_graph: bool = False, device_control: bool = True, coating: dict | None = None, core_radii=None):
     """Solve one squeeze case on the device; returns (SolverResult, backend).
     device_solve / use_graph: the Newton attempts on the device (linear solve,
     line search, update), after each increment's first as HIP-graph replays;
-    device_control: their convergence and divergence tests on the device too."""
+    device_control: their convergence and divergence tests on the device too;
+    coating: ContactConfig's coating fields, with the pairs' core radii."""
     r = float(case["radii"][0])
     sec = C.circle_section(r, W.E_STEEL, W.NU_STEEL)
     ctx = C.DeviceContext(case["X"], case["conn"], sec, radii=case["radii"], precision=precision)
     Md = E.ULCRBeamAssembler(case["X"], case["conn"], sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J, sec.kappa_y,
                              sec.kappa_z).lumped_mass_diag(W.RHO_STEEL)
     kcfg = K.ContactConfig(mu=case["mu"], k_t_ratio=case["ktr"], exclude_same_layer=True,
-                           elem_layer_map=W.layer_map(case["layer"]))
+                           elem_layer_map=W.layer_map(case["layer"]), **(coating or {}))
     b = S.DeviceBackend(ctx, M_diag=Md, rho_inf=case["rho_inf"], k_pen=case["k_pen"], smoothing_delta=case["sd"],
                         mu=case["mu"], k_t_ratio=case["ktr"], fixed_dofs=case["fixed"], config=kcfg,
-                        device_solve=device_solve, use_graph=use_graph, device_control=device_control)
+                        device_solve=device_solve, use_graph=use_graph, device_control=device_control,
+                        core_radii=core_radii)
     if net is not None:
         b.use_prescreen(net, elem_layer=case["layer"])
     if log:
