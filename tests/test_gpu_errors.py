@@ -8,7 +8,9 @@ zero-length element raises ValueError):
   (device attempts and host SuperLU attempts alike);
 - a zero-length element sets XK_FLAG_DEGENERATE and the drop-ins raise;
 - a tangent without stiffness sets XK_FLAG_SINGULAR in the device solve and
-  ends the Newton call as failed."""
+  ends the Newton call as failed;
+- argument errors of the device Newton control's coating probe and of the
+  optional per-attempt GA state come back as errors before any launch."""
 
 import numpy as np
 import pytest
@@ -132,3 +134,28 @@ def test_singular_tangent_flags_and_fails_the_call(gpu):
     assert b.flags & S.FLAG_SINGULAR
     assert b.control_reasons == [4]   # XK_NCR_SINGULAR: stopped on the device after the solve
     assert np.all(u == 0.0)    # no update after the failed solve
+
+
+@pytest.mark.gpu
+def test_control_probe_and_partial_ga_state_are_refused(gpu):
+    """Argument errors come back as errors, before any launch: the coating
+    probe of a device-controlled call without an attached control record, and
+    an assembly given only one of the two per-attempt GA state pointers."""
+    L, T, A = pkg("_lib"), pkg("time_integration"), pkg("assembly")
+    case, st, ctx = _strand()
+    ctx.set_coating(0.1 * case.k_pen, 0.0, 0.2, 0.5, None)
+    with pytest.raises(RuntimeError, match="no Newton control record attached"):
+        ctx.coating_probe_control(ctx.to_dev(np.zeros(ctx.ndof)))
+    ctx.set_coating(0.0)
+    E, C = pkg("elements"), pkg("context")
+    m = st.mesh
+    sec = C.circle_section(case.wire_radius, case.E, case.nu)
+    Md = E.ULCRBeamAssembler(m.node_coords, m.connectivity, sec.E, sec.G, sec.A, sec.Iy, sec.Iz, sec.J,
+                             sec.kappa_y, sec.kappa_z).lumped_mass_diag(case.rho)
+    ga = T.GeneralizedAlphaProcess(Md, rho_inf=0.8)
+    asm = A.StrandAssembly(ctx, k_pen=case.k_pen, smoothing_delta=case.smoothing_delta, mu=case.mu,
+                           f_ext=st.f_ext, fixed_dofs=st.fixed_dofs, time=ga, dt=1e-5)
+    asm.begin_increment(np.zeros(ctx.ndof))
+    asm._dyn.vel = None   # acc set, vel not
+    with pytest.raises(RuntimeError, match="acc and vel are both set or both null"):
+        asm.assemble(st.u)
