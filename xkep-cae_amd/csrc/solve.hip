@@ -425,6 +425,30 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restri
         const int k = tid + 256 * it, r = k & (LU_NB - 1), q = k / LU_NB;
         D[r][q] = (r < w && q < w) ? A[(size_t)(j0 + r) + (size_t)(j0 + q) * ld] : 0.0;
     }
+    // the workgroup's tile (rows below or columns right of the panel, which the
+    // diagonal block's factorization does not touch) is loaded now, so that its
+    // loads overlap the diagonal block's
+    const int rest = n - j0 - w;
+    const bool row_tile = (int)blockIdx.x < n_row_wg;
+    const int row0 = j0 + w + (int)blockIdx.x * LU_NB, mr = min(LU_NB, n - row0);
+    const int col0 = j0 + w + ((int)blockIdx.x - n_row_wg) * LU_NB, nc = min(LU_NB, n - col0);
+    if (rest > 0) {   // uniform
+        if (row_tile) {
+            // X[i][k] = A[row0 + i, j0 + k]
+#pragma unroll
+            for (int it = 0; it < CP; ++it) {
+                const int k = tid + 256 * it, i = k & (LU_NB - 1), q = k / LU_NB;
+                X[i][q] = (i < mr && q < w) ? A[(size_t)(row0 + i) + (size_t)(j0 + q) * ld] : 0.0;
+            }
+        } else {
+            // X[k][j] = A[j0 + k, col0 + j]
+#pragma unroll
+            for (int it = 0; it < CP; ++it) {
+                const int k = tid + 256 * it, q = k & (LU_NB - 1), j = k / LU_NB;
+                X[q][j] = (q < w && j < nc) ? A[(size_t)(j0 + q) + (size_t)(col0 + j) * ld] : 0.0;
+            }
+        }
+    }
     if (tid == 0) s_bad = -1;
     __syncthreads();
     // ---- diagonal block
@@ -546,18 +570,9 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restri
             *info = v;
         }
     }
-    const int rest = n - j0 - w;
     if (rest <= 0) return;
-    if ((int)blockIdx.x < n_row_wg) {
+    if (row_tile) {
         // ---- LU_NB rows of L21: X U11^-1, X[i][k] = A[row0 + i, j0 + k]
-        const int row0 = j0 + w + blockIdx.x * LU_NB;
-        const int mr = min(LU_NB, n - row0);
-#pragma unroll
-        for (int it = 0; it < CP; ++it) {
-            const int k = tid + 256 * it, i = k & (LU_NB - 1), q = k / LU_NB;
-            X[i][q] = (i < mr && q < w) ? A[(size_t)(row0 + i) + (size_t)(j0 + q) * ld] : 0.0;
-        }
-        __syncthreads();
 #pragma unroll 1
         for (int o = 0; o < w; o += LU_SB) {
             const int bw = FULL ? LU_SB : min(LU_SB, w - o);
@@ -622,14 +637,6 @@ __global__ __launch_bounds__(256) void sv_lu_panel_kernel(const double* __restri
         return;
     }
     // ---- LU_NB columns of U12: L11^-1 Y, X[k][j] = A[j0 + k, col0 + j]
-    const int col0 = j0 + w + (blockIdx.x - n_row_wg) * LU_NB;
-    const int nc = min(LU_NB, n - col0);
-#pragma unroll
-    for (int it = 0; it < CP; ++it) {
-        const int k = tid + 256 * it, q = k & (LU_NB - 1), j = k / LU_NB;
-        X[q][j] = (q < w && j < nc) ? A[(size_t)(j0 + q) + (size_t)(col0 + j) * ld] : 0.0;
-    }
-    __syncthreads();
 #pragma unroll 1
     for (int o = 0; o < w; o += LU_SB) {
         const int bw = FULL ? LU_SB : min(LU_SB, w - o);
