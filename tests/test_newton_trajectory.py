@@ -10,7 +10,11 @@ engages and the driver must fail the same way.
 
 On the CPU the driver runs over the oracle's FP64 assembly; on the GPU over
 the device assembly (FP64 and FP32-mixed tangents), with the host SuperLU
-solve or with the Newton attempts on the device (device linear solve)."""
+solve or with the Newton attempts on the device (device linear solve), the
+driver's tests on the host or on the device, eagerly or from HIP graphs.  At
+91 wires (the benchmark's frictional squeeze) the device-tested attempts take
+the trajectory of the driver on the oracle assembly
+(tests/golden/newton_91wire_mu03_oracle.npz)."""
 
 import numpy as np
 import pytest
