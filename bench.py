@@ -480,6 +480,49 @@ def coated_newton_loop(torch, wl):
     return out
 
 
+def prescreen_leg(r, net, K, torch, steps, warmup, label, workload):
+    """The learned prescreen on an assembled workload r (run()'s result): the
+    watch list's step, and that step with the verification pass after a call of
+    K assemblies and the per-increment features and selection spread over them."""
+    asm, ctx = r["asm"], r["ctx"]
+    asm.set_prescreen(net, elem_layer=asm.bench_layer)
+    asm.begin_increment(detect=False)          # features + selection of the increment
+    watched = asm.n_watch / max(ctx.n_pairs, 1)
+    t_feat, _ = _timed(torch, lambda: ctx.lp_features(asm.u, asm.lp_layer, asm.lp_scale))
+    t_sel, _ = _timed(torch, lambda: ctx.lp_select())
+    stages = profile_stages(asm, ctx, torch, 20)
+    ms, _ = time_graph(asm, torch, steps, warmup)
+    # the verification pass after a call of K assemblies: replay of their geometry for the pruned pairs
+    ctx.lp_select()
+    for _ in range(K):
+        asm.replay()
+    t_ver, _ = _timed(torch, lambda: ctx.lp_verify(asm.u), reps=1)
+    n_eval_watch = ctx.n_elems + asm.n_watch
+    per_attempt_ms = ms + (t_ver + t_feat + t_sel) / K
+    log(f"[bench] learned prescreen on {label}: watched {watched:.3f} of {ctx.n_pairs} pairs | "
+        f"{ms * 1e3:.1f} us/step (graph) | stages us {np.round(stages * 1e3, 1).tolist()} | verify after {K} "
+        f"assemblies {t_ver:.2f} ms, features {t_feat:.2f} ms, selection {t_sel:.2f} ms")
+    rec = {
+        "workload": workload,
+        "watched_fraction": round(watched, 4), "watched_pairs": int(asm.n_watch),
+        "us_per_step": round(ms * 1e3, 2), "stages_us": np.round(stages * 1e3, 2).tolist(),
+        "verify_ms_per_call": round(t_ver, 3), "features_ms_per_increment": round(t_feat, 3),
+        "select_ms_per_increment": round(t_sel, 3), "attempts_per_call": K,
+        "us_per_attempt_amortized": round(per_attempt_ms * 1e3, 2),
+        "value": round(n_eval_watch / (ms * 1e-3), 1),
+        "value_amortized": round(n_eval_watch / (per_attempt_ms * 1e-3), 1),
+        "speedup_vs_full_path": round(r["ms"] / ms, 3),
+        "speedup_vs_full_path_amortized": round(r["ms"] / per_attempt_ms, 3),
+        "value_note": "evals = elements + watched pairs (the narrowphase visits only those); amortized adds the "
+                      "verification pass (geometry replay of the call's assemblies for the pruned pairs) per call "
+                      "and the features and selection per increment, spread over the call's attempts",
+        "exactness": "contributing pairs always visited, pruned pairs' geometry replayed per assembly: R, K_T, "
+                     "n_active and every pair state identical to the full path"}
+    asm.set_prescreen(None)
+    ctx.lp_off()
+    return rec
+
+
 def side_measurements(batched, torch, wl, args):
     """Beside the headline: the learned prescreen on the batched 91-wire
     workload, the uneven 64-instance batch, and the 1000-wire strand (FP32-mixed,
@@ -492,45 +535,12 @@ def side_measurements(batched, torch, wl, args):
     out["coated_newton_loop_7wire"] = coated_newton_loop(torch, wl)
     att_per_call = loop["device"]["attempts"] / max(len(loop["calls"]), 1)
     P = importlib.import_module("xkep-cae_amd.prescreen")
-    if os.path.exists(P.WEIGHTS):
-        asm, ctx = batched["asm"], batched["ctx"]
-        net = P.PrescreenNet.load()
-        asm.set_prescreen(net, elem_layer=asm.bench_layer)
-        asm.begin_increment(detect=False)          # features + selection of the increment
-        watched = asm.n_watch / max(ctx.n_pairs, 1)
-        t_feat, _ = _timed(torch, lambda: ctx.lp_features(asm.u, asm.lp_layer, asm.lp_scale))
-        t_sel, _ = _timed(torch, lambda: ctx.lp_select())
-        stages = profile_stages(asm, ctx, torch, 20)
-        ms, _ = time_graph(asm, torch, args.steps, args.warmup)
-        # the verification pass after a call of K assemblies: replay of their geometry for the pruned pairs
-        K = int(round(att_per_call))
-        ctx.lp_select()
-        for _ in range(K):
-            asm.replay()
-        t_ver, _ = _timed(torch, lambda: ctx.lp_verify(asm.u), reps=1)
-        n_eval_watch = ctx.n_elems + asm.n_watch
-        per_attempt_ms = ms + (t_ver + t_feat + t_sel) / K
-        log(f"[bench] learned prescreen on mixed x{args.batch}: watched {watched:.3f} of {ctx.n_pairs} pairs | "
-            f"{ms * 1e3:.1f} us/step (graph) | stages us {np.round(stages * 1e3, 1).tolist()} | verify after {K} "
-            f"assemblies {t_ver:.2f} ms, features {t_feat:.2f} ms, selection {t_sel:.2f} ms")
-        out["learned_prescreen"] = {
-            "workload": f"same batch of {args.batch} x 91 wires, pair kernel over the learned watch list",
-            "watched_fraction": round(watched, 4), "watched_pairs": int(asm.n_watch),
-            "us_per_step": round(ms * 1e3, 2), "stages_us": np.round(stages * 1e3, 2).tolist(),
-            "verify_ms_per_call": round(t_ver, 3), "features_ms_per_increment": round(t_feat, 3),
-            "select_ms_per_increment": round(t_sel, 3), "attempts_per_call": K,
-            "us_per_attempt_amortized": round(per_attempt_ms * 1e3, 2),
-            "value": round(n_eval_watch / (ms * 1e-3), 1),
-            "value_amortized": round(n_eval_watch / (per_attempt_ms * 1e-3), 1),
-            "speedup_vs_full_path": round(batched["ms"] / ms, 3),
-            "speedup_vs_full_path_amortized": round(batched["ms"] / per_attempt_ms, 3),
-            "value_note": "evals = elements + watched pairs (the narrowphase visits only those); amortized adds the "
-                          "verification pass (geometry replay of the call's assemblies for the pruned pairs) per call "
-                          "and the features and selection per increment, spread over the call's attempts",
-            "exactness": "contributing pairs always visited, pruned pairs' geometry replayed per assembly: R, K_T, "
-                         "n_active and every pair state identical to the full path"}
-        asm.set_prescreen(None)
-        ctx.lp_off()
+    net = P.PrescreenNet.load() if os.path.exists(P.WEIGHTS) else None
+    K = int(round(att_per_call))
+    if net is not None:
+        out["learned_prescreen"] = prescreen_leg(
+            batched, net, K, torch, args.steps, args.warmup, f"mixed x{args.batch}",
+            f"same batch of {args.batch} x 91 wires, pair kernel over the learned watch list")
     cases4, _, b4, mu4 = wl.uneven_batch()
     r4 = run("mixed uneven x64", cases4[0], None, "mixed", torch, steps, args.warmup, batch=b4, mu_elem=mu4)
     out["uneven_batch_64"] = {"n_elem": r4["n_elem"], "n_pair": r4["n_pair"], "us_per_step": round(r4["ms"] * 1e3, 2),
@@ -543,6 +553,10 @@ def side_measurements(batched, torch, wl, args):
                          "value": round(r5["value"], 1), "stages_us": np.round(r5["stages"] * 1e3, 2).tolist(),
                          "increment_setup_ms": r5["inc"]["steady_ms"], "increment_first_ms": round(r5["t_inc"] * 1e3, 1),
                          "candidates_per_s": r5["inc"]["candidates_per_s"]}
+    if net is not None:
+        out["learned_prescreen_1000wire"] = prescreen_leg(
+            r5, net, K, torch, steps, args.warmup, "mixed 1000 wires",
+            "the 1000-wire strand, pair kernel over the learned watch list")
     del r5
     torch.cuda.empty_cache()
     out["coated_assembly"] = coated_assembly(wl.C3, batched["states"], torch)
