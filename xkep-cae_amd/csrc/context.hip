@@ -86,6 +86,9 @@ static void bind_coating(PairSoA& P, std::vector<DBuf>& b) {
     P.cneed = (int8_t*)b[PB_CNEED].p;
 }
 
+static_assert(PB_COUNT <= PERM_SET_MAX, "one permute_set launch moves every pair-state array");
+static_assert(PB_CRA + 9 == PB_COUNT, "the coating arrays are the last nine");
+
 static PairSoA soa_of(std::vector<DBuf>& b, int64_t cap) {
     PairSoA P{};
     P.na0 = (int*)b[PB_NA0].p; P.na1 = (int*)b[PB_NA1].p;
@@ -102,27 +105,43 @@ static PairSoA soa_of(std::vector<DBuf>& b, int64_t cap) {
     return P;
 }
 
+// The watch list and the coating are never on together (xk_lp_select refuses
+// the coating, xk_ctx_set_coating ends the watch list first), so the coating
+// arrays are neither compacted nor scattered back.
+static bool pb_coating(int k) { return k >= PB_CRA; }
+
 int Ctx::lp_compact(hipStream_t st) {
     wb.resize(PB_COUNT);
     const int64_t n = n_watch;
+    PermuteSet set;
     for (int k = 0; k < PB_COUNT; ++k) {
         XK_TRY(wb[k].ensure(PB_SZ[k] * (size_t)std::max<int64_t>(n, 1)));
+        if (pb_coating(k)) continue;
         const int comps = pb_comps(k);
-        XK_TRY(launch_permute((int)(PB_SZ[k] / comps), n, comps, pb[k].p, cap, wb[k].p, n,
-                              (const int*)lp_watch.p, 0, st));
+        set.a[set.count] = pb[k].p;
+        set.b[set.count] = wb[k].p;
+        set.esz[set.count] = (int)(PB_SZ[k] / comps);
+        set.comps[set.count] = comps;
+        ++set.count;
     }
+    XK_TRY(launch_permute_set(set, n, cap, n, (const int*)lp_watch.p, 0, st));
     PW = soa_of(wb, std::max<int64_t>(n, 1));
     return 0;
 }
 
 int Ctx::lp_sync(hipStream_t st, bool keep_on) {
     if (!lp_on) return 0;
+    PermuteSet set;
     for (int k = 0; k < PB_COUNT; ++k) {
-        if (!pb_written(k)) continue;
+        if (!pb_written(k) || pb_coating(k)) continue;
         const int comps = pb_comps(k);
-        XK_TRY(launch_permute((int)(PB_SZ[k] / comps), n_watch, comps, wb[k].p, std::max<int64_t>(n_watch, 1),
-                              pb[k].p, cap, (const int*)lp_watch.p, 1, st));
+        set.a[set.count] = wb[k].p;
+        set.b[set.count] = pb[k].p;
+        set.esz[set.count] = (int)(PB_SZ[k] / comps);
+        set.comps[set.count] = comps;
+        ++set.count;
     }
+    XK_TRY(launch_permute_set(set, n_watch, std::max<int64_t>(n_watch, 1), cap, (const int*)lp_watch.p, 1, st));
     if (!keep_on) lp_on = false;
     return 0;
 }

@@ -220,6 +220,49 @@ __global__ void permute_kernel(int64_t n, int comps, const T* __restrict__ a, in
     }
 }
 
+// Several arrays of one permutation in one launch: blockIdx.y picks the
+// array (element size and component count per array), each lane moves one
+// element's components as permute_kernel does.
+__global__ void permute_set_kernel(int64_t n, PermuteSet s, int64_t sa, int64_t sb, const int* __restrict__ idx,
+                                   int scatter) {
+    const int q = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t k = idx[j];
+    const int64_t src = scatter ? j : k, dst = scatter ? k : j;
+    const int comps = s.comps[q];
+    switch (s.esz[q]) {   // uniform over the workgroup
+        case 1: {
+            const int8_t* a = (const int8_t*)s.a[q];
+            int8_t* b = (int8_t*)s.b[q];
+            for (int c = 0; c < comps; ++c) b[c * sb + dst] = a[c * sa + src];
+            break;
+        }
+        case 4: {
+            const int* a = (const int*)s.a[q];
+            int* b = (int*)s.b[q];
+            for (int c = 0; c < comps; ++c) b[c * sb + dst] = a[c * sa + src];
+            break;
+        }
+        default: {
+            const double* a = (const double*)s.a[q];
+            double* b = (double*)s.b[q];
+            for (int c = 0; c < comps; ++c) b[c * sb + dst] = a[c * sa + src];
+            break;
+        }
+    }
+}
+
+hipError_t launch_permute_set(const PermuteSet& s, int64_t n, int64_t sa, int64_t sb, const int* idx, int scatter,
+                              hipStream_t st) {
+    if (n <= 0 || s.count <= 0) return hipSuccess;
+    for (int q = 0; q < s.count; ++q)
+        if (s.esz[q] != 1 && s.esz[q] != 4 && s.esz[q] != 8) return hipErrorInvalidValue;
+    const dim3 g((unsigned)((n + 255) / 256), (unsigned)s.count), blk(256);
+    hipLaunchKernelGGL(permute_set_kernel, g, blk, 0, st, n, s, sa, sb, idx, scatter);
+    return hipGetLastError();
+}
+
 hipError_t launch_permute(int esz, int64_t n, int comps, const void* a, int64_t sa, void* b, int64_t sb,
                           const int* idx, int scatter, hipStream_t st) {
     if (n <= 0) return hipSuccess;

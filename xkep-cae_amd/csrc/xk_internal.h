@@ -262,6 +262,17 @@ hipError_t launch_pair_replay(int64_t n_idx, const int* idx, const PairSoA& P, c
                               int64_t stride, int n_hist, hipStream_t st);
 hipError_t launch_lp_grad(int64_t n, const float* X, const float* y, const float* prm, float pos_w, float* partial,
                           float* grad, hipStream_t st);
+// arrays moved by one permute_set launch
+constexpr int PERM_SET_MAX = 32;
+struct PermuteSet {
+    int count = 0;
+    const void* a[PERM_SET_MAX];
+    void* b[PERM_SET_MAX];
+    int esz[PERM_SET_MAX];
+    int comps[PERM_SET_MAX];
+};
+hipError_t launch_permute_set(const PermuteSet& s, int64_t n, int64_t sa, int64_t sb, const int* idx, int scatter,
+                              hipStream_t st);
 hipError_t launch_permute(int esz, int64_t n, int comps, const void* a, int64_t sa, void* b, int64_t sb,
                           const int* idx, int scatter, hipStream_t st);
 hipError_t launch_lp_adam(float* prm, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
