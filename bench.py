@@ -346,20 +346,31 @@ def mixed_vs_fp64(case, states, torch):
     """The FP32-mixed assembly against the FP64 plain one on the same batch and
     state (one fresh assembly each, the same pattern): relative residual and
     tangent errors, the pairs' statuses (the active set) and the active count."""
-    res = {}
-    for prec in ("fp64", "mixed"):
+    res, geo = {}, {}
+    for prec in ("fp64", "mixed", "fp32"):
         ctx, asm, _ = setup(case, states, prec, torch)
         R, K, _, nact = asm.assemble()
         torch.cuda.synchronize()
         res[prec] = (R.cpu().numpy().astype(np.float64), K.cpu().numpy().astype(np.float64),
                      ctx.get_pairs()["status"].copy(), int(nact.item()))
+        pr = ctx.get_pairs()
+        geo[prec] = (pr["gap"].copy(), pr["normal"].copy(), pr["tangent1"].copy())
         del ctx, asm, R, K
         torch.cuda.empty_cache()
     (R64, K64, s64, n64), (R32, K32, s32, n32) = res["fp64"], res["mixed"]
+    Rf, Kf, sf, nf = res["fp32"]
+    rad = float(np.max(states[0].mesh.radii))
+    fp32 = {"R_rel_err": float(np.linalg.norm(Rf - R64) / np.linalg.norm(R64)),
+            "K_rel_err_frobenius": float(np.linalg.norm(Kf - K64) / np.linalg.norm(K64)),
+            "active_set_equal": bool(np.array_equal(s64, sf)), "n_active_fp32": nf,
+            "gap_max_abs_err_over_r": float(np.max(np.abs(geo["fp32"][0] - geo["fp64"][0]))) / rad,
+            "normal_max_abs_err": float(np.max(np.abs(geo["fp32"][1] - geo["fp64"][1]))),
+            "tangent1_max_abs_err": float(np.max(np.abs(geo["fp32"][2] - geo["fp64"][2])))}
     out = {"R_rel_err": float(np.linalg.norm(R32 - R64) / np.linalg.norm(R64)),
            "K_rel_err_frobenius": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
            "K_max_abs_err_over_max_K": float(np.max(np.abs(K32 - K64)) / np.abs(K64).max()),
            "active_set_equal": bool(np.array_equal(s64, s32)), "n_active_fp64": n64, "n_active_mixed": n32,
+           "fp32_pairs": fp32,
            "note": "FP32-mixed vs FP64 plain, one assembly of the headline batch at the same state; "
                    "the FP64 form equals the FP64 oracle within 1e-10 (tests)"}
     log(f"[bench] mixed vs fp64: {out}")
@@ -640,6 +651,11 @@ def main():
                   tune=not args.no_autotune)
     del plain_b["asm"], plain_b["ctx"]
     torch.cuda.empty_cache()
+    # the FP32 pair form (FP32 geometry of the non-contributing pairs) on the same batch
+    f32_b = run(f"fp32 x{args.batch}", case, states, "fp32", torch, args.steps, args.warmup,
+                tune=not args.no_autotune)
+    del f32_b["asm"], f32_b["ctx"]
+    torch.cuda.empty_cache()
     mixed_err = mixed_vs_fp64(case, states, torch)
 
     extra = {} if args.no_extra else side_measurements(batched, torch, wl, args)
@@ -710,6 +726,13 @@ def main():
         "fp64_batched": {"value": round(plain_b["value"], 1), "us_per_step": round(plain_b["ms"] * 1e3, 2),
                          "stages_us": np.round(plain_b["stages"] * 1e3, 2).tolist()},
         "mixed_vs_fp64": mixed_err,
+        "fp32_pairs_batched": {"value": round(f32_b["value"], 1), "us_per_step": round(f32_b["ms"] * 1e3, 2),
+                               "stages_us": np.round(f32_b["stages"] * 1e3, 2).tolist(),
+                               "speedup_vs_fp64_gpu": round(f32_b["value"] / plain_b["value"], 3),
+                               "vs_mixed": round(f32_b["value"] / value, 3),
+                               "note": "FP32 pair form: the mixed form plus the FP32 narrowphase and frame of the "
+                                       "non-contributing pairs, FP64 guard band at the hysteresis threshold; "
+                                       "its errors are mixed_vs_fp64.fp32_pairs"},
         "speedup_vs_fp64_gpu_single_instance": round(value / plain["value"], 3),
         "speedup_vs_fp64_gpu_single_instance_note": "batched FP32-mixed against one unbatched FP64 instance "
                                                     "(batching and precision change together)",

@@ -31,7 +31,13 @@ extern "C" {
 
 typedef struct xk_ctx xk_ctx;
 
-enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1 };
+/* XK_PREC_FP32: the mixed form plus the FP32 narrowphase and frame transport
+ * for pairs that contribute nothing (INACTIVE, no coating), with an FP64
+ * re-evaluation of every pair whose FP32 gap cannot decide the hysteresis
+ * threshold or whose segments are nearly parallel; contributing pairs, forces,
+ * residual and norm stay FP64.  Statuses and active sets equal the FP64 form's;
+ * the stored geometry of the FP32-evaluated pairs carries FP32 rounding. */
+enum { XK_PREC_FP64 = 0, XK_PREC_MIXED = 1, XK_PREC_FP32 = 2 };
 
 /* OR-ed into xk_ctx_create's precision: use the staged element path even when
  * the connectivity is made of element chains (the direct path is chosen

@@ -29,7 +29,7 @@ def test_1000_wire_candidates_and_assembly(gpu):
     dt = case.dt()
     out = {}
     x = m.node_coords + st.u.reshape(-1, 6)[:, :3]
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "fp32"):
         ctx = C.DeviceContext(m.node_coords, m.connectivity, s, radii=m.radii, precision=prec)
         ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
         ga.predict_device(ctx.torch.zeros(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev), dt)
@@ -59,7 +59,8 @@ def test_1000_wire_candidates_and_assembly(gpu):
     assert np.linalg.norm(R64 - ref.R) <= 1e-10 * np.linalg.norm(ref.R)
     assert sp.linalg.norm(K64 - ref.K) <= 1e-10 * sp.linalg.norm(ref.K)
     assert n64 == ref.n_active and np.array_equal(s64, ref.pairs.status)
-    Rm, Km, nm, sm, cm = out["mixed"]
-    assert np.array_equal(cm, c64)
-    assert np.linalg.norm(Rm - R64) <= 1e-6 * np.linalg.norm(R64)
-    assert nm == n64 and np.array_equal(sm, s64)
+    for prec in ("mixed", "fp32"):
+        Rm, Km, nm, sm, cm = out[prec]
+        assert np.array_equal(cm, c64), prec
+        assert np.linalg.norm(Rm - R64) <= 1e-6 * np.linalg.norm(R64), prec
+        assert nm == n64 and np.array_equal(sm, s64), prec

@@ -395,7 +395,7 @@ def test_mixed_precision_residual_and_active_set_at_91_wires(gpu, layers):
                              s.kappa_z).lumped_mass_diag(case.rho)
     dt = case.dt()
     outs = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "fp32"):
         ctx = C.DeviceContext(m.node_coords, m.connectivity, s, radii=m.radii, precision=prec)
         ga = T.GeneralizedAlphaProcess(Md, rho_inf=case.rho_inf)
         ga.predict_device(ctx.torch.zeros(ctx.ndof, dtype=ctx.torch.float64, device=ctx.dev), dt)
@@ -422,10 +422,11 @@ def test_mixed_precision_residual_and_active_set_at_91_wires(gpu, layers):
     assert srel(K64, ref.K) < FP64_TOL
     assert n64 == ref.n_active
     assert np.array_equal(s64, ref.pairs.status)
-    Rm, Km, nm, sm = outs["mixed"]
-    assert rel(Rm, R64) < MIXED_R_TOL
-    assert srel(Km, K64) < MIXED_K_TOL
-    assert nm == n64 and np.array_equal(sm, s64)
+    for prec in ("mixed", "fp32"):
+        Rm, Km, nm, sm = outs[prec]
+        assert rel(Rm, R64) < MIXED_R_TOL, prec
+        assert srel(Km, K64) < MIXED_K_TOL, prec
+        assert nm == n64 and np.array_equal(sm, s64), prec
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -30,8 +30,8 @@ from oracle import contact as ocon
 # first residual, hence the absolute floor relative to the call's first entry
 # (91 wires, FP64: the device solve and SuperLU differ by their rounding on a
 # capacitance of condition near 1e12; measured within 3e-6)
-HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3, "fp64_91": 1e-5}
-HIST_FLOOR = {"oracle": 1e-9, "fp64": 1e-9, "mixed": 1e-6, "fp64_91": 1e-9}
+HIST_RTOL = {"oracle": 1e-6, "fp64": 1e-6, "mixed": 1e-3, "fp32": 1e-3, "fp64_91": 1e-5}
+HIST_FLOOR = {"oracle": 1e-9, "fp64": 1e-9, "mixed": 1e-6, "fp32": 1e-6, "fp64_91": 1e-9}
 
 
 FIXTURES = ["newton_7wire.npz", "newton_7wire_mu03_f60.npz", "newton_7wire_mu03_f100.npz"]
@@ -157,7 +157,7 @@ def test_newton_trajectory_matches_reference_on_oracle_assembly(fixture):
 @pytest.mark.parametrize("fixture", FIXTURES)
 @pytest.mark.parametrize("mode", ["host_superlu", "device_hostctl", "device_graph_hostctl", "device_solve",
                                   "device_graph"])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_newton_trajectory_matches_reference_on_device(gpu, precision, mode, fixture):
     """The device assembly under the driver: with SuperLU on the host; with the
     Newton attempts on the device (device linear solve, line search and
@@ -225,7 +225,7 @@ def test_device_control_hands_missed_solves_to_the_host(gpu, use_graph):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_newton_trajectory_at_91_wires_matches_the_oracle_driver(gpu, precision):
     """The 91-wire strand of the benchmark (r = 0.4, L = 80, 150 elements per
     wire, lay angle 12 deg) squeezed by 0.1 per outer wire with Coulomb
@@ -236,7 +236,8 @@ def test_newton_trajectory_at_91_wires_matches_the_oracle_driver(gpu, precision)
     (tests/golden/newton_91wire_mu03_oracle.npz, written by
     tests/golden/make_oracle_trajectory.py; the oracle driver reproduces the
     reference solver's 7-wire trajectories exactly, and the reference's own
-    solver is too slow to run this strand), in FP64 and in FP32-mixed."""
+    solver is too slow to run this strand), in FP64, in FP32-mixed and in the
+    FP32 pair form (FP32 geometry of the non-contributing pairs)."""
     g, prm, cfg = case("newton_91wire_mu03_oracle.npz")
     S, C, Kc, wl = pkg("solver"), pkg("context"), pkg("contact"), pkg("workloads")
     r = float(g["radii"][0])

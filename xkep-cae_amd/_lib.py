@@ -22,7 +22,7 @@ vp = C.c_void_p
 
 PAIR_GEOM, PAIR_HYST, PAIR_HUBER, PAIR_FRIC, PAIR_TAN_C, PAIR_TAN_F = 1, 2, 4, 8, 16, 32
 PAIR_COAT_F, PAIR_COAT_FRIC, PAIR_COAT_K, PAIR_COAT_KF = 128, 256, 512, 1024
-PREC = {"fp64": 0, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "fp32": 2}
 OPT_STAGED = 256
 OPT_HOST_PATTERN = 512
 FLAG_COAT_UNPLACED = 8

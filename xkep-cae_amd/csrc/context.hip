@@ -470,7 +470,7 @@ int xk_ctx_create(xk_ctx** out, int n_nodes, int n_elems, const double* node_coo
     const bool force_staged = (precision & XK_OPT_STAGED) != 0;
     const bool host_pattern = (precision & XK_OPT_HOST_PATTERN) != 0;
     precision &= ~(XK_OPT_STAGED | XK_OPT_HOST_PATTERN);
-    if (precision != XK_PREC_FP64 && precision != XK_PREC_MIXED) {
+    if (precision != XK_PREC_FP64 && precision != XK_PREC_MIXED && precision != XK_PREC_FP32) {
         set_error("xk_ctx_create: unknown precision");
         return -1;
     }

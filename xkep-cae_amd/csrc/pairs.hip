@@ -45,6 +45,7 @@
 namespace xk {
 
 __device__ __forceinline__ double clip01(double x) { return fmin(fmax(x, 0.0), 1.0); }
+__device__ __forceinline__ float clip01f(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 __device__ __forceinline__ V3<double> node_x(const double* __restrict__ Xr, const double* __restrict__ u, int k) {
     V3<double> x{Xr[3 * k], Xr[3 * k + 1], Xr[3 * k + 2]};
@@ -59,10 +60,11 @@ __device__ __forceinline__ V3<double> node_du(const double* __restrict__ u, cons
     return a;
 }
 
-__device__ __forceinline__ V3<double> min_axis(V3<double> v) {
-    double ax = fabs(v.x), ay = fabs(v.y), az = fabs(v.z);
+template <class T>
+__device__ __forceinline__ V3<T> min_axis(V3<T> v) {
+    T ax = fabs(v.x), ay = fabs(v.y), az = fabs(v.z);
     int k = (ax <= ay && ax <= az) ? 0 : ((ay <= az) ? 1 : 2);
-    return V3<double>{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+    return V3<T>{T(k == 0 ? 1 : 0), T(k == 1 ? 1 : 0), T(k == 2 ? 1 : 0)};
 }
 
 // Clamped segment-segment closest points (_compute.py:55-151): s, t, the
@@ -98,56 +100,106 @@ __device__ __forceinline__ void closest_points(const V3<double>& xA0, const V3<d
 // Contact frame (_compute.py:154-261): n = nrm normalised, t1 = the previous
 // t1 parallel-transported along the minimal rotation pn -> n, projected and
 // normalised; else the previous t1 projected; else the min-|n| axis projected.
-// t2 = n x t1 is never stored.
-__device__ __forceinline__ void transport_frame(const V3<double>& nrm, const V3<double>& pn, const V3<double>& pt1,
-                                                V3<double>& n, V3<double>& t1) {
-    double nn_ = sqrt(dot(nrm, nrm));
-    nn_ = nn_ > 1e-30 ? nn_ : 1e-30;
-    n = V3<double>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
-    bool has_t = sqrt(dot(pt1, pt1)) > 1e-10;
-    bool has_n = sqrt(dot(pn, pn)) > 1e-10;
+// t2 = n x t1 is never stored.  T = float only in the FP32 pair form below.
+template <class T>
+__device__ __forceinline__ void transport_frame(const V3<T>& nrm, const V3<T>& pn, const V3<T>& pt1, V3<T>& n,
+                                                V3<T>& t1) {
+    T nn_ = sqrt(dot(nrm, nrm));
+    nn_ = nn_ > T(1e-30) ? nn_ : T(1e-30);
+    n = V3<T>{nrm.x / nn_, nrm.y / nn_, nrm.z / nn_};
+    bool has_t = sqrt(dot(pt1, pt1)) > T(1e-10);
+    bool has_n = sqrt(dot(pn, pn)) > T(1e-10);
     bool done = false;
     if (has_t && has_n) {
-        double on = sqrt(dot(pn, pn));
-        on = on > 1e-30 ? on : 1e-30;
-        V3<double> no{pn.x / on, pn.y / on, pn.z / on};
-        V3<double> v = cross(no, n);
-        double sv = sqrt(dot(v, v));
-        double cv = dot(no, n);
-        V3<double> tt = pt1;
-        if (sv >= 1e-12) {
-            V3<double> vxt = cross(v, pt1);
-            V3<double> vxvxt = cross(v, vxt);
-            double fac = (1.0 - cv) / (sv * sv);
+        T on = sqrt(dot(pn, pn));
+        on = on > T(1e-30) ? on : T(1e-30);
+        V3<T> no{pn.x / on, pn.y / on, pn.z / on};
+        V3<T> v = cross(no, n);
+        T sv = sqrt(dot(v, v));
+        T cv = dot(no, n);
+        V3<T> tt = pt1;
+        if (sv >= T(1e-12)) {
+            V3<T> vxt = cross(v, pt1);
+            V3<T> vxvxt = cross(v, vxt);
+            T fac = (T(1) - cv) / (sv * sv);
             tt = (pt1 + vxt) + fac * vxvxt;
         }
-        V3<double> raw = tt - dot(tt, n) * n;
-        double rn = sqrt(dot(raw, raw));
-        if (rn > 1e-10) {
-            t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+        V3<T> raw = tt - dot(tt, n) * n;
+        T rn = sqrt(dot(raw, raw));
+        if (rn > T(1e-10)) {
+            t1 = V3<T>{raw.x / rn, raw.y / rn, raw.z / rn};
             done = true;
         }
     }
     if (!done && has_t) {
-        V3<double> raw = pt1 - dot(pt1, n) * n;
-        double rn = sqrt(dot(raw, raw));
-        if (rn > 1e-10) {
-            t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+        V3<T> raw = pt1 - dot(pt1, n) * n;
+        T rn = sqrt(dot(raw, raw));
+        if (rn > T(1e-10)) {
+            t1 = V3<T>{raw.x / rn, raw.y / rn, raw.z / rn};
             done = true;
         }
     }
     if (!done) {
-        V3<double> ref = min_axis(n);
-        V3<double> raw = ref - dot(ref, n) * n;
-        double rn = sqrt(dot(raw, raw));
-        rn = rn > 1e-30 ? rn : 1e-30;
-        t1 = V3<double>{raw.x / rn, raw.y / rn, raw.z / rn};
+        V3<T> ref = min_axis(n);
+        V3<T> raw = ref - dot(ref, n) * n;
+        T rn = sqrt(dot(raw, raw));
+        rn = rn > T(1e-30) ? rn : T(1e-30);
+        t1 = V3<T>{raw.x / rn, raw.y / rn, raw.z / rn};
     }
 }
 
+// FP32 form of the narrowphase and frame of a pair that contributes nothing
+// (status INACTIVE, no coating).  The segment differences are formed in FP64
+// from the FP64 positions and rounded once; the closest point, distance and
+// frame transport then run in FP32, and s, t, gap, n, t1 are stored widened.
+// When `decide` (the hysteresis pass), the FP32 gap must show the pair stays
+// INACTIVE: kept only if gap > g_on + margin, where the margin bounds how far
+// the FP32 closest point can sit above the true minimum distance (the FP32
+// parameters move by about 2^-23 ac / det, the distance by at most the segment
+// lengths times that).  Nearly parallel or degenerate segments (det < 1e-4 ac,
+// a or c below 1e-10), whose FP32 parameters are ill-conditioned, are never
+// kept.  Returns false when the caller must evaluate the pair in FP64.
+__device__ __forceinline__ bool geom_f32(const V3<double>& xA0, const V3<double>& xA1, const V3<double>& xB0,
+                                         const V3<double>& xB1, double rsum, bool decide, double g_on,
+                                         const V3<double>& pn, const V3<double>& pt1, double& s_o, double& t_o,
+                                         double& gap_o, V3<double>& n_o, V3<double>& t1_o) {
+    auto f32 = [](const V3<double>& v) { return V3<float>{(float)v.x, (float)v.y, (float)v.z}; };
+    const V3<float> dA = f32(xA1 - xA0), dB = f32(xB1 - xB0), w0 = f32(xA0 - xB0);
+    const float a = dot(dA, dA), b = dot(dA, dB), c = dot(dB, dB), d = dot(dA, w0), e = dot(dB, w0);
+    const float ac = a * c, det = ac - b * b;
+    if (!(a > 1e-10f && c > 1e-10f && det >= 1e-4f * ac)) return false;
+    const float s_unc = (b * e - c * d) / det, t_unc = (a * e - b * d) / det;
+    float s = clip01f(s_unc), t = clip01f(t_unc);
+    if (s_unc < 0.0f || s_unc > 1.0f) t = clip01f((b * s + e) / c);
+    if (t_unc < 0.0f || t_unc > 1.0f) {
+        s = clip01f((b * t - d) / a);
+        t = clip01f((b * s + e) / c);
+    }
+    const V3<float> diff = (w0 + s * dA) - t * dB;
+    const float dist = sqrtf(dot(diff, diff));
+    const double gap = (double)dist - rsum;
+    if (decide) {
+        const double margin = 2.4e-6 * ((double)sqrtf(a) + (double)sqrtf(c)) * ((double)ac / (double)det) +
+                              1e-6 * (double)dist;
+        if (!(gap > g_on + margin)) return false;
+    }
+    const float sd = dist > 1e-30f ? dist : 1e-30f;
+    V3<float> nrm{diff.x / sd, diff.y / sd, diff.z / sd};
+    if (dist < 1e-30f) nrm = V3<float>{0.0f, 0.0f, 1.0f};
+    V3<float> n, t1;
+    transport_frame(nrm, f32(pn), f32(pt1), n, t1);
+    s_o = s;
+    t_o = t;
+    gap_o = gap;
+    n_o = V3<double>{n.x, n.y, n.z};
+    t1_o = V3<double>{t1.x, t1.y, t1.z};
+    return true;
+}
+
+// G32: the FP32 pair form (geom_f32 for pairs that contribute nothing).
 // COAT: the Kelvin-Voigt coating terms are compiled in (pp.coat_on); the
 // plain instantiation carries none of their registers.
-template <class GT, int BS, bool COAT>
+template <class GT, int BS, bool COAT, bool G32>
 __global__ void __launch_bounds__(BS)
 pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const double* __restrict__ u,
             const double* __restrict__ u_ref, int modes, PairParams pp, double* __restrict__ nf,
@@ -187,13 +239,17 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
             V3<double> xB0 = node_x(Xr, u, b0), xB1 = node_x(Xr, u, b1);
             double dist;
             V3<double> nrm;
-            closest_points(xA0, xA1, xB0, xB1, s, t, dist, nrm);
+            // COAT and G32 are never both set (launch_pairs_bs)
+            const bool in_f32 = G32 && st == ST_INACTIVE && geom &&
+                                geom_f32(xA0, xA1, xB0, xB1, ra + rb, (modes & PAIR_HYST) != 0, pp.g_on, pn, pt1, s, t,
+                                         gap, n, t1);
+            if (!in_f32) closest_points(xA0, xA1, xB0, xB1, s, t, dist, nrm);
             if (coat) {
                 const double cra = P.cra[i], crb = P.crb[i];
                 gap = dist - (cra + crb);
                 const double x = ((ra - cra) + (rb - crb)) - gap;
                 cc = x > 0.0 ? x : 0.0;
-            } else {
+            } else if (!in_f32) {
                 gap = dist - (ra + rb);
             }
             if (modes & PAIR_PROBE) {
@@ -201,7 +257,7 @@ pair_kernel(int64_t n_pairs, PairSoA P, const double* __restrict__ Xr, const dou
                 goto done_pair;
             }
 
-            transport_frame(nrm, pn, pt1, n, t1);
+            if (!in_f32) transport_frame(nrm, pn, pt1, n, t1);
             __builtin_nontemporal_store(s, &P.s[i]);
             __builtin_nontemporal_store(t, &P.t[i]);
             __builtin_nontemporal_store(gap, &P.gap[i]);
@@ -543,25 +599,28 @@ bool pair_block_ok(int bs) { return bs == 64 || bs == 128 || bs == 256 || bs == 
 template <class GT, int BS>
 static void launch_pairs_bs(int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u, const double* u_ref,
                             int modes, const PairParams& pp, double* nf, GT* sg, int* act_part, hipStream_t st,
-                            const int* idx) {
+                            const int* idx, bool g32) {
     dim3 blk(BS), grd((unsigned)xcd_padded(pair_blocks(n_pairs, BS)));
     if (pp.coat_on)
-        hipLaunchKernelGGL((pair_kernel<GT, BS, true>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg,
-                           act_part, idx);
+        hipLaunchKernelGGL((pair_kernel<GT, BS, true, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp,
+                           nf, sg, act_part, idx);
+    else if (g32)
+        hipLaunchKernelGGL((pair_kernel<GT, BS, false, true>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp,
+                           nf, sg, act_part, idx);
     else
-        hipLaunchKernelGGL((pair_kernel<GT, BS, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg,
-                           act_part, idx);
+        hipLaunchKernelGGL((pair_kernel<GT, BS, false, false>), grd, blk, 0, st, n_pairs, P, Xr, u, u_ref, modes, pp,
+                           nf, sg, act_part, idx);
 }
 
 template <class GT>
 static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                                  const double* u_ref, int modes, const PairParams& pp, double* nf, GT* sg,
-                                 int* act_part, hipStream_t st, const int* idx) {
+                                 int* act_part, hipStream_t st, const int* idx, bool g32) {
     switch (bs) {
-        case 64: launch_pairs_bs<GT, 64>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
-        case 128: launch_pairs_bs<GT, 128>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
-        case 256: launch_pairs_bs<GT, 256>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
-        case 512: launch_pairs_bs<GT, 512>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx); break;
+        case 64: launch_pairs_bs<GT, 64>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx, g32); break;
+        case 128: launch_pairs_bs<GT, 128>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx, g32); break;
+        case 256: launch_pairs_bs<GT, 256>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx, g32); break;
+        case 512: launch_pairs_bs<GT, 512>(n_pairs, P, Xr, u, u_ref, modes, pp, nf, sg, act_part, st, idx, g32); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -571,12 +630,13 @@ static hipError_t launch_pairs_t(int bs, int64_t n_pairs, const PairSoA& P, cons
 // solve factors the pairs' blocks from them, and the FP32 rounding of contact
 // blocks changes the Newton trajectory of frictional strands (measured at 91
 // wires); the mixed form rounds them to FP32 only where the gather writes K.
+// prec XK_PREC_FP32 (2) adds the FP32 geometry of non-contributing pairs.
 hipError_t launch_pairs(int prec, int64_t n_pairs, const PairSoA& P, const double* Xr, const double* u,
                         const double* u_ref, int modes, const PairParams& pp, double* nf, void* sg,
                         int* act_part, hipStream_t st, const int* idx, int bs) {
-    (void)prec;
     if (n_pairs <= 0) return hipSuccess;
-    return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, nf, (double*)sg, act_part, st, idx);
+    return launch_pairs_t<double>(bs, n_pairs, P, Xr, u, u_ref, modes, pp, nf, (double*)sg, act_part, st, idx,
+                                  prec == 2);
 }
 
 }  // namespace xk
